@@ -1,0 +1,194 @@
+"""CFPQ fixpoint benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 4] [--kernel auto]
+
+A step is one complete CFPQ fixpoint (all rules to the least fixpoint) of
+the chosen seeded workload, starting from the terminal matrices, which are
+resident in HBM before the timed region (each step first restores them with
+a device-side copy, which is inside the timed region).  The default workload
+is the 65,536-vertex ontology same-generation query (SG grammar, 7
+nonterminals, ~4.5e5 edges).
+
+value = effective Tbit-op/s = 2 N^3 x A_ref / (time per step), where A_ref
+is the number of add_mul calls the reference worklist driver makes on this
+workload (recorded in tests/golden/fixpoints.json by the CPU oracle).
+
+The JSON line also carries
+  roofline      the product kernel (k_product): algorithmic bytes per launch
+                (A rows 32 B, gathered B rows 32 B, staged B tiles 8 KiB,
+                target tile 8 KiB per item, written rows 32 B) / its mean
+                launch time (hipEvents on the engine's stream), vs 8 TB/s;
+  cpu_baseline  the CPU oracle's reference worklist (dense M4RM, k = 8) on the
+                same workload, run for a bounded time and expressed in the
+                same unit.
+"""
+import argparse
+import importlib
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+GOLDEN = os.path.join(ROOT, "tests", "golden", "fixpoints.json")
+TRAFFIC = os.path.join(ROOT, "profiles", "k_product_traffic.json")
+
+WORKLOADS = {
+    0: ("config0", "dyck_er_n1024"),
+    1: ("config1", "sg_forest_n16384"),
+    4: ("config4", "sg_forest_n65536_types8"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(q, a_ref, budget_s):
+    import oracle  # the checker; timed here only as the reported baseline
+
+    g = oracle.ParsedGrammar(q.grammar)
+    mats = oracle.initial_matrices(g, q.n, q.src, q.dst, q.label, q.labels)
+    calls, secs = oracle.fixpoint_worklist_budget(q.n, g.rules_array(), mats, budget_s, "m4rm")
+    per_call = secs / max(calls, 1)
+    value = 2.0 * q.n ** 3 * calls / secs / 1e12
+    return {
+        "value": round(value, 4),
+        "unit": "Tbit-op/s",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": f"first {calls} of {a_ref} add_mul calls of the reference worklist "
+                  f"(dense M4RM, k=8), {secs:.1f} s, {per_call:.2f} s per call; "
+                  f"extrapolated full fixpoint {per_call * a_ref:.0f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=4, choices=sorted(WORKLOADS))
+    ap.add_argument("--kernel", default="auto")
+    ap.add_argument("--iters-per-sync", type=int, default=0)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is designed but not built", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = importlib.import_module("cfpq-on-gpgpu_amd")
+    gen = importlib.import_module("cfpq-on-gpgpu_amd.gen")
+    cf = importlib.import_module("cfpq-on-gpgpu_amd.cfpq")
+    key, wname = WORKLOADS[args.config]
+    golden = json.load(open(GOLDEN))[key]
+    q = gen.config(args.config)
+    assert q.digest() == golden["digest"], "workload generator drifted from the golden fixture"
+    a_ref = int(golden["add_mul_calls"])
+    n = q.n
+    log(f"[bench] workload {wname}: n={n} edges={q.n_edges} A_ref={a_ref}")
+
+    eng = pkg.Engine(0, kernel=args.kernel, iters_per_sync=args.iters_per_sync)
+    gr = cf.Grammar(q.grammar, text=True)
+    mats = gr.load(eng, cf.Graph.from_query(q))
+    init = []
+    for m in mats:
+        c = eng.matrix(n)
+        c.copy_from(m)
+        init.append(c)
+    rules = gr.rule_triples()
+
+    def step():
+        for m, c in zip(mats, init):
+            m.copy_from(c)
+        return eng.fixpoint(rules, mats)
+
+    for w in range(args.warmup):
+        st = step()
+        log(f"[bench] warmup {w}: {st['fixpoint_ms']:.3f} ms, {st['iterations']} steps")
+    eng.sync()
+    stats = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        stats.append(step())
+    eng.sync()
+    wall = time.perf_counter() - t0
+    log(f"[bench] timed {args.steps} steps: {wall * 1e3:.2f} ms")
+
+    # size-independent parity check at full size: fingerprints + counts vs the oracle fixture
+    got = gr.results()
+    want = {k: tuple(v) for k, v in golden["fingerprints"].items()}
+    bit_exact = got == want
+    if not bit_exact:
+        log(f"[bench] MISMATCH vs oracle fixture: {got} vs {want}")
+
+    ms_per_step = wall * 1e3 / args.steps
+    fix_ms = [s["fixpoint_ms"] for s in stats]
+    value = 2.0 * n ** 3 * a_ref / (ms_per_step / 1e3) / 1e12
+
+    # roofline of the product kernel
+    launches = sum(s["product_launches"] for s in stats)
+    prod_ms = sum(s["product_ms"] for s in stats)
+    alg_bytes = sum(32 * (s["a_rows"] + s["b_rows"] + s["out_rows"]) + 8192 * (s["tile_products_dense"] + s["items"])
+                    for s in stats)
+    per_launch_bytes = alg_bytes / max(launches, 1)
+    per_launch_ms = prod_ms / max(launches, 1)
+    achieved = per_launch_bytes / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
+    traffic = None
+    if os.path.exists(TRAFFIC):
+        t = json.load(open(TRAFFIC))
+        if t.get("workload") == wname:
+            traffic = t.get("bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "k_product", "launch_ms": round(per_launch_ms, 5),
+                "alg_bytes_per_launch": round(per_launch_bytes)}
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline (budget {args.cpu_budget:.0f} s)")
+        cpu = cpu_baseline(q, a_ref, args.cpu_budget)
+
+    s0 = stats[-1]
+    line = {
+        "metric": "CFPQ fixpoint time (s) & eff. Tbit-op/s; batched/learned vs plain, bit-exact",
+        "value": round(value, 3),
+        "unit": "Tbit-op/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded generator, same inputs as the CPU oracle)",
+        "config": {"workload": wname, "n": n, "edges": q.n_edges, "nonterminals": gr.nonterminals_count,
+                   "rules": len(rules), "A_ref": a_ref, "engine": "plain delta fixpoint",
+                   "kernel": args.kernel, "parallelism": "single GPU"},
+        "fixpoint_s": round(statistics.median(fix_ms) / 1e3, 6),
+        "fixpoint_ms_device_median": round(statistics.median(fix_ms), 4),
+        "fixpoint_ms_device_std": round(statistics.pstdev(fix_ms), 4),
+        "iterations": s0["iterations"],
+        "tile_products": s0["tile_products"],
+        "bit_exact": bit_exact,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    gr.free()
+    for c in init:
+        c.free()
+    eng.close()
+    if not bit_exact:
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

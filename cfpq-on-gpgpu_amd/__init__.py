@@ -1,0 +1,266 @@
+"""MI355X (gfx950) CFPQ engine -- Python binding of the C ABI in include/cfpq_hip.h.
+
+The engine itself is ``libcfpq_hip.so`` (hand-written HIP kernels, built
+in-tree by ``make -C cfpq-on-gpgpu_amd``).  This module only marshals
+arguments through ``ctypes``; there is no CPU fallback: if the library is
+missing, importing :func:`lib` raises.
+
+The package directory name contains dashes, so import it with
+``importlib.import_module("cfpq-on-gpgpu_amd")``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcfpq_hip.so")
+
+KERNEL_AUTO, KERNEL_GATHER, KERNEL_LDS, KERNEL_M4R, KERNEL_MFMA = range(5)
+KERNELS = {"auto": KERNEL_AUTO, "gather": KERNEL_GATHER, "lds": KERNEL_LDS, "m4r": KERNEL_M4R, "mfma": KERNEL_MFMA}
+
+E_INVALID, E_DEVICE, E_NOMEM, E_LIMIT = -1, -2, -3, -4
+
+
+class CfpqError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cfpq error {code}: {msg}")
+        self.code = code
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [
+        ("kernel", ctypes.c_int32),
+        ("dense_bits", ctypes.c_uint32),
+        ("mfma_bits", ctypes.c_uint32),
+        ("max_iterations", ctypes.c_uint32),
+        ("final_full_pass", ctypes.c_int32),
+        ("iters_per_sync", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class Rule(ctypes.Structure):
+    _fields_ = [("head", ctypes.c_uint32), ("left", ctypes.c_uint32), ("right", ctypes.c_uint32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_uint32),
+        ("full_passes", ctypes.c_uint32),
+        ("new_bits", ctypes.c_uint64),
+        ("items", ctypes.c_uint64),
+        ("tile_products", ctypes.c_uint64),
+        ("tile_products_dense", ctypes.c_uint64),
+        ("tile_products_mfma", ctypes.c_uint64),
+        ("a_rows", ctypes.c_uint64),
+        ("b_rows", ctypes.c_uint64),
+        ("b_rows_lds", ctypes.c_uint64),
+        ("out_rows", ctypes.c_uint64),
+        ("fixpoint_ms", ctypes.c_double),
+        ("product_ms", ctypes.c_double),
+        ("product_launches", ctypes.c_uint32),
+        ("host_syncs", ctypes.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Query(ctypes.Structure):
+    _fields_ = [
+        ("mats", ctypes.POINTER(ctypes.c_void_p)),
+        ("nmats", ctypes.c_size_t),
+        ("rules", ctypes.POINTER(Rule)),
+        ("nrules", ctypes.c_size_t),
+    ]
+
+
+#: every symbol include/cfpq_hip.h declares
+EXPORTS = (
+    "cfpq_default_opts", "cfpq_create", "cfpq_destroy", "cfpq_last_error", "cfpq_set_opts", "cfpq_sync",
+    "cfpq_mat_alloc", "cfpq_mat_free", "cfpq_mat_n", "cfpq_mat_upload", "cfpq_mat_download",
+    "cfpq_mat_set_bits", "cfpq_mat_popcount", "cfpq_mat_copy", "cfpq_mat_fingerprint", "cfpq_add_mul", "cfpq_fixpoint",
+    "cfpq_batch_fixpoint",
+)
+
+_lib = None
+
+
+def lib():
+    """Load ``libcfpq_hip.so`` (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, u64, sz, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    sig = {
+        "cfpq_default_opts": (None, [ctypes.POINTER(Opts)]),
+        "cfpq_create": (i32, [i32, ctypes.POINTER(Opts), ctypes.POINTER(vp)]),
+        "cfpq_destroy": (i32, [vp]),
+        "cfpq_last_error": (ctypes.c_char_p, [vp]),
+        "cfpq_set_opts": (i32, [vp, ctypes.POINTER(Opts)]),
+        "cfpq_sync": (i32, [vp]),
+        "cfpq_mat_alloc": (i32, [vp, u32, ctypes.POINTER(vp)]),
+        "cfpq_mat_free": (i32, [vp, vp]),
+        "cfpq_mat_n": (u32, [vp]),
+        "cfpq_mat_upload": (i32, [vp, vp, u64p, sz]),
+        "cfpq_mat_download": (i32, [vp, vp, u64p, sz]),
+        "cfpq_mat_set_bits": (i32, [vp, vp, u32p, u32p, sz]),
+        "cfpq_mat_popcount": (i32, [vp, vp, u64p]),
+        "cfpq_mat_copy": (i32, [vp, vp, vp]),
+        "cfpq_mat_fingerprint": (i32, [vp, vp, u64p, u64p]),
+        "cfpq_add_mul": (i32, [vp, vp, vp, vp, ctypes.POINTER(i32)]),
+        "cfpq_fixpoint": (i32, [vp, ctypes.POINTER(Rule), sz, ctypes.POINTER(vp), sz, ctypes.POINTER(Stats)]),
+        "cfpq_batch_fixpoint": (i32, [vp, ctypes.POINTER(Query), sz, ctypes.POINTER(Stats)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def make_opts(kernel="auto", dense_bits=0, mfma_bits=0, max_iterations=0, final_full_pass=False,
+              iters_per_sync=0, use_graph=False) -> Opts:
+    o = Opts()
+    lib().cfpq_default_opts(ctypes.byref(o))
+    o.kernel = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+    o.dense_bits = dense_bits
+    o.mfma_bits = mfma_bits
+    o.max_iterations = max_iterations
+    o.final_full_pass = int(bool(final_full_pass))
+    o.iters_per_sync = iters_per_sync
+    o.use_graph = int(bool(use_graph))
+    return o
+
+
+def _rows_view(a: np.ndarray):
+    assert a.dtype == np.uint64 and a.ndim == 2 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), a.shape[1]
+
+
+class Engine:
+    """One device context (``cfpq_ctx``)."""
+
+    def __init__(self, device: int = 0, **opts):
+        self._L = lib()
+        self._h = ctypes.c_void_p()
+        o = make_opts(**opts)
+        rc = self._L.cfpq_create(device, ctypes.byref(o), ctypes.byref(self._h))
+        if rc != 0:
+            raise CfpqError(rc, "cfpq_create failed")
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise CfpqError(rc, self._L.cfpq_last_error(self._h).decode())
+
+    def set_opts(self, **opts):
+        o = make_opts(**opts)
+        self._check(self._L.cfpq_set_opts(self._h, ctypes.byref(o)))
+
+    def close(self):
+        if self._h:
+            self._L.cfpq_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def matrix(self, n: int) -> "DeviceMatrix":
+        return DeviceMatrix(self, n)
+
+    def sync(self):
+        self._check(self._L.cfpq_sync(self._h))
+
+    def add_mul(self, c: "DeviceMatrix", a: "DeviceMatrix", b: "DeviceMatrix") -> bool:
+        ch = ctypes.c_int(0)
+        self._check(self._L.cfpq_add_mul(self._h, c._h, a._h, b._h, ctypes.byref(ch)))
+        return bool(ch.value)
+
+    def fixpoint(self, rules, mats) -> dict:
+        rr = (Rule * max(1, len(rules)))(*[Rule(*map(int, r)) for r in rules])
+        mm = (ctypes.c_void_p * len(mats))(*[m._h.value for m in mats])
+        st = Stats()
+        self._check(self._L.cfpq_fixpoint(self._h, rr, len(rules), mm, len(mats), ctypes.byref(st)))
+        return st.as_dict()
+
+    def batch_fixpoint(self, queries) -> dict:
+        """queries: list of (rules, mats)."""
+        keep = []
+        qs = (Query * max(1, len(queries)))()
+        for i, (rules, mats) in enumerate(queries):
+            rr = (Rule * max(1, len(rules)))(*[Rule(*map(int, r)) for r in rules])
+            mm = (ctypes.c_void_p * len(mats))(*[m._h.value for m in mats])
+            keep += [rr, mm]
+            qs[i].mats = mm
+            qs[i].nmats = len(mats)
+            qs[i].rules = rr
+            qs[i].nrules = len(rules)
+        st = Stats()
+        self._check(self._L.cfpq_batch_fixpoint(self._h, qs, len(queries), ctypes.byref(st)))
+        return st.as_dict()
+
+
+class DeviceMatrix:
+    """An n x n bit matrix resident on the device (``cfpq_mat``)."""
+
+    def __init__(self, eng: Engine, n: int):
+        self.eng = eng
+        self.n = int(n)
+        self._h = ctypes.c_void_p()
+        eng._check(eng._L.cfpq_mat_alloc(eng._h, self.n, ctypes.byref(self._h)))
+
+    def free(self):
+        if self._h and self.eng._h:
+            self.eng._check(self.eng._L.cfpq_mat_free(self.eng._h, self._h))
+        self._h = ctypes.c_void_p()
+
+    @property
+    def wpr(self) -> int:
+        return (self.n + 63) // 64
+
+    def upload(self, rows: np.ndarray):
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        p, w = _rows_view(rows)
+        if rows.shape[0] != self.n:
+            raise ValueError("row count mismatch")
+        self.eng._check(self.eng._L.cfpq_mat_upload(self.eng._h, self._h, p, w))
+
+    def download(self) -> np.ndarray:
+        out = np.zeros((self.n, self.wpr), dtype=np.uint64)
+        p, w = _rows_view(out)
+        self.eng._check(self.eng._L.cfpq_mat_download(self.eng._h, self._h, p, w))
+        return out
+
+    def set_bits(self, rows, cols):
+        r = np.ascontiguousarray(rows, dtype=np.uint32)
+        c = np.ascontiguousarray(cols, dtype=np.uint32)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        self.eng._check(self.eng._L.cfpq_mat_set_bits(self.eng._h, self._h, r.ctypes.data_as(u32p),
+                                                      c.ctypes.data_as(u32p), r.shape[0]))
+
+    def popcount(self) -> int:
+        v = ctypes.c_uint64()
+        self.eng._check(self.eng._L.cfpq_mat_popcount(self.eng._h, self._h, ctypes.byref(v)))
+        return int(v.value)
+
+    def copy_from(self, src: "DeviceMatrix"):
+        self.eng._check(self.eng._L.cfpq_mat_copy(self.eng._h, self._h, src._h))
+
+    def fingerprint(self) -> tuple[int, int]:
+        h = ctypes.c_uint64()
+        c = ctypes.c_uint64()
+        self.eng._check(self.eng._L.cfpq_mat_fingerprint(self.eng._h, self._h, ctypes.byref(h), ctypes.byref(c)))
+        return int(h.value), int(c.value)

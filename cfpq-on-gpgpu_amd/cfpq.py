@@ -1,0 +1,163 @@
+"""Python mirror of the reference's cfpq-cpp interface (Grammar / Graph),
+driving the gfx950 engine through the C ABI.
+
+Same names, file formats and semantics as the reference:
+  Grammar(path)                       cfpq-cpp/Grammar.cpp:17-51
+  Graph(path)                         cfpq-cpp/Graph.cpp:11-20
+  Grammar.intersection_with_graph     cfpq-cpp/Grammar.h:23-81 -> (total_ms, algorithm_ms)
+  Grammar.print_results(path)         cfpq-cpp/Grammar.cpp:81-101 ("name hash count", name order)
+The C++ drop-in (include/cfpq/*.h) is the production host; this module is
+what the tests and the benchmark use.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import Engine
+
+
+class Grammar:
+    def __init__(self, source: str, text: bool = False):
+        if not text:
+            with open(source) as f:
+                source = f.read()
+        self.nonterminal_to_index: dict[str, int] = {}
+        self.epsilon_nonterminals: list[int] = []
+        self.terminal_to_nonterminals: dict[str, list[int]] = {}
+        self.rules: list[tuple[int, tuple[int, int]]] = []
+        for line in source.splitlines():
+            terms = line.split()
+            if not terms:
+                continue
+            head = self._index(terms[0])
+            if len(terms) == 1:
+                self.epsilon_nonterminals.append(head)
+            elif len(terms) == 2:
+                self.terminal_to_nonterminals.setdefault(terms[1], []).append(head)
+            elif len(terms) == 3:
+                b1 = self._index(terms[1])
+                b2 = self._index(terms[2])
+                self.rules.append((head, (b1, b2)))
+        self.vertices_count = 0
+        self.matrices = []
+
+    def _index(self, name: str) -> int:
+        if name not in self.nonterminal_to_index:
+            self.nonterminal_to_index[name] = len(self.nonterminal_to_index)
+        return self.nonterminal_to_index[name]
+
+    @property
+    def nonterminals_count(self) -> int:
+        return len(self.nonterminal_to_index)
+
+    def rule_triples(self) -> list[tuple[int, int, int]]:
+        return [(h, a, b) for h, (a, b) in self.rules]
+
+    def load(self, engine: Engine, graph: "Graph"):
+        """Allocate one device matrix per nonterminal and set terminal and
+        epsilon bits (cfpq-cpp/Grammar.h:26-51)."""
+        n = graph.vertices_count
+        self.vertices_count = n
+        self.matrices = [engine.matrix(n) for _ in range(self.nonterminals_count)]
+        rows = [[] for _ in range(self.nonterminals_count)]
+        cols = [[] for _ in range(self.nonterminals_count)]
+        for lab, (src, dst) in graph.edges_by_label().items():
+            for nt in self.terminal_to_nonterminals.get(lab, []):
+                rows[nt].append(src)
+                cols[nt].append(dst)
+        for nt in self.epsilon_nonterminals:
+            rows[nt].append(np.arange(n, dtype=np.uint32))
+            cols[nt].append(np.arange(n, dtype=np.uint32))
+        for nt, m in enumerate(self.matrices):
+            if rows[nt]:
+                m.set_bits(np.concatenate(rows[nt]), np.concatenate(cols[nt]))
+        return self.matrices
+
+    def intersection_with_graph(self, graph: "Graph", engine: Engine, mode: str = "fixpoint"):
+        """Returns (total_ms, algorithm_ms, stats).  mode "fixpoint" runs the
+        device-resident delta fixpoint; "worklist" runs the reference's
+        worklist loop (Grammar.h:58-69) over cfpq_add_mul."""
+        self.load(engine, graph)
+        t0 = time.perf_counter()
+        engine.sync()
+        ta = time.perf_counter()
+        if mode == "fixpoint":
+            stats = engine.fixpoint(self.rule_triples(), self.matrices)
+        elif mode == "worklist":
+            stats = {"add_mul_calls": self._worklist(engine)}
+        else:
+            raise ValueError(mode)
+        engine.sync()
+        tb = time.perf_counter()
+        return (tb - t0) * 1e3, (tb - ta) * 1e3, stats
+
+    def _worklist(self, engine: Engine) -> int:
+        with_nt = [[] for _ in range(self.nonterminals_count)]
+        todo = []
+        for i, (h, (a, b)) in enumerate(self.rules):
+            with_nt[a].append(i)
+            with_nt[b].append(i)
+            todo.append(i)
+        pending = set(todo)
+        calls = 0
+        while todo:
+            i = todo.pop(0)
+            pending.discard(i)
+            h, (a, b) = self.rules[i]
+            calls += 1
+            if engine.add_mul(self.matrices[h], self.matrices[a], self.matrices[b]):
+                for j in with_nt[h]:
+                    if j not in pending:
+                        pending.add(j)
+                        todo.append(j)
+        return calls
+
+    def results(self) -> dict[str, tuple[int, int]]:
+        """name -> (hash, count), in the reference's name order."""
+        return {name: self.matrices[idx].fingerprint() for name, idx in sorted(self.nonterminal_to_index.items(),
+                                                                                key=lambda kv: kv[0].encode())}
+
+    def print_results(self, path: str):
+        with open(path, "w") as f:
+            for name, (h, c) in self.results().items():
+                f.write(f"{name} {h} {c}\n")
+
+    def free(self):
+        for m in self.matrices:
+            m.free()
+        self.matrices = []
+
+
+class Graph:
+    def __init__(self, source=None, *, src=None, dst=None, labels=None, n=None):
+        """Either a path to a ``from label to`` edge list, or arrays
+        (src, dst, labels as a sequence of strings) plus an optional n."""
+        if source is not None:
+            s, l, d = [], [], []
+            with open(source) as f:
+                toks = f.read().split()
+            for i in range(0, len(toks) - 2, 3):
+                s.append(int(toks[i]))
+                l.append(toks[i + 1])
+                d.append(int(toks[i + 2]))
+            src, dst, labels = s, d, l
+        self.src = np.asarray(src, dtype=np.uint32)
+        self.dst = np.asarray(dst, dtype=np.uint32)
+        self.labels = list(labels)
+        vc = int(max(self.src.max(initial=0), self.dst.max(initial=0)) + 1) if self.src.size else 0
+        self.vertices_count = max(vc, n or 0)
+
+    @classmethod
+    def from_query(cls, q) -> "Graph":
+        names = np.asarray(q.labels, dtype=object)[q.label]
+        return cls(src=q.src, dst=q.dst, labels=names, n=q.n)
+
+    def edges_by_label(self) -> dict[str, tuple[np.ndarray, np.ndarray]]:
+        lab = np.asarray(self.labels, dtype=object)
+        out = {}
+        for name in dict.fromkeys(self.labels):
+            sel = lab == name
+            out[name] = (self.src[sel], self.dst[sel])
+        return out

@@ -1,0 +1,798 @@
+// Host side of the C ABI (include/cfpq_hip.h): contexts, device matrices,
+// the add_mul call and the device-resident delta fixpoint.
+//
+// The reference drives its fixpoint from the host, one add_mul per rule with
+// a flag round-trip and a device-wide sync each time (cfpq-cpp/Grammar.h:58-69,
+// cuda/naive-gpu/Multiplication.cu:145-166).  Here a fixpoint *step* evaluates
+// every rule at once on the device:
+//   new_H = (dL * M_R | M_L * dR) & ~M_H  for each rule H -> L R,
+// with d the bits added by the previous step, and the host only reads a small
+// counter block every `iters_per_sync` steps.  Steps after convergence are
+// empty (no work items), so enqueueing a few ahead is harmless.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cfpq_hip.h"
+#include "kernels.hpp"
+
+using namespace cfpq;
+
+struct cfpq_mat {
+    uint32_t n = 0;
+    uint32_t T = 0;
+    void *base = nullptr;  // single device allocation
+    size_t bytes = 0;
+    MatDev d = {};
+};
+
+struct cfpq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cfpq_opts opts = {};
+    std::string err;
+    int num_cu = 256;
+
+    // staging for uploads / downloads
+    void *stage = nullptr;
+    size_t stage_bytes = 0;
+
+    // step counters (ring) and their pinned host mirror
+    StepCounters *ring = nullptr;
+    StepCounters *ring_host = nullptr;
+    unsigned long long *scratch64 = nullptr;  // small reductions
+    unsigned long long *scratch64_host = nullptr;
+
+    // work lists
+    uint64_t *items = nullptr;
+    size_t items_cap = 0;
+    uint64_t *lists[2] = {nullptr, nullptr};
+    size_t lists_cap = 0;
+
+    // descriptor tables
+    MatDev *d_mats = nullptr;
+    size_t d_mats_cap = 0;
+    HeadDev *d_heads = nullptr;
+    size_t d_heads_cap = 0;
+    TermDev *d_terms = nullptr;  // [delta terms | full terms]
+    size_t d_terms_cap = 0;
+    uint32_t *d_ids = nullptr;   // id lists for clear/copy kernels
+    size_t d_ids_cap = 0;
+
+    // pool of all-zero matrices keyed by n
+    std::multimap<uint32_t, cfpq_mat *> pool;
+
+    // events
+    std::vector<hipEvent_t> ev;
+};
+
+// ---------------------------------------------------------------------------
+// error helpers
+// ---------------------------------------------------------------------------
+
+static int fail(cfpq_ctx *ctx, int code, const char *fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? CFPQ_E_NOMEM : CFPQ_E_DEVICE, "%s: %s (%s:%d)", \
+                        #call, hipGetErrorString(e_), __FILE__, __LINE__);                             \
+    } while (0)
+
+#define CHK(expr)                  \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != CFPQ_OK) return rc_; \
+    } while (0)
+
+template <class T>
+static int grow(cfpq_ctx *ctx, T **ptr, size_t *cap, size_t need) {
+    if (need <= *cap && *ptr) return CFPQ_OK;
+    if (*ptr) HIPCHK(hipFree(*ptr));
+    *ptr = nullptr;
+    size_t n = std::max<size_t>(need, 16);
+    n = std::max(n, *cap * 3 / 2);
+    HIPCHK(hipMalloc((void **)ptr, n * sizeof(T)));
+    *cap = n;
+    return CFPQ_OK;
+}
+
+static int launch_check(cfpq_ctx *ctx, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, CFPQ_E_DEVICE, "launch %s: %s", what, hipGetErrorString(e));
+    return CFPQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// lifecycle
+// ---------------------------------------------------------------------------
+
+extern "C" void cfpq_default_opts(cfpq_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->kernel = CFPQ_KERNEL_AUTO;
+    o->dense_bits = 0;
+    o->mfma_bits = 0;
+    o->max_iterations = 0;
+    o->final_full_pass = 0;
+    o->iters_per_sync = 0;
+    o->use_graph = 0;
+}
+
+static void normalise(cfpq_opts *o) {
+    if (o->dense_bits == 0) o->dense_bits = 2048;  // >= 8 bits per tile row on average
+    if (o->iters_per_sync <= 0) o->iters_per_sync = 4;
+    if (o->iters_per_sync > RING / 2) o->iters_per_sync = RING / 2;
+}
+
+extern "C" int cfpq_create(int device, const cfpq_opts *opts, cfpq_ctx **out) {
+    if (!out) return CFPQ_E_INVALID;
+    *out = nullptr;
+    cfpq_ctx *ctx = new (std::nothrow) cfpq_ctx();
+    if (!ctx) return CFPQ_E_NOMEM;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) {
+        delete ctx;
+        return CFPQ_E_DEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        delete ctx;
+        return CFPQ_E_INVALID;
+    }
+    ctx->device = device;
+    if (opts) ctx->opts = *opts;
+    else cfpq_default_opts(&ctx->opts);
+    normalise(&ctx->opts);
+    auto init = [&]() -> int {
+        HIPCHK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        ctx->num_cu = prop.multiProcessorCount;
+        HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        HIPCHK(hipMalloc((void **)&ctx->ring, sizeof(StepCounters) * RING));
+        HIPCHK(hipMemset(ctx->ring, 0, sizeof(StepCounters) * RING));
+        HIPCHK(hipHostMalloc((void **)&ctx->ring_host, sizeof(StepCounters) * RING, hipHostMallocDefault));
+        HIPCHK(hipMalloc((void **)&ctx->scratch64, sizeof(unsigned long long) * 4));
+        HIPCHK(hipHostMalloc((void **)&ctx->scratch64_host, sizeof(unsigned long long) * 4, hipHostMallocDefault));
+        return CFPQ_OK;
+    };
+    int rc = init();
+    if (rc != CFPQ_OK) {
+        cfpq_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return CFPQ_OK;
+}
+
+static void free_mat(cfpq_mat *m) {
+    if (!m) return;
+    if (m->base) (void)hipFree(m->base);
+    delete m;
+}
+
+extern "C" int cfpq_destroy(cfpq_ctx *ctx) {
+    if (!ctx) return CFPQ_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->pool) free_mat(kv.second);
+    ctx->pool.clear();
+    for (auto e : ctx->ev) (void)hipEventDestroy(e);
+    void *dev[] = {ctx->stage, ctx->ring, ctx->scratch64, ctx->items, ctx->lists[0], ctx->lists[1],
+                   ctx->d_mats, ctx->d_heads, ctx->d_terms, ctx->d_ids};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    if (ctx->ring_host) (void)hipHostFree(ctx->ring_host);
+    if (ctx->scratch64_host) (void)hipHostFree(ctx->scratch64_host);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CFPQ_OK;
+}
+
+extern "C" const char *cfpq_last_error(const cfpq_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" int cfpq_set_opts(cfpq_ctx *ctx, const cfpq_opts *opts) {
+    if (!ctx || !opts) return CFPQ_E_INVALID;
+    ctx->opts = *opts;
+    normalise(&ctx->opts);
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_sync(cfpq_ctx *ctx) {
+    if (!ctx) return CFPQ_E_INVALID;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFPQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// matrices
+// ---------------------------------------------------------------------------
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+static int mat_create(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
+    cfpq_mat *m = new (std::nothrow) cfpq_mat();
+    if (!m) return fail(ctx, CFPQ_E_NOMEM, "host allocation failed");
+    m->n = n;
+    m->T = tiles_per_side(n);
+    const size_t T = m->T, ow = occ_words(m->T);
+    const size_t b_bits = align256(T * T * TILE_WORDS * sizeof(uint64_t));
+    const size_t b_cnt = align256(T * T * sizeof(uint32_t));
+    const size_t b_rm = align256(T * T * 4 * sizeof(uint64_t));
+    const size_t b_occ = align256(T * ow * sizeof(uint32_t));
+    m->bytes = b_bits + b_cnt + b_rm + 2 * b_occ;
+    hipError_t e = hipMalloc(&m->base, m->bytes);
+    if (e != hipSuccess) {
+        delete m;
+        return fail(ctx, e == hipErrorOutOfMemory ? CFPQ_E_NOMEM : CFPQ_E_DEVICE, "hipMalloc(%zu): %s", (size_t)0,
+                    hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(m->base, 0, m->bytes, ctx->stream);
+    if (e != hipSuccess) {
+        free_mat(m);
+        return fail(ctx, CFPQ_E_DEVICE, "hipMemsetAsync: %s", hipGetErrorString(e));
+    }
+    char *b = (char *)m->base;
+    m->d.bits = (uint64_t *)b;
+    m->d.cnt = (uint32_t *)(b + b_bits);
+    m->d.rmask = (uint64_t *)(b + b_bits + b_cnt);
+    m->d.occR = (uint32_t *)(b + b_bits + b_cnt + b_rm);
+    m->d.occC = (uint32_t *)(b + b_bits + b_cnt + b_rm + b_occ);
+    m->d.T = m->T;
+    m->d.ow = (uint32_t)ow;
+    *out = m;
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_mat_alloc(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
+    if (!ctx || !out || n == 0) return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_alloc: bad argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    return mat_create(ctx, n, out);
+}
+
+extern "C" int cfpq_mat_free(cfpq_ctx *ctx, cfpq_mat *mat) {
+    if (!ctx || !mat) return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_free: bad argument");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    free_mat(mat);
+    return CFPQ_OK;
+}
+
+extern "C" uint32_t cfpq_mat_n(const cfpq_mat *mat) { return mat ? mat->n : 0; }
+
+static int ensure_stage(cfpq_ctx *ctx, size_t bytes) {
+    if (bytes <= ctx->stage_bytes) return CFPQ_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->stage) HIPCHK(hipFree(ctx->stage));
+    ctx->stage = nullptr;
+    ctx->stage_bytes = 0;
+    HIPCHK(hipMalloc(&ctx->stage, bytes));
+    ctx->stage_bytes = bytes;
+    return CFPQ_OK;
+}
+
+static uint32_t grid_for_tiles(cfpq_ctx *ctx, size_t tiles) {
+    size_t g = std::min<size_t>(tiles, (size_t)ctx->num_cu * 16);
+    return (uint32_t)std::max<size_t>(g, 1);
+}
+
+extern "C" int cfpq_mat_upload(cfpq_ctx *ctx, cfpq_mat *mat, const uint64_t *rows, size_t words_per_row) {
+    if (!ctx || !mat || !rows) return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_upload: bad argument");
+    const size_t wpr = (mat->n + 63) / 64;
+    if (words_per_row < wpr) return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_upload: words_per_row %zu < %zu", words_per_row, wpr);
+    HIPCHK(hipSetDevice(ctx->device));
+    CHK(ensure_stage(ctx, (size_t)mat->n * wpr * 8));
+    HIPCHK(hipMemcpy2DAsync(ctx->stage, wpr * 8, rows, words_per_row * 8, wpr * 8, mat->n, hipMemcpyHostToDevice,
+                            ctx->stream));
+    const size_t occ_bytes = (size_t)mat->T * mat->d.ow * 4;
+    HIPCHK(hipMemsetAsync(mat->d.occR, 0, occ_bytes, ctx->stream));
+    HIPCHK(hipMemsetAsync(mat->d.occC, 0, occ_bytes, ctx->stream));
+    hipLaunchKernelGGL(k_rows_to_tiles, dim3(grid_for_tiles(ctx, (size_t)mat->T * mat->T)), dim3(BLOCK), 0, ctx->stream,
+                       mat->d, (const uint64_t *)ctx->stage, mat->n, (uint32_t)wpr);
+    CHK(launch_check(ctx, "k_rows_to_tiles"));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_mat_download(cfpq_ctx *ctx, const cfpq_mat *mat, uint64_t *rows, size_t words_per_row) {
+    if (!ctx || !mat || !rows) return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_download: bad argument");
+    const size_t wpr = (mat->n + 63) / 64;
+    if (words_per_row < wpr) return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_download: words_per_row %zu < %zu", words_per_row, wpr);
+    HIPCHK(hipSetDevice(ctx->device));
+    CHK(ensure_stage(ctx, (size_t)mat->n * wpr * 8));
+    hipLaunchKernelGGL(k_tiles_to_rows, dim3(grid_for_tiles(ctx, (size_t)mat->T * mat->T)), dim3(BLOCK), 0, ctx->stream,
+                       mat->d, (uint64_t *)ctx->stage, mat->n, (uint32_t)wpr);
+    CHK(launch_check(ctx, "k_tiles_to_rows"));
+    HIPCHK(hipMemcpy2DAsync(rows, words_per_row * 8, ctx->stage, wpr * 8, wpr * 8, mat->n, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    if (words_per_row > wpr)
+        for (uint32_t i = 0; i < mat->n; ++i) std::memset(rows + (size_t)i * words_per_row + wpr, 0, (words_per_row - wpr) * 8);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_mat_set_bits(cfpq_ctx *ctx, cfpq_mat *mat, const uint32_t *rows, const uint32_t *cols, size_t count) {
+    if (!ctx || !mat || (count && (!rows || !cols))) return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_set_bits: bad argument");
+    for (size_t q = 0; q < count; ++q)
+        if (rows[q] >= mat->n || cols[q] >= mat->n)
+            return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_set_bits: bit (%u, %u) outside %u x %u", rows[q], cols[q], mat->n, mat->n);
+    if (!count) return CFPQ_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    CHK(ensure_stage(ctx, count * 8));
+    uint32_t *dr = (uint32_t *)ctx->stage, *dc = dr + count;
+    HIPCHK(hipMemcpyAsync(dr, rows, count * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dc, cols, count * 4, hipMemcpyHostToDevice, ctx->stream));
+    const uint32_t g = (uint32_t)std::min<size_t>((count + 255) / 256, (size_t)ctx->num_cu * 32);
+    hipLaunchKernelGGL(k_set_bits, dim3(g), dim3(256), 0, ctx->stream, mat->d, (const uint32_t *)dr, (const uint32_t *)dc, count);
+    CHK(launch_check(ctx, "k_set_bits"));
+    const size_t occ_bytes = (size_t)mat->T * mat->d.ow * 4;
+    HIPCHK(hipMemsetAsync(mat->d.occR, 0, occ_bytes, ctx->stream));
+    HIPCHK(hipMemsetAsync(mat->d.occC, 0, occ_bytes, ctx->stream));
+    hipLaunchKernelGGL(k_recount, dim3(grid_for_tiles(ctx, (size_t)mat->T * mat->T)), dim3(BLOCK), 0, ctx->stream, mat->d);
+    CHK(launch_check(ctx, "k_recount"));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_mat_popcount(cfpq_ctx *ctx, const cfpq_mat *mat, uint64_t *count) {
+    if (!ctx || !mat || !count) return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_popcount: bad argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemsetAsync(ctx->scratch64, 0, 8, ctx->stream));
+    const size_t tiles = (size_t)mat->T * mat->T;
+    const uint32_t g = (uint32_t)std::min<size_t>((tiles + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_popcount, dim3(g), dim3(256), 0, ctx->stream, mat->d, ctx->scratch64);
+    CHK(launch_check(ctx, "k_popcount"));
+    HIPCHK(hipMemcpyAsync(ctx->scratch64_host, ctx->scratch64, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *count = ctx->scratch64_host[0];
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_mat_fingerprint(cfpq_ctx *ctx, const cfpq_mat *mat, uint64_t *hash, uint64_t *count) {
+    if (!ctx || !mat || !hash || !count) return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_fingerprint: bad argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint32_t n = mat->n;
+    const uint64_t P = 1000000007ull;
+    std::vector<uint64_t> pw(2 * (size_t)n);
+    uint64_t a = 1, b = 1;
+    for (uint32_t e = 0; e < n; ++e) {
+        pw[e] = a;
+        pw[n + e] = b;
+        a = a * 11 % P;
+        b = b * 13 % P;
+    }
+    CHK(ensure_stage(ctx, pw.size() * 8));
+    HIPCHK(hipMemcpyAsync(ctx->stage, pw.data(), pw.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->scratch64, 0, 16, ctx->stream));
+    const uint64_t *p11 = (const uint64_t *)ctx->stage;
+    hipLaunchKernelGGL(k_fingerprint, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, mat->d, n, p11, p11 + n,
+                       ctx->scratch64);
+    CHK(launch_check(ctx, "k_fingerprint"));
+    HIPCHK(hipMemcpyAsync(ctx->scratch64_host, ctx->scratch64, 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *hash = ctx->scratch64_host[0] % P;
+    *count = ctx->scratch64_host[1];
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_mat_copy(cfpq_ctx *ctx, cfpq_mat *dst, const cfpq_mat *src) {
+    if (!ctx || !dst || !src || dst->n != src->n) return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_copy: bad argument");
+    if (dst == src) return CFPQ_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    CHK(grow(ctx, &ctx->d_mats, &ctx->d_mats_cap, 2));
+    CHK(grow(ctx, &ctx->d_ids, &ctx->d_ids_cap, 2));
+    const MatDev md[2] = {dst->d, src->d};
+    const uint32_t ids[2] = {0, 1};
+    HIPCHK(hipMemcpyAsync(ctx->d_mats, md, sizeof md, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_ids, ids, sizeof ids, hipMemcpyHostToDevice, ctx->stream));
+    const uint32_t g = grid_for_tiles(ctx, (size_t)dst->T * dst->T);
+    hipLaunchKernelGGL(k_clear_full, dim3(g, 1), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                       (const uint32_t *)ctx->d_ids, 1u);
+    hipLaunchKernelGGL(k_copy, dim3(g, 1), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                       (const uint32_t *)ctx->d_ids, (const uint32_t *)ctx->d_ids + 1, 1u);
+    CHK(launch_check(ctx, "cfpq_mat_copy"));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFPQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fixpoint machinery shared by add_mul, fixpoint and batch fixpoint
+// ---------------------------------------------------------------------------
+
+static int pool_take(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
+    auto it = ctx->pool.find(n);
+    if (it != ctx->pool.end()) {
+        *out = it->second;
+        ctx->pool.erase(it);
+        return CFPQ_OK;
+    }
+    return mat_create(ctx, n, out);
+}
+
+static void pool_give(cfpq_ctx *ctx, cfpq_mat *m) { ctx->pool.emplace(m->n, m); }
+
+static int ensure_events(cfpq_ctx *ctx, size_t n) {
+    while (ctx->ev.size() < n) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        ctx->ev.push_back(e);
+    }
+    return CFPQ_OK;
+}
+
+// A fixpoint problem laid out for the kernels.
+struct Plan {
+    std::vector<MatDev> mats;         // descriptor table
+    std::vector<HeadDev> heads;
+    std::vector<TermDev> terms_delta;  // semi-naive terms
+    std::vector<TermDev> terms_full;   // full pass terms (same head ranges, offset)
+    std::vector<uint32_t> all_delta[2];  // ids of every delta buffer of parity q (buffer Dq)
+    std::vector<uint32_t> copy_dst, copy_src;  // initial D0 := M
+    size_t max_tiles_per_head = 0;     // max T*T over heads
+    size_t total_head_tiles = 0;       // sum of T*T over heads
+};
+
+struct StepStats {
+    uint64_t new_bits = 0, items = 0, tp = 0, tpd = 0, tpm = 0, arows = 0, brows = 0, orows = 0, brows_lds = 0;
+};
+
+// Enqueue one step. parity q: deltas read from D_q, new bits to D_{1-q}.
+// prev_slot < 0: the delta consumed by this step is the initial one
+// (cleared in full); otherwise it was written by the step in prev_slot.
+static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, int prev_slot, bool full_pass,
+                        int ev_idx) {
+    StepCounters *c = ctx->ring + slot;
+    const uint32_t nh = (uint32_t)plan.heads.size();
+    const TermDev *terms = ctx->d_terms + (full_pass ? plan.terms_delta.size() : 0);
+    uint64_t *touched = ctx->lists[slot & 1];
+    hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
+    const uint32_t bx = (uint32_t)((plan.max_tiles_per_head + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                       (const HeadDev *)ctx->d_heads, terms, q, c, ctx->items);
+    const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
+    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[2 * ev_idx], ctx->stream));
+    hipLaunchKernelGGL(k_product, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                       (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
+                       ctx->opts.kernel == CFPQ_KERNEL_GATHER ? 0xffffffffu
+                       : (ctx->opts.kernel == CFPQ_KERNEL_AUTO ? ctx->opts.dense_bits : 0u));
+    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[2 * ev_idx + 1], ctx->stream));
+    hipLaunchKernelGGL(k_apply, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                       (const HeadDev *)ctx->d_heads, q, c, (const uint64_t *)touched);
+    // retire the delta this step consumed (buffer D_q)
+    const uint32_t *ids_q = ctx->d_ids + (q ? plan.all_delta[0].size() : 0);
+    const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
+    if (prev_slot < 0) {
+        hipLaunchKernelGGL(k_clear_full, dim3(std::max<uint32_t>(1, ctx->num_cu * 4 / std::max<uint32_t>(1, nids_q)),
+                                              std::min<uint32_t>(nids_q, 65535)),
+                           dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats, ids_q, nids_q);
+    } else {
+        hipLaunchKernelGGL(k_clear, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                           (const HeadDev *)ctx->d_heads, 1 - q, (const StepCounters *)(ctx->ring + prev_slot), c,
+                           (const uint64_t *)ctx->lists[prev_slot & 1]);
+        hipLaunchKernelGGL(k_clear_occ, dim3(4, std::min<uint32_t>(nids_q, 65535)), dim3(256), 0, ctx->stream,
+                           (const MatDev *)ctx->d_mats, ids_q, nids_q);
+    }
+    return launch_check(ctx, "fixpoint step");
+}
+
+static int upload_plan(cfpq_ctx *ctx, const Plan &plan) {
+    CHK(grow(ctx, &ctx->d_mats, &ctx->d_mats_cap, plan.mats.size()));
+    CHK(grow(ctx, &ctx->d_heads, &ctx->d_heads_cap, plan.heads.size()));
+    CHK(grow(ctx, &ctx->d_terms, &ctx->d_terms_cap, plan.terms_delta.size() + plan.terms_full.size()));
+    const size_t nid = plan.all_delta[0].size() + plan.all_delta[1].size() + 2 * plan.copy_dst.size();
+    CHK(grow(ctx, &ctx->d_ids, &ctx->d_ids_cap, nid));
+    CHK(grow(ctx, &ctx->items, &ctx->items_cap, plan.total_head_tiles));
+    if (plan.total_head_tiles > ctx->lists_cap || !ctx->lists[0]) {
+        size_t cap0 = ctx->lists_cap;
+        CHK(grow(ctx, &ctx->lists[0], &cap0, plan.total_head_tiles));
+        size_t cap1 = ctx->lists_cap;
+        CHK(grow(ctx, &ctx->lists[1], &cap1, plan.total_head_tiles));
+        ctx->lists_cap = std::min(cap0, cap1);
+    }
+    HIPCHK(hipMemcpyAsync(ctx->d_mats, plan.mats.data(), plan.mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_heads, plan.heads.data(), plan.heads.size() * sizeof(HeadDev), hipMemcpyHostToDevice, ctx->stream));
+    std::vector<TermDev> terms(plan.terms_delta);
+    terms.insert(terms.end(), plan.terms_full.begin(), plan.terms_full.end());
+    HIPCHK(hipMemcpyAsync(ctx->d_terms, terms.data(), terms.size() * sizeof(TermDev), hipMemcpyHostToDevice, ctx->stream));
+    std::vector<uint32_t> ids(plan.all_delta[0]);
+    ids.insert(ids.end(), plan.all_delta[1].begin(), plan.all_delta[1].end());
+    ids.insert(ids.end(), plan.copy_dst.begin(), plan.copy_dst.end());
+    ids.insert(ids.end(), plan.copy_src.begin(), plan.copy_src.end());
+    HIPCHK(hipMemcpyAsync(ctx->d_ids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->ring, 0, sizeof(StepCounters) * RING, ctx->stream));
+    // the tables above were copied from pageable host vectors that die here
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFPQ_OK;
+}
+
+static void accumulate(StepStats &s, const StepCounters &c) {
+    s.new_bits += c.new_bits;
+    s.items += c.n_items;
+    s.tp += c.tile_products;
+    s.tpd += c.tp_dense;
+    s.tpm += c.tp_mfma;
+    s.arows += c.a_rows;
+    s.brows += c.b_rows;
+    s.orows += c.out_rows;
+    s.brows_lds += c.b_rows_lds;
+}
+
+// Run delta steps to the fixpoint (plus the optional exact full pass).
+// The caller owns plan buffers; on return all delta buffers are zero again
+// (on success) -- on failure they are released instead of pooled.
+static int run_fixpoint(cfpq_ctx *ctx, const Plan &plan, cfpq_stats *stats, bool single_step_full) {
+    CHK(upload_plan(ctx, plan));
+    const uint32_t nd = (uint32_t)plan.all_delta[0].size();
+    // D0 := M (every initial bit is new)
+    if (!plan.copy_dst.empty()) {
+        const uint32_t ncp = (uint32_t)plan.copy_dst.size();
+        const uint32_t *dst = ctx->d_ids + nd + plan.all_delta[1].size();
+        hipLaunchKernelGGL(k_copy, dim3(std::max<uint32_t>(1, ctx->num_cu * 4 / ncp), std::min<uint32_t>(ncp, 65535)),
+                           dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats, dst, dst + ncp, ncp);
+        CHK(launch_check(ctx, "k_copy"));
+    }
+    const int per_sync = single_step_full ? 1 : ctx->opts.iters_per_sync;
+    CHK(ensure_events(ctx, 2 * (size_t)RING + 2));
+    hipEvent_t ev_begin = ctx->ev[2 * RING], ev_end = ctx->ev[2 * RING + 1];
+    HIPCHK(hipEventRecord(ev_begin, ctx->stream));
+    StepStats tot;
+    double product_ms = 0;
+    uint32_t steps = 0, syncs = 0, launches = 0, full_passes = 0;
+    int prev_slot = -1;
+    bool done = false;
+    bool pending_full = single_step_full;
+    const bool want_full = ctx->opts.final_full_pass != 0;
+    while (!done) {
+        const int k = pending_full ? 1 : per_sync;
+        std::vector<uint32_t> slots;
+        for (int s = 0; s < k; ++s) {
+            const uint32_t slot = steps % RING;
+            CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, pending_full, s));
+            ++launches;
+            slots.push_back(slot);
+            prev_slot = (int)slot;
+            ++steps;
+        }
+        // read the counters of the steps just run
+        HIPCHK(hipMemcpyAsync(ctx->ring_host, ctx->ring, sizeof(StepCounters) * RING, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ++syncs;
+        bool was_full = pending_full;
+        pending_full = false;
+        uint32_t used = 0;
+        for (int s = 0; s < k; ++s) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ctx->ev[2 * s], ctx->ev[2 * s + 1]));
+            product_ms += ms;
+            const StepCounters &c = ctx->ring_host[slots[s]];
+            accumulate(tot, c);
+            ++used;
+            if (c.new_bits == 0) {
+                // a step with no new bit: fixpoint of the semi-naive system
+                done = true;
+                break;
+            }
+        }
+        if (was_full) ++full_passes;
+        if (done) {
+            // steps enqueued after the empty one did nothing; count only up to it
+            steps -= (uint32_t)(k - used);
+            if (want_full && !was_full && !single_step_full) {
+                done = false;
+                pending_full = true;
+            }
+        }
+        if (single_step_full && was_full) done = true;
+        if (!done && ctx->opts.max_iterations && steps >= ctx->opts.max_iterations)
+            return fail(ctx, CFPQ_E_LIMIT, "fixpoint: iteration cap %u reached", ctx->opts.max_iterations);
+    }
+    // the last step's delta (D of the step after it) is empty; the delta it
+    // consumed was cleared by the step itself.  Retire the newest delta list
+    // so that every buffer is zero for the pool.
+    {
+        const int q = (int)(steps & 1);  // parity the next step would have
+        const uint32_t slot = steps % RING;
+        hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
+        const uint32_t *ids_q = ctx->d_ids + (q ? nd : 0);
+        const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
+        hipLaunchKernelGGL(k_clear, dim3(ctx->num_cu * 4), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                           (const HeadDev *)ctx->d_heads, 1 - q, (const StepCounters *)(ctx->ring + prev_slot),
+                           ctx->ring + slot, (const uint64_t *)ctx->lists[prev_slot & 1]);
+        hipLaunchKernelGGL(k_clear_occ, dim3(4, std::min<uint32_t>(nids_q, 65535)), dim3(256), 0, ctx->stream,
+                           (const MatDev *)ctx->d_mats, ids_q, nids_q);
+        CHK(launch_check(ctx, "final clear"));
+    }
+    HIPCHK(hipEventRecord(ev_end, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float total_ms = 0;
+    HIPCHK(hipEventElapsedTime(&total_ms, ev_begin, ev_end));
+    if (stats) {
+        stats->iterations = steps;
+        stats->full_passes = full_passes;
+        stats->new_bits = tot.new_bits;
+        stats->items = tot.items;
+        stats->tile_products = tot.tp;
+        stats->tile_products_dense = tot.tpd;
+        stats->tile_products_mfma = tot.tpm;
+        stats->a_rows = tot.arows;
+        stats->b_rows = tot.brows;
+        stats->out_rows = tot.orows;
+        stats->b_rows_lds = tot.brows_lds;
+        stats->fixpoint_ms = total_ms;
+        stats->product_ms = product_ms;
+        stats->product_launches = launches;
+        stats->host_syncs = syncs;
+    }
+    return CFPQ_OK;
+}
+
+// Build the plan for one query: matrix ids [base, base+nm) = M, then D0, D1.
+static void plan_query(Plan &plan, cfpq_mat *const *M, const std::vector<cfpq_mat *> &D0,
+                       const std::vector<cfpq_mat *> &D1, size_t nm, const cfpq_rule *rules, size_t nrules) {
+    const uint32_t base = (uint32_t)plan.mats.size();
+    for (size_t x = 0; x < nm; ++x) plan.mats.push_back(M[x]->d);
+    for (size_t x = 0; x < nm; ++x) plan.mats.push_back(D0[x]->d);
+    for (size_t x = 0; x < nm; ++x) plan.mats.push_back(D1[x]->d);
+    auto id_m = [&](uint32_t x) { return base + x; };
+    auto id_d = [&](int q, uint32_t x) { return base + (uint32_t)nm * (1 + q) + x; };
+    for (size_t x = 0; x < nm; ++x) {
+        plan.all_delta[0].push_back(id_d(0, (uint32_t)x));
+        plan.all_delta[1].push_back(id_d(1, (uint32_t)x));
+        plan.copy_dst.push_back(id_d(0, (uint32_t)x));
+        plan.copy_src.push_back(id_m((uint32_t)x));
+    }
+    const uint32_t T = M[0]->T;
+    // group rules by head (stable)
+    std::vector<std::vector<size_t>> by_head(nm);
+    for (size_t r = 0; r < nrules; ++r) by_head[rules[r].head].push_back(r);
+    for (size_t h = 0; h < nm; ++h) {
+        if (by_head[h].empty()) continue;
+        HeadDev hd;
+        hd.target = id_m((uint32_t)h);
+        hd.out[0] = id_d(1, (uint32_t)h);  // even step writes D1
+        hd.out[1] = id_d(0, (uint32_t)h);  // odd step writes D0
+        hd.t0 = (uint32_t)plan.terms_delta.size();
+        hd.T = T;
+        for (size_t r : by_head[h]) {
+            const uint32_t a = rules[r].left, b = rules[r].right;
+            TermDev t1, t2, f;
+            t1.left[0] = id_d(0, a); t1.left[1] = id_d(1, a);
+            t1.right[0] = id_m(b);   t1.right[1] = id_m(b);
+            t2.left[0] = id_m(a);    t2.left[1] = id_m(a);
+            t2.right[0] = id_d(0, b); t2.right[1] = id_d(1, b);
+            plan.terms_delta.push_back(t1);
+            plan.terms_delta.push_back(t2);
+            f.left[0] = f.left[1] = id_m(a);
+            f.right[0] = f.right[1] = id_m(b);
+            plan.terms_full.push_back(f);
+            plan.terms_full.push_back(f);  // keep the same index ranges as the delta table
+        }
+        hd.t1 = (uint32_t)plan.terms_delta.size();
+        plan.heads.push_back(hd);
+        plan.max_tiles_per_head = std::max<size_t>(plan.max_tiles_per_head, (size_t)T * T);
+        plan.total_head_tiles += (size_t)T * T;
+    }
+}
+
+static int check_query(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nrules, cfpq_mat *const *mats, size_t nmats) {
+    if (!mats || nmats == 0 || (nrules && !rules)) return fail(ctx, CFPQ_E_INVALID, "fixpoint: empty matrix list");
+    for (size_t x = 0; x < nmats; ++x) {
+        if (!mats[x]) return fail(ctx, CFPQ_E_INVALID, "fixpoint: matrix %zu is null", x);
+        if (mats[x]->n != mats[0]->n) return fail(ctx, CFPQ_E_INVALID, "fixpoint: matrices of different sizes");
+        for (size_t y = 0; y < x; ++y)
+            if (mats[y] == mats[x]) return fail(ctx, CFPQ_E_INVALID, "fixpoint: matrix %zu repeated", x);
+    }
+    for (size_t r = 0; r < nrules; ++r)
+        if (rules[r].head >= nmats || rules[r].left >= nmats || rules[r].right >= nmats)
+            return fail(ctx, CFPQ_E_INVALID, "fixpoint: rule %zu names a matrix >= %zu", r, nmats);
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_batch_fixpoint(cfpq_ctx *ctx, const cfpq_query *queries, size_t nqueries, cfpq_stats *stats) {
+    if (!ctx || (nqueries && !queries)) return fail(ctx, CFPQ_E_INVALID, "batch_fixpoint: bad argument");
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    if (nqueries == 0) return CFPQ_OK;
+    for (size_t qi = 0; qi < nqueries; ++qi)
+        CHK(check_query(ctx, queries[qi].rules, queries[qi].nrules, queries[qi].mats, queries[qi].nmats));
+    HIPCHK(hipSetDevice(ctx->device));
+    Plan plan;
+    std::vector<cfpq_mat *> taken;
+    int rc = CFPQ_OK;
+    for (size_t qi = 0; qi < nqueries && rc == CFPQ_OK; ++qi) {
+        const cfpq_query &q = queries[qi];
+        std::vector<cfpq_mat *> D0(q.nmats), D1(q.nmats);
+        for (size_t x = 0; x < q.nmats && rc == CFPQ_OK; ++x) {
+            rc = pool_take(ctx, q.mats[0]->n, &D0[x]);
+            if (rc == CFPQ_OK) taken.push_back(D0[x]);
+            if (rc == CFPQ_OK) rc = pool_take(ctx, q.mats[0]->n, &D1[x]);
+            if (rc == CFPQ_OK) taken.push_back(D1[x]);
+        }
+        if (rc == CFPQ_OK) plan_query(plan, q.mats, D0, D1, q.nmats, q.rules, q.nrules);
+    }
+    if (rc == CFPQ_OK && plan.heads.empty()) {
+        for (auto *m : taken) pool_give(ctx, m);
+        if (stats) stats->iterations = 0;
+        return CFPQ_OK;
+    }
+    if (rc == CFPQ_OK) rc = run_fixpoint(ctx, plan, stats, false);
+    if (rc == CFPQ_OK) {
+        for (auto *m : taken) pool_give(ctx, m);
+    } else {
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto *m : taken) free_mat(m);
+    }
+    return rc;
+}
+
+extern "C" int cfpq_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nrules, cfpq_mat *const *mats, size_t nmats,
+                             cfpq_stats *stats) {
+    if (!ctx) return CFPQ_E_INVALID;
+    cfpq_query q;
+    q.mats = mats;
+    q.nmats = nmats;
+    q.rules = rules;
+    q.nrules = nrules;
+    return cfpq_batch_fixpoint(ctx, &q, 1, stats);
+}
+
+extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const cfpq_mat *b, int *changed) {
+    if (!ctx || !c || !a || !b) return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul: bad argument");
+    if (a->n != c->n || b->n != c->n) return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul: size mismatch");
+    HIPCHK(hipSetDevice(ctx->device));
+    cfpq_mat *S = nullptr;
+    CHK(pool_take(ctx, c->n, &S));
+    // one full step:  S = (a*b) & ~c ;  c |= S ;  S cleared again
+    Plan plan;
+    plan.mats = {c->d, a->d, b->d, S->d};
+    HeadDev hd;
+    hd.target = 0;
+    hd.out[0] = hd.out[1] = 3;
+    hd.t0 = 0;
+    hd.t1 = 1;
+    hd.T = c->T;
+    plan.heads.push_back(hd);
+    TermDev t;
+    t.left[0] = t.left[1] = 1;
+    t.right[0] = t.right[1] = 2;
+    plan.terms_delta.push_back(t);
+    plan.terms_full.push_back(t);
+    plan.all_delta[0] = {3};
+    plan.all_delta[1] = {3};
+    plan.max_tiles_per_head = plan.total_head_tiles = (size_t)c->T * c->T;
+    int rc = upload_plan(ctx, plan);
+    if (rc == CFPQ_OK) rc = enqueue_step(ctx, plan, 0, 0, -1, false, -1);
+    // enqueue_step cleared nothing useful for S yet (prev_slot < 0 clears all
+    // non-empty tiles of "D_0" = S *after* apply), which is exactly the reset we need.
+    if (rc == CFPQ_OK) {
+        hipError_t e = hipMemcpyAsync(ctx->ring_host, ctx->ring, sizeof(StepCounters), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, CFPQ_E_DEVICE, "add_mul: %s", hipGetErrorString(e));
+    }
+    if (rc == CFPQ_OK) {
+        if (changed) *changed = ctx->ring_host[0].new_bits != 0;
+        pool_give(ctx, S);
+    } else {
+        (void)hipStreamSynchronize(ctx->stream);
+        free_mat(S);
+    }
+    return rc;
+}

@@ -1,0 +1,74 @@
+// Device-side layout of a bit-packed Boolean matrix (gfx950).
+//
+// An n x n matrix is padded to T = ceil(n/256) tiles per side.  Tile (I, J)
+// covers rows [256I, 256I+256) and columns [256J, 256J+256) and is stored as
+// 256 rows x 4 uint64 words (8 KiB, contiguous): word w of tile row r holds
+// columns 256J + 64w .. 256J + 64w + 63, bit b = column 256J + 64w + b.
+// Tiles are stored row-major by (I, J).  Per tile the engine also keeps
+//   cnt[I*T+J]          popcount of the tile,
+//   rmask[(I*T+J)*4+q]  bit r%64 of word q = r/64 set iff tile row r != 0,
+// and per tile row / tile column an occupancy bitset
+//   occR[I*ow + K/32] bit K%32  <=> tile (I, K) non-empty,
+//   occC[J*ow + I/32] bit I%32  <=> tile (I, J) non-empty,   ow = ceil(T/32).
+// Bits at rows or columns >= n are always zero.
+#pragma once
+#include <stdint.h>
+
+namespace cfpq {
+
+constexpr int TILE = 256;                 // tile edge in bits
+constexpr int TROW_WORDS = 4;             // uint64 words per tile row
+constexpr int TILE_WORDS = TILE * TROW_WORDS;
+constexpr uint32_t TILE_BITS = TILE * TILE;
+constexpr int BLOCK = 256;                // threads per work-group: one per tile row
+
+struct MatDev {
+    uint64_t *bits;
+    uint32_t *cnt;
+    uint64_t *rmask;
+    uint32_t *occR;
+    uint32_t *occC;
+    uint32_t T;
+    uint32_t ow;
+};
+
+// A group of rules sharing one head: the head's full matrix (mask), the
+// buffer new bits go to on even / odd steps, and its terms [t0, t1).
+struct HeadDev {
+    uint32_t target;
+    uint32_t out[2];
+    uint32_t t0, t1;
+    uint32_t T;
+};
+
+// One product term  left * right  contributing to a head; operands are
+// indices into the matrix-descriptor array, chosen by step parity.
+struct TermDev {
+    uint32_t left[2];
+    uint32_t right[2];
+};
+
+// Per-step counters (a ring of them lives in device memory).
+struct StepCounters {
+    uint32_t n_items;      // work items built
+    uint32_t item_next;    // product-kernel cursor
+    uint32_t n_touched;    // output tiles that received new bits
+    uint32_t apply_next;   // apply-kernel cursor
+    uint32_t clear_next;   // clear-kernel cursor
+    uint32_t pad0;
+    uint64_t new_bits;
+    uint64_t tile_products;
+    uint64_t tp_dense;
+    uint64_t tp_mfma;
+    uint64_t a_rows;
+    uint64_t b_rows;
+    uint64_t out_rows;     // C-tile rows that received new bits
+    uint64_t b_rows_lds;   // B rows OR-ed from LDS (dense path)
+};
+
+constexpr int RING = 64;  // step-counter ring length
+
+static inline uint32_t tiles_per_side(uint32_t n) { return (n + TILE - 1) / TILE; }
+static inline uint32_t occ_words(uint32_t T) { return (T + 31) / 32; }
+
+}  // namespace cfpq

@@ -1,0 +1,152 @@
+/*
+ * cfpq_hip.h -- C ABI of the MI355X (gfx950) CFPQ engine.
+ *
+ * Plain C: opaque handles, fixed-width integers, status codes.  No C++
+ * types, no exceptions and no exit() cross this boundary (the reference's
+ * GPU backends call exit() on any CUDA error, cuda/naive-gpu/Multiplication.cu:10-17).
+ *
+ * What each entry point replaces in the reference (JetBrains-Research/CFPQ-on-GPGPU):
+ *   cfpq_mat_alloc / cfpq_mat_free   gpuMatrix::allocate_device_matrix / deallocate
+ *                                    (cuda/naive-gpu/GpuMatrix.cpp:31-37,
+ *                                    Multiplication.cu:103-112)
+ *   cfpq_mat_upload                  gpuMatrix::transfer_to_gpu + the host mirror set_bit
+ *                                    (GpuMatrix.cpp:15-18,51-53; called from
+ *                                    gpuMatricesEnv::environment_preprocessing :56-64)
+ *   cfpq_mat_set_bits                Grammar::intersection_with_graph's set_bit loops
+ *                                    (cfpq-cpp/Grammar.h:41-51), batched
+ *   cfpq_mat_download                gpuMatrix::transfer_from_gpu (GpuMatrix.cpp:55-57;
+ *                                    environment_postprocessing :66-77)
+ *   cfpq_add_mul                     Matrix::add_mul (cfpq-cpp/Matrix.h:17), as
+ *                                    implemented by gpu_lib::matrix_product_add_wrapper
+ *                                    (cuda/naive-gpu/Multiplication.cu:145-166) and
+ *                                    the oracle M4riMatrix::add_mul (m4ri/m4ri_matrix.cpp:20-28)
+ *   cfpq_fixpoint                    the worklist loop of Grammar::intersection_with_graph
+ *                                    (cfpq-cpp/Grammar.h:58-69), moved onto the device
+ *   cfpq_batch_fixpoint              no counterpart: the reference runs one query per
+ *                                    process (m4ri/main.cpp:7-15, test_utils/runners.py:24)
+ *   cfpq_mat_fingerprint             Grammar::print_results (cfpq-cpp/Grammar.cpp:81-101)
+ *   cfpq_mat_copy                    the D2D copy MethodOf4RusMatrix::add_mul makes when
+ *                                    the head aliases an operand
+ *                                    (cuda/methodOf4RusBooleanSemiringGpu/methodOf4RusBooleanSemiringMatrix.cu:62-67)
+ *
+ * Conventions
+ *   - Every function returns 0 (CFPQ_OK) on success or a negative CFPQ_E* code;
+ *     cfpq_last_error(ctx) then describes the failure.
+ *   - The library owns device memory; the caller owns every host buffer.
+ *   - One context per host thread; calls on a context are serialised on the
+ *     context's HIP stream and return when their results are complete.
+ *   - Host bit matrices are row-major: n rows of `words_per_row` uint64
+ *     words, column j in word j/64 at bit j%64 (the m4ri convention used by
+ *     the reference's oracle backend).
+ */
+#ifndef CFPQ_HIP_H
+#define CFPQ_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFPQ_OK 0
+#define CFPQ_E_INVALID (-1)   /* bad argument: null handle, size mismatch, index out of range */
+#define CFPQ_E_DEVICE (-2)    /* a HIP runtime call failed */
+#define CFPQ_E_NOMEM (-3)     /* device or host allocation failed */
+#define CFPQ_E_LIMIT (-4)     /* iteration cap reached before the fixpoint */
+
+typedef struct cfpq_ctx cfpq_ctx;
+typedef struct cfpq_mat cfpq_mat;
+
+/* Product kernel choice for a tile-product (C tile |= A tile * B tile). */
+#define CFPQ_KERNEL_AUTO 0    /* per tile, by the popcount of the A tile */
+#define CFPQ_KERNEL_GATHER 1  /* bit-packed OR-of-rows, B rows gathered from HBM */
+#define CFPQ_KERNEL_LDS 2     /* bit-packed OR-of-rows, B tile staged in LDS */
+#define CFPQ_KERNEL_M4R 3     /* Four Russians, 8-bit tables in LDS */
+#define CFPQ_KERNEL_MFMA 4    /* exact int8 0/1 MFMA, i32 accumulation, > 0 */
+
+typedef struct cfpq_opts {
+    int32_t kernel;             /* CFPQ_KERNEL_*                                         */
+    uint32_t dense_bits;        /* AUTO: A-tile popcount at or above which the tile is dense
+                                   (0 = library default)                                 */
+    uint32_t mfma_bits;         /* AUTO: A-tile popcount at or above which MFMA is used
+                                   (0 = library default, 0xffffffff = never)             */
+    uint32_t max_iterations;    /* 0 = unlimited                                         */
+    int32_t final_full_pass;    /* 1: after the delta fixpoint, verify with one exact
+                                   full pass (every rule over full matrices)             */
+    int32_t iters_per_sync;     /* fixpoint steps enqueued between host flag reads (0 = default) */
+    int32_t use_graph;          /* 1: replay one captured hipGraph per fixpoint step     */
+    int32_t reserved[8];
+} cfpq_opts;
+
+typedef struct cfpq_rule {
+    uint32_t head, left, right; /* indices into the matrix array: head -> left right */
+} cfpq_rule;
+
+typedef struct cfpq_stats {
+    uint32_t iterations;        /* delta steps run, including the final empty one       */
+    uint32_t full_passes;       /* exact full passes run                                 */
+    uint64_t new_bits;          /* bits added to all matrices                            */
+    uint64_t items;             /* (head, C-tile) work items evaluated                   */
+    uint64_t tile_products;     /* (rule, C tile, k) tile-products evaluated, all kernels */
+    uint64_t tile_products_dense; /* of which on the LDS / M4R paths                     */
+    uint64_t tile_products_mfma;  /* of which on the MFMA path                           */
+    uint64_t a_rows;            /* non-empty A-tile rows visited                         */
+    uint64_t b_rows;            /* B rows gathered from HBM/L2 (one per set A bit, sparse path) */
+    uint64_t b_rows_lds;        /* B rows OR-ed from LDS (dense path)                    */
+    uint64_t out_rows;          /* C-tile rows that received new bits                    */
+    double fixpoint_ms;         /* device time of the whole call (hipEvent)              */
+    double product_ms;          /* device time inside the product kernel (hipEvent)      */
+    uint32_t product_launches;  /* product-kernel launches                               */
+    uint32_t host_syncs;        /* host waits on the device flag                         */
+} cfpq_stats;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+void cfpq_default_opts(cfpq_opts *opts);
+int cfpq_create(int device, const cfpq_opts *opts, cfpq_ctx **out);
+int cfpq_destroy(cfpq_ctx *ctx);
+const char *cfpq_last_error(const cfpq_ctx *ctx);
+int cfpq_set_opts(cfpq_ctx *ctx, const cfpq_opts *opts);
+int cfpq_sync(cfpq_ctx *ctx);
+
+/* ---- matrices ------------------------------------------------------------ */
+int cfpq_mat_alloc(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out);   /* all-zero n x n */
+int cfpq_mat_free(cfpq_ctx *ctx, cfpq_mat *mat);
+uint32_t cfpq_mat_n(const cfpq_mat *mat);
+int cfpq_mat_upload(cfpq_ctx *ctx, cfpq_mat *mat, const uint64_t *rows, size_t words_per_row);
+int cfpq_mat_download(cfpq_ctx *ctx, const cfpq_mat *mat, uint64_t *rows, size_t words_per_row);
+/* OR-in `count` bits (rows[i], cols[i]); indices must be < n. */
+int cfpq_mat_set_bits(cfpq_ctx *ctx, cfpq_mat *mat, const uint32_t *rows, const uint32_t *cols, size_t count);
+int cfpq_mat_popcount(cfpq_ctx *ctx, const cfpq_mat *mat, uint64_t *count);
+/* dst := src (same n), on the device. */
+int cfpq_mat_copy(cfpq_ctx *ctx, cfpq_mat *dst, const cfpq_mat *src);
+/* (hash, count) of Grammar::print_results for this matrix. */
+int cfpq_mat_fingerprint(cfpq_ctx *ctx, const cfpq_mat *mat, uint64_t *hash, uint64_t *count);
+
+/* ---- products and fixpoints --------------------------------------------- */
+/* c |= a * b over (OR, AND).  a and/or b may alias c: the product is taken over
+ * the operands as they were on entry.  *changed = 1 if c gained a bit. */
+int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const cfpq_mat *b, int *changed);
+
+/* Least fixpoint of  M[head] |= M[left] * M[right]  for all rules, in place on
+ * mats[0..nmats) (all of one size n).  stats may be NULL. */
+int cfpq_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nrules, cfpq_mat *const *mats,
+                  size_t nmats, cfpq_stats *stats);
+
+/* One independent query of a batch: its own matrices (one size n per query)
+ * and rules indexing them. */
+typedef struct cfpq_query {
+    cfpq_mat *const *mats;
+    size_t nmats;
+    const cfpq_rule *rules;
+    size_t nrules;
+} cfpq_query;
+
+/* Least fixpoints of all queries, advanced together: one set of kernel
+ * launches per step covers every query still changing. */
+int cfpq_batch_fixpoint(cfpq_ctx *ctx, const cfpq_query *queries, size_t nqueries, cfpq_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CFPQ_HIP_H */

@@ -1,0 +1,78 @@
+"""Writes tests/golden/fixpoints.json: expected per-nonterminal fingerprints
+(Grammar::print_results' ``name hash count``, cfpq-cpp/Grammar.cpp:81-101) of
+the five seeded workloads, computed by the CPU oracle.
+
+  * the semi-naive oracle fixpoint gives the fingerprints;
+  * where it is affordable, the reference worklist driver (oracle/_ref/cfpq_ref:
+    the reference's own cfpq-cpp sources + the oracle's M4RM matrix) is run on
+    the same files and must agree; its add_mul call count is recorded, since
+    the headline "effective bit-ops" figure is 2 N^3 per reference add_mul;
+  * elsewhere the add_mul count comes from the oracle's restatement of the
+    same worklist (cfpq-cpp/Grammar.h:58-69, identical std::unordered_set pop
+    order) with the row-gather product, which makes the same calls.
+Input digests pin the generators.
+
+usage: python tests/golden/make_golden.py [--skip-big]
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+import oracle  # noqa: E402
+
+gen = importlib.import_module("cfpq-on-gpgpu_amd.gen")
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fixpoints.json")
+
+
+def entry(q, ref_driver: bool, worklist: bool):
+    t0 = time.time()
+    g, mats, info = oracle.solve(q, "seminaive")
+    e = {
+        "name": q.name, "n": q.n, "edges": q.n_edges, "digest": q.digest(),
+        "fingerprints": {k: list(v) for k, v in oracle.fingerprints(g, mats, q.n).items()},
+        "seminaive_iterations": info["iterations"],
+    }
+    del mats
+    if worklist:
+        g2, mats2, info2 = oracle.solve(q, "rows")
+        assert oracle.fingerprints(g2, mats2, q.n) == {k: tuple(v) for k, v in e["fingerprints"].items()}
+        e["add_mul_calls"] = info2["add_mul_calls"]
+        del mats2
+    if ref_driver and os.path.exists(oracle.REF_BIN) and int(max(q.src.max(), q.dst.max())) + 1 == q.n:
+        with tempfile.TemporaryDirectory() as d:
+            gp, ep, op = (os.path.join(d, x) for x in ("g.txt", "e.txt", "o.txt"))
+            open(gp, "w").write(q.grammar)
+            open(ep, "w").write(q.graph_text())
+            ref, rinfo = oracle.run_reference_driver(gp, ep, op)
+        assert ref == {k: tuple(v) for k, v in e["fingerprints"].items()}, q.name
+        e["reference_driver"] = rinfo
+    e["oracle_seconds"] = round(time.time() - t0, 2)
+    print(q.name, e["oracle_seconds"], "s", flush=True)
+    return e
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--skip-big", action="store_true")
+    args = ap.parse_args()
+    out = {}
+    out["config0"] = entry(gen.config(0), ref_driver=True, worklist=True)
+    out["config1"] = entry(gen.config(1), ref_driver=True, worklist=True)
+    out["config2_n2048"] = entry(gen.dense_stress(2048), ref_driver=True, worklist=True)
+    out["config3_first16"] = [entry(q, ref_driver=False, worklist=False) for q in gen.config(3)[:16]]
+    if not args.skip_big:
+        out["config4"] = entry(gen.config(4), ref_driver=False, worklist=True)
+        out["config2_n4096"] = entry(gen.dense_stress(4096), ref_driver=False, worklist=False)
+    with open(OUT, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,204 @@
+"""Parity of the HIP engine with the CPU oracle, through the C ABI (needs a GPU).
+
+Bar: bit-exact.  Every product and fixpoint result is compared word for word
+(or by the reference fingerprint plus popcount at sizes where a full dump is
+large) with the oracle on the same seeded inputs.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = ["gather", "lds", "auto", "m4r", "mfma"]
+
+
+def rand_mat(rng, n, density):
+    return rng.random((n, n)) < density
+
+
+def upload(engine, oracle, d):
+    m = engine.matrix(d.shape[0])
+    m.upload(oracle.from_dense(d))
+    return m
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("n", [1, 63, 64, 255, 256, 257, 600])
+def test_add_mul_random(pkg, oracle, n, kernel):
+    eng = pkg.Engine(0, kernel=kernel)
+    try:
+        rng = np.random.default_rng(n * 7 + len(kernel))
+        for density in (0.0, 0.003, 0.05, 0.5, 1.0):
+            a, b, c = rand_mat(rng, n, density), rand_mat(rng, n, density), rand_mat(rng, n, 0.01)
+            A, B, C = oracle.from_dense(a), oracle.from_dense(b), oracle.from_dense(c)
+            want = C.copy()
+            want_changed = oracle.addmul_m4rm(want, A, B, n)
+            dA, dB, dC = upload(eng, oracle, a), upload(eng, oracle, b), upload(eng, oracle, c)
+            changed = eng.add_mul(dC, dA, dB)
+            got = dC.download()
+            assert np.array_equal(got, want), (n, density, kernel)
+            assert changed == want_changed
+            # metadata stays consistent: popcount and fingerprint agree with the oracle
+            assert dC.popcount() == int(np.unpackbits(want.view(np.uint8)).sum())
+            assert dC.fingerprint() == oracle.fingerprint(want, n)
+            for m in (dA, dB, dC):
+                m.free()
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("alias", ["c=a", "c=b", "c=a=b"])
+def test_add_mul_aliasing(pkg, oracle, engine, alias):
+    n = 300
+    rng = np.random.default_rng(5)
+    c, o = rand_mat(rng, n, 0.02), rand_mat(rng, n, 0.02)
+    C, O = oracle.from_dense(c), oracle.from_dense(o)
+    want = C.copy()
+    if alias == "c=a":
+        oracle.addmul_m4rm(want, C.copy(), O, n)
+    elif alias == "c=b":
+        oracle.addmul_m4rm(want, O, C.copy(), n)
+    else:
+        oracle.addmul_m4rm(want, C.copy(), C.copy(), n)
+    dC, dO = upload(engine, oracle, c), upload(engine, oracle, o)
+    if alias == "c=a":
+        engine.add_mul(dC, dC, dO)
+    elif alias == "c=b":
+        engine.add_mul(dC, dO, dC)
+    else:
+        engine.add_mul(dC, dC, dC)
+    assert np.array_equal(dC.download(), want)
+    dC.free()
+    dO.free()
+
+
+def test_upload_download_roundtrip(pkg, oracle, engine):
+    rng = np.random.default_rng(1)
+    for n in (1, 5, 64, 100, 256, 513, 1000):
+        d = rand_mat(rng, n, 0.1)
+        m = upload(engine, oracle, d)
+        assert np.array_equal(oracle.to_dense(m.download(), n), d)
+        m.free()
+
+
+def test_set_bits(pkg, oracle, engine):
+    rng = np.random.default_rng(2)
+    n = 777
+    r = rng.integers(0, n, 5000)
+    c = rng.integers(0, n, 5000)
+    m = engine.matrix(n)
+    m.set_bits(r, c)
+    m.set_bits(r[:10], c[:10])  # idempotent
+    want = oracle.zeros(n)
+    oracle.set_bits(want, r, c)
+    assert np.array_equal(m.download(), want)
+    assert m.fingerprint() == oracle.fingerprint(want, n)
+    with pytest.raises(pkg.CfpqError):
+        m.set_bits([n], [0])
+    m.free()
+
+
+def _fixpoint_vs_oracle(pkg, gen, oracle, q, method="seminaive", **opts):
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    g_ref, mats_ref, info = oracle.solve(q, method)
+    want = oracle.fingerprints(g_ref, mats_ref, q.n)
+    eng = pkg.Engine(0, **opts)
+    try:
+        gr = cf.Grammar(q.grammar, text=True)
+        total_ms, algo_ms, stats = gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+        got = gr.results()
+        assert got == want
+        # word-exact on sizes where a dump is cheap
+        if q.n <= 4096:
+            for name, idx in g_ref.nt_index.items():
+                assert np.array_equal(gr.matrices[gr.nonterminal_to_index[name]].download(), mats_ref[idx]), name
+        gr.free()
+        return stats
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("kernel", ["gather", "lds", "auto"])
+def test_fixpoint_dyck_er_1024(pkg, gen, oracle, kernel):
+    stats = _fixpoint_vs_oracle(pkg, gen, oracle, gen.config(0), kernel=kernel)
+    assert stats["iterations"] >= 2
+
+
+def test_fixpoint_sg_forest_16384(pkg, gen, oracle):
+    _fixpoint_vs_oracle(pkg, gen, oracle, gen.config(1))
+
+
+@pytest.mark.parametrize("n", [512, 1500])
+def test_fixpoint_dense_stress_scaled(pkg, gen, oracle, n):
+    _fixpoint_vs_oracle(pkg, gen, oracle, gen.dense_stress(n))
+
+
+def test_fixpoint_final_full_pass(pkg, gen, oracle):
+    stats = _fixpoint_vs_oracle(pkg, gen, oracle, gen.dyck_er(700, 2000, seed=3), final_full_pass=True)
+    assert stats["full_passes"] == 1
+
+
+@pytest.mark.parametrize("sync", [1, 3, 8])
+def test_fixpoint_sync_cadence(pkg, gen, oracle, sync):
+    _fixpoint_vs_oracle(pkg, gen, oracle, gen.dyck_er(400, 1200, seed=8), iters_per_sync=sync)
+
+
+def test_fixpoint_epsilon_and_self_rules(pkg, gen, oracle):
+    q = gen.dyck_er(300, 800, seed=21)
+    q.grammar = "S\nS A S1\nS1 S B\nS S S\nA a\nB b\n"
+    _fixpoint_vs_oracle(pkg, gen, oracle, q)
+
+
+def test_fixpoint_iteration_cap(pkg, gen, oracle):
+    eng = pkg.Engine(0, max_iterations=2, iters_per_sync=1)
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    try:
+        gr = cf.Grammar(gen.DYCK_GRAMMAR, text=True)
+        with pytest.raises(pkg.CfpqError) as ei:
+            gr.intersection_with_graph(cf.Graph.from_query(gen.config(0)), eng)
+        assert ei.value.code == pkg.E_LIMIT
+        # the context stays usable after the error
+        eng.set_opts()
+        _ = eng.matrix(10).popcount()
+    finally:
+        eng.close()
+
+
+def test_worklist_mode_matches(pkg, gen, oracle, engine):
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    q = gen.dyck_er(500, 1500, seed=12)
+    g_ref, mats_ref, _ = oracle.solve(q, "m4rm")
+    gr = cf.Grammar(q.grammar, text=True)
+    gr.intersection_with_graph(cf.Graph.from_query(q), engine, mode="worklist")
+    assert gr.results() == oracle.fingerprints(g_ref, mats_ref, q.n)
+    gr.free()
+
+
+def test_batch_matches_single(pkg, gen, oracle, engine):
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    qs = gen.batch_queries(12, seed=99, n_lo=256, n_hi=1500)
+    grs = []
+    for q in qs:
+        gr = cf.Grammar(q.grammar, text=True)
+        gr.load(engine, cf.Graph.from_query(q))
+        grs.append(gr)
+    stats = engine.batch_fixpoint([(gr.rule_triples(), gr.matrices) for gr in grs])
+    assert stats["iterations"] >= 2
+    for q, gr in zip(qs, grs):
+        g_ref, mats_ref, _ = oracle.solve(q, "seminaive")
+        assert gr.results() == oracle.fingerprints(g_ref, mats_ref, q.n), q.name
+        gr.free()
+
+
+def test_invalid_arguments(pkg, engine):
+    a = engine.matrix(10)
+    b = engine.matrix(11)
+    with pytest.raises(pkg.CfpqError) as ei:
+        engine.add_mul(a, a, b)
+    assert ei.value.code == pkg.E_INVALID
+    with pytest.raises(pkg.CfpqError):
+        engine.fixpoint([(0, 0, 5)], [a])
+    with pytest.raises(pkg.CfpqError):
+        engine.fixpoint([(0, 0, 1)], [a, b])
+    a.free()
+    b.free()
