@@ -1,0 +1,68 @@
+"""The C++ drop-in host (include/cfpq/*.h + cfpq-on-gpgpu_amd/host/*.cpp) on
+the CPU: its grammar/graph parsers, worklist driver and print_results must
+reproduce the reference driver's output file (oracle/_ref/cfpq_ref runs the
+reference's own cfpq-cpp sources) or, without it, the oracle's fingerprints."""
+import os
+import subprocess
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def host_cpu_bin(tmp_path_factory):
+    d = tmp_path_factory.mktemp("hostbin")
+    exe = str(d / "cfpq_host_cpu")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include", "cfpq"), "-o", exe,
+                    os.path.join(ROOT, "tests", "host", "cpu_backend_main.cpp"),
+                    os.path.join(ROOT, "cfpq-on-gpgpu_amd", "host", "Grammar.cpp"),
+                    os.path.join(ROOT, "cfpq-on-gpgpu_amd", "host", "Graph.cpp")], check=True)
+    return exe
+
+
+def read_results(path):
+    out = {}
+    for line in open(path):
+        name, h, c = line.split()
+        out[name] = (int(h), int(c))
+    return out
+
+
+@pytest.mark.parametrize("which", ["dyck", "sg", "random_cnf", "epsilon"])
+def test_host_matches_reference_output(host_cpu_bin, oracle, gen, which):
+    q = {"dyck": lambda: gen.dyck_er(300, 900, seed=41),
+         "sg": lambda: gen.sg_forest(1500, seed=42, types_per_vertex=3),
+         "random_cnf": lambda: gen.dense_stress(200, seed=43, n_nt=9, n_rules=16, mean_deg=2.0),
+         "epsilon": lambda: gen.dyck_er(150, 400, seed=44)}[which]()
+    if which == "epsilon":
+        q.grammar = "S\nS A S1\nS1 S B\nS S S\nA a\nB b\n"
+    with tempfile.TemporaryDirectory() as d:
+        gp, ep, o1, o2 = (os.path.join(d, x) for x in ("g.txt", "e.txt", "host.txt", "ref.txt"))
+        open(gp, "w").write(q.grammar)
+        open(ep, "w").write(q.graph_text())
+        subprocess.run([host_cpu_bin, gp, ep, o1], check=True, capture_output=True)
+        got = read_results(o1)
+        if os.path.exists(oracle.REF_BIN):
+            ref, _ = oracle.run_reference_driver(gp, ep, o2)
+            assert got == ref
+            # same line order as the reference writer (std::map name order)
+            assert [l.split()[0] for l in open(o1)] == [l.split()[0] for l in open(o2)]
+        q.n = int(max(q.src.max(), q.dst.max())) + 1
+        g, mats, _ = oracle.solve(q, "seminaive")
+        assert got == oracle.fingerprints(g, mats, q.n)
+
+
+def test_cli_fails_loudly_without_gpu(tmp_path):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    exe = os.path.join(ROOT, "cfpq-on-gpgpu_amd", "cfpq_hip")
+    if not os.path.exists(exe):
+        pytest.skip("cfpq_hip not built")
+    (tmp_path / "g.txt").write_text("S a\n")
+    (tmp_path / "e.txt").write_text("0 a 1\n")
+    p = subprocess.run([exe, str(tmp_path / "g.txt"), str(tmp_path / "e.txt"), str(tmp_path / "o.txt")],
+                       capture_output=True, text=True)
+    assert p.returncode != 0 and "cfpq_create" in p.stderr
