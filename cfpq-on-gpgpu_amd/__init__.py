@@ -35,6 +35,7 @@ class Opts(ctypes.Structure):
         ("kernel", ctypes.c_int32),
         ("dense_bits", ctypes.c_uint32),
         ("mfma_bits", ctypes.c_uint32),
+        ("m4r_bits", ctypes.c_uint32),
         ("max_iterations", ctypes.c_uint32),
         ("final_full_pass", ctypes.c_int32),
         ("iters_per_sync", ctypes.c_int32),
@@ -59,6 +60,7 @@ class Stats(ctypes.Structure):
         ("a_rows", ctypes.c_uint64),
         ("b_rows", ctypes.c_uint64),
         ("b_rows_lds", ctypes.c_uint64),
+        ("m4r_lookups", ctypes.c_uint64),
         ("out_rows", ctypes.c_uint64),
         ("fixpoint_ms", ctypes.c_double),
         ("product_ms", ctypes.c_double),
@@ -129,13 +131,14 @@ def lib():
     return L
 
 
-def make_opts(kernel="auto", dense_bits=0, mfma_bits=0, max_iterations=0, final_full_pass=False,
+def make_opts(kernel="auto", dense_bits=0, mfma_bits=0, m4r_bits=0, max_iterations=0, final_full_pass=False,
               iters_per_sync=0, use_graph=False) -> Opts:
     o = Opts()
     lib().cfpq_default_opts(ctypes.byref(o))
     o.kernel = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
     o.dense_bits = dense_bits
     o.mfma_bits = mfma_bits
+    o.m4r_bits = m4r_bits
     o.max_iterations = max_iterations
     o.final_full_pass = int(bool(final_full_pass))
     o.iters_per_sync = iters_per_sync

@@ -137,7 +137,8 @@ extern "C" void cfpq_default_opts(cfpq_opts *o) {
 }
 
 static void normalise(cfpq_opts *o) {
-    if (o->dense_bits == 0) o->dense_bits = 2048;  // >= 8 bits per tile row on average
+    if (o->dense_bits == 0) o->dense_bits = 2048;   // >= 8 bits per tile row on average
+    if (o->m4r_bits == 0) o->m4r_bits = 16384;      // >= 25% dense
     if (o->iters_per_sync <= 0) o->iters_per_sync = 4;
     if (o->iters_per_sync > RING / 2) o->iters_per_sync = RING / 2;
 }
@@ -452,8 +453,27 @@ struct Plan {
 };
 
 struct StepStats {
-    uint64_t new_bits = 0, items = 0, tp = 0, tpd = 0, tpm = 0, arows = 0, brows = 0, orows = 0, brows_lds = 0;
+    uint64_t new_bits = 0, items = 0, tp = 0, tpd = 0, tpm = 0, arows = 0, brows = 0, orows = 0, brows_lds = 0,
+             lookups = 0;
 };
+
+// A-tile popcount thresholds of the product kernel: below dense_threshold
+// gather B rows from HBM, from there LDS-staged OR-of-rows, from
+// m4r_threshold the Four Russians tables.
+static uint32_t dense_threshold(const cfpq_opts &o) {
+    switch (o.kernel) {
+    case CFPQ_KERNEL_GATHER: return 0xffffffffu;
+    case CFPQ_KERNEL_LDS: case CFPQ_KERNEL_M4R: case CFPQ_KERNEL_MFMA: return 0u;
+    default: return o.dense_bits;
+    }
+}
+static uint32_t m4r_threshold(const cfpq_opts &o) {
+    switch (o.kernel) {
+    case CFPQ_KERNEL_GATHER: case CFPQ_KERNEL_LDS: return 0xffffffffu;
+    case CFPQ_KERNEL_M4R: case CFPQ_KERNEL_MFMA: return 0u;
+    default: return o.m4r_bits;
+    }
+}
 
 // Enqueue one step. parity q: deltas read from D_q, new bits to D_{1-q}.
 // prev_slot < 0: the delta consumed by this step is the initial one
@@ -472,8 +492,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[2 * ev_idx], ctx->stream));
     hipLaunchKernelGGL(k_product, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                        (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
-                       ctx->opts.kernel == CFPQ_KERNEL_GATHER ? 0xffffffffu
-                       : (ctx->opts.kernel == CFPQ_KERNEL_AUTO ? ctx->opts.dense_bits : 0u));
+                       dense_threshold(ctx->opts), m4r_threshold(ctx->opts));
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[2 * ev_idx + 1], ctx->stream));
     hipLaunchKernelGGL(k_apply, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                        (const HeadDev *)ctx->d_heads, q, c, (const uint64_t *)touched);
@@ -534,6 +553,7 @@ static void accumulate(StepStats &s, const StepCounters &c) {
     s.brows += c.b_rows;
     s.orows += c.out_rows;
     s.brows_lds += c.b_rows_lds;
+    s.lookups += c.m4r_lookups;
 }
 
 // Run delta steps to the fixpoint (plus the optional exact full pass).
@@ -637,6 +657,7 @@ static int run_fixpoint(cfpq_ctx *ctx, const Plan &plan, cfpq_stats *stats, bool
         stats->b_rows = tot.brows;
         stats->out_rows = tot.orows;
         stats->b_rows_lds = tot.brows_lds;
+        stats->m4r_lookups = tot.lookups;
         stats->fixpoint_ms = total_ms;
         stats->product_ms = product_ms;
         stats->product_launches = launches;

@@ -36,6 +36,21 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Work-group-wide dynamic work cursor for the persistent kernels: thread 0
+// takes the next index and broadcasts it through LDS.  The barriers on both
+// sides are load-bearing.  They keep every divergent block of the caller's
+// loop body apart from the "thread 0" block here; without them the compiler
+// may fuse an "r == 0" block at the end of the body with the "r == 0" fetch
+// of the next iteration, turning the loop into a divergent inner loop in
+// which 255 threads re-read a stale index while thread 0 waits outside it --
+// a livelock on SIMT hardware.
+__device__ __forceinline__ uint32_t wg_next(uint32_t *cursor, uint32_t *s_idx) {
+    __syncthreads();
+    if (threadIdx.x == 0) *s_idx = atomicAdd(cursor, 1u);
+    __syncthreads();
+    return *s_idx;
+}
+
 // ---------------------------------------------------------------------------
 // step bookkeeping
 // ---------------------------------------------------------------------------
@@ -136,6 +151,55 @@ __device__ __forceinline__ void tp_lds(const uint64_t *__restrict__ At, const ui
     }
 }
 
+// Very dense A tile: "method of four Russians" with 4-bit keys.  The B tile
+// is staged in LDS, then all 64 tables of 16 OR-combinations (B rows
+// 4g..4g+3, 32 KiB) are built once; each row of A then costs one table
+// lookup per non-zero nibble instead of one row-OR per set bit.
+// Must be reached by the whole work-group.
+constexpr int M4R_GROUPS = TILE / 4;  // 64 tables of 16 entries
+__device__ __forceinline__ void tp_m4r(const uint64_t *__restrict__ At, const uint64_t *__restrict__ Bt,
+                                       uint64_t rm, int r, Row4 *Bs, Row4 *Tab, Row4 &acc, uint32_t &arows,
+                                       uint32_t &lookups) {
+    __syncthreads();
+    Bs[r] = load_row(Bt + r * TROW_WORDS);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < (M4R_GROUPS * 16) / BLOCK; ++q) {
+        const int e = q * BLOCK + r;     // entry index = g * 16 + key
+        const int g = e >> 4, key = e & 15;
+        Row4 v = {{0, 0, 0, 0}};
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if ((key >> b) & 1) {
+                const Row4 x = Bs[4 * g + b];
+                v.w[0] |= x.w[0];
+                v.w[1] |= x.w[1];
+                v.w[2] |= x.w[2];
+                v.w[3] |= x.w[3];
+            }
+        Tab[e] = v;
+    }
+    __syncthreads();
+    if (!((rm >> (r & 63)) & 1ull)) return;
+    ++arows;
+    const Row4 a = load_row(At + r * TROW_WORDS);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        uint64_t x = a.w[w];
+        while (x) {
+            const int nib = __builtin_ctzll(x) >> 2;   // lowest non-zero nibble
+            const int key = (int)((x >> (4 * nib)) & 15u);
+            x &= ~(15ull << (4 * nib));
+            const Row4 t = Tab[(w * 16 + nib) * 16 + key];
+            acc.w[0] |= t.w[0];
+            acc.w[1] |= t.w[1];
+            acc.w[2] |= t.w[2];
+            acc.w[3] |= t.w[3];
+            ++lookups;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // product kernel (persistent, dynamic item cursor)
 // ---------------------------------------------------------------------------
@@ -145,19 +209,16 @@ __global__ __launch_bounds__(BLOCK) void k_product(const MatDev *__restrict__ ma
                                                    const TermDev *__restrict__ terms, int parity,
                                                    StepCounters *__restrict__ c,
                                                    const uint64_t *__restrict__ items,
-                                                   uint64_t *__restrict__ touched, uint32_t dense_bits) {
+                                                   uint64_t *__restrict__ touched, uint32_t dense_bits,
+                                                   uint32_t m4r_bits) {
     __shared__ Row4 Bs[TILE];
+    __shared__ Row4 Tab[M4R_GROUPS * 16];
     __shared__ uint32_t s_idx;
     __shared__ uint32_t s_sum[4];
     const int r = threadIdx.x, wv = r >> 6, lane = r & 63;
     const uint32_t n_items = c->n_items;
-    uint32_t tp = 0, tpd = 0, arows = 0, brows = 0, brows_lds = 0, out_rows = 0;
-    for (;;) {
-        if (r == 0) s_idx = atomicAdd(&c->item_next, 1u);
-        __syncthreads();
-        const uint32_t idx = s_idx;
-        __syncthreads();
-        if (idx >= n_items) break;
+    uint32_t tp = 0, tpd = 0, arows = 0, brows = 0, brows_lds = 0, out_rows = 0, lookups = 0;
+    for (uint32_t idx = wg_next(&c->item_next, &s_idx); idx < n_items; idx = wg_next(&c->item_next, &s_idx)) {
         const uint64_t it = items[idx];
         const uint32_t h = uniform((uint32_t)(it >> 40));
         const uint32_t p = uniform((uint32_t)(it & 0xffffffffffull));
@@ -177,7 +238,11 @@ __global__ __launch_bounds__(BLOCK) void k_product(const MatDev *__restrict__ ma
                     const uint64_t *Bt = R.bits + (size_t)rt * TILE_WORDS;
                     const uint64_t rm = L.rmask[(size_t)lt * 4 + wv];
                     ++tp;
-                    if (uniform(L.cnt[lt]) >= dense_bits) {
+                    const uint32_t acnt = uniform(L.cnt[lt]);
+                    if (acnt >= m4r_bits) {
+                        ++tpd;
+                        tp_m4r(At, Bt, rm, r, Bs, Tab, acc, arows, lookups);
+                    } else if (acnt >= dense_bits) {
                         ++tpd;
                         tp_lds(At, Bt, rm, r, Bs, acc, arows, brows_lds);
                     } else {
@@ -219,8 +284,9 @@ __global__ __launch_bounds__(BLOCK) void k_product(const MatDev *__restrict__ ma
         }
     }
     const uint32_t ar = wave_sum(arows), br = wave_sum(brows), orow = wave_sum(out_rows);
-    const uint32_t brl = wave_sum(brows_lds);
+    const uint32_t brl = wave_sum(brows_lds), lk = wave_sum(lookups);
     if (lane == 0) {
+        if (lk) atomicAdd((unsigned long long *)&c->m4r_lookups, (unsigned long long)lk);
         if (brl) atomicAdd((unsigned long long *)&c->b_rows_lds, (unsigned long long)brl);
         if (ar) atomicAdd((unsigned long long *)&c->a_rows, (unsigned long long)ar);
         if (br) atomicAdd((unsigned long long *)&c->b_rows, (unsigned long long)br);
@@ -240,12 +306,7 @@ __global__ __launch_bounds__(BLOCK) void k_apply(const MatDev *__restrict__ mats
     __shared__ uint32_t s_idx;
     const int r = threadIdx.x, wv = r >> 6, lane = r & 63;
     const uint32_t n = c->n_touched;
-    for (;;) {
-        if (r == 0) s_idx = atomicAdd(&c->apply_next, 1u);
-        __syncthreads();
-        const uint32_t idx = s_idx;
-        __syncthreads();
-        if (idx >= n) break;
+    for (uint32_t idx = wg_next(&c->apply_next, &s_idx); idx < n; idx = wg_next(&c->apply_next, &s_idx)) {
         const uint64_t it = touched[idx];
         const uint32_t h = uniform((uint32_t)(it >> 40));
         const uint32_t p = uniform((uint32_t)(it & 0xffffffffffull));
@@ -286,12 +347,7 @@ __global__ __launch_bounds__(BLOCK) void k_clear(const MatDev *__restrict__ mats
     __shared__ uint32_t s_idx;
     const int r = threadIdx.x, wv = r >> 6, lane = r & 63;
     const uint32_t n = list_ctr->n_touched;
-    for (;;) {
-        if (r == 0) s_idx = atomicAdd(&c->clear_next, 1u);
-        __syncthreads();
-        const uint32_t idx = s_idx;
-        __syncthreads();
-        if (idx >= n) break;
+    for (uint32_t idx = wg_next(&c->clear_next, &s_idx); idx < n; idx = wg_next(&c->clear_next, &s_idx)) {
         const uint64_t it = list[idx];
         const uint32_t h = uniform((uint32_t)(it >> 40));
         const uint32_t p = uniform((uint32_t)(it & 0xffffffffffull));
@@ -312,12 +368,14 @@ __global__ __launch_bounds__(BLOCK) void k_clear_full(const MatDev *__restrict__
         const MatDev O = mats[ids[q]];
         const uint32_t tiles = O.T * O.T;
         for (uint32_t p = blockIdx.x; p < tiles; p += gridDim.x) {
-            if (!uniform(O.cnt[p])) continue;
-            const uint64_t rm = O.rmask[(size_t)p * 4 + wv];
-            if ((rm >> lane) & 1ull) store_row(O.bits + (size_t)p * TILE_WORDS + r * TROW_WORDS, Row4{{0, 0, 0, 0}});
-            __syncthreads();
-            if (lane == 0) O.rmask[(size_t)p * 4 + wv] = 0;
-            if (r == 0) O.cnt[p] = 0;
+            if (uniform(O.cnt[p])) {
+                const uint64_t rm = O.rmask[(size_t)p * 4 + wv];
+                if ((rm >> lane) & 1ull) store_row(O.bits + (size_t)p * TILE_WORDS + r * TROW_WORDS, Row4{{0, 0, 0, 0}});
+                __syncthreads();
+                if (lane == 0) O.rmask[(size_t)p * 4 + wv] = 0;
+                if (r == 0) O.cnt[p] = 0;
+                __syncthreads();  // keeps the divergent tail apart from the next tile (see wg_next)
+            }
         }
         const uint32_t ow_words = O.T * O.ow;
         for (uint32_t x = blockIdx.x * BLOCK + r; x < ow_words; x += gridDim.x * BLOCK) {

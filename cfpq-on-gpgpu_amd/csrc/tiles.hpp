@@ -64,6 +64,7 @@ struct StepCounters {
     uint64_t b_rows;
     uint64_t out_rows;     // C-tile rows that received new bits
     uint64_t b_rows_lds;   // B rows OR-ed from LDS (dense path)
+    uint64_t m4r_lookups;  // Four-Russians table lookups
 };
 
 constexpr int RING = 64;  // step-counter ring length

@@ -71,6 +71,8 @@ typedef struct cfpq_opts {
                                    (0 = library default)                                 */
     uint32_t mfma_bits;         /* AUTO: A-tile popcount at or above which MFMA is used
                                    (0 = library default, 0xffffffff = never)             */
+    uint32_t m4r_bits;          /* AUTO: A-tile popcount at or above which the Four
+                                   Russians tables are used (0 = library default)        */
     uint32_t max_iterations;    /* 0 = unlimited                                         */
     int32_t final_full_pass;    /* 1: after the delta fixpoint, verify with one exact
                                    full pass (every rule over full matrices)             */
@@ -94,6 +96,7 @@ typedef struct cfpq_stats {
     uint64_t a_rows;            /* non-empty A-tile rows visited                         */
     uint64_t b_rows;            /* B rows gathered from HBM/L2 (one per set A bit, sparse path) */
     uint64_t b_rows_lds;        /* B rows OR-ed from LDS (dense path)                    */
+    uint64_t m4r_lookups;       /* Four Russians table lookups (32 B each)               */
     uint64_t out_rows;          /* C-tile rows that received new bits                    */
     double fixpoint_ms;         /* device time of the whole call (hipEvent)              */
     double product_ms;          /* device time inside the product kernel (hipEvent)      */
