@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--config", type=int, default=4, choices=sorted(WORKLOADS))
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--iters-per-sync", type=int, default=0)
+    ap.add_argument("--sparse-batch", type=int, default=0)
+    ap.add_argument("--tune", action="store_true", help="autotune on this workload first (cached in tuned.json)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -94,7 +96,20 @@ def main():
     n = q.n
     log(f"[bench] workload {wname}: n={n} edges={q.n_edges} A_ref={a_ref}")
 
-    eng = pkg.Engine(0, kernel=args.kernel, iters_per_sync=args.iters_per_sync)
+    tune = importlib.import_module("cfpq-on-gpgpu_amd.tune")
+    if args.tune:
+        tune.tune(pkg, cf, q, log=log)
+        os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+        with open(tune.CACHE) as f, open(os.path.join(ROOT, "gpurun_out", "tuned.json"), "w") as g:
+            g.write(f.read())
+    opts = tune.tuned_opts(n, q.grammar)
+    opts["kernel"] = args.kernel
+    if args.iters_per_sync:
+        opts["iters_per_sync"] = args.iters_per_sync
+    if args.sparse_batch:
+        opts["sparse_batch"] = args.sparse_batch
+    log(f"[bench] engine options {opts}")
+    eng = pkg.Engine(0, **opts)
     gr = cf.Grammar(q.grammar, text=True)
     mats = gr.load(eng, cf.Graph.from_query(q))
     init = []
@@ -177,6 +192,10 @@ def main():
         "fixpoint_ms_device_std": round(statistics.pstdev(fix_ms), 4),
         "iterations": s0["iterations"],
         "tile_products": s0["tile_products"],
+        "work": {k: s0[k] for k in ("items", "tile_products", "tile_products_dense", "a_rows", "b_rows",
+                                    "b_rows_lds", "m4r_lookups", "out_rows", "new_bits", "product_launches",
+                                    "host_syncs")},
+        "engine_opts": opts,
         "bit_exact": bit_exact,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -40,7 +40,10 @@ class Opts(ctypes.Structure):
         ("final_full_pass", ctypes.c_int32),
         ("iters_per_sync", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("sparse_batch", ctypes.c_int32),
+        ("sparse_waves_per_cu", ctypes.c_int32),
+        ("sparse_chunk", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -132,7 +135,8 @@ def lib():
 
 
 def make_opts(kernel="auto", dense_bits=0, mfma_bits=0, m4r_bits=0, max_iterations=0, final_full_pass=False,
-              iters_per_sync=0, use_graph=False) -> Opts:
+              iters_per_sync=0, use_graph=False, sparse_batch=0, sparse_waves_per_cu=0,
+              sparse_chunk=0) -> Opts:
     o = Opts()
     lib().cfpq_default_opts(ctypes.byref(o))
     o.kernel = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
@@ -143,6 +147,9 @@ def make_opts(kernel="auto", dense_bits=0, mfma_bits=0, m4r_bits=0, max_iteratio
     o.final_full_pass = int(bool(final_full_pass))
     o.iters_per_sync = iters_per_sync
     o.use_graph = int(bool(use_graph))
+    o.sparse_batch = sparse_batch
+    o.sparse_waves_per_cu = sparse_waves_per_cu
+    o.sparse_chunk = sparse_chunk
     return o
 
 

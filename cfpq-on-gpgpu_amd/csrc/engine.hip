@@ -50,8 +50,10 @@ struct cfpq_ctx {
     unsigned long long *scratch64_host = nullptr;
 
     // work lists
-    uint64_t *items = nullptr;
+    uint64_t *items = nullptr;   // dense work items
     size_t items_cap = 0;
+    uint64_t *units = nullptr;   // sparse work units
+    size_t units_cap = 0;
     uint64_t *lists[2] = {nullptr, nullptr};
     size_t lists_cap = 0;
 
@@ -140,6 +142,9 @@ static void normalise(cfpq_opts *o) {
     if (o->dense_bits == 0) o->dense_bits = 2048;   // >= 8 bits per tile row on average
     if (o->m4r_bits == 0) o->m4r_bits = 16384;      // >= 25% dense
     if (o->iters_per_sync <= 0) o->iters_per_sync = 4;
+    if (o->sparse_batch != 1 && o->sparse_batch != 2 && o->sparse_batch != 4) o->sparse_batch = 2;
+    if (o->sparse_waves_per_cu <= 0) o->sparse_waves_per_cu = 32;
+    if (o->sparse_chunk <= 0) o->sparse_chunk = 1;
     if (o->iters_per_sync > RING / 2) o->iters_per_sync = RING / 2;
 }
 
@@ -197,7 +202,7 @@ extern "C" int cfpq_destroy(cfpq_ctx *ctx) {
     for (auto &kv : ctx->pool) free_mat(kv.second);
     ctx->pool.clear();
     for (auto e : ctx->ev) (void)hipEventDestroy(e);
-    void *dev[] = {ctx->stage, ctx->ring, ctx->scratch64, ctx->items, ctx->lists[0], ctx->lists[1],
+    void *dev[] = {ctx->stage, ctx->ring, ctx->scratch64, ctx->items, ctx->units, ctx->lists[0], ctx->lists[1],
                    ctx->d_mats, ctx->d_heads, ctx->d_terms, ctx->d_ids};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -240,8 +245,9 @@ static int mat_create(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
     const size_t b_bits = align256(T * T * TILE_WORDS * sizeof(uint64_t));
     const size_t b_cnt = align256(T * T * sizeof(uint32_t));
     const size_t b_rm = align256(T * T * 4 * sizeof(uint64_t));
-    const size_t b_occ = align256(T * ow * sizeof(uint32_t));
-    m->bytes = b_bits + b_cnt + b_rm + 2 * b_occ;
+    const size_t b_cq = align256(T * T * QUARTERS * TROW_WORDS * sizeof(uint64_t));
+    const size_t b_occ = align256((2 + QUARTERS + 1) * T * ow * sizeof(uint32_t));  // occR, occC, occQ, occD
+    m->bytes = b_bits + b_cnt + b_rm + b_cq + b_occ;
     hipError_t e = hipMalloc(&m->base, m->bytes);
     if (e != hipSuccess) {
         delete m;
@@ -257,8 +263,11 @@ static int mat_create(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
     m->d.bits = (uint64_t *)b;
     m->d.cnt = (uint32_t *)(b + b_bits);
     m->d.rmask = (uint64_t *)(b + b_bits + b_cnt);
-    m->d.occR = (uint32_t *)(b + b_bits + b_cnt + b_rm);
-    m->d.occC = (uint32_t *)(b + b_bits + b_cnt + b_rm + b_occ);
+    m->d.colq = (uint64_t *)(b + b_bits + b_cnt + b_rm);
+    m->d.occR = (uint32_t *)(b + b_bits + b_cnt + b_rm + b_cq);
+    m->d.occC = m->d.occR + T * ow;
+    m->d.occQ = m->d.occC + T * ow;
+    m->d.occD = m->d.occQ + QUARTERS * T * ow;
     m->d.T = m->T;
     m->d.ow = (uint32_t)ow;
     *out = m;
@@ -304,9 +313,8 @@ extern "C" int cfpq_mat_upload(cfpq_ctx *ctx, cfpq_mat *mat, const uint64_t *row
     CHK(ensure_stage(ctx, (size_t)mat->n * wpr * 8));
     HIPCHK(hipMemcpy2DAsync(ctx->stage, wpr * 8, rows, words_per_row * 8, wpr * 8, mat->n, hipMemcpyHostToDevice,
                             ctx->stream));
-    const size_t occ_bytes = (size_t)mat->T * mat->d.ow * 4;
+    const size_t occ_bytes = (size_t)(2 + QUARTERS + 1) * mat->T * mat->d.ow * 4;
     HIPCHK(hipMemsetAsync(mat->d.occR, 0, occ_bytes, ctx->stream));
-    HIPCHK(hipMemsetAsync(mat->d.occC, 0, occ_bytes, ctx->stream));
     hipLaunchKernelGGL(k_rows_to_tiles, dim3(grid_for_tiles(ctx, (size_t)mat->T * mat->T)), dim3(BLOCK), 0, ctx->stream,
                        mat->d, (const uint64_t *)ctx->stage, mat->n, (uint32_t)wpr);
     CHK(launch_check(ctx, "k_rows_to_tiles"));
@@ -345,9 +353,8 @@ extern "C" int cfpq_mat_set_bits(cfpq_ctx *ctx, cfpq_mat *mat, const uint32_t *r
     const uint32_t g = (uint32_t)std::min<size_t>((count + 255) / 256, (size_t)ctx->num_cu * 32);
     hipLaunchKernelGGL(k_set_bits, dim3(g), dim3(256), 0, ctx->stream, mat->d, (const uint32_t *)dr, (const uint32_t *)dc, count);
     CHK(launch_check(ctx, "k_set_bits"));
-    const size_t occ_bytes = (size_t)mat->T * mat->d.ow * 4;
+    const size_t occ_bytes = (size_t)(2 + QUARTERS + 1) * mat->T * mat->d.ow * 4;
     HIPCHK(hipMemsetAsync(mat->d.occR, 0, occ_bytes, ctx->stream));
-    HIPCHK(hipMemsetAsync(mat->d.occC, 0, occ_bytes, ctx->stream));
     hipLaunchKernelGGL(k_recount, dim3(grid_for_tiles(ctx, (size_t)mat->T * mat->T)), dim3(BLOCK), 0, ctx->stream, mat->d);
     CHK(launch_check(ctx, "k_recount"));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -450,6 +457,8 @@ struct Plan {
     std::vector<uint32_t> copy_dst, copy_src;  // initial D0 := M
     size_t max_tiles_per_head = 0;     // max T*T over heads
     size_t total_head_tiles = 0;       // sum of T*T over heads
+    size_t max_occ_words = 0;          // max T*ow over matrices
+    size_t ids_all_off = 0;            // offset of the "every matrix" id list in d_ids
 };
 
 struct StepStats {
@@ -463,8 +472,8 @@ struct StepStats {
 static uint32_t dense_threshold(const cfpq_opts &o) {
     switch (o.kernel) {
     case CFPQ_KERNEL_GATHER: return 0xffffffffu;
-    case CFPQ_KERNEL_LDS: case CFPQ_KERNEL_M4R: case CFPQ_KERNEL_MFMA: return 0u;
-    default: return o.dense_bits;
+    case CFPQ_KERNEL_LDS: case CFPQ_KERNEL_M4R: case CFPQ_KERNEL_MFMA: return 1u;
+    default: return std::max<uint32_t>(o.dense_bits, 1u);
     }
 }
 static uint32_t m4r_threshold(const cfpq_opts &o) {
@@ -485,12 +494,37 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const TermDev *terms = ctx->d_terms + (full_pass ? plan.terms_delta.size() : 0);
     uint64_t *touched = ctx->lists[slot & 1];
     hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
+    // dense-tile bitsets of every matrix of the plan (operands of this step)
+    const uint32_t nall = (uint32_t)plan.mats.size();
+    const uint32_t *ids_all = ctx->d_ids + plan.ids_all_off;
+    hipLaunchKernelGGL(k_dense_occ, dim3(std::max<uint32_t>(1, (uint32_t)((plan.max_occ_words + 255) / 256)),
+                                         std::min<uint32_t>(nall, 65535)),
+                       dim3(256), 0, ctx->stream, (const MatDev *)ctx->d_mats, ids_all, nall,
+                       dense_threshold(ctx->opts));
     const uint32_t bx = (uint32_t)((plan.max_tiles_per_head + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                       (const HeadDev *)ctx->d_heads, terms, q, c, ctx->items);
+                       (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items);
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[2 * ev_idx], ctx->stream));
-    hipLaunchKernelGGL(k_product, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+    const dim3 sgrid((uint32_t)ctx->num_cu * (uint32_t)ctx->opts.sparse_waves_per_cu);
+    switch (ctx->opts.sparse_batch) {
+    case 1:
+        hipLaunchKernelGGL(k_product_sparse<1>, sgrid, dim3(WAVE), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                           (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->units, touched,
+                           (uint32_t)ctx->opts.sparse_chunk);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_product_sparse<2>, sgrid, dim3(WAVE), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                           (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->units, touched,
+                           (uint32_t)ctx->opts.sparse_chunk);
+        break;
+    default:
+        hipLaunchKernelGGL(k_product_sparse<4>, sgrid, dim3(WAVE), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                           (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->units, touched,
+                           (uint32_t)ctx->opts.sparse_chunk);
+        break;
+    }
+    hipLaunchKernelGGL(k_product_dense, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                        (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
                        dense_threshold(ctx->opts), m4r_threshold(ctx->opts));
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[2 * ev_idx + 1], ctx->stream));
@@ -513,13 +547,18 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     return launch_check(ctx, "fixpoint step");
 }
 
-static int upload_plan(cfpq_ctx *ctx, const Plan &plan) {
+static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
+    plan.ids_all_off = plan.all_delta[0].size() + plan.all_delta[1].size() + 2 * plan.copy_dst.size();
+    plan.max_occ_words = 0;
+    for (const MatDev &m : plan.mats) plan.max_occ_words = std::max<size_t>(plan.max_occ_words, (size_t)m.T * m.ow);
     CHK(grow(ctx, &ctx->d_mats, &ctx->d_mats_cap, plan.mats.size()));
     CHK(grow(ctx, &ctx->d_heads, &ctx->d_heads_cap, plan.heads.size()));
     CHK(grow(ctx, &ctx->d_terms, &ctx->d_terms_cap, plan.terms_delta.size() + plan.terms_full.size()));
-    const size_t nid = plan.all_delta[0].size() + plan.all_delta[1].size() + 2 * plan.copy_dst.size();
+    const size_t nid = plan.all_delta[0].size() + plan.all_delta[1].size() + 2 * plan.copy_dst.size() +
+                       plan.mats.size();
     CHK(grow(ctx, &ctx->d_ids, &ctx->d_ids_cap, nid));
     CHK(grow(ctx, &ctx->items, &ctx->items_cap, plan.total_head_tiles));
+    CHK(grow(ctx, &ctx->units, &ctx->units_cap, QUARTERS * plan.total_head_tiles));
     if (plan.total_head_tiles > ctx->lists_cap || !ctx->lists[0]) {
         size_t cap0 = ctx->lists_cap;
         CHK(grow(ctx, &ctx->lists[0], &cap0, plan.total_head_tiles));
@@ -536,6 +575,7 @@ static int upload_plan(cfpq_ctx *ctx, const Plan &plan) {
     ids.insert(ids.end(), plan.all_delta[1].begin(), plan.all_delta[1].end());
     ids.insert(ids.end(), plan.copy_dst.begin(), plan.copy_dst.end());
     ids.insert(ids.end(), plan.copy_src.begin(), plan.copy_src.end());
+    for (uint32_t x = 0; x < (uint32_t)plan.mats.size(); ++x) ids.push_back(x);  // at plan.ids_all_off
     HIPCHK(hipMemcpyAsync(ctx->d_ids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->ring, 0, sizeof(StepCounters) * RING, ctx->stream));
     // the tables above were copied from pageable host vectors that die here
@@ -545,7 +585,7 @@ static int upload_plan(cfpq_ctx *ctx, const Plan &plan) {
 
 static void accumulate(StepStats &s, const StepCounters &c) {
     s.new_bits += c.new_bits;
-    s.items += c.n_items;
+    s.items += c.n_units + c.n_dense;
     s.tp += c.tile_products;
     s.tpd += c.tp_dense;
     s.tpm += c.tp_mfma;
@@ -559,7 +599,7 @@ static void accumulate(StepStats &s, const StepCounters &c) {
 // Run delta steps to the fixpoint (plus the optional exact full pass).
 // The caller owns plan buffers; on return all delta buffers are zero again
 // (on success) -- on failure they are released instead of pooled.
-static int run_fixpoint(cfpq_ctx *ctx, const Plan &plan, cfpq_stats *stats, bool single_step_full) {
+static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool single_step_full) {
     CHK(upload_plan(ctx, plan));
     const uint32_t nd = (uint32_t)plan.all_delta[0].size();
     // D0 := M (every initial bit is new)

@@ -1,14 +1,19 @@
 // gfx950 kernels of the delta fixpoint over tiled bit-packed matrices.
 //
 // One step of the fixpoint (all rules at once, Jacobi style) is
-//   k_begin    zero this step's counters
-//   k_build    list the (head, C tile) items whose product can be non-empty
-//   k_product  persistent: for each item, OR the tile-products of every term
-//              into registers, keep only bits not yet in the head, write them
-//              to the head's "next delta" buffer
-//   k_apply    persistent: M[head] |= next delta, tile metadata updated
-//   k_clear    persistent: zero the tiles of the delta consumed by this step
-// Work groups are 256 threads; thread r owns row r of the current C tile.
+//   k_begin           zero this step's counters
+//   k_dense_occ       mark the operand tiles heavy enough for the LDS paths
+//   k_build           list the work: a (head, C tile) item goes to the dense
+//                     list if some term has a heavy A tile for it, otherwise
+//                     each non-empty 64-row quarter becomes a sparse unit
+//   k_product_sparse  one wavefront per unit, no LDS, no barriers: gathers the
+//                     B rows named by the set bits of its A rows
+//   k_product_dense   one 256-thread work-group per item: B tiles staged in
+//                     LDS, OR-of-rows or Four Russians per k-tile
+//   k_apply           M[head] |= next delta, tile metadata updated
+//   k_clear           zero the delta tiles consumed by this step
+// Both product kernels keep only bits not yet in the head and write them to
+// the head's "next delta" buffer; the first writer of a tile lists it.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "tiles.hpp"
@@ -25,6 +30,16 @@ __device__ __forceinline__ Row4 load_row(const uint64_t *p) {
 __device__ __forceinline__ void store_row(uint64_t *p, const Row4 &v) {
     *reinterpret_cast<Row4 *>(p) = v;
 }
+__device__ __forceinline__ void or_row(Row4 &a, const Row4 &b) {
+    a.w[0] |= b.w[0];
+    a.w[1] |= b.w[1];
+    a.w[2] |= b.w[2];
+    a.w[3] |= b.w[3];
+}
+__device__ __forceinline__ bool row_nonzero(const Row4 &a) { return (a.w[0] | a.w[1] | a.w[2] | a.w[3]) != 0; }
+__device__ __forceinline__ uint32_t row_popc(const Row4 &a) {
+    return (uint32_t)(__popcll(a.w[0]) + __popcll(a.w[1]) + __popcll(a.w[2]) + __popcll(a.w[3]));
+}
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
@@ -34,6 +49,16 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Wave-uniform read of memory that no thread writes during the current
+// kernel (operand metadata, work lists): going through the constant address
+// space lets the compiler use scalar loads through the scalar cache, which
+// is invalidated at every kernel launch.  Never use it for data the running
+// kernel modifies.
+template <class T>
+__device__ __forceinline__ T ld_const(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
 }
 
 // Work-group-wide dynamic work cursor for the persistent kernels: thread 0
@@ -51,6 +76,50 @@ __device__ __forceinline__ uint32_t wg_next(uint32_t *cursor, uint32_t *s_idx) {
     return *s_idx;
 }
 
+// Wavefront-wide cursor (single-wave work-groups; no barrier involved):
+// claims `chunk` consecutive indices per atomic.
+__device__ __forceinline__ uint32_t wave_next(uint32_t *cursor, uint32_t chunk) {
+    uint32_t v = 0;
+    if ((threadIdx.x & 63) == 0) v = atomicAdd(cursor, chunk);
+    return uniform(v);
+}
+
+// OR of a row over the 64 lanes of the wavefront (all lanes get the result).
+__device__ __forceinline__ Row4 wave_or_row(Row4 v) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        uint32_t lo = (uint32_t)v.w[w], hi = (uint32_t)(v.w[w] >> 32);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo |= __shfl_xor(lo, o, 64);
+            hi |= __shfl_xor(hi, o, 64);
+        }
+        v.w[w] = ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t *colq_of(const MatDev &M, uint32_t p, uint32_t q) {
+    return M.colq + ((size_t)p * QUARTERS + q) * TROW_WORDS;
+}
+__device__ __forceinline__ Row4 ld_const_row(const uint64_t *p) {
+    const __attribute__((address_space(4))) uint64_t *c = (const __attribute__((address_space(4))) uint64_t *)p;
+    Row4 r;
+    r.w[0] = c[0];
+    r.w[1] = c[1];
+    r.w[2] = c[2];
+    r.w[3] = c[3];
+    return r;
+}
+
+__device__ __forceinline__ void set_occ(const MatDev &M, uint32_t I, uint32_t J) {
+    atomicOr(&M.occR[I * M.ow + (J >> 5)], 1u << (J & 31));
+    atomicOr(&M.occC[J * M.ow + (I >> 5)], 1u << (I & 31));
+}
+__device__ __forceinline__ void set_occq(const MatDev &M, uint32_t q, uint32_t I, uint32_t J) {
+    atomicOr(&M.occQ[(q * M.T + I) * M.ow + (J >> 5)], 1u << (J & 31));
+}
+
 // ---------------------------------------------------------------------------
 // step bookkeeping
 // ---------------------------------------------------------------------------
@@ -62,47 +131,207 @@ __global__ void k_begin(StepCounters *ring, uint32_t slot) {
     }
 }
 
-// Item = (head << 40) | tile index.  A C tile is listed when some term has
-// a k with both operand tiles non-empty and the head tile is not already full.
-__global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats,
-                                                 const HeadDev *__restrict__ heads,
-                                                 const TermDev *__restrict__ terms, int parity,
-                                                 StepCounters *__restrict__ c,
-                                                 uint64_t *__restrict__ items) {
-    const uint32_t h = blockIdx.y;
-    const HeadDev hd = heads[h];
-    const uint32_t T = hd.T;
-    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
-    bool want = false;
-    if (p < T * T) {
-        const uint32_t I = p / T, J = p - I * T;
-        if (mats[hd.target].cnt[p] < TILE_BITS) {
-            for (uint32_t t = hd.t0; t < hd.t1 && !want; ++t) {
-                const MatDev L = mats[terms[t].left[parity]];
-                const MatDev R = mats[terms[t].right[parity]];
-                for (uint32_t w = 0; w < L.ow; ++w)
-                    if (L.occR[I * L.ow + w] & R.occC[J * R.ow + w]) {
-                        want = true;
-                        break;
-                    }
-            }
+// occD of matrices ids[0..nid): one thread per 32-tile word of a tile row.
+__global__ void k_dense_occ(const MatDev *__restrict__ mats, const uint32_t *__restrict__ ids, uint32_t nid,
+                            uint32_t dense_bits) {
+    for (uint32_t q = blockIdx.y; q < nid; q += gridDim.y) {
+        const MatDev M = mats[ids[q]];
+        const uint32_t words = M.T * M.ow;
+        for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < words; x += gridDim.x * blockDim.x) {
+            const uint32_t I = x / M.ow, w = x - (x / M.ow) * M.ow;
+            uint32_t bits = 0;
+            const uint32_t k0 = w * 32, k1 = (k0 + 32 < M.T) ? k0 + 32 : M.T;
+            for (uint32_t K = k0; K < k1; ++K)
+                if (M.cnt[I * M.T + K] >= dense_bits) bits |= 1u << (K - k0);
+            M.occD[x] = bits;
         }
     }
+}
+
+// Wave-aggregated append of `val` for the lanes with `want`.
+__device__ __forceinline__ void wave_append(bool want, uint64_t val, uint32_t *count, uint64_t *list) {
     const uint64_t bal = __ballot(want);
     if (!bal) return;
     const int lane = threadIdx.x & 63;
     const int leader = __ffsll((unsigned long long)bal) - 1;
     uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&c->n_items, (uint32_t)__popcll(bal));
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(bal));
     base = __shfl(base, leader, 64);
-    if (want) items[base + __popcll(bal & ((1ull << lane) - 1ull))] = ((uint64_t)h << 40) | p;
+    if (want) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = val;
+}
+
+// One thread per (head, C tile).  Skips full head tiles.
+__global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats,
+                                                 const HeadDev *__restrict__ heads,
+                                                 const TermDev *__restrict__ terms, int parity,
+                                                 StepCounters *__restrict__ c, uint64_t *__restrict__ units,
+                                                 uint64_t *__restrict__ dense_items) {
+    const uint32_t h = blockIdx.y;
+    const uint32_t T = uniform(heads[h].T), target = uniform(heads[h].target);
+    const uint32_t t0 = uniform(heads[h].t0), t1 = uniform(heads[h].t1);
+    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
+    bool dense = false;
+    uint32_t qmask = 0;
+    if (p < T * T) {
+        const uint32_t I = p / T, J = p - I * T;
+        if (mats[target].cnt[p] < TILE_BITS) {
+            for (uint32_t t = t0; t < t1; ++t) {
+                const MatDev L = mats[uniform(terms[t].left[parity])];
+                const MatDev R = mats[uniform(terms[t].right[parity])];
+                for (uint32_t w = 0; w < L.ow; ++w) {
+                    const uint32_t col = R.occC[J * R.ow + w];
+                    if (!col) continue;
+                    if (L.occD[I * L.ow + w] & col) dense = true;
+#pragma unroll
+                    for (uint32_t q = 0; q < QUARTERS; ++q)
+                        if (L.occQ[(q * T + I) * L.ow + w] & col) qmask |= 1u << q;
+                }
+            }
+        }
+    }
+    wave_append(dense, ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
+#pragma unroll
+    for (uint32_t q = 0; q < QUARTERS; ++q)
+        wave_append(!dense && ((qmask >> q) & 1u), ((uint64_t)h << 40) | ((uint64_t)p << 2) | q, &c->n_units, units);
 }
 
 // ---------------------------------------------------------------------------
-// tile-products:  acc (row r of the C tile) |= A tile row r * B tile
+// sparse product: one wavefront per (head, C tile, quarter)
 // ---------------------------------------------------------------------------
 
-// Sparse A tile: gather the B rows named by A's set bits straight from HBM/L2.
+// SPARSE_BATCH: candidate k-tiles whose loads are issued together (1, 2 or 4;
+// more hides more latency per wave but costs registers, hence occupancy).
+template <int SPARSE_BATCH>
+__global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restrict__ mats,
+                                                         const HeadDev *__restrict__ heads,
+                                                         const TermDev *__restrict__ terms, int parity,
+                                                         StepCounters *__restrict__ c,
+                                                         const uint64_t *__restrict__ units,
+                                                         uint64_t *__restrict__ touched, uint32_t chunk) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t n_units = c->n_units;
+    uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0;
+    for (uint32_t u0 = wave_next(&c->unit_next, chunk); u0 < n_units; u0 = wave_next(&c->unit_next, chunk)) {
+    const uint32_t u1 = (u0 + chunk < n_units) ? u0 + chunk : n_units;
+    for (uint32_t u = u0; u < u1; ++u) {
+        const uint64_t it = ld_const(units + u);
+        const uint32_t h = uniform((uint32_t)(it >> 40));
+        const uint32_t p = uniform((uint32_t)((it & ITEM_TILE_MASK) >> 2));
+        const uint32_t q = uniform((uint32_t)(it & 3u));
+        const uint32_t T = ld_const(&heads[h].T), t0 = ld_const(&heads[h].t0), t1 = ld_const(&heads[h].t1);
+        const uint32_t I = p / T, J = p - (p / T) * T;
+        const uint32_t row = q * WAVE + lane;
+        Row4 acc = {{0, 0, 0, 0}};
+        for (uint32_t t = t0; t < t1; ++t) {
+            const MatDev L = mats[ld_const(&terms[t].left[parity])];
+            const MatDev R = mats[ld_const(&terms[t].right[parity])];
+            for (uint32_t w = 0; w < L.ow; ++w) {
+                uint32_t kb = uniform(ld_const(&L.occQ[(q * T + I) * L.ow + w]) & ld_const(&R.occC[J * R.ow + w]));
+                while (kb) {
+                    // Up to SPARSE_BATCH candidate k-tiles at once, so that their
+                    // dependent loads (row masks, then A rows, then B rows) are
+                    // in flight together instead of one latency chain each.
+                    uint32_t lts[SPARSE_BATCH], rts[SPARSE_BATCH];
+                    uint32_t nk = 0;
+#pragma unroll
+                    for (int j = 0; j < SPARSE_BATCH; ++j) {
+                        const uint32_t K = w * 32 + (kb ? __builtin_ctz(kb) : 0);
+                        lts[j] = I * T + K;
+                        rts[j] = K * T + J;
+                        if (kb) {
+                            kb &= kb - 1;
+                            nk = j + 1;
+                        }
+                    }
+                    uint64_t rm[SPARSE_BATCH];
+                    Row4 brm[SPARSE_BATCH];
+#pragma unroll
+                    for (int j = 0; j < SPARSE_BATCH; ++j) {
+                        rm[j] = 0;
+                        brm[j] = Row4{{0, 0, 0, 0}};
+                        if ((uint32_t)j < nk) {
+                            // all three loads are independent and go out together
+                            brm[j] = ld_const_row(R.rmask + (size_t)rts[j] * 4);  // non-empty rows of the B tile
+                            const Row4 cu = ld_const_row(colq_of(L, lts[j], q));  // columns used by this A quarter
+                            const uint64_t arm = ld_const(&L.rmask[(size_t)lts[j] * 4 + q]);
+                            const bool hit = ((cu.w[0] & brm[j].w[0]) | (cu.w[1] & brm[j].w[1]) |
+                                              (cu.w[2] & brm[j].w[2]) | (cu.w[3] & brm[j].w[3])) != 0;
+                            rm[j] = hit ? arm : 0;
+                        }
+                    }
+                    Row4 a[SPARSE_BATCH];
+#pragma unroll
+                    for (int j = 0; j < SPARSE_BATCH; ++j) {
+                        a[j] = Row4{{0, 0, 0, 0}};
+                        if ((rm[j] >> lane) & 1ull) {
+                            ++arows;
+                            a[j] = load_row(L.bits + (size_t)lts[j] * TILE_WORDS + row * TROW_WORDS);
+                        }
+                    }
+                    tp += nk;
+#pragma unroll
+                    for (int j = 0; j < SPARSE_BATCH; ++j) {
+                        const uint64_t *Bt = R.bits + (size_t)rts[j] * TILE_WORDS;
+#pragma unroll
+                        for (int ww = 0; ww < 4; ++ww) {
+                            uint64_t x = a[j].w[ww] & brm[j].w[ww];  // only B rows that hold bits
+                            while (x) {
+                                const int k = ww * 64 + __builtin_ctzll(x);
+                                x &= x - 1;
+                                or_row(acc, load_row(Bt + k * TROW_WORDS));
+                                ++brows;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        const MatDev O = mats[uniform(heads[h].out[parity])];
+        const uint64_t *Mbits = mats[uniform(heads[h].target)].bits;
+        Row4 nw = {{0, 0, 0, 0}};
+        if (row_nonzero(acc)) {
+            const Row4 m = load_row(Mbits + (size_t)p * TILE_WORDS + row * TROW_WORDS);
+            nw.w[0] = acc.w[0] & ~m.w[0];
+            nw.w[1] = acc.w[1] & ~m.w[1];
+            nw.w[2] = acc.w[2] & ~m.w[2];
+            nw.w[3] = acc.w[3] & ~m.w[3];
+            if (row_nonzero(nw)) {
+                store_row(O.bits + (size_t)p * TILE_WORDS + row * TROW_WORDS, nw);
+                ++out_rows;
+            }
+        }
+        const uint64_t bal = __ballot(row_nonzero(nw));
+        const uint32_t pc = wave_sum(row_popc(nw));
+        Row4 cu = {{0, 0, 0, 0}};
+        if (bal) cu = wave_or_row(nw);
+        if (bal && lane == 0) {
+            O.rmask[(size_t)p * 4 + q] = bal;
+            store_row(colq_of(O, p, q), cu);
+            set_occq(O, q, I, J);
+            const uint32_t old = atomicAdd(&O.cnt[p], pc);
+            if (old == 0) {
+                set_occ(O, I, J);
+                touched[atomicAdd(&c->n_touched, 1u)] = ((uint64_t)h << 40) | p;
+            }
+            atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)pc);
+        }
+    }
+    }
+    const uint32_t ar = wave_sum(arows), br = wave_sum(brows), orow = wave_sum(out_rows);
+    if (lane == 0) {
+        if (tp) atomicAdd((unsigned long long *)&c->tile_products, (unsigned long long)tp);
+        if (ar) atomicAdd((unsigned long long *)&c->a_rows, (unsigned long long)ar);
+        if (br) atomicAdd((unsigned long long *)&c->b_rows, (unsigned long long)br);
+        if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dense product: one 256-thread work-group per (head, C tile); thread r owns
+// row r of the tile
+// ---------------------------------------------------------------------------
+
+// A tile with few bits: gather the B rows named by A's set bits from HBM/L2.
 __device__ __forceinline__ void tp_gather(const uint64_t *__restrict__ At, const uint64_t *__restrict__ Bt,
                                           uint64_t rm, int r, Row4 &acc, uint32_t &arows, uint32_t &brows) {
     if (!((rm >> (r & 63)) & 1ull)) return;
@@ -114,17 +343,13 @@ __device__ __forceinline__ void tp_gather(const uint64_t *__restrict__ At, const
         while (x) {
             const int k = w * 64 + __builtin_ctzll(x);
             x &= x - 1;
-            const Row4 b = load_row(Bt + k * TROW_WORDS);
-            acc.w[0] |= b.w[0];
-            acc.w[1] |= b.w[1];
-            acc.w[2] |= b.w[2];
-            acc.w[3] |= b.w[3];
+            or_row(acc, load_row(Bt + k * TROW_WORDS));
             ++brows;
         }
     }
 }
 
-// Dense A tile: stage the B tile in LDS (8 KiB, one coalesced row per
+// Heavier A tile: stage the B tile in LDS (8 KiB, one coalesced row per
 // thread), then OR rows from LDS.  Must be reached by the whole work-group.
 __device__ __forceinline__ void tp_lds(const uint64_t *__restrict__ At, const uint64_t *__restrict__ Bt,
                                        uint64_t rm, int r, Row4 *Bs, Row4 &acc, uint32_t &arows,
@@ -141,11 +366,7 @@ __device__ __forceinline__ void tp_lds(const uint64_t *__restrict__ At, const ui
         while (x) {
             const int k = w * 64 + __builtin_ctzll(x);
             x &= x - 1;
-            const Row4 b = Bs[k];
-            acc.w[0] |= b.w[0];
-            acc.w[1] |= b.w[1];
-            acc.w[2] |= b.w[2];
-            acc.w[3] |= b.w[3];
+            or_row(acc, Bs[k]);
             ++brows_lds;
         }
     }
@@ -170,13 +391,7 @@ __device__ __forceinline__ void tp_m4r(const uint64_t *__restrict__ At, const ui
         Row4 v = {{0, 0, 0, 0}};
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-            if ((key >> b) & 1) {
-                const Row4 x = Bs[4 * g + b];
-                v.w[0] |= x.w[0];
-                v.w[1] |= x.w[1];
-                v.w[2] |= x.w[2];
-                v.w[3] |= x.w[3];
-            }
+            if ((key >> b) & 1) or_row(v, Bs[4 * g + b]);
         Tab[e] = v;
     }
     __syncthreads();
@@ -190,44 +405,36 @@ __device__ __forceinline__ void tp_m4r(const uint64_t *__restrict__ At, const ui
             const int nib = __builtin_ctzll(x) >> 2;   // lowest non-zero nibble
             const int key = (int)((x >> (4 * nib)) & 15u);
             x &= ~(15ull << (4 * nib));
-            const Row4 t = Tab[(w * 16 + nib) * 16 + key];
-            acc.w[0] |= t.w[0];
-            acc.w[1] |= t.w[1];
-            acc.w[2] |= t.w[2];
-            acc.w[3] |= t.w[3];
+            or_row(acc, Tab[(w * 16 + nib) * 16 + key]);
             ++lookups;
         }
     }
 }
 
-// ---------------------------------------------------------------------------
-// product kernel (persistent, dynamic item cursor)
-// ---------------------------------------------------------------------------
-
-__global__ __launch_bounds__(BLOCK) void k_product(const MatDev *__restrict__ mats,
-                                                   const HeadDev *__restrict__ heads,
-                                                   const TermDev *__restrict__ terms, int parity,
-                                                   StepCounters *__restrict__ c,
-                                                   const uint64_t *__restrict__ items,
-                                                   uint64_t *__restrict__ touched, uint32_t dense_bits,
-                                                   uint32_t m4r_bits) {
+__global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restrict__ mats,
+                                                         const HeadDev *__restrict__ heads,
+                                                         const TermDev *__restrict__ terms, int parity,
+                                                         StepCounters *__restrict__ c,
+                                                         const uint64_t *__restrict__ items,
+                                                         uint64_t *__restrict__ touched, uint32_t dense_bits,
+                                                         uint32_t m4r_bits) {
     __shared__ Row4 Bs[TILE];
     __shared__ Row4 Tab[M4R_GROUPS * 16];
     __shared__ uint32_t s_idx;
     __shared__ uint32_t s_sum[4];
     const int r = threadIdx.x, wv = r >> 6, lane = r & 63;
-    const uint32_t n_items = c->n_items;
+    const uint32_t n_items = c->n_dense;
     uint32_t tp = 0, tpd = 0, arows = 0, brows = 0, brows_lds = 0, out_rows = 0, lookups = 0;
-    for (uint32_t idx = wg_next(&c->item_next, &s_idx); idx < n_items; idx = wg_next(&c->item_next, &s_idx)) {
+    for (uint32_t idx = wg_next(&c->dense_next, &s_idx); idx < n_items; idx = wg_next(&c->dense_next, &s_idx)) {
         const uint64_t it = items[idx];
         const uint32_t h = uniform((uint32_t)(it >> 40));
-        const uint32_t p = uniform((uint32_t)(it & 0xffffffffffull));
-        const HeadDev hd = heads[h];
-        const uint32_t T = hd.T, I = p / T, J = p - (p / T) * T;
+        const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
+        const uint32_t T = uniform(heads[h].T), t0 = uniform(heads[h].t0), t1 = uniform(heads[h].t1);
+        const uint32_t I = p / T, J = p - (p / T) * T;
         Row4 acc = {{0, 0, 0, 0}};
-        for (uint32_t t = hd.t0; t < hd.t1; ++t) {
-            const MatDev L = mats[terms[t].left[parity]];
-            const MatDev R = mats[terms[t].right[parity]];
+        for (uint32_t t = t0; t < t1; ++t) {
+            const MatDev L = mats[uniform(terms[t].left[parity])];
+            const MatDev R = mats[uniform(terms[t].right[parity])];
             for (uint32_t w = 0; w < L.ow; ++w) {
                 uint32_t kb = uniform(L.occR[I * L.ow + w] & R.occC[J * R.ow + w]);
                 while (kb) {
@@ -251,23 +458,27 @@ __global__ __launch_bounds__(BLOCK) void k_product(const MatDev *__restrict__ ma
                 }
             }
         }
-        const MatDev M = mats[hd.target];
-        const MatDev O = mats[hd.out[parity]];
+        const MatDev O = mats[uniform(heads[h].out[parity])];
+        const uint64_t *Mbits = mats[uniform(heads[h].target)].bits;
         const size_t row_off = (size_t)p * TILE_WORDS + r * TROW_WORDS;
-        const Row4 m = load_row(M.bits + row_off);
-        Row4 nw;
-        nw.w[0] = acc.w[0] & ~m.w[0];
-        nw.w[1] = acc.w[1] & ~m.w[1];
-        nw.w[2] = acc.w[2] & ~m.w[2];
-        nw.w[3] = acc.w[3] & ~m.w[3];
-        const bool nz = (nw.w[0] | nw.w[1] | nw.w[2] | nw.w[3]) != 0;
+        Row4 nw = {{0, 0, 0, 0}};
+        if (row_nonzero(acc)) {
+            const Row4 m = load_row(Mbits + row_off);
+            nw.w[0] = acc.w[0] & ~m.w[0];
+            nw.w[1] = acc.w[1] & ~m.w[1];
+            nw.w[2] = acc.w[2] & ~m.w[2];
+            nw.w[3] = acc.w[3] & ~m.w[3];
+        }
+        const bool nz = row_nonzero(nw);
         if (nz) store_row(O.bits + row_off, nw);
-        const uint64_t bal = __ballot(nz);
         out_rows += nz;
-        const uint32_t pc = wave_sum((uint32_t)(__popcll(nw.w[0]) + __popcll(nw.w[1]) + __popcll(nw.w[2]) +
-                                                __popcll(nw.w[3])));
+        const uint64_t bal = __ballot(nz);
+        const uint32_t pc = wave_sum(row_popc(nw));
+        const Row4 cu = wave_or_row(nw);
         if (lane == 0) {
             O.rmask[(size_t)p * 4 + wv] = bal;
+            store_row(colq_of(O, p, wv), cu);
+            if (bal) set_occq(O, wv, I, J);
             s_sum[wv] = pc;
         }
         __syncthreads();
@@ -275,10 +486,8 @@ __global__ __launch_bounds__(BLOCK) void k_product(const MatDev *__restrict__ ma
             const uint32_t total = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
             if (total) {
                 O.cnt[p] = total;
-                atomicOr(&O.occR[I * O.ow + (J >> 5)], 1u << (J & 31));
-                atomicOr(&O.occC[J * O.ow + (I >> 5)], 1u << (I & 31));
-                const uint32_t pos = atomicAdd(&c->n_touched, 1u);
-                touched[pos] = it;
+                set_occ(O, I, J);
+                touched[atomicAdd(&c->n_touched, 1u)] = it;
                 atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)total);
             }
         }
@@ -286,11 +495,11 @@ __global__ __launch_bounds__(BLOCK) void k_product(const MatDev *__restrict__ ma
     const uint32_t ar = wave_sum(arows), br = wave_sum(brows), orow = wave_sum(out_rows);
     const uint32_t brl = wave_sum(brows_lds), lk = wave_sum(lookups);
     if (lane == 0) {
-        if (lk) atomicAdd((unsigned long long *)&c->m4r_lookups, (unsigned long long)lk);
-        if (brl) atomicAdd((unsigned long long *)&c->b_rows_lds, (unsigned long long)brl);
         if (ar) atomicAdd((unsigned long long *)&c->a_rows, (unsigned long long)ar);
         if (br) atomicAdd((unsigned long long *)&c->b_rows, (unsigned long long)br);
         if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
+        if (brl) atomicAdd((unsigned long long *)&c->b_rows_lds, (unsigned long long)brl);
+        if (lk) atomicAdd((unsigned long long *)&c->m4r_lookups, (unsigned long long)lk);
     }
     if (r == 0 && tp) {
         atomicAdd((unsigned long long *)&c->tile_products, (unsigned long long)tp);
@@ -298,7 +507,11 @@ __global__ __launch_bounds__(BLOCK) void k_product(const MatDev *__restrict__ ma
     }
 }
 
-// M[head] |= next-delta tile, for every tile the product kernel wrote.
+// ---------------------------------------------------------------------------
+// apply / clear
+// ---------------------------------------------------------------------------
+
+// M[head] |= next-delta tile, for every tile a product kernel wrote.
 __global__ __launch_bounds__(BLOCK) void k_apply(const MatDev *__restrict__ mats,
                                                  const HeadDev *__restrict__ heads, int parity,
                                                  StepCounters *__restrict__ c,
@@ -309,30 +522,29 @@ __global__ __launch_bounds__(BLOCK) void k_apply(const MatDev *__restrict__ mats
     for (uint32_t idx = wg_next(&c->apply_next, &s_idx); idx < n; idx = wg_next(&c->apply_next, &s_idx)) {
         const uint64_t it = touched[idx];
         const uint32_t h = uniform((uint32_t)(it >> 40));
-        const uint32_t p = uniform((uint32_t)(it & 0xffffffffffull));
-        const HeadDev hd = heads[h];
-        const MatDev M = mats[hd.target];
-        const MatDev O = mats[hd.out[parity]];
+        const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
+        const MatDev M = mats[uniform(heads[h].target)];
+        const MatDev O = mats[uniform(heads[h].out[parity])];
+        const uint32_t T = uniform(heads[h].T), I = p / T, J = p - (p / T) * T;
         const uint64_t rm = O.rmask[(size_t)p * 4 + wv];
         if ((rm >> lane) & 1ull) {
             const size_t off = (size_t)p * TILE_WORDS + r * TROW_WORDS;
             Row4 m = load_row(M.bits + off);
-            const Row4 o = load_row(O.bits + off);
-            m.w[0] |= o.w[0];
-            m.w[1] |= o.w[1];
-            m.w[2] |= o.w[2];
-            m.w[3] |= o.w[3];
+            or_row(m, load_row(O.bits + off));
             store_row(M.bits + off, m);
         }
-        if (lane == 0) M.rmask[(size_t)p * 4 + wv] |= rm;
+        if (lane == 0 && rm) {
+            const uint64_t old = M.rmask[(size_t)p * 4 + wv];
+            M.rmask[(size_t)p * 4 + wv] = old | rm;
+            Row4 cu = load_row(colq_of(M, p, wv));
+            or_row(cu, load_row(colq_of(O, p, wv)));
+            store_row(colq_of(M, p, wv), cu);
+            if (!old) set_occq(M, wv, I, J);
+        }
         if (r == 0) {
-            const uint32_t T = hd.T, I = p / T, J = p - (p / T) * T;
             const uint32_t old = M.cnt[p];
             M.cnt[p] = old + O.cnt[p];
-            if (!old) {
-                atomicOr(&M.occR[I * M.ow + (J >> 5)], 1u << (J & 31));
-                atomicOr(&M.occC[J * M.ow + (I >> 5)], 1u << (I & 31));
-            }
+            if (!old) set_occ(M, I, J);
         }
     }
 }
@@ -350,14 +562,34 @@ __global__ __launch_bounds__(BLOCK) void k_clear(const MatDev *__restrict__ mats
     for (uint32_t idx = wg_next(&c->clear_next, &s_idx); idx < n; idx = wg_next(&c->clear_next, &s_idx)) {
         const uint64_t it = list[idx];
         const uint32_t h = uniform((uint32_t)(it >> 40));
-        const uint32_t p = uniform((uint32_t)(it & 0xffffffffffull));
-        const MatDev O = mats[heads[h].out[parity]];
+        const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
+        const MatDev O = mats[uniform(heads[h].out[parity])];
         const uint64_t rm = O.rmask[(size_t)p * 4 + wv];
         if ((rm >> lane) & 1ull) store_row(O.bits + (size_t)p * TILE_WORDS + r * TROW_WORDS, Row4{{0, 0, 0, 0}});
         __syncthreads();
-        if (lane == 0) O.rmask[(size_t)p * 4 + wv] = 0;
+        if (lane == 0) {
+            O.rmask[(size_t)p * 4 + wv] = 0;
+            store_row(colq_of(O, p, wv), Row4{{0, 0, 0, 0}});
+        }
         if (r == 0) O.cnt[p] = 0;
     }
+}
+
+// Zero the occupancy bitsets (all four kinds) of one matrix, grid-strided.
+__device__ __forceinline__ void zero_occ(const MatDev &O, uint32_t x0, uint32_t stride) {
+    const uint32_t words = O.T * O.ow;
+    for (uint32_t x = x0; x < words; x += stride) {
+        O.occR[x] = 0;
+        O.occC[x] = 0;
+        O.occD[x] = 0;
+#pragma unroll
+        for (int q = 0; q < QUARTERS; ++q) O.occQ[q * words + x] = 0;
+    }
+}
+
+__global__ void k_clear_occ(const MatDev *__restrict__ mats, const uint32_t *__restrict__ ids, uint32_t nid) {
+    for (uint32_t q = blockIdx.y; q < nid; q += gridDim.y)
+        zero_occ(mats[ids[q]], blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // Zero every non-empty tile and the metadata of matrices ids[0..nid).
@@ -372,32 +604,19 @@ __global__ __launch_bounds__(BLOCK) void k_clear_full(const MatDev *__restrict__
                 const uint64_t rm = O.rmask[(size_t)p * 4 + wv];
                 if ((rm >> lane) & 1ull) store_row(O.bits + (size_t)p * TILE_WORDS + r * TROW_WORDS, Row4{{0, 0, 0, 0}});
                 __syncthreads();
-                if (lane == 0) O.rmask[(size_t)p * 4 + wv] = 0;
+                if (lane == 0) {
+                    O.rmask[(size_t)p * 4 + wv] = 0;
+                    store_row(colq_of(O, p, wv), Row4{{0, 0, 0, 0}});
+                }
                 if (r == 0) O.cnt[p] = 0;
                 __syncthreads();  // keeps the divergent tail apart from the next tile (see wg_next)
             }
         }
-        const uint32_t ow_words = O.T * O.ow;
-        for (uint32_t x = blockIdx.x * BLOCK + r; x < ow_words; x += gridDim.x * BLOCK) {
-            O.occR[x] = 0;
-            O.occC[x] = 0;
-        }
+        zero_occ(O, blockIdx.x * BLOCK + r, gridDim.x * BLOCK);
     }
 }
 
-// Zero the occupancy bitsets of matrices ids[0..nid).
-__global__ void k_clear_occ(const MatDev *__restrict__ mats, const uint32_t *__restrict__ ids, uint32_t nid) {
-    for (uint32_t q = blockIdx.y; q < nid; q += gridDim.y) {
-        const MatDev O = mats[ids[q]];
-        const uint32_t words = O.T * O.ow;
-        for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < words; x += gridDim.x * blockDim.x) {
-            O.occR[x] = 0;
-            O.occC[x] = 0;
-        }
-    }
-}
-
-// dst := src (same T) for the matrices pairs (dst_ids[q], src_ids[q]); only
+// dst := src (same T) for the matrix pairs (dst_ids[q], src_ids[q]); only
 // non-empty tiles are copied, dst must be all-zero on entry.
 __global__ __launch_bounds__(BLOCK) void k_copy(const MatDev *__restrict__ mats, const uint32_t *__restrict__ dst_ids,
                                                 const uint32_t *__restrict__ src_ids, uint32_t nid) {
@@ -412,12 +631,16 @@ __global__ __launch_bounds__(BLOCK) void k_copy(const MatDev *__restrict__ mats,
             const size_t off = (size_t)p * TILE_WORDS + r * TROW_WORDS;
             store_row(D.bits + off, load_row(S.bits + off));
             if (r < 4) D.rmask[(size_t)p * 4 + r] = S.rmask[(size_t)p * 4 + r];
+            if (r < QUARTERS * TROW_WORDS) D.colq[(size_t)p * QUARTERS * TROW_WORDS + r] = S.colq[(size_t)p * QUARTERS * TROW_WORDS + r];
             if (r == 0) D.cnt[p] = cn;
         }
-        const uint32_t ow_words = S.T * S.ow;
-        for (uint32_t x = blockIdx.x * BLOCK + r; x < ow_words; x += gridDim.x * BLOCK) {
+        const uint32_t words = S.T * S.ow;
+        for (uint32_t x = blockIdx.x * BLOCK + r; x < words; x += gridDim.x * BLOCK) {
             D.occR[x] = S.occR[x];
             D.occC[x] = S.occC[x];
+            D.occD[x] = S.occD[x];
+#pragma unroll
+            for (int qq = 0; qq < QUARTERS; ++qq) D.occQ[qq * words + x] = S.occQ[qq * words + x];
         }
     }
 }
@@ -429,22 +652,21 @@ __global__ __launch_bounds__(BLOCK) void k_copy(const MatDev *__restrict__ mats,
 // Recompute cnt / rmask / occupancy of one tile from its bits (whole WG).
 __device__ __forceinline__ void tile_meta(const MatDev &M, uint32_t p, const Row4 &v, uint32_t *s_sum) {
     const int r = threadIdx.x, wv = r >> 6, lane = r & 63;
-    const bool nz = (v.w[0] | v.w[1] | v.w[2] | v.w[3]) != 0;
-    const uint64_t bal = __ballot(nz);
-    const uint32_t pc = wave_sum((uint32_t)(__popcll(v.w[0]) + __popcll(v.w[1]) + __popcll(v.w[2]) + __popcll(v.w[3])));
+    const uint64_t bal = __ballot(row_nonzero(v));
+    const uint32_t pc = wave_sum(row_popc(v));
+    const Row4 cu = wave_or_row(v);
+    const uint32_t I = p / M.T, J = p - (p / M.T) * M.T;
     if (lane == 0) {
         M.rmask[(size_t)p * 4 + wv] = bal;
+        store_row(colq_of(M, p, wv), cu);
+        if (bal) set_occq(M, wv, I, J);
         s_sum[wv] = pc;
     }
     __syncthreads();
     if (r == 0) {
         const uint32_t total = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
         M.cnt[p] = total;
-        if (total) {
-            const uint32_t I = p / M.T, J = p - (p / M.T) * M.T;
-            atomicOr(&M.occR[I * M.ow + (J >> 5)], 1u << (J & 31));
-            atomicOr(&M.occC[J * M.ow + (I >> 5)], 1u << (I & 31));
-        }
+        if (total) set_occ(M, I, J);
     }
     __syncthreads();
 }

@@ -4,12 +4,17 @@
 // covers rows [256I, 256I+256) and columns [256J, 256J+256) and is stored as
 // 256 rows x 4 uint64 words (8 KiB, contiguous): word w of tile row r holds
 // columns 256J + 64w .. 256J + 64w + 63, bit b = column 256J + 64w + b.
-// Tiles are stored row-major by (I, J).  Per tile the engine also keeps
+// Tiles are stored row-major by (I, J).  A tile's rows fall in four
+// "quarters" of 64 rows, one per wavefront.  Per tile the engine keeps
 //   cnt[I*T+J]          popcount of the tile,
 //   rmask[(I*T+J)*4+q]  bit r%64 of word q = r/64 set iff tile row r != 0,
-// and per tile row / tile column an occupancy bitset
-//   occR[I*ow + K/32] bit K%32  <=> tile (I, K) non-empty,
-//   occC[J*ow + I/32] bit I%32  <=> tile (I, J) non-empty,   ow = ceil(T/32).
+//   colq[((I*T+J)*4+q)*4 .. +4]  OR of the rows of quarter q (256 column bits),
+// and occupancy bitsets over tile rows / columns (ow = ceil(T/32) words each):
+//   occR[I*ow + K/32]           bit K%32 <=> tile (I, K) non-empty,
+//   occC[J*ow + I/32]           bit I%32 <=> tile (I, J) non-empty,
+//   occQ[(q*T + I)*ow + K/32]   bit K%32 <=> quarter q of tile (I, K) non-empty,
+//   occD[I*ow + K/32]           bit K%32 <=> tile (I, K) holds at least the
+//                               dense-path threshold of bits (rebuilt per step).
 // Bits at rows or columns >= n are always zero.
 #pragma once
 #include <stdint.h>
@@ -20,14 +25,19 @@ constexpr int TILE = 256;                 // tile edge in bits
 constexpr int TROW_WORDS = 4;             // uint64 words per tile row
 constexpr int TILE_WORDS = TILE * TROW_WORDS;
 constexpr uint32_t TILE_BITS = TILE * TILE;
-constexpr int BLOCK = 256;                // threads per work-group: one per tile row
+constexpr int BLOCK = 256;                // threads per dense work-group: one per tile row
+constexpr int WAVE = 64;                  // gfx950 wavefront
+constexpr int QUARTERS = TILE / WAVE;     // 64-row quarters per tile
 
 struct MatDev {
     uint64_t *bits;
     uint32_t *cnt;
     uint64_t *rmask;
+    uint64_t *colq;
     uint32_t *occR;
     uint32_t *occC;
+    uint32_t *occQ;
+    uint32_t *occD;
     uint32_t T;
     uint32_t ow;
 };
@@ -50,26 +60,32 @@ struct TermDev {
 
 // Per-step counters (a ring of them lives in device memory).
 struct StepCounters {
-    uint32_t n_items;      // work items built
-    uint32_t item_next;    // product-kernel cursor
+    uint32_t n_units;      // sparse work units (head, tile, quarter) built
+    uint32_t unit_next;    // sparse product cursor
+    uint32_t n_dense;      // dense work items (head, tile) built
+    uint32_t dense_next;   // dense product cursor
     uint32_t n_touched;    // output tiles that received new bits
     uint32_t apply_next;   // apply-kernel cursor
     uint32_t clear_next;   // clear-kernel cursor
     uint32_t pad0;
     uint64_t new_bits;
-    uint64_t tile_products;
-    uint64_t tp_dense;
-    uint64_t tp_mfma;
-    uint64_t a_rows;
-    uint64_t b_rows;
-    uint64_t out_rows;     // C-tile rows that received new bits
-    uint64_t b_rows_lds;   // B rows OR-ed from LDS (dense path)
-    uint64_t m4r_lookups;  // Four-Russians table lookups
+    uint64_t tile_products;   // (term, C tile, k) products evaluated (sparse: per quarter)
+    uint64_t tp_dense;        // of which on the LDS / Four Russians paths
+    uint64_t tp_mfma;         // of which on the MFMA path
+    uint64_t a_rows;          // non-empty A rows visited
+    uint64_t b_rows;          // B rows gathered from HBM/L2
+    uint64_t out_rows;        // C-tile rows that received new bits
+    uint64_t b_rows_lds;      // B rows OR-ed from LDS
+    uint64_t m4r_lookups;     // Four-Russians table lookups
 };
 
 constexpr int RING = 64;  // step-counter ring length
 
 static inline uint32_t tiles_per_side(uint32_t n) { return (n + TILE - 1) / TILE; }
 static inline uint32_t occ_words(uint32_t T) { return (T + 31) / 32; }
+
+// Item encodings: dense item = (head << 40) | tile; sparse unit =
+// (head << 40) | (tile << 2) | quarter.
+constexpr uint64_t ITEM_TILE_MASK = 0xffffffffffull;
 
 }  // namespace cfpq

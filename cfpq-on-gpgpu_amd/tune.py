@@ -1,0 +1,114 @@
+"""Autotuner for the fixpoint engine's launch and crossover parameters.
+
+For a workload it times complete fixpoints (device time from the engine's own
+hipEvents, median of a few runs) while walking the parameters one at a time
+(coordinate descent, best value kept before moving to the next parameter):
+
+  sparse_batch         k-tiles whose loads the sparse product issues together
+  sparse_waves_per_cu  persistent waves of the sparse product per CU
+  sparse_chunk         work units a sparse-product wave claims per atomic
+  dense_bits           A-tile popcount from which a tile goes to the LDS kernel
+  m4r_bits             A-tile popcount from which the Four Russians tables are used
+  iters_per_sync       fixpoint steps enqueued per host read of the counters
+
+Every candidate is checked against the first run's fingerprints, so a tuning
+run is also a parity run.  Results are cached in ``tuned.json`` next to this
+file, keyed by workload class (log2 of N, grammar family), and
+:func:`tuned_opts` returns the cached choice for a query.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import math
+import os
+import statistics
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tuned.json")
+
+SPACE = [
+    ("sparse_batch", [1, 2, 4]),
+    ("sparse_waves_per_cu", [8, 16, 32]),
+    ("sparse_chunk", [1, 4, 16]),
+    ("dense_bits", [512, 2048, 8192, 0xFFFFFFF0]),
+    ("m4r_bits", [8192, 16384, 32768, 0xFFFFFFF0]),
+    ("iters_per_sync", [2, 4, 8]),
+]
+DEFAULTS = {"sparse_batch": 2, "sparse_waves_per_cu": 32, "sparse_chunk": 1, "dense_bits": 2048, "m4r_bits": 16384,
+            "iters_per_sync": 4}
+
+
+def workload_key(n: int, grammar_text: str) -> str:
+    fam = hashlib.sha1(" ".join(sorted(grammar_text.split())).encode()).hexdigest()[:8]
+    return f"n2^{int(round(math.log2(max(n, 2))))}:{fam}"
+
+
+def load_cache() -> dict:
+    if os.path.exists(CACHE):
+        with open(CACHE) as f:
+            return json.load(f)
+    return {}
+
+
+def tuned_opts(n: int, grammar_text: str) -> dict:
+    """Cached parameters for this workload class (library defaults if none)."""
+    ent = load_cache().get(workload_key(n, grammar_text))
+    return dict(ent["opts"]) if ent else {}
+
+
+def tune(pkg, cf, query, repeats: int = 3, log=print) -> dict:
+    """Tune on `query` (a gen.Query); returns the best option dict and stores it."""
+    eng = pkg.Engine(0)
+    gr = cf.Grammar(query.grammar, text=True)
+    mats = gr.load(eng, cf.Graph.from_query(query))
+    init = []
+    for m in mats:
+        c = eng.matrix(query.n)
+        c.copy_from(m)
+        init.append(c)
+    rules = gr.rule_triples()
+    ref = None
+
+    def run(opts) -> float:
+        nonlocal ref
+        eng.set_opts(**opts)
+        times = []
+        for _ in range(repeats):
+            for m, c in zip(mats, init):
+                m.copy_from(c)
+            st = eng.fixpoint(rules, mats)
+            times.append(st["fixpoint_ms"])
+        res = gr.results()
+        if ref is None:
+            ref = res
+        elif res != ref:
+            raise AssertionError(f"tuning candidate {opts} changed the result")
+        return statistics.median(times)
+
+    best = dict(DEFAULTS)
+    best_ms = run(best)
+    log(f"[tune] start {best} -> {best_ms:.3f} ms")
+    trace = [{"opts": dict(best), "ms": best_ms}]
+    for name, values in SPACE:
+        for v in values:
+            if v == best[name]:
+                continue
+            cand = dict(best)
+            cand[name] = v
+            ms = run(cand)
+            trace.append({"opts": dict(cand), "ms": ms})
+            log(f"[tune] {name}={v} -> {ms:.3f} ms")
+            if ms < best_ms * 0.98:
+                best, best_ms = cand, ms
+    for c in init:
+        c.free()
+    gr.free()
+    eng.close()
+    key = workload_key(query.n, query.grammar)
+    cache = load_cache()
+    cache[key] = {"workload": query.name, "opts": best, "ms": round(best_ms, 4), "trace": trace}
+    with open(CACHE, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+    log(f"[tune] best {best} -> {best_ms:.3f} ms (cached as {key})")
+    return best

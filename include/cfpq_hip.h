@@ -78,7 +78,10 @@ typedef struct cfpq_opts {
                                    full pass (every rule over full matrices)             */
     int32_t iters_per_sync;     /* fixpoint steps enqueued between host flag reads (0 = default) */
     int32_t use_graph;          /* 1: replay one captured hipGraph per fixpoint step     */
-    int32_t reserved[8];
+    int32_t sparse_batch;       /* k-tiles per batch in the sparse product: 1, 2 or 4 (0 = default) */
+    int32_t sparse_waves_per_cu; /* persistent sparse-product waves per CU (0 = default)   */
+    int32_t sparse_chunk;       /* work units a sparse-product wave claims per atomic (0 = default) */
+    int32_t reserved[5];
 } cfpq_opts;
 
 typedef struct cfpq_rule {
