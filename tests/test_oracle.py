@@ -199,3 +199,50 @@ def test_reference_driver_agrees(oracle, gen, which):
     assert oracle.fingerprints(g, mats, q.n) == ref
     # same worklist order as the reference => same number of add_mul calls
     assert winfo["add_mul_calls"] == info["add_mul_calls"]
+
+
+def _cyk_path(g, labels_on_path):
+    """CYK over every substring of the word spelled by a simple path: on a path
+    graph, CFPQ result NT(i, j) holds iff NT derives the word between i and j."""
+    n = len(labels_on_path) + 1
+    table = {}  # (i, j) -> set of nonterminals deriving word[i:j]
+    for i, lab in enumerate(labels_on_path):
+        table[(i, i + 1)] = set(g.term.get(lab, []))
+    for length in range(2, n):
+        for i in range(0, n - length):
+            j = i + length
+            cell = set()
+            for k in range(i + 1, j):
+                left, right = table[(i, k)], table[(k, j)]
+                if not left or not right:
+                    continue
+                for h, a, b in g.rules:
+                    if a in left and b in right:
+                        cell.add(h)
+            table[(i, j)] = cell
+    return table
+
+
+@pytest.mark.parametrize("which", ["dyck", "random_cnf"])
+def test_path_graph_equals_cyk(oracle, gen, which):
+    # CFPQ on a chain 0 -> 1 -> ... -> L is CYK over all substrings of its word
+    rng = np.random.default_rng(17 if which == "dyck" else 18)
+    L = 40
+    if which == "dyck":
+        grammar, labels = gen.DYCK_GRAMMAR, gen.DYCK_LABELS
+    else:
+        q0 = gen.dense_stress(64, seed=19, n_nt=6, n_rules=14, mean_deg=2.0)
+        grammar, labels = q0.grammar, q0.labels
+    lab = rng.integers(0, len(labels), size=L).astype(np.uint8)
+    src = np.arange(L, dtype=np.uint32)
+    q = gen.Query("path", L + 1, grammar, src, src + 1, lab, labels)
+    g, mats, _ = oracle.solve(q, "m4rm")
+    table = _cyk_path(g, [labels[x] for x in lab])
+    assert sum(len(c) for (i, j), c in table.items() if j - i >= 2) > 0  # non-trivial derivations exist
+    for name, idx in g.nt_index.items():
+        got = oracle.to_dense(mats[idx], L + 1)
+        want = np.zeros((L + 1, L + 1), bool)
+        for (i, j), cell in table.items():
+            if idx in cell:
+                want[i, j] = True
+        assert np.array_equal(got, want), name
