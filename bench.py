@@ -14,10 +14,13 @@ is the number of add_mul calls the reference worklist driver makes on this
 workload (recorded in tests/golden/fixpoints.json by the CPU oracle).
 
 The JSON line also carries
-  roofline      the product kernel (k_product): algorithmic bytes per launch
-                (A rows 32 B, gathered B rows 32 B, staged B tiles 8 KiB,
-                target tile 8 KiB per item, written rows 32 B) / its mean
-                launch time (hipEvents on the engine's stream), vs 8 TB/s;
+  roofline      the dominant product kernel, timed per launch with hipEvents
+                on the engine's stream:
+                  sparse / dense (HBM bound): algorithmic bytes per launch --
+                    A rows 32 B, gathered B rows 32 B, written C rows 32 B,
+                    B tiles staged in LDS 8 KiB -- / mean launch time, vs 8 TB/s;
+                  MFMA (matrix-core bound): 2 x 256^3 int8 ops per tile-product
+                    / mean launch time, vs 5.0 POP/s dense int8;
   cpu_baseline  the CPU oracle's reference worklist (dense M4RM, k = 8) on the
                 same workload, run for a bounded time and expressed in the
                 same unit.
@@ -33,13 +36,15 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0
 GOLDEN = os.path.join(ROOT, "tests", "golden", "fixpoints.json")
 TRAFFIC = os.path.join(ROOT, "profiles", "k_product_traffic.json")
 
+HBM_PEAK_GBS = 8000.0
+MFMA_I8_PEAK_TOPS = 5000.0
 WORKLOADS = {
     0: ("config0", "dyck_er_n1024"),
     1: ("config1", "sg_forest_n16384"),
+    2: ("config2", "dense_stress_n32768"),
     4: ("config4", "sg_forest_n65536_types8"),
 }
 
@@ -65,6 +70,36 @@ def cpu_baseline(q, a_ref, budget_s):
                   f"(dense M4RM, k=8), {secs:.1f} s, {per_call:.2f} s per call; "
                   f"extrapolated full fixpoint {per_call * a_ref:.0f} s",
     }
+
+
+def product_roofline(stats, wname):
+    """Roofline of the product kernel that took the most device time."""
+    launches = max(sum(s["product_launches"] for s in stats), 1)
+    t_sparse = sum(s["sparse_ms"] for s in stats)
+    t_dense = sum(s["dense_ms"] for s in stats)
+    t_mfma = sum(s["mfma_ms"] for s in stats)
+    if t_mfma >= t_sparse + t_dense:
+        ops = sum(s["tile_products_mfma"] for s in stats) * 2.0 * 256 ** 3
+        executed = sum(s["mfma_instr"] for s in stats) * 2.0 * 32 ** 3
+        per_launch_ms = t_mfma / launches
+        achieved = ops / launches / (per_launch_ms / 1e3) / 1e12 if per_launch_ms > 0 else 0.0
+        return {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_I8_PEAK_TOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / MFMA_I8_PEAK_TOPS, 5), "traffic": None, "kernel": "k_product_mfma",
+                "ops": "int8 MACs x2", "launch_ms": round(per_launch_ms, 5),
+                "alg_ops_per_launch": round(ops / launches), "executed_ops_per_launch": round(executed / launches)}
+    kern = "k_product_sparse" if t_dense == 0 else "k_product_sparse+k_product_dense"
+    alg_bytes = sum(32 * (s["a_rows"] + s["b_rows"] + s["out_rows"]) + 8192 * s["tile_products_dense"] for s in stats)
+    per_launch_ms = (t_sparse + t_dense) / launches
+    per_launch_bytes = alg_bytes / launches
+    achieved = per_launch_bytes / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
+    traffic = None
+    if os.path.exists(TRAFFIC):
+        t = json.load(open(TRAFFIC))
+        if t.get("workload") == wname:
+            traffic = t.get("bytes_per_launch")
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kern,
+            "launch_ms": round(per_launch_ms, 5), "alg_bytes_per_launch": round(per_launch_bytes)}
 
 
 def main():
@@ -147,23 +182,7 @@ def main():
     fix_ms = [s["fixpoint_ms"] for s in stats]
     value = 2.0 * n ** 3 * a_ref / (ms_per_step / 1e3) / 1e12
 
-    # roofline of the product kernel
-    launches = sum(s["product_launches"] for s in stats)
-    prod_ms = sum(s["product_ms"] for s in stats)
-    alg_bytes = sum(32 * (s["a_rows"] + s["b_rows"] + s["out_rows"]) + 8192 * (s["tile_products_dense"] + s["items"])
-                    for s in stats)
-    per_launch_bytes = alg_bytes / max(launches, 1)
-    per_launch_ms = prod_ms / max(launches, 1)
-    achieved = per_launch_bytes / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
-    traffic = None
-    if os.path.exists(TRAFFIC):
-        t = json.load(open(TRAFFIC))
-        if t.get("workload") == wname:
-            traffic = t.get("bytes_per_launch")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "k_product", "launch_ms": round(per_launch_ms, 5),
-                "alg_bytes_per_launch": round(per_launch_bytes)}
+    roofline = product_roofline(stats, wname)
 
     cpu = None
     if not args.no_cpu:
@@ -192,9 +211,10 @@ def main():
         "fixpoint_ms_device_std": round(statistics.pstdev(fix_ms), 4),
         "iterations": s0["iterations"],
         "tile_products": s0["tile_products"],
-        "work": {k: s0[k] for k in ("items", "tile_products", "tile_products_dense", "a_rows", "b_rows",
-                                    "b_rows_lds", "m4r_lookups", "out_rows", "new_bits", "product_launches",
-                                    "host_syncs")},
+        "work": {k: s0[k] for k in ("items", "tile_products", "tile_products_dense", "tile_products_mfma",
+                                    "a_rows", "b_rows", "b_rows_lds", "m4r_lookups", "mfma_instr", "out_rows",
+                                    "new_bits", "product_launches", "host_syncs")},
+        "kernel_ms": {k: round(s0[k], 4) for k in ("sparse_ms", "dense_ms", "mfma_ms")},
         "engine_opts": opts,
         "bit_exact": bit_exact,
         "roofline": roofline,

@@ -69,6 +69,10 @@ class Stats(ctypes.Structure):
         ("product_ms", ctypes.c_double),
         ("product_launches", ctypes.c_uint32),
         ("host_syncs", ctypes.c_uint32),
+        ("mfma_instr", ctypes.c_uint64),
+        ("sparse_ms", ctypes.c_double),
+        ("dense_ms", ctypes.c_double),
+        ("mfma_ms", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:

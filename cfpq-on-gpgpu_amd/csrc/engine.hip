@@ -54,6 +54,8 @@ struct cfpq_ctx {
     size_t items_cap = 0;
     uint64_t *units = nullptr;   // sparse work units
     size_t units_cap = 0;
+    uint64_t *mitems = nullptr;  // MFMA work items
+    size_t mitems_cap = 0;
     uint64_t *lists[2] = {nullptr, nullptr};
     size_t lists_cap = 0;
 
@@ -141,6 +143,7 @@ extern "C" void cfpq_default_opts(cfpq_opts *o) {
 static void normalise(cfpq_opts *o) {
     if (o->dense_bits == 0) o->dense_bits = 2048;   // >= 8 bits per tile row on average
     if (o->m4r_bits == 0) o->m4r_bits = 16384;      // >= 25% dense
+    if (o->mfma_bits == 0) o->mfma_bits = 32768;    // >= 50% dense
     if (o->iters_per_sync <= 0) o->iters_per_sync = 4;
     if (o->sparse_batch != 1 && o->sparse_batch != 2 && o->sparse_batch != 4) o->sparse_batch = 2;
     if (o->sparse_waves_per_cu <= 0) o->sparse_waves_per_cu = 32;
@@ -202,7 +205,7 @@ extern "C" int cfpq_destroy(cfpq_ctx *ctx) {
     for (auto &kv : ctx->pool) free_mat(kv.second);
     ctx->pool.clear();
     for (auto e : ctx->ev) (void)hipEventDestroy(e);
-    void *dev[] = {ctx->stage, ctx->ring, ctx->scratch64, ctx->items, ctx->units, ctx->lists[0], ctx->lists[1],
+    void *dev[] = {ctx->stage, ctx->ring, ctx->scratch64, ctx->items, ctx->units, ctx->mitems, ctx->lists[0], ctx->lists[1],
                    ctx->d_mats, ctx->d_heads, ctx->d_terms, ctx->d_ids};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -246,7 +249,7 @@ static int mat_create(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
     const size_t b_cnt = align256(T * T * sizeof(uint32_t));
     const size_t b_rm = align256(T * T * 4 * sizeof(uint64_t));
     const size_t b_cq = align256(T * T * QUARTERS * TROW_WORDS * sizeof(uint64_t));
-    const size_t b_occ = align256((2 + QUARTERS + 1) * T * ow * sizeof(uint32_t));  // occR, occC, occQ, occD
+    const size_t b_occ = align256(OCC_ARRAYS * T * ow * sizeof(uint32_t));  // occR, occC, occQ, occD, occX
     m->bytes = b_bits + b_cnt + b_rm + b_cq + b_occ;
     hipError_t e = hipMalloc(&m->base, m->bytes);
     if (e != hipSuccess) {
@@ -268,6 +271,7 @@ static int mat_create(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
     m->d.occC = m->d.occR + T * ow;
     m->d.occQ = m->d.occC + T * ow;
     m->d.occD = m->d.occQ + QUARTERS * T * ow;
+    m->d.occX = m->d.occD + T * ow;
     m->d.T = m->T;
     m->d.ow = (uint32_t)ow;
     *out = m;
@@ -313,7 +317,7 @@ extern "C" int cfpq_mat_upload(cfpq_ctx *ctx, cfpq_mat *mat, const uint64_t *row
     CHK(ensure_stage(ctx, (size_t)mat->n * wpr * 8));
     HIPCHK(hipMemcpy2DAsync(ctx->stage, wpr * 8, rows, words_per_row * 8, wpr * 8, mat->n, hipMemcpyHostToDevice,
                             ctx->stream));
-    const size_t occ_bytes = (size_t)(2 + QUARTERS + 1) * mat->T * mat->d.ow * 4;
+    const size_t occ_bytes = (size_t)OCC_ARRAYS * mat->T * mat->d.ow * 4;
     HIPCHK(hipMemsetAsync(mat->d.occR, 0, occ_bytes, ctx->stream));
     hipLaunchKernelGGL(k_rows_to_tiles, dim3(grid_for_tiles(ctx, (size_t)mat->T * mat->T)), dim3(BLOCK), 0, ctx->stream,
                        mat->d, (const uint64_t *)ctx->stage, mat->n, (uint32_t)wpr);
@@ -353,7 +357,7 @@ extern "C" int cfpq_mat_set_bits(cfpq_ctx *ctx, cfpq_mat *mat, const uint32_t *r
     const uint32_t g = (uint32_t)std::min<size_t>((count + 255) / 256, (size_t)ctx->num_cu * 32);
     hipLaunchKernelGGL(k_set_bits, dim3(g), dim3(256), 0, ctx->stream, mat->d, (const uint32_t *)dr, (const uint32_t *)dc, count);
     CHK(launch_check(ctx, "k_set_bits"));
-    const size_t occ_bytes = (size_t)(2 + QUARTERS + 1) * mat->T * mat->d.ow * 4;
+    const size_t occ_bytes = (size_t)OCC_ARRAYS * mat->T * mat->d.ow * 4;
     HIPCHK(hipMemsetAsync(mat->d.occR, 0, occ_bytes, ctx->stream));
     hipLaunchKernelGGL(k_recount, dim3(grid_for_tiles(ctx, (size_t)mat->T * mat->T)), dim3(BLOCK), 0, ctx->stream, mat->d);
     CHK(launch_check(ctx, "k_recount"));
@@ -463,23 +467,34 @@ struct Plan {
 
 struct StepStats {
     uint64_t new_bits = 0, items = 0, tp = 0, tpd = 0, tpm = 0, arows = 0, brows = 0, orows = 0, brows_lds = 0,
-             lookups = 0;
+             lookups = 0, mfma_instr = 0;
 };
 
-// A-tile popcount thresholds of the product kernel: below dense_threshold
-// gather B rows from HBM, from there LDS-staged OR-of-rows, from
-// m4r_threshold the Four Russians tables.
+// A-tile popcount thresholds of the product kernels.  A (head, C tile) item
+// goes to the MFMA kernel if some term has an A tile with >= mfma_threshold
+// bits, else to the dense kernel if some A tile has >= dense_threshold bits,
+// else to the sparse kernel.  In the dense kernel, A tiles below
+// dense_threshold gather B rows from HBM, from there LDS-staged OR-of-rows,
+// from m4r_threshold the Four Russians tables.
 static uint32_t dense_threshold(const cfpq_opts &o) {
     switch (o.kernel) {
-    case CFPQ_KERNEL_GATHER: return 0xffffffffu;
-    case CFPQ_KERNEL_LDS: case CFPQ_KERNEL_M4R: case CFPQ_KERNEL_MFMA: return 1u;
+    case CFPQ_KERNEL_GATHER: case CFPQ_KERNEL_MFMA: return 0xffffffffu;
+    case CFPQ_KERNEL_LDS: case CFPQ_KERNEL_M4R: return 1u;
     default: return std::max<uint32_t>(o.dense_bits, 1u);
+    }
+}
+static uint32_t mfma_threshold(const cfpq_opts &o) {
+    switch (o.kernel) {
+    case CFPQ_KERNEL_MFMA: return 1u;
+    case CFPQ_KERNEL_AUTO: return std::max<uint32_t>(o.mfma_bits, 1u);
+    default: return 0xffffffffu;
     }
 }
 static uint32_t m4r_threshold(const cfpq_opts &o) {
     switch (o.kernel) {
     case CFPQ_KERNEL_GATHER: case CFPQ_KERNEL_LDS: return 0xffffffffu;
-    case CFPQ_KERNEL_M4R: case CFPQ_KERNEL_MFMA: return 0u;
+    case CFPQ_KERNEL_M4R: return 0u;
+    case CFPQ_KERNEL_MFMA: return 0xffffffffu;
     default: return o.m4r_bits;
     }
 }
@@ -500,12 +515,13 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     hipLaunchKernelGGL(k_dense_occ, dim3(std::max<uint32_t>(1, (uint32_t)((plan.max_occ_words + 255) / 256)),
                                          std::min<uint32_t>(nall, 65535)),
                        dim3(256), 0, ctx->stream, (const MatDev *)ctx->d_mats, ids_all, nall,
-                       dense_threshold(ctx->opts));
+                       dense_threshold(ctx->opts), mfma_threshold(ctx->opts));
     const uint32_t bx = (uint32_t)((plan.max_tiles_per_head + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                       (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items);
+                       (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems);
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
-    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[2 * ev_idx], ctx->stream));
+    // product kernels, timed: ev[4i] sparse ev[4i+1] dense ev[4i+2] mfma ev[4i+3]
+    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx], ctx->stream));
     const dim3 sgrid((uint32_t)ctx->num_cu * (uint32_t)ctx->opts.sparse_waves_per_cu);
     switch (ctx->opts.sparse_batch) {
     case 1:
@@ -524,10 +540,15 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                            (uint32_t)ctx->opts.sparse_chunk);
         break;
     }
+    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 1], ctx->stream));
     hipLaunchKernelGGL(k_product_dense, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                        (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
                        dense_threshold(ctx->opts), m4r_threshold(ctx->opts));
-    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[2 * ev_idx + 1], ctx->stream));
+    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 2], ctx->stream));
+    hipLaunchKernelGGL(k_product_mfma, dim3((uint32_t)ctx->num_cu), dim3(MFMA_BLOCK), 0, ctx->stream,
+                       (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, terms, q, c,
+                       (const uint64_t *)ctx->mitems, touched);
+    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 3], ctx->stream));
     hipLaunchKernelGGL(k_apply, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                        (const HeadDev *)ctx->d_heads, q, c, (const uint64_t *)touched);
     // retire the delta this step consumed (buffer D_q)
@@ -559,6 +580,7 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     CHK(grow(ctx, &ctx->d_ids, &ctx->d_ids_cap, nid));
     CHK(grow(ctx, &ctx->items, &ctx->items_cap, plan.total_head_tiles));
     CHK(grow(ctx, &ctx->units, &ctx->units_cap, QUARTERS * plan.total_head_tiles));
+    CHK(grow(ctx, &ctx->mitems, &ctx->mitems_cap, plan.total_head_tiles));
     if (plan.total_head_tiles > ctx->lists_cap || !ctx->lists[0]) {
         size_t cap0 = ctx->lists_cap;
         CHK(grow(ctx, &ctx->lists[0], &cap0, plan.total_head_tiles));
@@ -585,7 +607,8 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
 
 static void accumulate(StepStats &s, const StepCounters &c) {
     s.new_bits += c.new_bits;
-    s.items += c.n_units + c.n_dense;
+    s.items += c.n_units + c.n_dense + c.n_mfma;
+    s.mfma_instr += c.mfma_instr;
     s.tp += c.tile_products;
     s.tpd += c.tp_dense;
     s.tpm += c.tp_mfma;
@@ -611,11 +634,11 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         CHK(launch_check(ctx, "k_copy"));
     }
     const int per_sync = single_step_full ? 1 : ctx->opts.iters_per_sync;
-    CHK(ensure_events(ctx, 2 * (size_t)RING + 2));
-    hipEvent_t ev_begin = ctx->ev[2 * RING], ev_end = ctx->ev[2 * RING + 1];
+    CHK(ensure_events(ctx, 4 * (size_t)RING + 2));
+    hipEvent_t ev_begin = ctx->ev[4 * RING], ev_end = ctx->ev[4 * RING + 1];
     HIPCHK(hipEventRecord(ev_begin, ctx->stream));
     StepStats tot;
-    double product_ms = 0;
+    double product_ms = 0, part_ms[3] = {0, 0, 0};
     uint32_t steps = 0, syncs = 0, launches = 0, full_passes = 0;
     int prev_slot = -1;
     bool done = false;
@@ -640,9 +663,12 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         pending_full = false;
         uint32_t used = 0;
         for (int s = 0; s < k; ++s) {
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, ctx->ev[2 * s], ctx->ev[2 * s + 1]));
-            product_ms += ms;
+            for (int k2 = 0; k2 < 3; ++k2) {
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, ctx->ev[4 * s + k2], ctx->ev[4 * s + k2 + 1]));
+                part_ms[k2] += ms;
+                product_ms += ms;
+            }
             const StepCounters &c = ctx->ring_host[slots[s]];
             accumulate(tot, c);
             ++used;
@@ -702,6 +728,10 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         stats->product_ms = product_ms;
         stats->product_launches = launches;
         stats->host_syncs = syncs;
+        stats->mfma_instr = tot.mfma_instr;
+        stats->sparse_ms = part_ms[0];
+        stats->dense_ms = part_ms[1];
+        stats->mfma_ms = part_ms[2];
     }
     return CFPQ_OK;
 }

@@ -2,14 +2,17 @@
 //
 // One step of the fixpoint (all rules at once, Jacobi style) is
 //   k_begin           zero this step's counters
-//   k_dense_occ       mark the operand tiles heavy enough for the LDS paths
-//   k_build           list the work: a (head, C tile) item goes to the dense
-//                     list if some term has a heavy A tile for it, otherwise
+//   k_dense_occ       mark the operand tiles heavy enough for the LDS / MFMA paths
+//   k_build           list the work: a (head, C tile) item goes to the MFMA
+//                     list if some term has an MFMA-heavy A tile for it, else
+//                     to the dense list if some A tile is heavy, otherwise
 //                     each non-empty 64-row quarter becomes a sparse unit
 //   k_product_sparse  one wavefront per unit, no LDS, no barriers: gathers the
 //                     B rows named by the set bits of its A rows
 //   k_product_dense   one 256-thread work-group per item: B tiles staged in
 //                     LDS, OR-of-rows or Four Russians per k-tile
+//   k_product_mfma    one 512-thread work-group per item: exact int8 0/1
+//                     matrix-core products with int32 accumulation, > 0
 //   k_apply           M[head] |= next delta, tile metadata updated
 //   k_clear           zero the delta tiles consumed by this step
 // Both product kernels keep only bits not yet in the head and write them to
@@ -131,19 +134,23 @@ __global__ void k_begin(StepCounters *ring, uint32_t slot) {
     }
 }
 
-// occD of matrices ids[0..nid): one thread per 32-tile word of a tile row.
+// occD / occX of matrices ids[0..nid): one thread per 32-tile word of a tile row.
 __global__ void k_dense_occ(const MatDev *__restrict__ mats, const uint32_t *__restrict__ ids, uint32_t nid,
-                            uint32_t dense_bits) {
+                            uint32_t dense_bits, uint32_t mfma_bits) {
     for (uint32_t q = blockIdx.y; q < nid; q += gridDim.y) {
         const MatDev M = mats[ids[q]];
         const uint32_t words = M.T * M.ow;
         for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < words; x += gridDim.x * blockDim.x) {
             const uint32_t I = x / M.ow, w = x - (x / M.ow) * M.ow;
-            uint32_t bits = 0;
+            uint32_t bits = 0, xbits = 0;
             const uint32_t k0 = w * 32, k1 = (k0 + 32 < M.T) ? k0 + 32 : M.T;
-            for (uint32_t K = k0; K < k1; ++K)
-                if (M.cnt[I * M.T + K] >= dense_bits) bits |= 1u << (K - k0);
+            for (uint32_t K = k0; K < k1; ++K) {
+                const uint32_t cn = M.cnt[I * M.T + K];
+                if (cn >= dense_bits) bits |= 1u << (K - k0);
+                if (cn >= mfma_bits) xbits |= 1u << (K - k0);
+            }
             M.occD[x] = bits;
+            M.occX[x] = xbits;
         }
     }
 }
@@ -165,12 +172,13 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
                                                  const HeadDev *__restrict__ heads,
                                                  const TermDev *__restrict__ terms, int parity,
                                                  StepCounters *__restrict__ c, uint64_t *__restrict__ units,
-                                                 uint64_t *__restrict__ dense_items) {
+                                                 uint64_t *__restrict__ dense_items,
+                                                 uint64_t *__restrict__ mfma_items) {
     const uint32_t h = blockIdx.y;
     const uint32_t T = uniform(heads[h].T), target = uniform(heads[h].target);
     const uint32_t t0 = uniform(heads[h].t0), t1 = uniform(heads[h].t1);
     const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
-    bool dense = false;
+    bool dense = false, mfma = false;
     uint32_t qmask = 0;
     if (p < T * T) {
         const uint32_t I = p / T, J = p - I * T;
@@ -182,6 +190,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
                     const uint32_t col = R.occC[J * R.ow + w];
                     if (!col) continue;
                     if (L.occD[I * L.ow + w] & col) dense = true;
+                    if (L.occX[I * L.ow + w] & col) mfma = true;
 #pragma unroll
                     for (uint32_t q = 0; q < QUARTERS; ++q)
                         if (L.occQ[(q * T + I) * L.ow + w] & col) qmask |= 1u << q;
@@ -189,7 +198,9 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
             }
         }
     }
-    wave_append(dense, ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
+    wave_append(mfma, ((uint64_t)h << 40) | p, &c->n_mfma, mfma_items);
+    wave_append(dense && !mfma, ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
+    dense = dense || mfma;
 #pragma unroll
     for (uint32_t q = 0; q < QUARTERS; ++q)
         wave_append(!dense && ((qmask >> q) & 1u), ((uint64_t)h << 40) | ((uint64_t)p << 2) | q, &c->n_units, units);
@@ -508,6 +519,218 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
 }
 
 // ---------------------------------------------------------------------------
+// MFMA product: one 512-thread work-group (8 waves) per (head, C tile)
+// ---------------------------------------------------------------------------
+//
+// C tile (256 x 256) = sum over terms and k-tiles of A tile * B tile, as an
+// exact integer product of 0/1 int8 operands with int32 accumulation
+// (v_mfma_i32_32x32x32_i8), then "> 0".  Wave w owns rows 64*(w%4) .. +63 and
+// columns 128*(w/4) .. +127 of the C tile: 2 x 4 blocks of 32 x 32, 128
+// accumulator registers per lane, kept across all k-tiles of the item.
+//
+// Operands are expanded from the bit-packed tiles on the fly:
+//   * A: lane l of a 32-row block holds row l%32, and the 16 consecutive k of
+//     half (l/32) of the current 32-k block, one byte per bit;
+//   * B: the B tile is staged in LDS, each wave transposes two 64 x 64 bit
+//     blocks across its lanes (six butterfly stages), and writes the columns
+//     as bytes to BX[column][k] (64 KiB; 16-byte chunks XOR-swizzled by the
+//     column so that a 32-lane fragment read touches distinct banks).
+// A and B bytes are placed by the same (lane/32, byte) -> k rule, which is all
+// the matrix core needs for the contraction; the result layout is the
+// documented 32 x 32 one: lane l, register r holds row 8*(r/4) + 4*(l/32) + r%4,
+// column l%32.
+constexpr int MFMA_BLOCK = 512;
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
+
+// bit e of x (e < 4) -> byte e (0 or 1): the four shifted copies x << 7e do
+// not overlap, so the product has no carries.
+__device__ __forceinline__ int32_t bytes_of_nibble(uint32_t x) {
+    return (int32_t)(((x & 15u) * 0x00204081u) & 0x01010101u);
+}
+__device__ __forceinline__ i32x4 bytes_of_16(uint32_t x) {
+    i32x4 r;
+    r[0] = bytes_of_nibble(x);
+    r[1] = bytes_of_nibble(x >> 4);
+    r[2] = bytes_of_nibble(x >> 8);
+    r[3] = bytes_of_nibble(x >> 12);
+    return r;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// 64 x 64 bit transpose across the wave: on entry lane r holds row r (bit c =
+// element (r, c)), on exit lane c holds column c (bit r = element (r, c)).
+// Stage j swaps the off-diagonal j x j sub-blocks of every 2j x 2j block.
+__device__ __forceinline__ uint64_t wave_transpose64(uint64_t v) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                              0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int j = 32 >> s;
+        const uint64_t m0 = keep[s];
+        const uint64_t p = shfl_xor64(v, j);
+        v = (lane & j) ? ((v & ~m0) | ((p & ~m0) >> j)) : ((v & m0) | ((p & m0) << j));
+    }
+    return v;
+}
+
+// word index of the 16-byte chunk `chunk` (0..15, along k) of column n in BX
+__device__ __forceinline__ int bx_index(int n, int chunk) { return n * 64 + ((chunk ^ (n & 15)) << 2); }
+
+__global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__restrict__ mats,
+                                                             const HeadDev *__restrict__ heads,
+                                                             const TermDev *__restrict__ terms, int parity,
+                                                             StepCounters *__restrict__ c,
+                                                             const uint64_t *__restrict__ items,
+                                                             uint64_t *__restrict__ touched) {
+    __shared__ __attribute__((aligned(16))) uint32_t BX[TILE * TILE / 4];  // 64 KiB
+    __shared__ Row4 Bs[TILE];  // B tile bits; the result bits in the epilogue
+    __shared__ uint32_t s_idx;
+    __shared__ uint32_t s_sum[4];
+    const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+    const int rs = wv & 3, ch = wv >> 2;
+    const uint32_t n_items = c->n_mfma;
+    uint32_t tp = 0, instr = 0, out_rows = 0;
+    for (uint32_t idx = wg_next(&c->mfma_next, &s_idx); idx < n_items; idx = wg_next(&c->mfma_next, &s_idx)) {
+        const uint64_t it = items[idx];
+        const uint32_t h = uniform((uint32_t)(it >> 40));
+        const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
+        const uint32_t T = uniform(heads[h].T), t0 = uniform(heads[h].t0), t1 = uniform(heads[h].t1);
+        const uint32_t I = p / T, J = p - (p / T) * T;
+        i32x16 acc[2][4];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = i32x16{};
+        for (uint32_t tt = t0; tt < t1; ++tt) {
+            const MatDev L = mats[uniform(terms[tt].left[parity])];
+            const MatDev R = mats[uniform(terms[tt].right[parity])];
+            for (uint32_t w = 0; w < L.ow; ++w) {
+                uint32_t kb = uniform(L.occR[I * L.ow + w] & R.occC[J * R.ow + w]);
+                while (kb) {
+                    const uint32_t K = w * 32 + __builtin_ctz(kb);
+                    kb &= kb - 1;
+                    const uint64_t *At = L.bits + (size_t)(I * T + K) * TILE_WORDS;
+                    const uint64_t *Bt = R.bits + (size_t)(K * T + J) * TILE_WORDS;
+                    ++tp;
+                    const Row4 a0 = load_row(At + (rs * 64 + (lane & 31)) * TROW_WORDS);
+                    const Row4 a1 = load_row(At + (rs * 64 + 32 + (lane & 31)) * TROW_WORDS);
+                    __syncthreads();  // the previous k-tile's readers of Bs / BX are done
+                    if (t < TILE) Bs[t] = load_row(Bt + t * TROW_WORDS);
+                    __syncthreads();
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) {
+                        const int blk = 2 * wv + i, kb64 = blk >> 2, cb64 = blk & 3;
+                        const uint64_t col = wave_transpose64(Bs[kb64 * 64 + lane].w[cb64]);
+                        const int n = cb64 * 64 + lane;  // this lane's B column; bits = k in kb64*64 ..
+#pragma unroll
+                        for (int cq = 0; cq < 4; ++cq)
+                            *reinterpret_cast<i32x4 *>(&BX[bx_index(n, kb64 * 4 + cq)]) =
+                                bytes_of_16((uint32_t)(col >> (16 * cq)) & 0xFFFFu);
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int k32 = 0; k32 < 8; ++k32) {
+                        const int sh = (k32 & 1) * 32 + (lane >> 5) * 16;
+                        const uint32_t x0 = (uint32_t)(a0.w[k32 >> 1] >> sh) & 0xFFFFu;
+                        const uint32_t x1 = (uint32_t)(a1.w[k32 >> 1] >> sh) & 0xFFFFu;
+                        const bool any0 = __ballot(x0 != 0) != 0, any1 = __ballot(x1 != 0) != 0;
+                        if (!any0 && !any1) continue;
+                        const i32x4 fa0 = bytes_of_16(x0), fa1 = bytes_of_16(x1);
+                        const int chunk = k32 * 2 + (lane >> 5);
+#pragma unroll
+                        for (int cb = 0; cb < 4; ++cb) {
+                            const int n = ch * 128 + cb * 32 + (lane & 31);
+                            const i32x4 fb = *reinterpret_cast<const i32x4 *>(&BX[bx_index(n, chunk)]);
+                            if (any0) acc[0][cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa0, fb, acc[0][cb], 0, 0, 0);
+                            if (any1) acc[1][cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa1, fb, acc[1][cb], 0, 0, 0);
+                        }
+                        instr += 4 * ((uint32_t)any0 + (uint32_t)any1);
+                    }
+                }
+            }
+        }
+        // "> 0" per accumulator register, packed by ballots: register r of block
+        // (rb, cb) gives columns cb*32 .. +31 of rows m = 8*(r/4) + r%4 (lanes
+        // 0..31) and m + 4 (lanes 32..63).  Piece 2k / 2k+1 of ballot k goes to
+        // lane (2k)%64 / (2k+1)%64 of pieces[(2k)/64].
+        uint32_t pieces[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const uint64_t b = __ballot(acc[rb][cb][r] > 0);
+                    const int k = (rb * 4 + cb) * 16 + r;
+                    if (lane == ((2 * k) & 63)) pieces[(2 * k) >> 6] = (uint32_t)b;
+                    if (lane == ((2 * k + 1) & 63)) pieces[(2 * k + 1) >> 6] = (uint32_t)(b >> 32);
+                }
+        __syncthreads();  // every wave is past its last read of Bs
+        uint32_t *Cs = reinterpret_cast<uint32_t *>(Bs);  // [256 rows][8 dwords]
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int P = 64 * j + lane, k = P >> 1, half = P & 1;
+            const int rb = k >> 6, cb = (k >> 4) & 3, r = k & 15;
+            const int row = rs * 64 + rb * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+            Cs[row * 8 + ch * 4 + cb] = pieces[j];
+        }
+        __syncthreads();
+        if (t < TILE) {
+            const Row4 accb = Bs[t];
+            const MatDev O = mats[uniform(heads[h].out[parity])];
+            const uint64_t *Mbits = mats[uniform(heads[h].target)].bits;
+            const size_t row_off = (size_t)p * TILE_WORDS + t * TROW_WORDS;
+            Row4 nw = {{0, 0, 0, 0}};
+            if (row_nonzero(accb)) {
+                const Row4 m = load_row(Mbits + row_off);
+                nw.w[0] = accb.w[0] & ~m.w[0];
+                nw.w[1] = accb.w[1] & ~m.w[1];
+                nw.w[2] = accb.w[2] & ~m.w[2];
+                nw.w[3] = accb.w[3] & ~m.w[3];
+            }
+            const bool nz = row_nonzero(nw);
+            if (nz) store_row(O.bits + row_off, nw);
+            out_rows += nz;
+            const uint64_t bal = __ballot(nz);
+            const uint32_t pc = wave_sum(row_popc(nw));
+            const Row4 cu = wave_or_row(nw);
+            if (lane == 0) {
+                O.rmask[(size_t)p * 4 + wv] = bal;
+                store_row(colq_of(O, p, wv), cu);
+                if (bal) set_occq(O, wv, I, J);
+                s_sum[wv] = pc;
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            const uint32_t total = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
+            if (total) {
+                const MatDev O = mats[heads[h].out[parity]];
+                O.cnt[p] = total;
+                set_occ(O, I, J);
+                touched[atomicAdd(&c->n_touched, 1u)] = it;
+                atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)total);
+            }
+        }
+    }
+    const uint32_t orow = wave_sum(out_rows);
+    if (lane == 0) {
+        if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
+        if (instr) atomicAdd((unsigned long long *)&c->mfma_instr, (unsigned long long)instr);
+    }
+    if (t == 0 && tp) {
+        atomicAdd((unsigned long long *)&c->tile_products, (unsigned long long)tp);
+        atomicAdd((unsigned long long *)&c->tp_mfma, (unsigned long long)tp);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // apply / clear
 // ---------------------------------------------------------------------------
 
@@ -582,6 +805,7 @@ __device__ __forceinline__ void zero_occ(const MatDev &O, uint32_t x0, uint32_t 
         O.occR[x] = 0;
         O.occC[x] = 0;
         O.occD[x] = 0;
+        O.occX[x] = 0;
 #pragma unroll
         for (int q = 0; q < QUARTERS; ++q) O.occQ[q * words + x] = 0;
     }
@@ -639,6 +863,7 @@ __global__ __launch_bounds__(BLOCK) void k_copy(const MatDev *__restrict__ mats,
             D.occR[x] = S.occR[x];
             D.occC[x] = S.occC[x];
             D.occD[x] = S.occD[x];
+            D.occX[x] = S.occX[x];
 #pragma unroll
             for (int qq = 0; qq < QUARTERS; ++qq) D.occQ[qq * words + x] = S.occQ[qq * words + x];
         }

@@ -14,7 +14,8 @@
 //   occC[J*ow + I/32]           bit I%32 <=> tile (I, J) non-empty,
 //   occQ[(q*T + I)*ow + K/32]   bit K%32 <=> quarter q of tile (I, K) non-empty,
 //   occD[I*ow + K/32]           bit K%32 <=> tile (I, K) holds at least the
-//                               dense-path threshold of bits (rebuilt per step).
+//                               dense-path threshold of bits (rebuilt per step),
+//   occX[I*ow + K/32]           the same for the MFMA threshold.
 // Bits at rows or columns >= n are always zero.
 #pragma once
 #include <stdint.h>
@@ -28,6 +29,7 @@ constexpr uint32_t TILE_BITS = TILE * TILE;
 constexpr int BLOCK = 256;                // threads per dense work-group: one per tile row
 constexpr int WAVE = 64;                  // gfx950 wavefront
 constexpr int QUARTERS = TILE / WAVE;     // 64-row quarters per tile
+constexpr int OCC_ARRAYS = 2 + QUARTERS + 2;  // occR, occC, occQ[4], occD, occX
 
 struct MatDev {
     uint64_t *bits;
@@ -38,6 +40,7 @@ struct MatDev {
     uint32_t *occC;
     uint32_t *occQ;
     uint32_t *occD;
+    uint32_t *occX;
     uint32_t T;
     uint32_t ow;
 };
@@ -67,6 +70,8 @@ struct StepCounters {
     uint32_t n_touched;    // output tiles that received new bits
     uint32_t apply_next;   // apply-kernel cursor
     uint32_t clear_next;   // clear-kernel cursor
+    uint32_t n_mfma;       // MFMA work items (head, tile) built
+    uint32_t mfma_next;    // MFMA product cursor
     uint32_t pad0;
     uint64_t new_bits;
     uint64_t tile_products;   // (term, C tile, k) products evaluated (sparse: per quarter)
@@ -77,6 +82,7 @@ struct StepCounters {
     uint64_t out_rows;        // C-tile rows that received new bits
     uint64_t b_rows_lds;      // B rows OR-ed from LDS
     uint64_t m4r_lookups;     // Four-Russians table lookups
+    uint64_t mfma_instr;      // v_mfma_i32_32x32x32_i8 issued (per wave)
 };
 
 constexpr int RING = 64;  // step-counter ring length

@@ -9,6 +9,7 @@ hipEvents, median of a few runs) while walking the parameters one at a time
   sparse_chunk         work units a sparse-product wave claims per atomic
   dense_bits           A-tile popcount from which a tile goes to the LDS kernel
   m4r_bits             A-tile popcount from which the Four Russians tables are used
+  mfma_bits            A-tile popcount from which a tile goes to the int8 MFMA kernel
   iters_per_sync       fixpoint steps enqueued per host read of the counters
 
 Every candidate is checked against the first run's fingerprints, so a tuning
@@ -33,10 +34,11 @@ SPACE = [
     ("sparse_chunk", [1, 4, 16]),
     ("dense_bits", [512, 2048, 8192, 0xFFFFFFF0]),
     ("m4r_bits", [8192, 16384, 32768, 0xFFFFFFF0]),
+    ("mfma_bits", [8192, 16384, 32768, 0xFFFFFFF0]),
     ("iters_per_sync", [2, 4, 8]),
 ]
 DEFAULTS = {"sparse_batch": 2, "sparse_waves_per_cu": 32, "sparse_chunk": 1, "dense_bits": 2048, "m4r_bits": 16384,
-            "iters_per_sync": 4}
+            "mfma_bits": 32768, "iters_per_sync": 4}
 
 
 def workload_key(n: int, grammar_text: str) -> str:

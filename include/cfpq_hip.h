@@ -62,8 +62,9 @@ typedef struct cfpq_mat cfpq_mat;
 #define CFPQ_KERNEL_AUTO 0    /* per tile, by the popcount of the A tile */
 #define CFPQ_KERNEL_GATHER 1  /* bit-packed OR-of-rows, B rows gathered from HBM */
 #define CFPQ_KERNEL_LDS 2     /* bit-packed OR-of-rows, B tile staged in LDS */
-#define CFPQ_KERNEL_M4R 3     /* Four Russians, 8-bit tables in LDS */
-#define CFPQ_KERNEL_MFMA 4    /* exact int8 0/1 MFMA, i32 accumulation, > 0 */
+#define CFPQ_KERNEL_M4R 3     /* Four Russians, 4-bit tables in LDS */
+#define CFPQ_KERNEL_MFMA 4    /* exact int8 0/1 matrix-core product (v_mfma_i32_32x32x32_i8),
+                                 i32 accumulation, > 0 */
 
 typedef struct cfpq_opts {
     int32_t kernel;             /* CFPQ_KERNEL_*                                         */
@@ -103,8 +104,12 @@ typedef struct cfpq_stats {
     uint64_t out_rows;          /* C-tile rows that received new bits                    */
     double fixpoint_ms;         /* device time of the whole call (hipEvent)              */
     double product_ms;          /* device time inside the product kernel (hipEvent)      */
-    uint32_t product_launches;  /* product-kernel launches                               */
+    uint32_t product_launches;  /* fixpoint steps whose product kernels were timed       */
     uint32_t host_syncs;        /* host waits on the device flag                         */
+    uint64_t mfma_instr;        /* v_mfma_i32_32x32x32_i8 issued (per wave; 2*32^3 ops each) */
+    double sparse_ms;           /* device time in the sparse (wave-per-quarter) product  */
+    double dense_ms;            /* device time in the LDS / Four Russians product        */
+    double mfma_ms;             /* device time in the MFMA product                       */
 } cfpq_stats;
 
 /* ---- lifecycle ---------------------------------------------------------- */

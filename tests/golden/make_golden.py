@@ -13,6 +13,12 @@ the five seeded workloads, computed by the CPU oracle.
 Input digests pin the generators.
 
 usage: python tests/golden/make_golden.py [--skip-big]
+       python tests/golden/make_golden.py --add config2|config3   (merge one big entry)
+
+  config2  the full 32,768-vertex dense-stress workload: fingerprints and
+           add_mul count from the oracle's reference worklist with dense M4RM
+           (its matrices saturate, so the row-gather methods are slower);
+  config3  all 512 batch queries, semi-naive oracle per query.
 """
 import argparse
 import importlib
@@ -57,10 +63,35 @@ def entry(q, ref_driver: bool, worklist: bool):
     return e
 
 
+def entry_m4rm(q):
+    t0 = time.time()
+    g, mats, info = oracle.solve(q, "m4rm")
+    e = {
+        "name": q.name, "n": q.n, "edges": q.n_edges, "digest": q.digest(),
+        "fingerprints": {k: list(v) for k, v in oracle.fingerprints(g, mats, q.n).items()},
+        "add_mul_calls": info["add_mul_calls"],
+        "oracle_seconds": round(time.time() - t0, 2),
+    }
+    print(q.name, e["oracle_seconds"], "s", flush=True)
+    return e
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-big", action="store_true")
+    ap.add_argument("--add", choices=["config2", "config3"])
     args = ap.parse_args()
+    if args.add:
+        with open(OUT) as f:
+            out = json.load(f)
+        if args.add == "config2":
+            out["config2"] = entry_m4rm(gen.config(2))
+        else:
+            out["config3"] = [entry(q, ref_driver=False, worklist=False) for q in gen.config(3)]
+        with open(OUT, "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+        print("updated", OUT)
+        return
     out = {}
     out["config0"] = entry(gen.config(0), ref_driver=True, worklist=True)
     out["config1"] = entry(gen.config(1), ref_driver=True, worklist=True)

@@ -47,8 +47,9 @@ def test_add_mul_random(pkg, oracle, n, kernel):
         eng.close()
 
 
+@pytest.mark.parametrize("kernel", ["auto", "mfma"])
 @pytest.mark.parametrize("alias", ["c=a", "c=b", "c=a=b"])
-def test_add_mul_aliasing(pkg, oracle, engine, alias):
+def test_add_mul_aliasing(pkg, oracle, kernel, alias):
     n = 300
     rng = np.random.default_rng(5)
     c, o = rand_mat(rng, n, 0.02), rand_mat(rng, n, 0.02)
@@ -60,16 +61,17 @@ def test_add_mul_aliasing(pkg, oracle, engine, alias):
         oracle.addmul_m4rm(want, O, C.copy(), n)
     else:
         oracle.addmul_m4rm(want, C.copy(), C.copy(), n)
-    dC, dO = upload(engine, oracle, c), upload(engine, oracle, o)
-    if alias == "c=a":
-        engine.add_mul(dC, dC, dO)
-    elif alias == "c=b":
-        engine.add_mul(dC, dO, dC)
-    else:
-        engine.add_mul(dC, dC, dC)
-    assert np.array_equal(dC.download(), want)
-    dC.free()
-    dO.free()
+    with pkg.Engine(0, kernel=kernel) as engine:
+        dC, dO = upload(engine, oracle, c), upload(engine, oracle, o)
+        if alias == "c=a":
+            engine.add_mul(dC, dC, dO)
+        elif alias == "c=b":
+            engine.add_mul(dC, dO, dC)
+        else:
+            engine.add_mul(dC, dC, dC)
+        assert np.array_equal(dC.download(), want)
+        dC.free()
+        dO.free()
 
 
 def test_upload_download_roundtrip(pkg, oracle, engine):
@@ -129,8 +131,35 @@ def test_fixpoint_sg_forest_16384(pkg, gen, oracle):
 
 
 @pytest.mark.parametrize("n", [512, 1500])
-def test_fixpoint_dense_stress_scaled(pkg, gen, oracle, n):
-    _fixpoint_vs_oracle(pkg, gen, oracle, gen.dense_stress(n))
+@pytest.mark.parametrize("kernel", ["auto", "mfma", "m4r"])
+def test_fixpoint_dense_stress_scaled(pkg, gen, oracle, n, kernel):
+    stats = _fixpoint_vs_oracle(pkg, gen, oracle, gen.dense_stress(n), kernel=kernel)
+    if kernel == "mfma":
+        assert stats["tile_products_mfma"] == stats["tile_products"] > 0
+        assert stats["mfma_instr"] > 0
+
+
+def test_fixpoint_mixed_paths(pkg, gen, oracle):
+    # thresholds that send some items to each of the three product kernels
+    stats = _fixpoint_vs_oracle(pkg, gen, oracle, gen.dense_stress(1500), kernel="auto", dense_bits=64,
+                                m4r_bits=8192, mfma_bits=30000)
+    assert stats["tile_products_mfma"] > 0 and stats["tile_products_dense"] > 0
+    assert stats["tile_products"] > stats["tile_products_mfma"] + stats["tile_products_dense"]
+
+
+def test_fixpoint_dense_stress_4096_golden(pkg, gen):
+    import json
+    import os
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixpoints.json")))["config2_n4096"]
+    q = gen.dense_stress(4096)
+    assert q.digest() == golden["digest"]
+    for kernel in ("auto", "mfma"):
+        with pkg.Engine(0, kernel=kernel) as eng:
+            gr = cf.Grammar(q.grammar, text=True)
+            gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+            assert gr.results() == {k: tuple(v) for k, v in golden["fingerprints"].items()}, kernel
+            gr.free()
 
 
 def test_fixpoint_final_full_pass(pkg, gen, oracle):
