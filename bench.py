@@ -107,7 +107,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=4, choices=sorted(WORKLOADS))
+    ap.add_argument("--config", type=int, default=4, choices=sorted(WORKLOADS) + [3])
+    ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs per step")
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--iters-per-sync", type=int, default=0)
     ap.add_argument("--sparse-batch", type=int, default=0)
@@ -123,6 +124,8 @@ def main():
     pkg = importlib.import_module("cfpq-on-gpgpu_amd")
     gen = importlib.import_module("cfpq-on-gpgpu_amd.gen")
     cf = importlib.import_module("cfpq-on-gpgpu_amd.cfpq")
+    if args.config == 3:
+        return bench_batch(args, pkg, gen, cf)
     key, wname = WORKLOADS[args.config]
     golden = json.load(open(GOLDEN))[key]
     q = gen.config(args.config)
@@ -141,6 +144,8 @@ def main():
     opts["kernel"] = args.kernel
     if args.iters_per_sync:
         opts["iters_per_sync"] = args.iters_per_sync
+    if args.graph:
+        opts["use_graph"] = True
     if args.sparse_batch:
         opts["sparse_batch"] = args.sparse_batch
     log(f"[bench] engine options {opts}")
@@ -224,6 +229,101 @@ def main():
     gr.free()
     for c in init:
         c.free()
+    eng.close()
+    if not bit_exact:
+        sys.exit(1)
+
+
+def bench_batch(args, pkg, gen, cf):
+    """512 independent queries (log-uniform N in [256, 8192], Dyck or SG
+    family): one batched fixpoint per step vs the same queries one call each."""
+    qs = gen.config(3)
+    allg = json.load(open(GOLDEN))
+    golden = allg.get("config3") or allg["config3_first16"]
+    for q, g in zip(qs, golden):
+        assert q.digest() == g["digest"], "workload generator drifted from the golden fixture"
+    opts = {"kernel": args.kernel}
+    if args.iters_per_sync:
+        opts["iters_per_sync"] = args.iters_per_sync
+    if args.graph:
+        opts["use_graph"] = True
+    log(f"[bench] batch of {len(qs)} queries, N in [{min(q.n for q in qs)}, {max(q.n for q in qs)}], "
+        f"engine options {opts}")
+    eng = pkg.Engine(0, **opts)
+    grs, init = [], []
+    for q in qs:
+        gr = cf.Grammar(q.grammar, text=True)
+        mats = gr.load(eng, cf.Graph.from_query(q))
+        cp = []
+        for m in mats:
+            c = eng.matrix(q.n)
+            c.copy_from(m)
+            cp.append(c)
+        grs.append(gr)
+        init.append(cp)
+    work = [(gr.rule_triples(), gr.matrices) for gr in grs]
+
+    def restore():
+        for gr, cp in zip(grs, init):
+            for m, c in zip(gr.matrices, cp):
+                m.copy_from(c)
+        eng.sync()
+
+    def timed(fn):
+        restore()
+        t0 = time.perf_counter()
+        st = fn()
+        eng.sync()
+        return (time.perf_counter() - t0) * 1e3, st
+
+    batched = lambda: eng.batch_fixpoint(work)
+    for w in range(args.warmup):
+        ms, st = timed(batched)
+        log(f"[bench] warmup {w}: {ms:.2f} ms, {st['iterations']} steps")
+    runs = [timed(batched) for _ in range(args.steps)]
+    ms_per_step = sum(r[0] for r in runs) / len(runs)
+    st = runs[-1][1]
+    got = [gr.results() for gr in grs]
+    want = [{k: tuple(v) for k, v in g["fingerprints"].items()} for g in golden]
+    bit_exact = all(a == b for a, b in zip(got, want))
+    checked = min(len(got), len(want))
+
+    def sequential():
+        tot = {"fixpoint_ms": 0.0, "iterations": 0}
+        for rules, mats in work:
+            s1 = eng.fixpoint(rules, mats)
+            tot["fixpoint_ms"] += s1["fixpoint_ms"]
+            tot["iterations"] += s1["iterations"]
+        return tot
+    seq_ms, seq = timed(sequential)
+    bit_exact = bit_exact and all(gr.results() == w for gr, w in zip(grs, want))
+    log(f"[bench] batched {ms_per_step:.2f} ms/step, sequential {seq_ms:.2f} ms, bit_exact {bit_exact} "
+        f"({checked} queries checked against the oracle fixture)")
+    line = {
+        "metric": "CFPQ fixpoint time (s) & eff. Tbit-op/s; batched/learned vs plain, bit-exact",
+        "value": round(len(qs) / (ms_per_step / 1e3), 3),
+        "unit": "queries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (seeded generator, same inputs as the CPU oracle)",
+        "config": {"workload": "batch512_n256_8192_dyck_sg", "queries": len(qs), "engine": "batched delta fixpoint",
+                   "graph": bool(args.graph), "parallelism": "single GPU"},
+        "batched_fixpoint_ms_device": round(st["fixpoint_ms"], 4),
+        "batched_steps": st["iterations"],
+        "sequential_ms": round(seq_ms, 4),
+        "sequential_fixpoint_ms_device": round(seq["fixpoint_ms"], 4),
+        "sequential_steps_total": seq["iterations"],
+        "speedup_batched_vs_sequential": round(seq_ms / ms_per_step, 3),
+        "bit_exact": bit_exact,
+        "queries_checked": checked,
+    }
+    print(json.dumps(line), flush=True)
+    for gr in grs:
+        gr.free()
+    for cp in init:
+        for c in cp:
+            c.free()
     eng.close()
     if not bit_exact:
         sys.exit(1)

@@ -74,6 +74,11 @@ struct cfpq_ctx {
 
     // events
     std::vector<hipEvent_t> ev;
+
+    // captured fixpoint steps: graph_exec[variant * RING + slot], valid while
+    // the launch configuration equals graph_sig
+    std::vector<hipGraphExec_t> graph_exec;
+    std::vector<uint64_t> graph_sig;
 };
 
 // ---------------------------------------------------------------------------
@@ -205,6 +210,8 @@ extern "C" int cfpq_destroy(cfpq_ctx *ctx) {
     for (auto &kv : ctx->pool) free_mat(kv.second);
     ctx->pool.clear();
     for (auto e : ctx->ev) (void)hipEventDestroy(e);
+    for (auto g : ctx->graph_exec)
+        if (g) (void)hipGraphExecDestroy(g);
     void *dev[] = {ctx->stage, ctx->ring, ctx->scratch64, ctx->items, ctx->units, ctx->mitems, ctx->lists[0], ctx->lists[1],
                    ctx->d_mats, ctx->d_heads, ctx->d_terms, ctx->d_ids};
     for (void *p : dev)
@@ -619,6 +626,63 @@ static void accumulate(StepStats &s, const StepCounters &c) {
     s.lookups += c.m4r_lookups;
 }
 
+// ---------------------------------------------------------------------------
+// captured steps
+// ---------------------------------------------------------------------------
+//
+// A step's launches depend only on the plan's launch configuration (table
+// pointers, grid sizes, thresholds) and on the ring slot, which fixes the
+// parity and the previous slot.  With use_graph, each (variant, slot) step
+// is captured once into a hipGraph and replayed; the cache is dropped when
+// the configuration changes.  Variants: 0 = first step of a fixpoint (clears
+// the initial delta in full), 1 = delta step, 2 = exact full-pass step.
+// Steps contain no memset nodes: counters are zeroed by a kernel (k_begin).
+
+static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &plan) {
+    const cfpq_opts &o = ctx->opts;
+    return {(uint64_t)(uintptr_t)ctx->d_mats, (uint64_t)(uintptr_t)ctx->d_heads, (uint64_t)(uintptr_t)ctx->d_terms,
+            (uint64_t)(uintptr_t)ctx->d_ids, (uint64_t)(uintptr_t)ctx->units, (uint64_t)(uintptr_t)ctx->items,
+            (uint64_t)(uintptr_t)ctx->mitems, (uint64_t)(uintptr_t)ctx->lists[0], (uint64_t)(uintptr_t)ctx->lists[1],
+            (uint64_t)(uintptr_t)ctx->ring, plan.heads.size(), plan.mats.size(), plan.terms_delta.size(),
+            plan.all_delta[0].size(), plan.all_delta[1].size(), plan.max_tiles_per_head, plan.max_occ_words,
+            plan.ids_all_off, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
+            (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk};
+}
+
+static int drop_graphs(cfpq_ctx *ctx) {
+    for (auto &g : ctx->graph_exec)
+        if (g) {
+            HIPCHK(hipGraphExecDestroy(g));
+            g = nullptr;
+        }
+    return CFPQ_OK;
+}
+
+// Enqueue one step through the graph cache.
+static int launch_step_graph(cfpq_ctx *ctx, const Plan &plan, uint32_t slot, int prev_slot, bool full_pass) {
+    const int variant = full_pass ? 2 : (prev_slot < 0 ? 0 : 1);
+    hipGraphExec_t &ex = ctx->graph_exec[(size_t)variant * RING + slot];
+    if (!ex) {
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(ctx, plan, (int)(slot & 1), slot, prev_slot, full_pass, -1);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc != CFPQ_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(ctx, CFPQ_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            ex = nullptr;
+            return fail(ctx, CFPQ_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        }
+    }
+    HIPCHK(hipGraphLaunch(ex, ctx->stream));
+    return CFPQ_OK;
+}
+
 // Run delta steps to the fixpoint (plus the optional exact full pass).
 // The caller owns plan buffers; on return all delta buffers are zero again
 // (on success) -- on failure they are released instead of pooled.
@@ -634,6 +698,16 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         CHK(launch_check(ctx, "k_copy"));
     }
     const int per_sync = single_step_full ? 1 : ctx->opts.iters_per_sync;
+    // graph replay needs the parity to follow the slot (RING is even)
+    const bool use_graph = ctx->opts.use_graph != 0 && !single_step_full;
+    if (use_graph) {
+        std::vector<uint64_t> sig = launch_signature(ctx, plan);
+        if (sig != ctx->graph_sig || ctx->graph_exec.size() != 3 * (size_t)RING) {
+            CHK(drop_graphs(ctx));
+            ctx->graph_exec.assign(3 * (size_t)RING, nullptr);
+            ctx->graph_sig = sig;
+        }
+    }
     CHK(ensure_events(ctx, 4 * (size_t)RING + 2));
     hipEvent_t ev_begin = ctx->ev[4 * RING], ev_end = ctx->ev[4 * RING + 1];
     HIPCHK(hipEventRecord(ev_begin, ctx->stream));
@@ -649,7 +723,8 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         std::vector<uint32_t> slots;
         for (int s = 0; s < k; ++s) {
             const uint32_t slot = steps % RING;
-            CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, pending_full, s));
+            if (use_graph) CHK(launch_step_graph(ctx, plan, slot, prev_slot, pending_full));
+            else CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, pending_full, s));
             ++launches;
             slots.push_back(slot);
             prev_slot = (int)slot;
@@ -663,7 +738,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         pending_full = false;
         uint32_t used = 0;
         for (int s = 0; s < k; ++s) {
-            for (int k2 = 0; k2 < 3; ++k2) {
+            for (int k2 = 0; k2 < 3 && !use_graph; ++k2) {
                 float ms = 0;
                 HIPCHK(hipEventElapsedTime(&ms, ctx->ev[4 * s + k2], ctx->ev[4 * s + k2 + 1]));
                 part_ms[k2] += ms;

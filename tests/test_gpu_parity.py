@@ -248,3 +248,34 @@ def test_fixpoint_sg_forest_65536_golden(pkg, gen):
         assert gr.results() == {k: tuple(v) for k, v in golden["fingerprints"].items()}
         assert stats["full_passes"] == 1
         gr.free()
+
+
+@pytest.mark.parametrize("which", ["dyck", "dense"])
+def test_fixpoint_graph_replay(pkg, gen, oracle, which):
+    """Steps captured as hipGraphs and replayed: several fixpoints in a row on
+    one context (graphs reused), with eager work in between, each checked."""
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    q = gen.dyck_er(900, 3000, seed=4) if which == "dyck" else gen.dense_stress(700)
+    g_ref, mats_ref, _ = oracle.solve(q, "seminaive")
+    want = oracle.fingerprints(g_ref, mats_ref, q.n)
+    with pkg.Engine(0, use_graph=True, iters_per_sync=3) as eng:
+        gr = cf.Grammar(q.grammar, text=True)
+        mats = gr.load(eng, cf.Graph.from_query(q))
+        init = []
+        for m in mats:
+            c = eng.matrix(q.n)
+            c.copy_from(m)
+            init.append(c)
+        for rep in range(3):
+            for m, c in zip(mats, init):
+                m.copy_from(c)
+            st = eng.fixpoint(gr.rule_triples(), mats)
+            assert gr.results() == want, rep
+            # eager work on the same stream between replays
+            x = eng.matrix(q.n)
+            eng.add_mul(x, mats[0], mats[0])
+            x.free()
+        assert st["iterations"] >= 2
+        for c in init:
+            c.free()
+        gr.free()
