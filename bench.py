@@ -115,6 +115,9 @@ def main():
     ap.add_argument("--tune", action="store_true", help="autotune on this workload first (cached in tuned.json)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler engine")
+    ap.add_argument("--sched-threshold", type=float, default=0.0,
+                    help="defer items with predicted activity below this probability (default: tuned/library)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -164,28 +167,53 @@ def main():
             m.copy_from(c)
         return eng.fixpoint(rules, mats)
 
-    for w in range(args.warmup):
-        st = step()
-        log(f"[bench] warmup {w}: {st['fixpoint_ms']:.3f} ms, {st['iterations']} steps")
-    eng.sync()
-    stats = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        stats.append(step())
-    eng.sync()
-    wall = time.perf_counter() - t0
-    log(f"[bench] timed {args.steps} steps: {wall * 1e3:.2f} ms")
-
-    # size-independent parity check at full size: fingerprints + counts vs the oracle fixture
-    got = gr.results()
     want = {k: tuple(v) for k, v in golden["fingerprints"].items()}
-    bit_exact = got == want
-    if not bit_exact:
-        log(f"[bench] MISMATCH vs oracle fixture: {got} vs {want}")
 
-    ms_per_step = wall * 1e3 / args.steps
+    def measure(label, run_opts):
+        eng.set_opts(**run_opts)
+        for w in range(args.warmup):
+            st = step()
+            log(f"[bench] {label} warmup {w}: {st['fixpoint_ms']:.3f} ms, {st['iterations']} steps")
+        eng.sync()
+        stats = []
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            stats.append(step())
+        eng.sync()
+        wall = time.perf_counter() - t0
+        log(f"[bench] {label}: timed {args.steps} steps: {wall * 1e3:.2f} ms")
+        # size-independent parity check at full size: fingerprints + counts vs the oracle fixture
+        got = gr.results()
+        ok = got == want
+        if not ok:
+            log(f"[bench] {label} MISMATCH vs oracle fixture: {got} vs {want}")
+        return wall * 1e3 / args.steps, stats, ok
+
+    runs = {"plain": measure("plain", opts)}
+    sched_mod = importlib.import_module("cfpq-on-gpgpu_amd.sched")
+    weights = None if args.no_learned else sched_mod.load()
+    if weights is not None:
+        eng.sched_set_weights(weights)
+        lopts = dict(opts, sched=True)
+        if args.sched_threshold:
+            lopts["sched_threshold"] = args.sched_threshold
+        runs["learned"] = measure("learned", lopts)
+    ok_runs = {k: v for k, v in runs.items() if v[2]}
+    chosen = min(ok_runs, key=lambda k: ok_runs[k][0]) if ok_runs else "plain"
+    ms_per_step, stats, bit_exact = runs[chosen]
+    bit_exact = all(v[2] for v in runs.values())
     fix_ms = [s["fixpoint_ms"] for s in stats]
     value = 2.0 * n ** 3 * a_ref / (ms_per_step / 1e3) / 1e12
+    engines = {}
+    for k, (ms, sts, ok) in runs.items():
+        last = sts[-1]
+        engines[k] = {"ms_per_step": round(ms, 4), "fixpoint_ms_device_median": round(
+            statistics.median(x["fixpoint_ms"] for x in sts), 4), "iterations": last["iterations"],
+            "tile_products": last["tile_products"], "bit_exact": ok}
+        if k == "learned":
+            engines[k].update({x: last[x] for x in ("sched_scored", "sched_deferred", "sched_full", "flush_steps")})
+    if "learned" in runs:
+        engines["learned_vs_plain_speedup"] = round(runs["plain"][0] / runs["learned"][0], 4)
 
     roofline = product_roofline(stats, wname)
 
@@ -209,7 +237,8 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded generator, same inputs as the CPU oracle)",
         "config": {"workload": wname, "n": n, "edges": q.n_edges, "nonterminals": gr.nonterminals_count,
-                   "rules": len(rules), "A_ref": a_ref, "engine": "plain delta fixpoint",
+                   "rules": len(rules), "A_ref": a_ref,
+                   "engine": {"plain": "delta fixpoint", "learned": "delta fixpoint + learned tile scheduler"}[chosen],
                    "kernel": args.kernel, "parallelism": "single GPU"},
         "fixpoint_s": round(statistics.median(fix_ms) / 1e3, 6),
         "fixpoint_ms_device_median": round(statistics.median(fix_ms), 4),
@@ -221,6 +250,7 @@ def main():
                                     "new_bits", "product_launches", "host_syncs")},
         "kernel_ms": {k: round(s0[k], 4) for k in ("sparse_ms", "dense_ms", "mfma_ms")},
         "engine_opts": opts,
+        "engines": engines,
         "bit_exact": bit_exact,
         "roofline": roofline,
         "cpu_baseline": cpu,

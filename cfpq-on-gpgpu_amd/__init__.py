@@ -43,7 +43,9 @@ class Opts(ctypes.Structure):
         ("sparse_batch", ctypes.c_int32),
         ("sparse_waves_per_cu", ctypes.c_int32),
         ("sparse_chunk", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("sched", ctypes.c_int32),
+        ("sched_threshold", ctypes.c_uint32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -73,6 +75,11 @@ class Stats(ctypes.Structure):
         ("sparse_ms", ctypes.c_double),
         ("dense_ms", ctypes.c_double),
         ("mfma_ms", ctypes.c_double),
+        ("sched_scored", ctypes.c_uint64),
+        ("sched_deferred", ctypes.c_uint64),
+        ("sched_full", ctypes.c_uint64),
+        ("flush_steps", ctypes.c_uint32),
+        ("pad0", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -93,8 +100,13 @@ EXPORTS = (
     "cfpq_default_opts", "cfpq_create", "cfpq_destroy", "cfpq_last_error", "cfpq_set_opts", "cfpq_sync",
     "cfpq_mat_alloc", "cfpq_mat_free", "cfpq_mat_n", "cfpq_mat_upload", "cfpq_mat_download",
     "cfpq_mat_set_bits", "cfpq_mat_popcount", "cfpq_mat_copy", "cfpq_mat_fingerprint", "cfpq_add_mul", "cfpq_fixpoint",
-    "cfpq_batch_fixpoint",
+    "cfpq_batch_fixpoint", "cfpq_sched_set_weights", "cfpq_sched_get_weights", "cfpq_sched_init",
+    "cfpq_sched_eval", "cfpq_sched_train", "cfpq_sched_trace_begin", "cfpq_sched_trace_read",
+    "cfpq_sched_trace_end",
 )
+
+SCHED_PARAMS = 5825
+SCHED_FEATURES = 8
 
 _lib = None
 
@@ -110,6 +122,7 @@ def lib():
     vp, u32, u64, sz, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int
     u64p = ctypes.POINTER(ctypes.c_uint64)
     u32p = ctypes.POINTER(ctypes.c_uint32)
+    fp = ctypes.POINTER(ctypes.c_float)
     sig = {
         "cfpq_default_opts": (None, [ctypes.POINTER(Opts)]),
         "cfpq_create": (i32, [i32, ctypes.POINTER(Opts), ctypes.POINTER(vp)]),
@@ -129,6 +142,14 @@ def lib():
         "cfpq_add_mul": (i32, [vp, vp, vp, vp, ctypes.POINTER(i32)]),
         "cfpq_fixpoint": (i32, [vp, ctypes.POINTER(Rule), sz, ctypes.POINTER(vp), sz, ctypes.POINTER(Stats)]),
         "cfpq_batch_fixpoint": (i32, [vp, ctypes.POINTER(Query), sz, ctypes.POINTER(Stats)]),
+        "cfpq_sched_set_weights": (i32, [vp, ctypes.POINTER(ctypes.c_uint16), sz]),
+        "cfpq_sched_get_weights": (i32, [vp, ctypes.POINTER(ctypes.c_uint16), sz]),
+        "cfpq_sched_init": (i32, [vp, u64]),
+        "cfpq_sched_eval": (i32, [vp, ctypes.POINTER(ctypes.c_uint16), fp, u32p, fp, sz, fp, fp, fp]),
+        "cfpq_sched_train": (i32, [vp, fp, u32p, fp, sz, u32, u32, ctypes.c_float, u64, fp]),
+        "cfpq_sched_trace_begin": (i32, [vp, sz]),
+        "cfpq_sched_trace_read": (i32, [vp, fp, u32p, fp, sz, ctypes.POINTER(sz)]),
+        "cfpq_sched_trace_end": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -140,7 +161,7 @@ def lib():
 
 def make_opts(kernel="auto", dense_bits=0, mfma_bits=0, m4r_bits=0, max_iterations=0, final_full_pass=False,
               iters_per_sync=0, use_graph=False, sparse_batch=0, sparse_waves_per_cu=0,
-              sparse_chunk=0) -> Opts:
+              sparse_chunk=0, sched=False, sched_threshold=0) -> Opts:
     o = Opts()
     lib().cfpq_default_opts(ctypes.byref(o))
     o.kernel = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
@@ -154,6 +175,9 @@ def make_opts(kernel="auto", dense_bits=0, mfma_bits=0, m4r_bits=0, max_iteratio
     o.sparse_batch = sparse_batch
     o.sparse_waves_per_cu = sparse_waves_per_cu
     o.sparse_chunk = sparse_chunk
+    o.sched = int(bool(sched))
+    # a float in (0, 1) is a probability; an int is already in units of 1/65536
+    o.sched_threshold = int(round(sched_threshold * 65536)) if isinstance(sched_threshold, float) else sched_threshold
     return o
 
 
@@ -225,6 +249,70 @@ class Engine:
         st = Stats()
         self._check(self._L.cfpq_batch_fixpoint(self._h, qs, len(queries), ctypes.byref(st)))
         return st.as_dict()
+
+
+    # ---- learned tile scheduler ------------------------------------------
+    def sched_set_weights(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.uint16)
+        self._check(self._L.cfpq_sched_set_weights(self._h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                                    w.size))
+
+    def sched_get_weights(self) -> np.ndarray:
+        w = np.zeros(SCHED_PARAMS, np.uint16)
+        self._check(self._L.cfpq_sched_get_weights(self._h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                                    w.size))
+        return w
+
+    def sched_init(self, seed: int = 0):
+        self._check(self._L.cfpq_sched_init(self._h, ctypes.c_uint64(seed)))
+
+    def sched_eval(self, w, x, rule, y, want_grad=True):
+        """(logits, loss, grad) of the network with bf16 parameters w."""
+        w = np.ascontiguousarray(w, dtype=np.uint16)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        rule = np.ascontiguousarray(rule, dtype=np.uint32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        n = y.shape[0]
+        logits = np.zeros(max(n, 1), np.float32)
+        loss = ctypes.c_float()
+        grad = np.zeros(SCHED_PARAMS, np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(self._L.cfpq_sched_eval(self._h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                             x.ctypes.data_as(fp), rule.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                             y.ctypes.data_as(fp), n, logits.ctypes.data_as(fp), ctypes.byref(loss),
+                                             grad.ctypes.data_as(fp) if want_grad else None))
+        return logits[:n], float(loss.value), (grad if want_grad else None)
+
+    def sched_train(self, x, rule, y, epochs=8, batch=4096, lr=3e-3, seed=0) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        rule = np.ascontiguousarray(rule, dtype=np.uint32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = np.zeros(epochs, np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(self._L.cfpq_sched_train(self._h, x.ctypes.data_as(fp),
+                                              rule.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                              y.ctypes.data_as(fp), y.shape[0], epochs, batch, ctypes.c_float(lr),
+                                              ctypes.c_uint64(seed), loss.ctypes.data_as(fp)))
+        return loss
+
+    def trace_begin(self, capacity: int):
+        self._check(self._L.cfpq_sched_trace_begin(self._h, capacity))
+
+    def trace_read(self, capacity: int):
+        """(x, rule, y, rows recorded) of the trace collected so far."""
+        x = np.zeros((capacity, SCHED_FEATURES), np.float32)
+        rule = np.zeros(capacity, np.uint32)
+        y = np.zeros(capacity, np.float32)
+        cnt = ctypes.c_size_t()
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(self._L.cfpq_sched_trace_read(self._h, x.ctypes.data_as(fp),
+                                                   rule.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                                   y.ctypes.data_as(fp), capacity, ctypes.byref(cnt)))
+        n = min(cnt.value, capacity)
+        return x[:n], rule[:n], y[:n], cnt.value
+
+    def trace_end(self):
+        self._check(self._L.cfpq_sched_trace_end(self._h))
 
 
 class DeviceMatrix:

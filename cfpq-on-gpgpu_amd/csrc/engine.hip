@@ -20,7 +20,7 @@
 #include <vector>
 
 #include "cfpq_hip.h"
-#include "kernels.hpp"
+#include "sched.hpp"
 
 using namespace cfpq;
 
@@ -79,6 +79,23 @@ struct cfpq_ctx {
     // the launch configuration equals graph_sig
     std::vector<hipGraphExec_t> graph_exec;
     std::vector<uint64_t> graph_sig;
+
+    // learned scheduler
+    uint16_t *sched_w = nullptr;      // bf16 parameters on the device
+    bool sched_have = false;
+    uint32_t *dirty = nullptr;        // deferred-tile bitmaps, one per head
+    size_t dirty_cap = 0;
+    uint32_t *dirty_off = nullptr;    // word offset of each head's bitmap
+    size_t dirty_off_cap = 0;
+    // trace rows
+    float *tr_x = nullptr, *tr_y = nullptr;
+    uint32_t *tr_rule = nullptr, *tr_item = nullptr, *tr_count = nullptr;
+    size_t tr_cap = 0;
+    // training state
+    float *t_master = nullptr, *t_m1 = nullptr, *t_m2 = nullptr, *t_grad = nullptr, *t_loss = nullptr;
+    float *t_x = nullptr, *t_y = nullptr;
+    uint32_t *t_rule = nullptr;
+    size_t t_cap = 0, t_x_cap = 0, t_y_cap = 0, t_rule_cap = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -154,6 +171,7 @@ static void normalise(cfpq_opts *o) {
     if (o->sparse_waves_per_cu <= 0) o->sparse_waves_per_cu = 32;
     if (o->sparse_chunk <= 0) o->sparse_chunk = 1;
     if (o->iters_per_sync > RING / 2) o->iters_per_sync = RING / 2;
+    if (o->sched_threshold == 0) o->sched_threshold = 6554;  // P(active) < 0.1
 }
 
 extern "C" int cfpq_create(int device, const cfpq_opts *opts, cfpq_ctx **out) {
@@ -213,7 +231,9 @@ extern "C" int cfpq_destroy(cfpq_ctx *ctx) {
     for (auto g : ctx->graph_exec)
         if (g) (void)hipGraphExecDestroy(g);
     void *dev[] = {ctx->stage, ctx->ring, ctx->scratch64, ctx->items, ctx->units, ctx->mitems, ctx->lists[0], ctx->lists[1],
-                   ctx->d_mats, ctx->d_heads, ctx->d_terms, ctx->d_ids};
+                   ctx->d_mats, ctx->d_heads, ctx->d_terms, ctx->d_ids, ctx->sched_w, ctx->dirty, ctx->dirty_off,
+                   ctx->tr_x, ctx->tr_y, ctx->tr_rule, ctx->tr_item, ctx->tr_count, ctx->t_master, ctx->t_m1,
+                   ctx->t_m2, ctx->t_grad, ctx->t_loss, ctx->t_x, ctx->t_y, ctx->t_rule};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (ctx->ring_host) (void)hipHostFree(ctx->ring_host);
@@ -462,14 +482,16 @@ static int ensure_events(cfpq_ctx *ctx, size_t n) {
 struct Plan {
     std::vector<MatDev> mats;         // descriptor table
     std::vector<HeadDev> heads;
-    std::vector<TermDev> terms_delta;  // semi-naive terms
-    std::vector<TermDev> terms_full;   // full pass terms (same head ranges, offset)
+    std::vector<TermDev> terms_delta;  // semi-naive terms (two per rule)
+    std::vector<TermDev> terms_full;   // full terms (one per rule), stored after the delta terms
     std::vector<uint32_t> all_delta[2];  // ids of every delta buffer of parity q (buffer Dq)
     std::vector<uint32_t> copy_dst, copy_src;  // initial D0 := M
     size_t max_tiles_per_head = 0;     // max T*T over heads
     size_t total_head_tiles = 0;       // sum of T*T over heads
     size_t max_occ_words = 0;          // max T*ow over matrices
     size_t ids_all_off = 0;            // offset of the "every matrix" id list in d_ids
+    std::vector<uint32_t> dirty_off;   // per head: word offset of its deferred-tile bitmap
+    size_t dirty_words = 0;
 };
 
 struct StepStats {
@@ -509,11 +531,33 @@ static uint32_t m4r_threshold(const cfpq_opts &o) {
 // Enqueue one step. parity q: deltas read from D_q, new bits to D_{1-q}.
 // prev_slot < 0: the delta consumed by this step is the initial one
 // (cleared in full); otherwise it was written by the step in prev_slot.
+static bool sched_on(const cfpq_ctx *ctx) { return ctx->opts.sched != 0 && ctx->sched_have; }
+
+static SchedArgs sched_args(const cfpq_ctx *ctx, bool flush) {
+    SchedArgs a = {};
+    a.w = sched_on(ctx) ? ctx->sched_w : nullptr;
+    a.dirty = ctx->dirty;
+    a.dirty_off = ctx->dirty_off;
+    const double pth = std::min(0.999999, std::max(1e-6, ctx->opts.sched_threshold / 65536.0));
+    a.logit_threshold = (float)std::log(pth / (1.0 - pth));
+    a.defer = sched_on(ctx) && !flush;
+    a.flush = flush;
+    if (ctx->tr_cap) {
+        a.trace_x = ctx->tr_x;
+        a.trace_rule = ctx->tr_rule;
+        a.trace_item = ctx->tr_item;
+        a.trace_y = ctx->tr_y;
+        a.trace_count = ctx->tr_count;
+        a.trace_cap = (uint32_t)ctx->tr_cap;
+    }
+    return a;
+}
+
 static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, int prev_slot, bool full_pass,
-                        int ev_idx) {
+                        int ev_idx, bool flush = false) {
     StepCounters *c = ctx->ring + slot;
     const uint32_t nh = (uint32_t)plan.heads.size();
-    const TermDev *terms = ctx->d_terms + (full_pass ? plan.terms_delta.size() : 0);
+    const TermDev *terms = ctx->d_terms;
     uint64_t *touched = ctx->lists[slot & 1];
     hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
     // dense-tile bitsets of every matrix of the plan (operands of this step)
@@ -524,8 +568,19 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                        dim3(256), 0, ctx->stream, (const MatDev *)ctx->d_mats, ids_all, nall,
                        dense_threshold(ctx->opts), mfma_threshold(ctx->opts));
     const uint32_t bx = (uint32_t)((plan.max_tiles_per_head + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                       (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems);
+    const bool use_sched = !full_pass && plan.dirty_words && (sched_on(ctx) || ctx->tr_cap || flush);
+    const SchedArgs sa = sched_args(ctx, flush);
+    if (use_sched) {
+        if (ctx->tr_cap)
+            hipLaunchKernelGGL(k_trace_begin, dim3(1), dim3(64), 0, ctx->stream, c, (const uint32_t *)ctx->tr_count,
+                               (uint32_t)ctx->tr_cap);
+        hipLaunchKernelGGL(k_build_sched, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                           (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems, sa);
+    } else {
+        hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                           (const HeadDev *)ctx->d_heads, terms, q, (int)full_pass, c, ctx->units, ctx->items,
+                           ctx->mitems);
+    }
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
     // product kernels, timed: ev[4i] sparse ev[4i+1] dense ev[4i+2] mfma ev[4i+3]
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx], ctx->stream));
@@ -556,6 +611,9 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                        (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, terms, q, c,
                        (const uint64_t *)ctx->mitems, touched);
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 3], ctx->stream));
+    if (use_sched && ctx->tr_cap)
+        hipLaunchKernelGGL(k_trace_label, dim3((uint32_t)ctx->num_cu), dim3(256), 0, ctx->stream,
+                           (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, (const StepCounters *)c, sa);
     hipLaunchKernelGGL(k_apply, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                        (const HeadDev *)ctx->d_heads, q, c, (const uint64_t *)touched);
     // retire the delta this step consumed (buffer D_q)
@@ -581,6 +639,11 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     for (const MatDev &m : plan.mats) plan.max_occ_words = std::max<size_t>(plan.max_occ_words, (size_t)m.T * m.ow);
     CHK(grow(ctx, &ctx->d_mats, &ctx->d_mats_cap, plan.mats.size()));
     CHK(grow(ctx, &ctx->d_heads, &ctx->d_heads_cap, plan.heads.size()));
+    std::vector<HeadDev> heads(plan.heads);
+    for (HeadDev &hd : heads) {  // full terms follow the delta terms in the device table
+        hd.f0 += (uint32_t)plan.terms_delta.size();
+        hd.f1 += (uint32_t)plan.terms_delta.size();
+    }
     CHK(grow(ctx, &ctx->d_terms, &ctx->d_terms_cap, plan.terms_delta.size() + plan.terms_full.size()));
     const size_t nid = plan.all_delta[0].size() + plan.all_delta[1].size() + 2 * plan.copy_dst.size() +
                        plan.mats.size();
@@ -596,7 +659,7 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
         ctx->lists_cap = std::min(cap0, cap1);
     }
     HIPCHK(hipMemcpyAsync(ctx->d_mats, plan.mats.data(), plan.mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->d_heads, plan.heads.data(), plan.heads.size() * sizeof(HeadDev), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_heads, heads.data(), heads.size() * sizeof(HeadDev), hipMemcpyHostToDevice, ctx->stream));
     std::vector<TermDev> terms(plan.terms_delta);
     terms.insert(terms.end(), plan.terms_full.begin(), plan.terms_full.end());
     HIPCHK(hipMemcpyAsync(ctx->d_terms, terms.data(), terms.size() * sizeof(TermDev), hipMemcpyHostToDevice, ctx->stream));
@@ -607,6 +670,17 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     for (uint32_t x = 0; x < (uint32_t)plan.mats.size(); ++x) ids.push_back(x);  // at plan.ids_all_off
     HIPCHK(hipMemcpyAsync(ctx->d_ids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->ring, 0, sizeof(StepCounters) * RING, ctx->stream));
+    plan.dirty_off.clear();
+    plan.dirty_words = 0;
+    for (const HeadDev &hd : plan.heads) {
+        plan.dirty_off.push_back((uint32_t)plan.dirty_words);
+        plan.dirty_words += ((size_t)hd.T * hd.T + 31) / 32;
+    }
+    CHK(grow(ctx, &ctx->dirty, &ctx->dirty_cap, plan.dirty_words));
+    CHK(grow(ctx, &ctx->dirty_off, &ctx->dirty_off_cap, plan.dirty_off.size()));
+    HIPCHK(hipMemcpyAsync(ctx->dirty_off, plan.dirty_off.data(), plan.dirty_off.size() * 4, hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->dirty, 0, plan.dirty_words * 4, ctx->stream));
     // the tables above were copied from pageable host vectors that die here
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return CFPQ_OK;
@@ -635,7 +709,8 @@ static void accumulate(StepStats &s, const StepCounters &c) {
 // parity and the previous slot.  With use_graph, each (variant, slot) step
 // is captured once into a hipGraph and replayed; the cache is dropped when
 // the configuration changes.  Variants: 0 = first step of a fixpoint (clears
-// the initial delta in full), 1 = delta step, 2 = exact full-pass step.
+// the initial delta in full), 1 = delta step, 2 = exact full-pass step,
+// 3 = scheduler flush step.
 // Steps contain no memset nodes: counters are zeroed by a kernel (k_begin).
 
 static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &plan) {
@@ -646,7 +721,9 @@ static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &p
             (uint64_t)(uintptr_t)ctx->ring, plan.heads.size(), plan.mats.size(), plan.terms_delta.size(),
             plan.all_delta[0].size(), plan.all_delta[1].size(), plan.max_tiles_per_head, plan.max_occ_words,
             plan.ids_all_off, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
-            (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk};
+            (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk, (uint64_t)o.sched, o.sched_threshold,
+            (uint64_t)ctx->sched_have, (uint64_t)(uintptr_t)ctx->sched_w, (uint64_t)(uintptr_t)ctx->dirty,
+            (uint64_t)(uintptr_t)ctx->dirty_off, (uint64_t)ctx->tr_cap, (uint64_t)(uintptr_t)ctx->tr_x};
 }
 
 static int drop_graphs(cfpq_ctx *ctx) {
@@ -659,12 +736,14 @@ static int drop_graphs(cfpq_ctx *ctx) {
 }
 
 // Enqueue one step through the graph cache.
-static int launch_step_graph(cfpq_ctx *ctx, const Plan &plan, uint32_t slot, int prev_slot, bool full_pass) {
-    const int variant = full_pass ? 2 : (prev_slot < 0 ? 0 : 1);
+constexpr int GRAPH_VARIANTS = 4;
+static int launch_step_graph(cfpq_ctx *ctx, const Plan &plan, uint32_t slot, int prev_slot, bool full_pass,
+                             bool flush) {
+    const int variant = full_pass ? 2 : flush ? 3 : (prev_slot < 0 ? 0 : 1);
     hipGraphExec_t &ex = ctx->graph_exec[(size_t)variant * RING + slot];
     if (!ex) {
         HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(ctx, plan, (int)(slot & 1), slot, prev_slot, full_pass, -1);
+        int rc = enqueue_step(ctx, plan, (int)(slot & 1), slot, prev_slot, full_pass, -1, flush);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if (rc != CFPQ_OK) {
@@ -702,9 +781,9 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
     const bool use_graph = ctx->opts.use_graph != 0 && !single_step_full;
     if (use_graph) {
         std::vector<uint64_t> sig = launch_signature(ctx, plan);
-        if (sig != ctx->graph_sig || ctx->graph_exec.size() != 3 * (size_t)RING) {
+        if (sig != ctx->graph_sig || ctx->graph_exec.size() != GRAPH_VARIANTS * (size_t)RING) {
             CHK(drop_graphs(ctx));
-            ctx->graph_exec.assign(3 * (size_t)RING, nullptr);
+            ctx->graph_exec.assign(GRAPH_VARIANTS * (size_t)RING, nullptr);
             ctx->graph_sig = sig;
         }
     }
@@ -713,18 +792,22 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
     HIPCHK(hipEventRecord(ev_begin, ctx->stream));
     StepStats tot;
     double product_ms = 0, part_ms[3] = {0, 0, 0};
-    uint32_t steps = 0, syncs = 0, launches = 0, full_passes = 0;
+    uint32_t steps = 0, syncs = 0, launches = 0, full_passes = 0, flush_steps = 0;
+    uint64_t sched_scored = 0, sched_deferred = 0, sched_full = 0;
+    int64_t dirty_total = 0;  // tiles deferred and not yet evaluated in full
     int prev_slot = -1;
     bool done = false;
-    bool pending_full = single_step_full;
+    // next batch: 0 = delta steps, 1 = one exact full-pass step, 2 = one flush step
+    int pending = single_step_full ? 1 : 0;
     const bool want_full = ctx->opts.final_full_pass != 0;
     while (!done) {
-        const int k = pending_full ? 1 : per_sync;
+        const int kind = pending;
+        const int k = kind ? 1 : per_sync;
         std::vector<uint32_t> slots;
         for (int s = 0; s < k; ++s) {
             const uint32_t slot = steps % RING;
-            if (use_graph) CHK(launch_step_graph(ctx, plan, slot, prev_slot, pending_full));
-            else CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, pending_full, s));
+            if (use_graph) CHK(launch_step_graph(ctx, plan, slot, prev_slot, kind == 1, kind == 2));
+            else CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, kind == 1, s, kind == 2));
             ++launches;
             slots.push_back(slot);
             prev_slot = (int)slot;
@@ -734,35 +817,47 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         HIPCHK(hipMemcpyAsync(ctx->ring_host, ctx->ring, sizeof(StepCounters) * RING, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
         ++syncs;
-        bool was_full = pending_full;
-        pending_full = false;
+        pending = 0;
         uint32_t used = 0;
+        bool quiet = false;  // a step added no bit
         for (int s = 0; s < k; ++s) {
+            const StepCounters &c = ctx->ring_host[slots[s]];
+            // every executed step counts for the deferred-tile balance (a step
+            // after a quiet one still retires dirty tiles whose head tile is full)
+            dirty_total += (int64_t)c.dirty_new - (int64_t)c.dirty_cleared;
+            if (quiet) continue;
             for (int k2 = 0; k2 < 3 && !use_graph; ++k2) {
                 float ms = 0;
                 HIPCHK(hipEventElapsedTime(&ms, ctx->ev[4 * s + k2], ctx->ev[4 * s + k2 + 1]));
                 part_ms[k2] += ms;
                 product_ms += ms;
             }
-            const StepCounters &c = ctx->ring_host[slots[s]];
             accumulate(tot, c);
+            sched_scored += c.n_scored;
+            sched_deferred += c.n_deferred;
+            sched_full += c.n_full;
             ++used;
-            if (c.new_bits == 0) {
-                // a step with no new bit: fixpoint of the semi-naive system
+            if (c.new_bits == 0) quiet = true;
+        }
+        if (kind == 1) ++full_passes;
+        if (kind == 2) ++flush_steps;
+        // steps enqueued after a quiet one found no delta; count only up to it
+        steps -= (uint32_t)(k - used);
+        if (quiet) {
+            if (dirty_total > 0) {
+                // deferred tiles remain: evaluate them all in full before
+                // anything is declared converged
+                pending = 2;
+            } else if (single_step_full) {
                 done = true;
-                break;
+            } else if (want_full && kind != 1) {
+                pending = 1;  // the exact full pass certifies the fixpoint
+            } else {
+                done = true;
             }
+        } else if (single_step_full && kind == 1) {
+            done = true;
         }
-        if (was_full) ++full_passes;
-        if (done) {
-            // steps enqueued after the empty one did nothing; count only up to it
-            steps -= (uint32_t)(k - used);
-            if (want_full && !was_full && !single_step_full) {
-                done = false;
-                pending_full = true;
-            }
-        }
-        if (single_step_full && was_full) done = true;
         if (!done && ctx->opts.max_iterations && steps >= ctx->opts.max_iterations)
             return fail(ctx, CFPQ_E_LIMIT, "fixpoint: iteration cap %u reached", ctx->opts.max_iterations);
     }
@@ -807,6 +902,10 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         stats->sparse_ms = part_ms[0];
         stats->dense_ms = part_ms[1];
         stats->mfma_ms = part_ms[2];
+        stats->sched_scored = sched_scored;
+        stats->sched_deferred = sched_deferred;
+        stats->sched_full = sched_full;
+        stats->flush_steps = flush_steps;
     }
     return CFPQ_OK;
 }
@@ -837,6 +936,7 @@ static void plan_query(Plan &plan, cfpq_mat *const *M, const std::vector<cfpq_ma
         hd.out[0] = id_d(1, (uint32_t)h);  // even step writes D1
         hd.out[1] = id_d(0, (uint32_t)h);  // odd step writes D0
         hd.t0 = (uint32_t)plan.terms_delta.size();
+        hd.f0 = (uint32_t)plan.terms_full.size();  // rebased past the delta terms in upload_plan
         hd.T = T;
         for (size_t r : by_head[h]) {
             const uint32_t a = rules[r].left, b = rules[r].right;
@@ -845,14 +945,16 @@ static void plan_query(Plan &plan, cfpq_mat *const *M, const std::vector<cfpq_ma
             t1.right[0] = id_m(b);   t1.right[1] = id_m(b);
             t2.left[0] = id_m(a);    t2.left[1] = id_m(a);
             t2.right[0] = id_d(0, b); t2.right[1] = id_d(1, b);
+            t1.rule = t2.rule = (uint32_t)r;
             plan.terms_delta.push_back(t1);
             plan.terms_delta.push_back(t2);
             f.left[0] = f.left[1] = id_m(a);
             f.right[0] = f.right[1] = id_m(b);
+            f.rule = (uint32_t)r;
             plan.terms_full.push_back(f);
-            plan.terms_full.push_back(f);  // keep the same index ranges as the delta table
         }
         hd.t1 = (uint32_t)plan.terms_delta.size();
+        hd.f1 = (uint32_t)plan.terms_full.size();
         plan.heads.push_back(hd);
         plan.max_tiles_per_head = std::max<size_t>(plan.max_tiles_per_head, (size_t)T * T);
         plan.total_head_tiles += (size_t)T * T;
@@ -934,11 +1036,14 @@ extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const
     hd.out[0] = hd.out[1] = 3;
     hd.t0 = 0;
     hd.t1 = 1;
+    hd.f0 = 0;  // rebased to 1 by upload_plan
+    hd.f1 = 1;
     hd.T = c->T;
     plan.heads.push_back(hd);
     TermDev t;
     t.left[0] = t.left[1] = 1;
     t.right[0] = t.right[1] = 2;
+    t.rule = 0;
     plan.terms_delta.push_back(t);
     plan.terms_full.push_back(t);
     plan.all_delta[0] = {3};
@@ -961,4 +1066,225 @@ extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const
         free_mat(S);
     }
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// learned tile scheduler: weights, evaluation, training, traces
+// ---------------------------------------------------------------------------
+
+static int ensure_sched_weights(cfpq_ctx *ctx) {
+    if (!ctx->sched_w) HIPCHK(hipMalloc((void **)&ctx->sched_w, MLP_PARAMS * sizeof(uint16_t)));
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_sched_set_weights(cfpq_ctx *ctx, const uint16_t *w, size_t count) {
+    if (!ctx || !w || count != (size_t)MLP_PARAMS)
+        return fail(ctx, CFPQ_E_INVALID, "cfpq_sched_set_weights: need %d bf16 values", MLP_PARAMS);
+    HIPCHK(hipSetDevice(ctx->device));
+    CHK(ensure_sched_weights(ctx));
+    HIPCHK(hipMemcpyAsync(ctx->sched_w, w, MLP_PARAMS * 2, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->sched_have = true;
+    // the master copy used by training restarts from these weights
+    if (ctx->t_master) {
+        std::vector<float> f(MLP_PARAMS);
+        for (int i = 0; i < MLP_PARAMS; ++i) f[i] = bf16_to_f32(w[i]);
+        HIPCHK(hipMemcpy(ctx->t_master, f.data(), MLP_PARAMS * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(ctx->t_m1, 0, MLP_PARAMS * 4));
+        HIPCHK(hipMemset(ctx->t_m2, 0, MLP_PARAMS * 4));
+    }
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_sched_get_weights(cfpq_ctx *ctx, uint16_t *w, size_t count) {
+    if (!ctx || !w || count != (size_t)MLP_PARAMS)
+        return fail(ctx, CFPQ_E_INVALID, "cfpq_sched_get_weights: need room for %d bf16 values", MLP_PARAMS);
+    if (!ctx->sched_have) return fail(ctx, CFPQ_E_INVALID, "cfpq_sched_get_weights: no weights set");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpyAsync(w, ctx->sched_w, MLP_PARAMS * 2, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_sched_init(cfpq_ctx *ctx, uint64_t seed) {
+    if (!ctx) return CFPQ_E_INVALID;
+    // uniform(-a, a) with a = sqrt(6 / fan_in) for the weight matrices (He
+    // uniform), small uniform embeddings, zero biases; SplitMix64 stream
+    uint64_t st = seed;
+    auto next = [&st]() {
+        uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    };
+    auto unif = [&](float a) { return (float)(((next() >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0) * a; };
+    std::vector<uint16_t> w(MLP_PARAMS, 0);
+    for (int i = 0; i < MLP_RULES * MLP_E; ++i) w[MLP_OFF_E + i] = f32_to_bf16(unif(0.5f));
+    for (int i = 0; i < MLP_IN * MLP_H; ++i) w[MLP_OFF_W1 + i] = f32_to_bf16(unif(std::sqrt(6.f / MLP_IN)));
+    for (int i = 0; i < MLP_H * MLP_H; ++i) w[MLP_OFF_W2 + i] = f32_to_bf16(unif(std::sqrt(6.f / MLP_H)));
+    for (int i = 0; i < MLP_H; ++i) w[MLP_OFF_W3 + i] = f32_to_bf16(unif(std::sqrt(3.f / MLP_H)));
+    return cfpq_sched_set_weights(ctx, w.data(), w.size());
+}
+
+static int upload_samples(cfpq_ctx *ctx, const float *x, const uint32_t *rule, const float *y, size_t n) {
+    CHK(grow(ctx, &ctx->t_x, &ctx->t_x_cap, n * MLP_F));
+    CHK(grow(ctx, &ctx->t_rule, &ctx->t_rule_cap, n));
+    CHK(grow(ctx, &ctx->t_y, &ctx->t_y_cap, n));
+    HIPCHK(hipMemcpyAsync(ctx->t_x, x, n * MLP_F * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->t_rule, rule, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->t_y, y, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    return CFPQ_OK;
+}
+
+static int ensure_train_state(cfpq_ctx *ctx) {
+    if (ctx->t_master) return CFPQ_OK;
+    for (float **p : {&ctx->t_master, &ctx->t_m1, &ctx->t_m2, &ctx->t_grad})
+        HIPCHK(hipMalloc((void **)p, MLP_PARAMS * sizeof(float)));
+    HIPCHK(hipMalloc((void **)&ctx->t_loss, 64 * sizeof(float)));
+    HIPCHK(hipMemset(ctx->t_m1, 0, MLP_PARAMS * 4));
+    HIPCHK(hipMemset(ctx->t_m2, 0, MLP_PARAMS * 4));
+    if (ctx->sched_have) {
+        std::vector<uint16_t> w(MLP_PARAMS);
+        HIPCHK(hipMemcpy(w.data(), ctx->sched_w, MLP_PARAMS * 2, hipMemcpyDeviceToHost));
+        std::vector<float> f(MLP_PARAMS);
+        for (int i = 0; i < MLP_PARAMS; ++i) f[i] = bf16_to_f32(w[i]);
+        HIPCHK(hipMemcpy(ctx->t_master, f.data(), MLP_PARAMS * 4, hipMemcpyHostToDevice));
+    }
+    return CFPQ_OK;
+}
+
+static uint32_t mlp_grid(cfpq_ctx *ctx, size_t n) {
+    return (uint32_t)std::max<size_t>(1, std::min<size_t>((n + MLP_BLOCK - 1) / MLP_BLOCK, (size_t)ctx->num_cu * 8));
+}
+
+extern "C" int cfpq_sched_eval(cfpq_ctx *ctx, const uint16_t *w, const float *x, const uint32_t *rule, const float *y,
+                               size_t n, float *logits, float *loss, float *grad) {
+    if (!ctx || !w || (n && (!x || !rule || !y)) || n > 0xffffffffu)
+        return fail(ctx, CFPQ_E_INVALID, "cfpq_sched_eval: bad argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    CHK(ensure_train_state(ctx));
+    uint16_t *dw = nullptr;
+    float *dlog = nullptr;
+    HIPCHK(hipMalloc((void **)&dw, MLP_PARAMS * 2));
+    int rc = CFPQ_OK;
+    auto run = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(dw, w, MLP_PARAMS * 2, hipMemcpyHostToDevice, ctx->stream));
+        if (n) CHK(upload_samples(ctx, x, rule, y, n));
+        if (logits && n) HIPCHK(hipMalloc((void **)&dlog, n * 4));
+        hipLaunchKernelGGL(k_fill_f32, dim3(24), dim3(256), 0, ctx->stream, ctx->t_grad, (uint32_t)MLP_PARAMS, 0.f);
+        hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, ctx->stream, ctx->t_loss, 1u, 0.f);
+        if (n)
+            hipLaunchKernelGGL(k_mlp_eval, dim3(mlp_grid(ctx, n)), dim3(MLP_BLOCK), 0, ctx->stream,
+                               (const uint16_t *)dw, (const float *)ctx->t_x, (const uint32_t *)ctx->t_rule,
+                               (const float *)ctx->t_y, (uint32_t)n, 1.f / (float)n, 1.f / (float)n, dlog,
+                               ctx->t_loss, grad ? ctx->t_grad : nullptr);
+        CHK(launch_check(ctx, "k_mlp_eval"));
+        if (logits && n) HIPCHK(hipMemcpyAsync(logits, dlog, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        if (loss) HIPCHK(hipMemcpyAsync(loss, ctx->t_loss, 4, hipMemcpyDeviceToHost, ctx->stream));
+        if (grad) HIPCHK(hipMemcpyAsync(grad, ctx->t_grad, MLP_PARAMS * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return CFPQ_OK;
+    };
+    rc = run();
+    (void)hipStreamSynchronize(ctx->stream);
+    if (dlog) (void)hipFree(dlog);
+    (void)hipFree(dw);
+    return rc;
+}
+
+extern "C" int cfpq_sched_train(cfpq_ctx *ctx, const float *x, const uint32_t *rule, const float *y, size_t n,
+                                uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *loss) {
+    if (!ctx || !x || !rule || !y || n == 0 || n > 0xffffffffu || epochs == 0 || epochs > 64)
+        return fail(ctx, CFPQ_E_INVALID, "cfpq_sched_train: bad argument (n > 0, 1 <= epochs <= 64)");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->sched_have) CHK(cfpq_sched_init(ctx, seed));
+    CHK(ensure_train_state(ctx));
+    if (batch == 0) batch = 4096;
+    // one fixed shuffle of the samples (host, from the seed); batches are
+    // contiguous slices of it
+    std::vector<uint32_t> perm(n);
+    for (size_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
+    uint64_t st = seed ^ 0x5851f42d4c957f2dull;
+    for (size_t i = n - 1; i > 0; --i) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        std::swap(perm[i], perm[(st >> 33) % (i + 1)]);
+    }
+    std::vector<float> sx(n * MLP_F), sy(n);
+    std::vector<uint32_t> sr(n);
+    for (size_t i = 0; i < n; ++i) {
+        std::memcpy(&sx[i * MLP_F], x + (size_t)perm[i] * MLP_F, MLP_F * 4);
+        sr[i] = rule[perm[i]];
+        sy[i] = y[perm[i]];
+    }
+    CHK(upload_samples(ctx, sx.data(), sr.data(), sy.data(), n));
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, ctx->stream, ctx->t_loss, 64u, 0.f);
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    uint64_t t = 0;
+    const size_t nb = (n + batch - 1) / batch;
+    for (uint32_t e = 0; e < epochs; ++e) {
+        for (size_t bi = 0; bi < nb; ++bi) {
+            const size_t b0 = bi * batch, bn = std::min<size_t>(batch, n - b0);
+            ++t;
+            hipLaunchKernelGGL(k_fill_f32, dim3(24), dim3(256), 0, ctx->stream, ctx->t_grad, (uint32_t)MLP_PARAMS, 0.f);
+            // gradient of the batch mean; the epoch's mean loss accumulates in t_loss[e]
+            hipLaunchKernelGGL(k_mlp_eval, dim3(mlp_grid(ctx, bn)), dim3(MLP_BLOCK), 0, ctx->stream,
+                               (const uint16_t *)ctx->sched_w, (const float *)(ctx->t_x + b0 * MLP_F),
+                               (const uint32_t *)(ctx->t_rule + b0), (const float *)(ctx->t_y + b0), (uint32_t)bn,
+                               1.f / (float)bn, 1.f / (float)n, (float *)nullptr, ctx->t_loss + e, ctx->t_grad);
+            const float c1 = 1.f - (float)std::pow((double)b1, (double)t);
+            const float c2 = 1.f - (float)std::pow((double)b2, (double)t);
+            hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, ctx->t_master,
+                               ctx->sched_w, ctx->t_m1, ctx->t_m2, (const float *)ctx->t_grad, lr, b1, b2, c1, c2,
+                               eps);
+            CHK(launch_check(ctx, "training step"));
+        }
+    }
+    if (loss) HIPCHK(hipMemcpyAsync(loss, ctx->t_loss, epochs * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_sched_trace_begin(cfpq_ctx *ctx, size_t capacity) {
+    if (!ctx || capacity == 0 || capacity > 0x7fffffffu) return fail(ctx, CFPQ_E_INVALID, "cfpq_sched_trace_begin: bad capacity");
+    HIPCHK(hipSetDevice(ctx->device));
+    CHK(cfpq_sched_trace_end(ctx));
+    HIPCHK(hipMalloc((void **)&ctx->tr_x, capacity * MLP_F * 4));
+    HIPCHK(hipMalloc((void **)&ctx->tr_y, capacity * 4));
+    HIPCHK(hipMalloc((void **)&ctx->tr_rule, capacity * 4));
+    HIPCHK(hipMalloc((void **)&ctx->tr_item, 2 * capacity * 4));
+    HIPCHK(hipMalloc((void **)&ctx->tr_count, 4));
+    HIPCHK(hipMemsetAsync(ctx->tr_count, 0, 4, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->tr_cap = capacity;
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_sched_trace_read(cfpq_ctx *ctx, float *x, uint32_t *rule, float *y, size_t capacity, size_t *count) {
+    if (!ctx || !count || (capacity && (!x || !rule || !y))) return fail(ctx, CFPQ_E_INVALID, "cfpq_sched_trace_read: bad argument");
+    *count = 0;
+    if (!ctx->tr_cap) return CFPQ_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    uint32_t cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, ctx->tr_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const size_t n = std::min<size_t>(std::min<size_t>(cnt, ctx->tr_cap), capacity);
+    if (n) {
+        HIPCHK(hipMemcpyAsync(x, ctx->tr_x, n * MLP_F * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(rule, ctx->tr_rule, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(y, ctx->tr_y, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    *count = std::min<size_t>(cnt, ctx->tr_cap);
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_sched_trace_end(cfpq_ctx *ctx) {
+    if (!ctx) return CFPQ_E_INVALID;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (void *p : {(void *)ctx->tr_x, (void *)ctx->tr_y, (void *)ctx->tr_rule, (void *)ctx->tr_item, (void *)ctx->tr_count})
+        if (p) HIPCHK(hipFree(p));
+    ctx->tr_x = ctx->tr_y = nullptr;
+    ctx->tr_rule = ctx->tr_item = ctx->tr_count = nullptr;
+    ctx->tr_cap = 0;
+    return CFPQ_OK;
 }

@@ -167,43 +167,54 @@ __device__ __forceinline__ void wave_append(bool want, uint64_t val, uint32_t *c
     if (want) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = val;
 }
 
+// Candidate scan of the terms [t0, t1) for C tile (I, J): which quarters have
+// candidates, whether some A tile is heavy enough for the dense / MFMA paths.
+__device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, const TermDev *__restrict__ terms,
+                                           uint32_t t0, uint32_t t1, int parity, uint32_t T, uint32_t I, uint32_t J,
+                                           uint32_t &qmask, bool &dense, bool &mfma) {
+    for (uint32_t t = t0; t < t1; ++t) {
+        const MatDev L = mats[terms[t].left[parity]];
+        const MatDev R = mats[terms[t].right[parity]];
+        for (uint32_t w = 0; w < L.ow; ++w) {
+            const uint32_t col = R.occC[J * R.ow + w];
+            if (!col) continue;
+            if (L.occD[I * L.ow + w] & col) dense = true;
+            if (L.occX[I * L.ow + w] & col) mfma = true;
+#pragma unroll
+            for (uint32_t q = 0; q < QUARTERS; ++q)
+                if (L.occQ[(q * T + I) * L.ow + w] & col) qmask |= 1u << q;
+        }
+    }
+}
+
 // One thread per (head, C tile).  Skips full head tiles.
 __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats,
                                                  const HeadDev *__restrict__ heads,
-                                                 const TermDev *__restrict__ terms, int parity,
+                                                 const TermDev *__restrict__ terms, int parity, int full_pass,
                                                  StepCounters *__restrict__ c, uint64_t *__restrict__ units,
                                                  uint64_t *__restrict__ dense_items,
                                                  uint64_t *__restrict__ mfma_items) {
     const uint32_t h = blockIdx.y;
     const uint32_t T = uniform(heads[h].T), target = uniform(heads[h].target);
-    const uint32_t t0 = uniform(heads[h].t0), t1 = uniform(heads[h].t1);
+    const uint32_t t0 = uniform(full_pass ? heads[h].f0 : heads[h].t0);
+    const uint32_t t1 = uniform(full_pass ? heads[h].f1 : heads[h].t1);
+    const uint64_t flag = full_pass ? ITEM_FULL : 0;
     const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
     bool dense = false, mfma = false;
     uint32_t qmask = 0;
     if (p < T * T) {
         const uint32_t I = p / T, J = p - I * T;
         if (mats[target].cnt[p] < TILE_BITS) {
-            for (uint32_t t = t0; t < t1; ++t) {
-                const MatDev L = mats[uniform(terms[t].left[parity])];
-                const MatDev R = mats[uniform(terms[t].right[parity])];
-                for (uint32_t w = 0; w < L.ow; ++w) {
-                    const uint32_t col = R.occC[J * R.ow + w];
-                    if (!col) continue;
-                    if (L.occD[I * L.ow + w] & col) dense = true;
-                    if (L.occX[I * L.ow + w] & col) mfma = true;
-#pragma unroll
-                    for (uint32_t q = 0; q < QUARTERS; ++q)
-                        if (L.occQ[(q * T + I) * L.ow + w] & col) qmask |= 1u << q;
-                }
-            }
+            scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma);
         }
     }
-    wave_append(mfma, ((uint64_t)h << 40) | p, &c->n_mfma, mfma_items);
-    wave_append(dense && !mfma, ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
+    wave_append(mfma, flag | ((uint64_t)h << 40) | p, &c->n_mfma, mfma_items);
+    wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
 #pragma unroll
     for (uint32_t q = 0; q < QUARTERS; ++q)
-        wave_append(!dense && ((qmask >> q) & 1u), ((uint64_t)h << 40) | ((uint64_t)p << 2) | q, &c->n_units, units);
+        wave_append(!dense && ((qmask >> q) & 1u), flag | ((uint64_t)h << 40) | ((uint64_t)p << 2) | q, &c->n_units,
+                    units);
 }
 
 // ---------------------------------------------------------------------------
@@ -226,10 +237,13 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
     const uint32_t u1 = (u0 + chunk < n_units) ? u0 + chunk : n_units;
     for (uint32_t u = u0; u < u1; ++u) {
         const uint64_t it = ld_const(units + u);
-        const uint32_t h = uniform((uint32_t)(it >> 40));
+        const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)((it & ITEM_TILE_MASK) >> 2));
         const uint32_t q = uniform((uint32_t)(it & 3u));
-        const uint32_t T = ld_const(&heads[h].T), t0 = ld_const(&heads[h].t0), t1 = ld_const(&heads[h].t1);
+        const bool full = (it & ITEM_FULL) != 0;
+        const uint32_t T = ld_const(&heads[h].T);
+        const uint32_t t0 = full ? ld_const(&heads[h].f0) : ld_const(&heads[h].t0);
+        const uint32_t t1 = full ? ld_const(&heads[h].f1) : ld_const(&heads[h].t1);
         const uint32_t I = p / T, J = p - (p / T) * T;
         const uint32_t row = q * WAVE + lane;
         Row4 acc = {{0, 0, 0, 0}};
@@ -438,9 +452,11 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
     uint32_t tp = 0, tpd = 0, arows = 0, brows = 0, brows_lds = 0, out_rows = 0, lookups = 0;
     for (uint32_t idx = wg_next(&c->dense_next, &s_idx); idx < n_items; idx = wg_next(&c->dense_next, &s_idx)) {
         const uint64_t it = items[idx];
-        const uint32_t h = uniform((uint32_t)(it >> 40));
+        const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
-        const uint32_t T = uniform(heads[h].T), t0 = uniform(heads[h].t0), t1 = uniform(heads[h].t1);
+        const bool full = (it & ITEM_FULL) != 0;
+        const uint32_t T = uniform(heads[h].T);
+        const uint32_t t0 = uniform(full ? heads[h].f0 : heads[h].t0), t1 = uniform(full ? heads[h].f1 : heads[h].t1);
         const uint32_t I = p / T, J = p - (p / T) * T;
         Row4 acc = {{0, 0, 0, 0}};
         for (uint32_t t = t0; t < t1; ++t) {
@@ -598,9 +614,11 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
     uint32_t tp = 0, instr = 0, out_rows = 0;
     for (uint32_t idx = wg_next(&c->mfma_next, &s_idx); idx < n_items; idx = wg_next(&c->mfma_next, &s_idx)) {
         const uint64_t it = items[idx];
-        const uint32_t h = uniform((uint32_t)(it >> 40));
+        const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
-        const uint32_t T = uniform(heads[h].T), t0 = uniform(heads[h].t0), t1 = uniform(heads[h].t1);
+        const bool full = (it & ITEM_FULL) != 0;
+        const uint32_t T = uniform(heads[h].T);
+        const uint32_t t0 = uniform(full ? heads[h].f0 : heads[h].t0), t1 = uniform(full ? heads[h].f1 : heads[h].t1);
         const uint32_t I = p / T, J = p - (p / T) * T;
         i32x16 acc[2][4];
 #pragma unroll
@@ -744,7 +762,7 @@ __global__ __launch_bounds__(BLOCK) void k_apply(const MatDev *__restrict__ mats
     const uint32_t n = c->n_touched;
     for (uint32_t idx = wg_next(&c->apply_next, &s_idx); idx < n; idx = wg_next(&c->apply_next, &s_idx)) {
         const uint64_t it = touched[idx];
-        const uint32_t h = uniform((uint32_t)(it >> 40));
+        const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
         const MatDev M = mats[uniform(heads[h].target)];
         const MatDev O = mats[uniform(heads[h].out[parity])];
@@ -784,7 +802,7 @@ __global__ __launch_bounds__(BLOCK) void k_clear(const MatDev *__restrict__ mats
     const uint32_t n = list_ctr->n_touched;
     for (uint32_t idx = wg_next(&c->clear_next, &s_idx); idx < n; idx = wg_next(&c->clear_next, &s_idx)) {
         const uint64_t it = list[idx];
-        const uint32_t h = uniform((uint32_t)(it >> 40));
+        const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
         const MatDev O = mats[uniform(heads[h].out[parity])];
         const uint64_t rm = O.rmask[(size_t)p * 4 + wv];

@@ -46,19 +46,24 @@ struct MatDev {
 };
 
 // A group of rules sharing one head: the head's full matrix (mask), the
-// buffer new bits go to on even / odd steps, and its terms [t0, t1).
+// buffer new bits go to on even / odd steps, its semi-naive terms [t0, t1)
+// (two per rule: dL*M_R then M_L*dR) and its full terms [f0, f1) (M_L*M_R,
+// one per rule), both ranges into one term table.
 struct HeadDev {
     uint32_t target;
     uint32_t out[2];
     uint32_t t0, t1;
+    uint32_t f0, f1;
     uint32_t T;
 };
 
 // One product term  left * right  contributing to a head; operands are
-// indices into the matrix-descriptor array, chosen by step parity.
+// indices into the matrix-descriptor array, chosen by step parity.  `rule`
+// is the rule's index in its query (the scheduler's rule id).
 struct TermDev {
     uint32_t left[2];
     uint32_t right[2];
+    uint32_t rule;
 };
 
 // Per-step counters (a ring of them lives in device memory).
@@ -72,7 +77,12 @@ struct StepCounters {
     uint32_t clear_next;   // clear-kernel cursor
     uint32_t n_mfma;       // MFMA work items (head, tile) built
     uint32_t mfma_next;    // MFMA product cursor
-    uint32_t pad0;
+    uint32_t n_deferred;   // (head, tile) items the scheduler deferred
+    uint32_t n_full;       // (head, tile) items evaluated over full operands
+    uint32_t n_scored;     // (rule, tile) items scored by the scheduler
+    uint32_t trace0;       // first trace row written by this step
+    uint32_t dirty_new;    // tiles that became dirty (deferred) in this step
+    uint32_t dirty_cleared; // dirty tiles evaluated in full (or found full) in this step
     uint64_t new_bits;
     uint64_t tile_products;   // (term, C tile, k) products evaluated (sparse: per quarter)
     uint64_t tp_dense;        // of which on the LDS / Four Russians paths
@@ -91,7 +101,10 @@ static inline uint32_t tiles_per_side(uint32_t n) { return (n + TILE - 1) / TILE
 static inline uint32_t occ_words(uint32_t T) { return (T + 31) / 32; }
 
 // Item encodings: dense item = (head << 40) | tile; sparse unit =
-// (head << 40) | (tile << 2) | quarter.
+// (head << 40) | (tile << 2) | quarter; bit 63 set = evaluate the head's
+// full terms (M_L * M_R) instead of its semi-naive terms.
 constexpr uint64_t ITEM_TILE_MASK = 0xffffffffffull;
+constexpr uint64_t ITEM_FULL = 1ull << 63;
+__host__ __device__ inline uint32_t item_head(uint64_t it) { return (uint32_t)((it >> 40) & 0x7fffffu); }
 
 }  // namespace cfpq

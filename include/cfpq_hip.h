@@ -82,7 +82,11 @@ typedef struct cfpq_opts {
     int32_t sparse_batch;       /* k-tiles per batch in the sparse product: 1, 2 or 4 (0 = default) */
     int32_t sparse_waves_per_cu; /* persistent sparse-product waves per CU (0 = default)   */
     int32_t sparse_chunk;       /* work units a sparse-product wave claims per atomic (0 = default) */
-    int32_t reserved[5];
+    int32_t sched;              /* 1: the learned tile scheduler may defer (head, tile) work
+                                   (needs weights: cfpq_sched_set_weights / _train)     */
+    uint32_t sched_threshold;   /* defer an item whose predicted activity probability is
+                                   below sched_threshold / 65536 (0 = library default)   */
+    int32_t reserved[3];
 } cfpq_opts;
 
 typedef struct cfpq_rule {
@@ -110,6 +114,11 @@ typedef struct cfpq_stats {
     double sparse_ms;           /* device time in the sparse (wave-per-quarter) product  */
     double dense_ms;            /* device time in the LDS / Four Russians product        */
     double mfma_ms;             /* device time in the MFMA product                       */
+    uint64_t sched_scored;      /* (rule, tile) items scored by the learned scheduler    */
+    uint64_t sched_deferred;    /* (head, tile) items it deferred                        */
+    uint64_t sched_full;        /* deferred tiles later evaluated over full operands     */
+    uint32_t flush_steps;       /* steps that evaluated every deferred tile (no deferral) */
+    uint32_t pad0;
 } cfpq_stats;
 
 /* ---- lifecycle ---------------------------------------------------------- */
@@ -156,6 +165,33 @@ typedef struct cfpq_query {
 /* Least fixpoints of all queries, advanced together: one set of kernel
  * launches per step covers every query still changing. */
 int cfpq_batch_fixpoint(cfpq_ctx *ctx, const cfpq_query *queries, size_t nqueries, cfpq_stats *stats);
+
+/* ---- learned tile scheduler ----------------------------------------------
+ * No counterpart in the reference.  A tile-activity network (16 inputs:
+ * 8 tile features + 8-dim rule embedding; 64-64 ReLU hidden; 1 logit),
+ * bf16 parameters, fp32 arithmetic.  CFPQ_SCHED_PARAMS parameters:
+ * E[64][8], W1[16][64], b1[64], W2[64][64], b2[64], W3[64], b3. */
+#define CFPQ_SCHED_PARAMS 5825
+#define CFPQ_SCHED_FEATURES 8
+int cfpq_sched_set_weights(cfpq_ctx *ctx, const uint16_t *bf16, size_t count);
+int cfpq_sched_get_weights(cfpq_ctx *ctx, uint16_t *bf16, size_t count);
+/* Deterministic initialisation (scaled uniform, from `seed`). */
+int cfpq_sched_init(cfpq_ctx *ctx, uint64_t seed);
+/* Forward (logits, may be NULL), mean binary cross-entropy (loss, may be NULL)
+ * and its fp32 gradient (grad[CFPQ_SCHED_PARAMS], may be NULL) of the network
+ * with parameters w (bf16) on n samples: x[n][8], rule[n], y[n] in {0, 1}. */
+int cfpq_sched_eval(cfpq_ctx *ctx, const uint16_t *w, const float *x, const uint32_t *rule, const float *y,
+                    size_t n, float *logits, float *loss, float *grad);
+/* Adam on the context's network over the samples, on the device; loss[epoch]
+ * receives each epoch's mean training loss (may be NULL). */
+int cfpq_sched_train(cfpq_ctx *ctx, const float *x, const uint32_t *rule, const float *y, size_t n,
+                     uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *loss);
+/* Trace collection: while enabled, every fixpoint step records one row per
+ * scored (rule, tile) item -- features, rule id and whether the item's head
+ * tile gained a bit in that step -- up to `capacity` rows. */
+int cfpq_sched_trace_begin(cfpq_ctx *ctx, size_t capacity);
+int cfpq_sched_trace_read(cfpq_ctx *ctx, float *x, uint32_t *rule, float *y, size_t capacity, size_t *count);
+int cfpq_sched_trace_end(cfpq_ctx *ctx);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,111 @@
+"""The learned tile scheduler on the GPU (needs a GPU), through the C ABI.
+
+  * the HIP forward / backward kernels against the fp64 CPU statement of the
+    network (oracle/mlp.py) at the same bf16 parameters;
+  * training on the device lowers the loss and learns a separable rule;
+  * traces recorded by the plain fixpoint are well formed;
+  * with the scheduler deferring work, every fixpoint stays bit-exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(seed, n):
+    rng = np.random.default_rng(seed)
+    w = mlp.f32_to_bf16(rng.normal(0, 0.3, mlp.PARAMS).astype(np.float32))
+    x = rng.random((n, mlp.F)).astype(np.float32)
+    rule = rng.integers(0, 70, n).astype(np.uint32)
+    y = (rng.random(n) < 0.3).astype(np.float32)
+    return w, x, rule, y
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000, 5000])
+def test_forward_and_gradient_match_fp64(engine, n):
+    w, x, rule, y = _problem(n, n)
+    logits, loss, grad = engine.sched_eval(w, x, rule, y)
+    p = mlp.bf16_to_f64(w)
+    want_loss, want_grad, want_z = mlp.loss_and_grad(p, x.astype(np.float64), rule, y.astype(np.float64))
+    # fp32 arithmetic against fp64: tolerance scaled by the magnitudes involved
+    assert np.allclose(logits, want_z, rtol=1e-4, atol=1e-4)
+    assert abs(loss - want_loss) <= 1e-5 + 1e-5 * abs(want_loss)
+    scale = np.abs(want_grad).max()
+    assert np.abs(grad - want_grad).max() <= 1e-4 * scale + 1e-7
+    # every block of the layout receives gradient where fp64 says so
+    for a, b in [(mlp.OFF_E, mlp.OFF_W1), (mlp.OFF_W1, mlp.OFF_B1), (mlp.OFF_W2, mlp.OFF_B2), (mlp.OFF_W3, mlp.OFF_B3)]:
+        assert np.array_equal(grad[a:b] != 0, want_grad[a:b] != 0) or n >= 1000
+
+
+def test_training_learns_a_separable_rule(engine):
+    rng = np.random.default_rng(5)
+    n = 20000
+    x = rng.random((n, mlp.F)).astype(np.float32)
+    rule = rng.integers(0, 6, n).astype(np.uint32)
+    y = ((x[:, 0] + 0.5 * x[:, 3] > 0.8) ^ (rule == 2)).astype(np.float32)
+    engine.sched_init(1)
+    losses = engine.sched_train(x, rule, y, epochs=12, batch=1024, lr=3e-3, seed=1)
+    assert losses[-1] < 0.5 * losses[0]
+    logits, _, _ = engine.sched_eval(engine.sched_get_weights(), x, rule, y, want_grad=False)
+    acc = np.mean((logits > 0) == (y > 0.5))
+    assert acc > 0.95, acc
+
+
+def test_trace_rows(pkg, gen, engine):
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    q = gen.dense_stress(900)
+    engine.trace_begin(1 << 20)
+    gr = cf.Grammar(q.grammar, text=True)
+    gr.intersection_with_graph(cf.Graph.from_query(q), engine)
+    x, rule, y, count = engine.trace_read(1 << 20)
+    engine.trace_end()
+    gr.free()
+    assert count == x.shape[0] > 0
+    assert set(np.unique(y)) <= {0.0, 1.0} and 0 < y.mean() < 1
+    assert np.isfinite(x).all() and (x >= 0).all()
+    assert rule.max() < len(q.grammar.splitlines())
+
+
+@pytest.mark.parametrize("threshold", [0.3, 0.9, 0.999])
+@pytest.mark.parametrize("which", ["dense", "dyck", "sg"])
+def test_scheduled_fixpoint_is_bit_exact(pkg, gen, oracle, which, threshold):
+    """Deferral on (an untrained network and thresholds that defer much of the
+    work): results must not change."""
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    q = {"dense": lambda: gen.dense_stress(1200), "dyck": lambda: gen.dyck_er(1500, 5000, seed=6),
+         "sg": lambda: gen.sg_forest(6000, seed=7, types_per_vertex=3)}[which]()
+    g_ref, mats_ref, _ = oracle.solve(q, "seminaive")
+    want = oracle.fingerprints(g_ref, mats_ref, q.n)
+    for graph in (False, True):
+        with pkg.Engine(0, sched=True, sched_threshold=threshold, use_graph=graph) as eng:
+            eng.sched_init(11)
+            gr = cf.Grammar(q.grammar, text=True)
+            _, _, st = gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+            assert gr.results() == want, (which, threshold, graph)
+            assert st["sched_scored"] > 0
+            if st["sched_deferred"]:
+                assert st["flush_steps"] >= 1 and st["sched_full"] > 0
+            gr.free()
+
+
+def test_scheduled_fixpoint_with_final_full_pass(pkg, gen, oracle):
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    q = gen.dense_stress(800)
+    g_ref, mats_ref, _ = oracle.solve(q, "seminaive")
+    with pkg.Engine(0, sched=True, sched_threshold=0.9, final_full_pass=True) as eng:
+        eng.sched_init(3)
+        gr = cf.Grammar(q.grammar, text=True)
+        _, _, st = gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+        assert gr.results() == oracle.fingerprints(g_ref, mats_ref, q.n)
+        assert st["full_passes"] == 1 and st["sched_deferred"] > 0
+        gr.free()
+
+
+def test_weights_roundtrip_and_errors(pkg, engine):
+    w = mlp.f32_to_bf16(np.linspace(-1, 1, mlp.PARAMS).astype(np.float32))
+    engine.sched_set_weights(w)
+    assert np.array_equal(engine.sched_get_weights(), w)
+    with pytest.raises(pkg.CfpqError):
+        engine.sched_set_weights(w[:10])
