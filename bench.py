@@ -116,6 +116,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler engine")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run one plain device copy of MIB MiB (torch), the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--sched-threshold", type=float, default=0.0,
                     help="defer items with predicted activity below this probability (default: tuned/library)")
     args = ap.parse_args()
@@ -124,6 +127,14 @@ def main():
               "multi-GPU sharding is designed but not built", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        import torch
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        del src, dst
     pkg = importlib.import_module("cfpq-on-gpgpu_amd")
     gen = importlib.import_module("cfpq-on-gpgpu_amd.gen")
     cf = importlib.import_module("cfpq-on-gpgpu_amd.cfpq")
@@ -247,7 +258,8 @@ def main():
         "tile_products": s0["tile_products"],
         "work": {k: s0[k] for k in ("items", "tile_products", "tile_products_dense", "tile_products_mfma",
                                     "a_rows", "b_rows", "b_rows_lds", "m4r_lookups", "mfma_instr", "out_rows",
-                                    "new_bits", "product_launches", "host_syncs")},
+                                    "new_bits", "product_launches", "host_syncs", "cand_hits",
+                                    "max_unit_cands")},
         "kernel_ms": {k: round(s0[k], 4) for k in ("sparse_ms", "dense_ms", "mfma_ms")},
         "engine_opts": opts,
         "engines": engines,

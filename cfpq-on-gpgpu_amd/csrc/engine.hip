@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -485,7 +486,6 @@ struct Plan {
     std::vector<TermDev> terms_delta;  // semi-naive terms (two per rule)
     std::vector<TermDev> terms_full;   // full terms (one per rule), stored after the delta terms
     std::vector<uint32_t> all_delta[2];  // ids of every delta buffer of parity q (buffer Dq)
-    std::vector<uint32_t> copy_dst, copy_src;  // initial D0 := M
     size_t max_tiles_per_head = 0;     // max T*T over heads
     size_t total_head_tiles = 0;       // sum of T*T over heads
     size_t max_occ_words = 0;          // max T*ow over matrices
@@ -496,7 +496,8 @@ struct Plan {
 
 struct StepStats {
     uint64_t new_bits = 0, items = 0, tp = 0, tpd = 0, tpm = 0, arows = 0, brows = 0, orows = 0, brows_lds = 0,
-             lookups = 0, mfma_instr = 0;
+             lookups = 0, mfma_instr = 0, cand_hits = 0;
+    uint32_t max_unit_cands = 0;
 };
 
 // A-tile popcount thresholds of the product kernels.  A (head, C tile) item
@@ -585,22 +586,18 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     // product kernels, timed: ev[4i] sparse ev[4i+1] dense ev[4i+2] mfma ev[4i+3]
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx], ctx->stream));
     const dim3 sgrid((uint32_t)ctx->num_cu * (uint32_t)ctx->opts.sparse_waves_per_cu);
-    switch (ctx->opts.sparse_batch) {
-    case 1:
-        hipLaunchKernelGGL(k_product_sparse<1>, sgrid, dim3(WAVE), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                           (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->units, touched,
-                           (uint32_t)ctx->opts.sparse_chunk);
-        break;
-    case 2:
-        hipLaunchKernelGGL(k_product_sparse<2>, sgrid, dim3(WAVE), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                           (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->units, touched,
-                           (uint32_t)ctx->opts.sparse_chunk);
-        break;
-    default:
-        hipLaunchKernelGGL(k_product_sparse<4>, sgrid, dim3(WAVE), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                           (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->units, touched,
-                           (uint32_t)ctx->opts.sparse_chunk);
-        break;
+    const uint32_t chunk = (uint32_t)ctx->opts.sparse_chunk;
+    const MatDev *dm = ctx->d_mats;
+    const HeadDev *dh = ctx->d_heads;
+    const uint64_t *un = ctx->units;
+    if (ctx->opts.sparse_mode == 0) {
+        hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk);
+    } else if (ctx->opts.sparse_batch == 1) {
+        hipLaunchKernelGGL(k_product_sparse<1>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk);
+    } else if (ctx->opts.sparse_batch == 2) {
+        hipLaunchKernelGGL(k_product_sparse<2>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk);
+    } else {
+        hipLaunchKernelGGL(k_product_sparse<4>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk);
     }
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 1], ctx->stream));
     hipLaunchKernelGGL(k_product_dense, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
@@ -619,7 +616,9 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     // retire the delta this step consumed (buffer D_q)
     const uint32_t *ids_q = ctx->d_ids + (q ? plan.all_delta[0].size() : 0);
     const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
-    if (prev_slot < 0) {
+    if (prev_slot < 0 && full_pass) {
+        // first step of a fixpoint over full terms: its delta buffer was never filled
+    } else if (prev_slot < 0) {
         hipLaunchKernelGGL(k_clear_full, dim3(std::max<uint32_t>(1, ctx->num_cu * 4 / std::max<uint32_t>(1, nids_q)),
                                               std::min<uint32_t>(nids_q, 65535)),
                            dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats, ids_q, nids_q);
@@ -634,7 +633,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
 }
 
 static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
-    plan.ids_all_off = plan.all_delta[0].size() + plan.all_delta[1].size() + 2 * plan.copy_dst.size();
+    plan.ids_all_off = plan.all_delta[0].size() + plan.all_delta[1].size();
     plan.max_occ_words = 0;
     for (const MatDev &m : plan.mats) plan.max_occ_words = std::max<size_t>(plan.max_occ_words, (size_t)m.T * m.ow);
     CHK(grow(ctx, &ctx->d_mats, &ctx->d_mats_cap, plan.mats.size()));
@@ -645,8 +644,7 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
         hd.f1 += (uint32_t)plan.terms_delta.size();
     }
     CHK(grow(ctx, &ctx->d_terms, &ctx->d_terms_cap, plan.terms_delta.size() + plan.terms_full.size()));
-    const size_t nid = plan.all_delta[0].size() + plan.all_delta[1].size() + 2 * plan.copy_dst.size() +
-                       plan.mats.size();
+    const size_t nid = plan.all_delta[0].size() + plan.all_delta[1].size() + plan.mats.size();
     CHK(grow(ctx, &ctx->d_ids, &ctx->d_ids_cap, nid));
     CHK(grow(ctx, &ctx->items, &ctx->items_cap, plan.total_head_tiles));
     CHK(grow(ctx, &ctx->units, &ctx->units_cap, QUARTERS * plan.total_head_tiles));
@@ -665,8 +663,6 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     HIPCHK(hipMemcpyAsync(ctx->d_terms, terms.data(), terms.size() * sizeof(TermDev), hipMemcpyHostToDevice, ctx->stream));
     std::vector<uint32_t> ids(plan.all_delta[0]);
     ids.insert(ids.end(), plan.all_delta[1].begin(), plan.all_delta[1].end());
-    ids.insert(ids.end(), plan.copy_dst.begin(), plan.copy_dst.end());
-    ids.insert(ids.end(), plan.copy_src.begin(), plan.copy_src.end());
     for (uint32_t x = 0; x < (uint32_t)plan.mats.size(); ++x) ids.push_back(x);  // at plan.ids_all_off
     HIPCHK(hipMemcpyAsync(ctx->d_ids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->ring, 0, sizeof(StepCounters) * RING, ctx->stream));
@@ -690,6 +686,8 @@ static void accumulate(StepStats &s, const StepCounters &c) {
     s.new_bits += c.new_bits;
     s.items += c.n_units + c.n_dense + c.n_mfma;
     s.mfma_instr += c.mfma_instr;
+    s.cand_hits += c.cand_hits;
+    s.max_unit_cands = std::max(s.max_unit_cands, c.max_unit_cands);
     s.tp += c.tile_products;
     s.tpd += c.tp_dense;
     s.tpm += c.tp_mfma;
@@ -739,7 +737,7 @@ static int drop_graphs(cfpq_ctx *ctx) {
 constexpr int GRAPH_VARIANTS = 4;
 static int launch_step_graph(cfpq_ctx *ctx, const Plan &plan, uint32_t slot, int prev_slot, bool full_pass,
                              bool flush) {
-    const int variant = full_pass ? 2 : flush ? 3 : (prev_slot < 0 ? 0 : 1);
+    const int variant = prev_slot < 0 ? 0 : full_pass ? 2 : flush ? 3 : 1;
     hipGraphExec_t &ex = ctx->graph_exec[(size_t)variant * RING + slot];
     if (!ex) {
         HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -768,14 +766,8 @@ static int launch_step_graph(cfpq_ctx *ctx, const Plan &plan, uint32_t slot, int
 static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool single_step_full) {
     CHK(upload_plan(ctx, plan));
     const uint32_t nd = (uint32_t)plan.all_delta[0].size();
-    // D0 := M (every initial bit is new)
-    if (!plan.copy_dst.empty()) {
-        const uint32_t ncp = (uint32_t)plan.copy_dst.size();
-        const uint32_t *dst = ctx->d_ids + nd + plan.all_delta[1].size();
-        hipLaunchKernelGGL(k_copy, dim3(std::max<uint32_t>(1, ctx->num_cu * 4 / ncp), std::min<uint32_t>(ncp, 65535)),
-                           dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats, dst, dst + ncp, ncp);
-        CHK(launch_check(ctx, "k_copy"));
-    }
+    // (the first step reads the full matrices, so the initial delta buffer
+    // D0 stays empty: every initial bit is "new" through the full terms)
     const int per_sync = single_step_full ? 1 : ctx->opts.iters_per_sync;
     // graph replay needs the parity to follow the slot (RING is even)
     const bool use_graph = ctx->opts.use_graph != 0 && !single_step_full;
@@ -795,6 +787,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
     uint32_t steps = 0, syncs = 0, launches = 0, full_passes = 0, flush_steps = 0;
     uint64_t sched_scored = 0, sched_deferred = 0, sched_full = 0;
     int64_t dirty_total = 0;  // tiles deferred and not yet evaluated in full
+    static const bool step_log = std::getenv("CFPQ_STEP_LOG") != nullptr;  // per-step counters to stderr
     int prev_slot = -1;
     bool done = false;
     // next batch: 0 = delta steps, 1 = one exact full-pass step, 2 = one flush step
@@ -806,8 +799,11 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         std::vector<uint32_t> slots;
         for (int s = 0; s < k; ++s) {
             const uint32_t slot = steps % RING;
-            if (use_graph) CHK(launch_step_graph(ctx, plan, slot, prev_slot, kind == 1, kind == 2));
-            else CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, kind == 1, s, kind == 2));
+            // the first step evaluates every rule once over the full matrices
+            // (with d = M the two semi-naive terms would repeat the same product)
+            const bool full = kind == 1 || steps == 0;
+            if (use_graph) CHK(launch_step_graph(ctx, plan, slot, prev_slot, full, kind == 2));
+            else CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, full, s, kind == 2));
             ++launches;
             slots.push_back(slot);
             prev_slot = (int)slot;
@@ -826,12 +822,20 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             // after a quiet one still retires dirty tiles whose head tile is full)
             dirty_total += (int64_t)c.dirty_new - (int64_t)c.dirty_cleared;
             if (quiet) continue;
+            float step_ms[3] = {0, 0, 0};
             for (int k2 = 0; k2 < 3 && !use_graph; ++k2) {
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, ctx->ev[4 * s + k2], ctx->ev[4 * s + k2 + 1]));
-                part_ms[k2] += ms;
-                product_ms += ms;
+                HIPCHK(hipEventElapsedTime(&step_ms[k2], ctx->ev[4 * s + k2], ctx->ev[4 * s + k2 + 1]));
+                part_ms[k2] += step_ms[k2];
+                product_ms += step_ms[k2];
             }
+            if (step_log)
+                std::fprintf(stderr,
+                             "[cfpq step %u] units %u dense %u mfma %u touched %u new_bits %llu cand %llu hits %llu "
+                             "max_unit %u a_rows %llu b_rows %llu | sparse %.3f dense %.3f mfma %.3f ms\n",
+                             steps - k + s, c.n_units, c.n_dense, c.n_mfma, c.n_touched,
+                             (unsigned long long)c.new_bits, (unsigned long long)c.tile_products,
+                             (unsigned long long)c.cand_hits, c.max_unit_cands, (unsigned long long)c.a_rows,
+                             (unsigned long long)c.b_rows, step_ms[0], step_ms[1], step_ms[2]);
             accumulate(tot, c);
             sched_scored += c.n_scored;
             sched_deferred += c.n_deferred;
@@ -906,6 +910,8 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         stats->sched_deferred = sched_deferred;
         stats->sched_full = sched_full;
         stats->flush_steps = flush_steps;
+        stats->cand_hits = tot.cand_hits;
+        stats->max_unit_cands = tot.max_unit_cands;
     }
     return CFPQ_OK;
 }
@@ -922,8 +928,6 @@ static void plan_query(Plan &plan, cfpq_mat *const *M, const std::vector<cfpq_ma
     for (size_t x = 0; x < nm; ++x) {
         plan.all_delta[0].push_back(id_d(0, (uint32_t)x));
         plan.all_delta[1].push_back(id_d(1, (uint32_t)x));
-        plan.copy_dst.push_back(id_d(0, (uint32_t)x));
-        plan.copy_src.push_back(id_m((uint32_t)x));
     }
     const uint32_t T = M[0]->T;
     // group rules by head (stable)

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
                                                          uint64_t *__restrict__ touched, uint32_t chunk) {
     const int lane = threadIdx.x & 63;
     const uint32_t n_units = c->n_units;
-    uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0;
+    uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0, hits = 0, max_cands = 0;
     for (uint32_t u0 = wave_next(&c->unit_next, chunk); u0 < n_units; u0 = wave_next(&c->unit_next, chunk)) {
     const uint32_t u1 = (u0 + chunk < n_units) ? u0 + chunk : n_units;
     for (uint32_t u = u0; u < u1; ++u) {
@@ -246,6 +246,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
         const uint32_t t1 = full ? ld_const(&heads[h].f1) : ld_const(&heads[h].t1);
         const uint32_t I = p / T, J = p - (p / T) * T;
         const uint32_t row = q * WAVE + lane;
+        const uint32_t tp_unit0 = tp;
         Row4 acc = {{0, 0, 0, 0}};
         for (uint32_t t = t0; t < t1; ++t) {
             const MatDev L = mats[ld_const(&terms[t].left[parity])];
@@ -282,6 +283,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
                             const bool hit = ((cu.w[0] & brm[j].w[0]) | (cu.w[1] & brm[j].w[1]) |
                                               (cu.w[2] & brm[j].w[2]) | (cu.w[3] & brm[j].w[3])) != 0;
                             rm[j] = hit ? arm : 0;
+                            hits += rm[j] != 0;
                         }
                     }
                     Row4 a[SPARSE_BATCH];
@@ -311,6 +313,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
                 }
             }
         }
+        max_cands = max(max_cands, tp - tp_unit0);
         const MatDev O = mats[uniform(heads[h].out[parity])];
         const uint64_t *Mbits = mats[uniform(heads[h].target)].bits;
         Row4 nw = {{0, 0, 0, 0}};
@@ -348,6 +351,215 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
         if (ar) atomicAdd((unsigned long long *)&c->a_rows, (unsigned long long)ar);
         if (br) atomicAdd((unsigned long long *)&c->b_rows, (unsigned long long)br);
         if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
+        if (hits) atomicAdd((unsigned long long *)&c->cand_hits, (unsigned long long)hits);
+        if (max_cands) atomicMax(&c->max_unit_cands, max_cands);
+    }
+}
+
+// Inclusive prefix sum over the 64 lanes.
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Position of the j-th (0-based) set bit of m (j < popcount(m)), by halving.
+__device__ __forceinline__ int select_bit(uint64_t m, uint32_t j) {
+    int r = 0;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(m & ((1ull << s) - 1ull));
+        if (j >= c) {
+            j -= c;
+            m >>= s;
+            r += s;
+        }
+    }
+    return r;
+}
+
+// Lane-parallel variant: the same work unit (head, C tile, 64-row quarter),
+// but the wave's lanes work on different candidate k-tiles at once.
+//   filter  lane l tests k-tile (w + l/32)*32 + l%32 of a pair of occupancy
+//           words: vector loads of the B tile's row mask, the A quarter's
+//           column union and its row mask; a ballot gives the hits;
+//   gather  the hits' (candidate, non-empty A row) pairs are numbered by a
+//           prefix sum over the lanes and dealt out 64 at a time: a lane
+//           finds its pair's candidate by binary search and the row by bit
+//           selection, loads the A row, ORs the B rows named by its bits
+//           (up to four B rows in flight) and merges the result into the
+//           unit's row accumulator in LDS (ds_or_b64);
+//   epilogue  lane r owns row r of the quarter, as in the batched kernel.
+// 64 candidates are filtered per load round and 64 A rows are in flight per
+// wave, however the rows are spread over the candidates.
+__global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__restrict__ mats,
+                                                            const HeadDev *__restrict__ heads,
+                                                            const TermDev *__restrict__ terms, int parity,
+                                                            StepCounters *__restrict__ c,
+                                                            const uint64_t *__restrict__ units,
+                                                            uint64_t *__restrict__ touched, uint32_t chunk) {
+    __shared__ Row4 acc_s[WAVE];
+    __shared__ Row4 s_brm[WAVE];
+    __shared__ uint64_t s_arm[WAVE];
+    __shared__ uint32_t s_k[WAVE], s_end[WAVE];
+    const int lane = threadIdx.x & 63;
+    const uint32_t n_units = c->n_units;
+    uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0, hits_n = 0, max_cands = 0;
+    for (uint32_t u0 = wave_next(&c->unit_next, chunk); u0 < n_units; u0 = wave_next(&c->unit_next, chunk)) {
+    const uint32_t u1 = (u0 + chunk < n_units) ? u0 + chunk : n_units;
+    for (uint32_t u = u0; u < u1; ++u) {
+        const uint64_t it = ld_const(units + u);
+        const uint32_t h = uniform(item_head(it));
+        const uint32_t p = uniform((uint32_t)((it & ITEM_TILE_MASK) >> 2));
+        const uint32_t q = uniform((uint32_t)(it & 3u));
+        const bool full = (it & ITEM_FULL) != 0;
+        const uint32_t T = ld_const(&heads[h].T);
+        const uint32_t t0 = full ? ld_const(&heads[h].f0) : ld_const(&heads[h].t0);
+        const uint32_t t1 = full ? ld_const(&heads[h].f1) : ld_const(&heads[h].t1);
+        const uint32_t I = p / T, J = p - (p / T) * T;
+        const uint32_t tp_unit0 = tp;
+        acc_s[lane] = Row4{{0, 0, 0, 0}};
+        __syncthreads();
+        for (uint32_t t = t0; t < t1; ++t) {
+            const MatDev L = mats[ld_const(&terms[t].left[parity])];
+            const MatDev R = mats[ld_const(&terms[t].right[parity])];
+            for (uint32_t w = 0; w < L.ow; w += 2) {
+                const uint32_t kb0 = ld_const(&L.occQ[(q * T + I) * L.ow + w]) & ld_const(&R.occC[J * R.ow + w]);
+                const uint32_t kb1 = (w + 1 < L.ow)
+                                         ? (ld_const(&L.occQ[(q * T + I) * L.ow + w + 1]) &
+                                            ld_const(&R.occC[J * R.ow + w + 1]))
+                                         : 0u;
+                if (!(kb0 | kb1)) continue;
+                tp += __popc(kb0) + __popc(kb1);
+                const uint32_t K = (w + (lane >> 5)) * 32 + (lane & 31);
+                const bool mine = (((lane < 32) ? kb0 : kb1) >> (lane & 31)) & 1u;
+                uint64_t arm = 0;
+                Row4 brm = {{0, 0, 0, 0}};
+                if (mine) {
+                    const size_t lt = (size_t)I * T + K, rt = (size_t)K * T + J;
+                    brm = load_row(R.rmask + rt * 4);
+                    const Row4 cu = load_row(colq_of(L, (uint32_t)lt, q));
+                    arm = L.rmask[lt * 4 + q];
+                    const bool hit = ((cu.w[0] & brm.w[0]) | (cu.w[1] & brm.w[1]) | (cu.w[2] & brm.w[2]) |
+                                      (cu.w[3] & brm.w[3])) != 0;
+                    if (!hit) arm = 0;
+                }
+                // stage the hits in LDS in lane order and spread their
+                // (candidate, A row) pairs over the lanes
+                const bool hit = arm != 0;
+                const uint64_t hmask = __ballot(hit);
+                if (!hmask) continue;
+                const uint32_t nh = __popcll(hmask);
+                const uint32_t cnt = hit ? (uint32_t)__popcll(arm) : 0u;
+                const uint32_t incl = wave_scan_incl(cnt);
+                const uint32_t total = __shfl(incl, 63, 64);
+                if (hit) {
+                    const uint32_t slot = __popcll(hmask & ((1ull << lane) - 1ull));
+                    s_arm[slot] = arm;
+                    s_k[slot] = K;
+                    s_end[slot] = incl;
+                    s_brm[slot] = brm;
+                }
+                hits_n += hit;
+                arows += cnt;
+                __syncthreads();
+                for (uint32_t base = 0; base < total; base += WAVE) {
+                    const uint32_t g = base + lane;
+                    if (g < total) {
+                        uint32_t lo = 0, hi = nh - 1;  // owner: first slot whose range ends past g
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (s_end[mid] > g) hi = mid;
+                            else lo = mid + 1;
+                        }
+                        const uint64_t am = s_arm[lo];
+                        const int r = select_bit(am, g - (s_end[lo] - (uint32_t)__popcll(am)));
+                        const uint32_t Ko = s_k[lo];
+                        const Row4 bm = s_brm[lo];
+                        const size_t lt = (size_t)I * T + Ko, rt = (size_t)Ko * T + J;
+                        const Row4 a = load_row(L.bits + lt * TILE_WORDS + ((size_t)q * WAVE + r) * TROW_WORDS);
+                        const uint64_t *Bt = R.bits + rt * TILE_WORDS;
+                        Row4 v = {{0, 0, 0, 0}};
+#pragma unroll
+                        for (int ww = 0; ww < 4; ++ww) {
+                            uint64_t x = a.w[ww] & bm.w[ww];
+                            while (x) {
+                                // up to four B rows in flight
+                                int ks[4];
+                                int nk = 0;
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) {
+                                    ks[j] = ww * 64 + (x ? __builtin_ctzll(x) : 0);
+                                    if (x) {
+                                        x &= x - 1;
+                                        nk = j + 1;
+                                    }
+                                }
+                                Row4 b[4];
+#pragma unroll
+                                for (int j = 0; j < 4; ++j)
+                                    b[j] = j < nk ? load_row(Bt + ks[j] * TROW_WORDS) : Row4{{0, 0, 0, 0}};
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) or_row(v, b[j]);
+                                brows += nk;
+                            }
+                        }
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            if (v.w[i]) atomicOr((unsigned long long *)&acc_s[r].w[i], (unsigned long long)v.w[i]);
+                    }
+                }
+                __syncthreads();  // the staging arrays are refilled by the next filter round
+            }
+        }
+        __syncthreads();
+        const Row4 acc = acc_s[lane];
+        max_cands = max(max_cands, tp - tp_unit0);
+        const uint32_t row = q * WAVE + lane;
+        const MatDev O = mats[uniform(heads[h].out[parity])];
+        const uint64_t *Mbits = mats[uniform(heads[h].target)].bits;
+        Row4 nw = {{0, 0, 0, 0}};
+        if (row_nonzero(acc)) {
+            const Row4 m = load_row(Mbits + (size_t)p * TILE_WORDS + row * TROW_WORDS);
+            nw.w[0] = acc.w[0] & ~m.w[0];
+            nw.w[1] = acc.w[1] & ~m.w[1];
+            nw.w[2] = acc.w[2] & ~m.w[2];
+            nw.w[3] = acc.w[3] & ~m.w[3];
+            if (row_nonzero(nw)) {
+                store_row(O.bits + (size_t)p * TILE_WORDS + row * TROW_WORDS, nw);
+                ++out_rows;
+            }
+        }
+        const uint64_t bal = __ballot(row_nonzero(nw));
+        const uint32_t pc = wave_sum(row_popc(nw));
+        Row4 cu = {{0, 0, 0, 0}};
+        if (bal) cu = wave_or_row(nw);
+        if (bal && lane == 0) {
+            O.rmask[(size_t)p * 4 + q] = bal;
+            store_row(colq_of(O, p, q), cu);
+            set_occq(O, q, I, J);
+            const uint32_t old = atomicAdd(&O.cnt[p], pc);
+            if (old == 0) {
+                set_occ(O, I, J);
+                touched[atomicAdd(&c->n_touched, 1u)] = ((uint64_t)h << 40) | p;
+            }
+            atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)pc);
+        }
+        __syncthreads();  // the accumulator is reset by the next unit
+    }
+    }
+    const uint32_t ar = wave_sum(arows), br = wave_sum(brows), orow = wave_sum(out_rows), hs = wave_sum(hits_n);
+    if (lane == 0) {
+        if (tp) atomicAdd((unsigned long long *)&c->tile_products, (unsigned long long)tp);
+        if (ar) atomicAdd((unsigned long long *)&c->a_rows, (unsigned long long)ar);
+        if (br) atomicAdd((unsigned long long *)&c->b_rows, (unsigned long long)br);
+        if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
+        if (hs) atomicAdd((unsigned long long *)&c->cand_hits, (unsigned long long)hs);
+        if (max_cands) atomicMax(&c->max_unit_cands, max_cands);
     }
 }
 

@@ -83,6 +83,7 @@ struct StepCounters {
     uint32_t trace0;       // first trace row written by this step
     uint32_t dirty_new;    // tiles that became dirty (deferred) in this step
     uint32_t dirty_cleared; // dirty tiles evaluated in full (or found full) in this step
+    uint32_t max_unit_cands; // most candidate k-tiles of one sparse unit
     uint64_t new_bits;
     uint64_t tile_products;   // (term, C tile, k) products evaluated (sparse: per quarter)
     uint64_t tp_dense;        // of which on the LDS / Four Russians paths
@@ -93,6 +94,7 @@ struct StepCounters {
     uint64_t b_rows_lds;      // B rows OR-ed from LDS
     uint64_t m4r_lookups;     // Four-Russians table lookups
     uint64_t mfma_instr;      // v_mfma_i32_32x32x32_i8 issued (per wave)
+    uint64_t cand_hits;       // sparse candidates that passed the row / column filter
 };
 
 constexpr int RING = 64;  // step-counter ring length

@@ -4,7 +4,8 @@ For a workload it times complete fixpoints (device time from the engine's own
 hipEvents, median of a few runs) while walking the parameters one at a time
 (coordinate descent, best value kept before moving to the next parameter):
 
-  sparse_batch         k-tiles whose loads the sparse product issues together
+  sparse_mode          sparse product: lane-parallel candidates (0) or row-per-lane (1)
+  sparse_batch         k-tiles whose loads the row-per-lane sparse product issues together
   sparse_waves_per_cu  persistent waves of the sparse product per CU
   sparse_chunk         work units a sparse-product wave claims per atomic
   dense_bits           A-tile popcount from which a tile goes to the LDS kernel
@@ -29,6 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned.json")
 
 SPACE = [
+    ("sparse_mode", [0, 1]),
     ("sparse_batch", [1, 2, 4]),
     ("sparse_waves_per_cu", [8, 16, 32]),
     ("sparse_chunk", [1, 4, 16]),
@@ -37,7 +39,7 @@ SPACE = [
     ("mfma_bits", [8192, 16384, 32768, 0xFFFFFFF0]),
     ("iters_per_sync", [2, 4, 8]),
 ]
-DEFAULTS = {"sparse_batch": 2, "sparse_waves_per_cu": 32, "sparse_chunk": 1, "dense_bits": 2048, "m4r_bits": 16384,
+DEFAULTS = {"sparse_mode": 0, "sparse_batch": 2, "sparse_waves_per_cu": 32, "sparse_chunk": 1, "dense_bits": 2048, "m4r_bits": 16384,
             "mfma_bits": 32768, "iters_per_sync": 4}
 
 

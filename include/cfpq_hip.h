@@ -86,7 +86,9 @@ typedef struct cfpq_opts {
                                    (needs weights: cfpq_sched_set_weights / _train)     */
     uint32_t sched_threshold;   /* defer an item whose predicted activity probability is
                                    below sched_threshold / 65536 (0 = library default)   */
-    int32_t reserved[3];
+    int32_t sparse_mode;        /* sparse product: 0 = lane-parallel candidates (default),
+                                   1 = row-per-lane with sparse_batch candidates per round */
+    int32_t reserved[2];
 } cfpq_opts;
 
 typedef struct cfpq_rule {
@@ -118,7 +120,8 @@ typedef struct cfpq_stats {
     uint64_t sched_deferred;    /* (head, tile) items it deferred                        */
     uint64_t sched_full;        /* deferred tiles later evaluated over full operands     */
     uint32_t flush_steps;       /* steps that evaluated every deferred tile (no deferral) */
-    uint32_t pad0;
+    uint32_t max_unit_cands;    /* most candidate k-tiles one sparse work unit visited   */
+    uint64_t cand_hits;         /* sparse candidates that passed the row / column filter */
 } cfpq_stats;
 
 /* ---- lifecycle ---------------------------------------------------------- */
