@@ -554,8 +554,11 @@ static SchedArgs sched_args(const cfpq_ctx *ctx, bool flush) {
     return a;
 }
 
+// fuse: merge new rows into the head matrices inside the product kernels
+// (fixpoint steps); add_mul keeps a separate apply pass so that aliased
+// operands are read as they were on entry.
 static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, int prev_slot, bool full_pass,
-                        int ev_idx, bool flush = false) {
+                        int ev_idx, bool flush = false, bool fuse = true) {
     StepCounters *c = ctx->ring + slot;
     const uint32_t nh = (uint32_t)plan.heads.size();
     const TermDev *terms = ctx->d_terms;
@@ -590,29 +593,36 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const MatDev *dm = ctx->d_mats;
     const HeadDev *dh = ctx->d_heads;
     const uint64_t *un = ctx->units;
+    const int fz = (fuse && ctx->opts.fuse_apply) ? 1 : 0;
+    fuse = fz != 0;
     if (ctx->opts.sparse_mode == 0) {
-        hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk);
+        hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
+                           fz);
     } else if (ctx->opts.sparse_batch == 1) {
-        hipLaunchKernelGGL(k_product_sparse<1>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk);
+        hipLaunchKernelGGL(k_product_sparse<1>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
+                           fz);
     } else if (ctx->opts.sparse_batch == 2) {
-        hipLaunchKernelGGL(k_product_sparse<2>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk);
+        hipLaunchKernelGGL(k_product_sparse<2>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
+                           fz);
     } else {
-        hipLaunchKernelGGL(k_product_sparse<4>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk);
+        hipLaunchKernelGGL(k_product_sparse<4>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
+                           fz);
     }
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 1], ctx->stream));
     hipLaunchKernelGGL(k_product_dense, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                        (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
-                       dense_threshold(ctx->opts), m4r_threshold(ctx->opts));
+                       dense_threshold(ctx->opts), m4r_threshold(ctx->opts), fz);
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 2], ctx->stream));
     hipLaunchKernelGGL(k_product_mfma, dim3((uint32_t)ctx->num_cu), dim3(MFMA_BLOCK), 0, ctx->stream,
                        (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, terms, q, c,
-                       (const uint64_t *)ctx->mitems, touched);
+                       (const uint64_t *)ctx->mitems, touched, fz);
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 3], ctx->stream));
     if (use_sched && ctx->tr_cap)
         hipLaunchKernelGGL(k_trace_label, dim3((uint32_t)ctx->num_cu), dim3(256), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, (const StepCounters *)c, sa);
-    hipLaunchKernelGGL(k_apply, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                       (const HeadDev *)ctx->d_heads, q, c, (const uint64_t *)touched);
+    if (!fuse)
+        hipLaunchKernelGGL(k_apply, dim3((uint32_t)ctx->num_cu * 8), dim3(WAVE), 0, ctx->stream,
+                           (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, c, (const uint64_t *)touched);
     // retire the delta this step consumed (buffer D_q)
     const uint32_t *ids_q = ctx->d_ids + (q ? plan.all_delta[0].size() : 0);
     const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
@@ -623,9 +633,9 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                                               std::min<uint32_t>(nids_q, 65535)),
                            dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats, ids_q, nids_q);
     } else {
-        hipLaunchKernelGGL(k_clear, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                           (const HeadDev *)ctx->d_heads, 1 - q, (const StepCounters *)(ctx->ring + prev_slot), c,
-                           (const uint64_t *)ctx->lists[prev_slot & 1]);
+        hipLaunchKernelGGL(k_clear, dim3((uint32_t)ctx->num_cu * 8), dim3(WAVE), 0, ctx->stream,
+                           (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, 1 - q,
+                           (const StepCounters *)(ctx->ring + prev_slot), c, (const uint64_t *)ctx->lists[prev_slot & 1]);
         hipLaunchKernelGGL(k_clear_occ, dim3(4, std::min<uint32_t>(nids_q, 65535)), dim3(256), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, ids_q, nids_q);
     }
@@ -720,6 +730,7 @@ static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &p
             plan.all_delta[0].size(), plan.all_delta[1].size(), plan.max_tiles_per_head, plan.max_occ_words,
             plan.ids_all_off, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
             (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk, (uint64_t)o.sched, o.sched_threshold,
+            (uint64_t)o.sparse_mode, (uint64_t)o.fuse_apply,
             (uint64_t)ctx->sched_have, (uint64_t)(uintptr_t)ctx->sched_w, (uint64_t)(uintptr_t)ctx->dirty,
             (uint64_t)(uintptr_t)ctx->dirty_off, (uint64_t)ctx->tr_cap, (uint64_t)(uintptr_t)ctx->tr_x};
 }
@@ -874,9 +885,10 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
         const uint32_t *ids_q = ctx->d_ids + (q ? nd : 0);
         const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
-        hipLaunchKernelGGL(k_clear, dim3(ctx->num_cu * 4), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                           (const HeadDev *)ctx->d_heads, 1 - q, (const StepCounters *)(ctx->ring + prev_slot),
-                           ctx->ring + slot, (const uint64_t *)ctx->lists[prev_slot & 1]);
+        hipLaunchKernelGGL(k_clear, dim3((uint32_t)ctx->num_cu * 8), dim3(WAVE), 0, ctx->stream,
+                           (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, 1 - q,
+                           (const StepCounters *)(ctx->ring + prev_slot), ctx->ring + slot,
+                           (const uint64_t *)ctx->lists[prev_slot & 1]);
         hipLaunchKernelGGL(k_clear_occ, dim3(4, std::min<uint32_t>(nids_q, 65535)), dim3(256), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, ids_q, nids_q);
         CHK(launch_check(ctx, "final clear"));
@@ -1054,7 +1066,7 @@ extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const
     plan.all_delta[1] = {3};
     plan.max_tiles_per_head = plan.total_head_tiles = (size_t)c->T * c->T;
     int rc = upload_plan(ctx, plan);
-    if (rc == CFPQ_OK) rc = enqueue_step(ctx, plan, 0, 0, -1, false, -1);
+    if (rc == CFPQ_OK) rc = enqueue_step(ctx, plan, 0, 0, -1, false, -1, false, /*fuse=*/false);
     // enqueue_step cleared nothing useful for S yet (prev_slot < 0 clears all
     // non-empty tiles of "D_0" = S *after* apply), which is exactly the reset we need.
     if (rc == CFPQ_OK) {

@@ -79,6 +79,21 @@ __device__ __forceinline__ uint32_t wg_next(uint32_t *cursor, uint32_t *s_idx) {
     return *s_idx;
 }
 
+// Work-group-uniform copy of a value read from memory that other
+// work-groups may be updating during this kernel (the head matrices'
+// occupancy and counts, with the head update fused into the products): every
+// wave must see the same value, or the waves take different paths and their
+// barriers no longer pair up.  Thread 0's read is broadcast through one of two
+// LDS slots, alternating with `n` (a per-thread call count), which lets one
+// barrier per call suffice: a slot is rewritten only after every thread has
+// passed the next call's barrier, i.e. after it has read the slot.
+__device__ __forceinline__ uint32_t wg_bcast(uint32_t v, uint32_t *slots, uint32_t &n) {
+    uint32_t *slot = slots + (n++ & 1u);
+    if (threadIdx.x == 0) *slot = v;
+    __syncthreads();
+    return *slot;
+}
+
 // Wavefront-wide cursor (single-wave work-groups; no barrier involved):
 // claims `chunk` consecutive indices per atomic.
 __device__ __forceinline__ uint32_t wave_next(uint32_t *cursor, uint32_t chunk) {
@@ -121,6 +136,62 @@ __device__ __forceinline__ void set_occ(const MatDev &M, uint32_t I, uint32_t J)
 }
 __device__ __forceinline__ void set_occq(const MatDev &M, uint32_t q, uint32_t I, uint32_t J) {
     atomicOr(&M.occQ[(q * M.T + I) * M.ow + (J >> 5)], 1u << (J & 31));
+}
+
+// Epilogue of one 64-row quarter q of C tile p = (I, J) of a head: the new
+// rows (acc & ~M) go to the output delta O and, with `fuse`, are OR-ed into
+// the head matrix M right away (fixpoint steps only: an operand read later in
+// the same step may then already hold them, which only adds facts that are
+// true, while every new row still reaches the next step through O).  Each
+// (head, tile, quarter) has exactly one writer per step, so the row and the
+// quarter's metadata are updated without atomics; the tile count is shared
+// by the quarters.  Whole wave; lane r owns row 64q + r.  Returns the
+// quarter's row mask and new-bit count (all lanes).
+struct QuarterOut {
+    uint64_t bal;
+    uint32_t pc;
+};
+__device__ __forceinline__ QuarterOut quarter_epilogue(const MatDev &O, const MatDev &M, uint32_t p, uint32_t q,
+                                                       uint32_t I, uint32_t J, const Row4 &acc, int fuse,
+                                                       uint32_t &out_rows) {
+    const int lane = threadIdx.x & 63;
+    const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
+    Row4 nw = {{0, 0, 0, 0}};
+    if (row_nonzero(acc)) {
+        Row4 m = load_row(M.bits + off);
+        nw.w[0] = acc.w[0] & ~m.w[0];
+        nw.w[1] = acc.w[1] & ~m.w[1];
+        nw.w[2] = acc.w[2] & ~m.w[2];
+        nw.w[3] = acc.w[3] & ~m.w[3];
+        if (row_nonzero(nw)) {
+            store_row(O.bits + off, nw);
+            if (fuse) {
+                or_row(m, nw);
+                store_row(M.bits + off, m);
+            }
+            ++out_rows;
+        }
+    }
+    const uint64_t bal = __ballot(row_nonzero(nw));
+    const uint32_t pc = wave_sum(row_popc(nw));
+    if (bal) {
+        const Row4 cu = wave_or_row(nw);
+        if (lane == 0) {
+            O.rmask[(size_t)p * 4 + q] = bal;
+            store_row(colq_of(O, p, q), cu);
+            set_occq(O, q, I, J);
+            if (fuse) {
+                const uint64_t old = M.rmask[(size_t)p * 4 + q];
+                M.rmask[(size_t)p * 4 + q] = old | bal;
+                Row4 mc = load_row(colq_of(M, p, q));
+                or_row(mc, cu);
+                store_row(colq_of(M, p, q), mc);
+                if (!old) set_occq(M, q, I, J);
+                if (atomicAdd(&M.cnt[p], pc) == 0) set_occ(M, I, J);
+            }
+        }
+    }
+    return QuarterOut{bal, pc};
 }
 
 // ---------------------------------------------------------------------------
@@ -229,7 +300,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
                                                          const TermDev *__restrict__ terms, int parity,
                                                          StepCounters *__restrict__ c,
                                                          const uint64_t *__restrict__ units,
-                                                         uint64_t *__restrict__ touched, uint32_t chunk) {
+                                                         uint64_t *__restrict__ touched, uint32_t chunk,
+                                                         int fuse) {
     const int lane = threadIdx.x & 63;
     const uint32_t n_units = c->n_units;
     uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0, hits = 0, max_cands = 0;
@@ -315,33 +387,15 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
         }
         max_cands = max(max_cands, tp - tp_unit0);
         const MatDev O = mats[uniform(heads[h].out[parity])];
-        const uint64_t *Mbits = mats[uniform(heads[h].target)].bits;
-        Row4 nw = {{0, 0, 0, 0}};
-        if (row_nonzero(acc)) {
-            const Row4 m = load_row(Mbits + (size_t)p * TILE_WORDS + row * TROW_WORDS);
-            nw.w[0] = acc.w[0] & ~m.w[0];
-            nw.w[1] = acc.w[1] & ~m.w[1];
-            nw.w[2] = acc.w[2] & ~m.w[2];
-            nw.w[3] = acc.w[3] & ~m.w[3];
-            if (row_nonzero(nw)) {
-                store_row(O.bits + (size_t)p * TILE_WORDS + row * TROW_WORDS, nw);
-                ++out_rows;
-            }
-        }
-        const uint64_t bal = __ballot(row_nonzero(nw));
-        const uint32_t pc = wave_sum(row_popc(nw));
-        Row4 cu = {{0, 0, 0, 0}};
-        if (bal) cu = wave_or_row(nw);
-        if (bal && lane == 0) {
-            O.rmask[(size_t)p * 4 + q] = bal;
-            store_row(colq_of(O, p, q), cu);
-            set_occq(O, q, I, J);
-            const uint32_t old = atomicAdd(&O.cnt[p], pc);
+        const MatDev M = mats[uniform(heads[h].target)];
+        const QuarterOut qo = quarter_epilogue(O, M, p, q, I, J, acc, fuse, out_rows);
+        if (qo.bal && lane == 0) {
+            const uint32_t old = atomicAdd(&O.cnt[p], qo.pc);
             if (old == 0) {
                 set_occ(O, I, J);
                 touched[atomicAdd(&c->n_touched, 1u)] = ((uint64_t)h << 40) | p;
             }
-            atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)pc);
+            atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)qo.pc);
         }
     }
     }
@@ -401,7 +455,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                                             const TermDev *__restrict__ terms, int parity,
                                                             StepCounters *__restrict__ c,
                                                             const uint64_t *__restrict__ units,
-                                                            uint64_t *__restrict__ touched, uint32_t chunk) {
+                                                            uint64_t *__restrict__ touched, uint32_t chunk,
+                                                            int fuse) {
     __shared__ Row4 acc_s[WAVE];
     __shared__ Row4 s_brm[WAVE];
     __shared__ uint64_t s_arm[WAVE];
@@ -519,35 +574,16 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         __syncthreads();
         const Row4 acc = acc_s[lane];
         max_cands = max(max_cands, tp - tp_unit0);
-        const uint32_t row = q * WAVE + lane;
         const MatDev O = mats[uniform(heads[h].out[parity])];
-        const uint64_t *Mbits = mats[uniform(heads[h].target)].bits;
-        Row4 nw = {{0, 0, 0, 0}};
-        if (row_nonzero(acc)) {
-            const Row4 m = load_row(Mbits + (size_t)p * TILE_WORDS + row * TROW_WORDS);
-            nw.w[0] = acc.w[0] & ~m.w[0];
-            nw.w[1] = acc.w[1] & ~m.w[1];
-            nw.w[2] = acc.w[2] & ~m.w[2];
-            nw.w[3] = acc.w[3] & ~m.w[3];
-            if (row_nonzero(nw)) {
-                store_row(O.bits + (size_t)p * TILE_WORDS + row * TROW_WORDS, nw);
-                ++out_rows;
-            }
-        }
-        const uint64_t bal = __ballot(row_nonzero(nw));
-        const uint32_t pc = wave_sum(row_popc(nw));
-        Row4 cu = {{0, 0, 0, 0}};
-        if (bal) cu = wave_or_row(nw);
-        if (bal && lane == 0) {
-            O.rmask[(size_t)p * 4 + q] = bal;
-            store_row(colq_of(O, p, q), cu);
-            set_occq(O, q, I, J);
-            const uint32_t old = atomicAdd(&O.cnt[p], pc);
+        const MatDev M = mats[uniform(heads[h].target)];
+        const QuarterOut qo = quarter_epilogue(O, M, p, q, I, J, acc, fuse, out_rows);
+        if (qo.bal && lane == 0) {
+            const uint32_t old = atomicAdd(&O.cnt[p], qo.pc);
             if (old == 0) {
                 set_occ(O, I, J);
                 touched[atomicAdd(&c->n_touched, 1u)] = ((uint64_t)h << 40) | p;
             }
-            atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)pc);
+            atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)qo.pc);
         }
         __syncthreads();  // the accumulator is reset by the next unit
     }
@@ -654,11 +690,13 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
                                                          StepCounters *__restrict__ c,
                                                          const uint64_t *__restrict__ items,
                                                          uint64_t *__restrict__ touched, uint32_t dense_bits,
-                                                         uint32_t m4r_bits) {
+                                                         uint32_t m4r_bits, int fuse) {
     __shared__ Row4 Bs[TILE];
     __shared__ Row4 Tab[M4R_GROUPS * 16];
     __shared__ uint32_t s_idx;
     __shared__ uint32_t s_sum[4];
+    __shared__ uint32_t s_bc[2];
+    uint32_t nbc = 0;
     const int r = threadIdx.x, wv = r >> 6, lane = r & 63;
     const uint32_t n_items = c->n_dense;
     uint32_t tp = 0, tpd = 0, arows = 0, brows = 0, brows_lds = 0, out_rows = 0, lookups = 0;
@@ -675,7 +713,7 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
             const MatDev L = mats[uniform(terms[t].left[parity])];
             const MatDev R = mats[uniform(terms[t].right[parity])];
             for (uint32_t w = 0; w < L.ow; ++w) {
-                uint32_t kb = uniform(L.occR[I * L.ow + w] & R.occC[J * R.ow + w]);
+                uint32_t kb = wg_bcast(L.occR[I * L.ow + w] & R.occC[J * R.ow + w], s_bc, nbc);
                 while (kb) {
                     const uint32_t K = w * 32 + __builtin_ctz(kb);
                     kb &= kb - 1;
@@ -684,7 +722,7 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
                     const uint64_t *Bt = R.bits + (size_t)rt * TILE_WORDS;
                     const uint64_t rm = L.rmask[(size_t)lt * 4 + wv];
                     ++tp;
-                    const uint32_t acnt = uniform(L.cnt[lt]);
+                    const uint32_t acnt = wg_bcast(L.cnt[lt], s_bc, nbc);
                     if (acnt >= m4r_bits) {
                         ++tpd;
                         tp_m4r(At, Bt, rm, r, Bs, Tab, acc, arows, lookups);
@@ -698,28 +736,9 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
             }
         }
         const MatDev O = mats[uniform(heads[h].out[parity])];
-        const uint64_t *Mbits = mats[uniform(heads[h].target)].bits;
-        const size_t row_off = (size_t)p * TILE_WORDS + r * TROW_WORDS;
-        Row4 nw = {{0, 0, 0, 0}};
-        if (row_nonzero(acc)) {
-            const Row4 m = load_row(Mbits + row_off);
-            nw.w[0] = acc.w[0] & ~m.w[0];
-            nw.w[1] = acc.w[1] & ~m.w[1];
-            nw.w[2] = acc.w[2] & ~m.w[2];
-            nw.w[3] = acc.w[3] & ~m.w[3];
-        }
-        const bool nz = row_nonzero(nw);
-        if (nz) store_row(O.bits + row_off, nw);
-        out_rows += nz;
-        const uint64_t bal = __ballot(nz);
-        const uint32_t pc = wave_sum(row_popc(nw));
-        const Row4 cu = wave_or_row(nw);
-        if (lane == 0) {
-            O.rmask[(size_t)p * 4 + wv] = bal;
-            store_row(colq_of(O, p, wv), cu);
-            if (bal) set_occq(O, wv, I, J);
-            s_sum[wv] = pc;
-        }
+        const MatDev M = mats[uniform(heads[h].target)];
+        const QuarterOut qo = quarter_epilogue(O, M, p, wv, I, J, acc, fuse, out_rows);
+        if (lane == 0) s_sum[wv] = qo.pc;
         __syncthreads();
         if (r == 0) {
             const uint32_t total = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
@@ -815,11 +834,13 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
                                                              const TermDev *__restrict__ terms, int parity,
                                                              StepCounters *__restrict__ c,
                                                              const uint64_t *__restrict__ items,
-                                                             uint64_t *__restrict__ touched) {
+                                                             uint64_t *__restrict__ touched, int fuse) {
     __shared__ __attribute__((aligned(16))) uint32_t BX[TILE * TILE / 4];  // 64 KiB
     __shared__ Row4 Bs[TILE];  // B tile bits; the result bits in the epilogue
     __shared__ uint32_t s_idx;
     __shared__ uint32_t s_sum[4];
+    __shared__ uint32_t s_bc[2];
+    uint32_t nbc = 0;
     const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
     const int rs = wv & 3, ch = wv >> 2;
     const uint32_t n_items = c->n_mfma;
@@ -841,7 +862,7 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
             const MatDev L = mats[uniform(terms[tt].left[parity])];
             const MatDev R = mats[uniform(terms[tt].right[parity])];
             for (uint32_t w = 0; w < L.ow; ++w) {
-                uint32_t kb = uniform(L.occR[I * L.ow + w] & R.occC[J * R.ow + w]);
+                uint32_t kb = wg_bcast(L.occR[I * L.ow + w] & R.occC[J * R.ow + w], s_bc, nbc);
                 while (kb) {
                     const uint32_t K = w * 32 + __builtin_ctz(kb);
                     kb &= kb - 1;
@@ -914,28 +935,9 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
         if (t < TILE) {
             const Row4 accb = Bs[t];
             const MatDev O = mats[uniform(heads[h].out[parity])];
-            const uint64_t *Mbits = mats[uniform(heads[h].target)].bits;
-            const size_t row_off = (size_t)p * TILE_WORDS + t * TROW_WORDS;
-            Row4 nw = {{0, 0, 0, 0}};
-            if (row_nonzero(accb)) {
-                const Row4 m = load_row(Mbits + row_off);
-                nw.w[0] = accb.w[0] & ~m.w[0];
-                nw.w[1] = accb.w[1] & ~m.w[1];
-                nw.w[2] = accb.w[2] & ~m.w[2];
-                nw.w[3] = accb.w[3] & ~m.w[3];
-            }
-            const bool nz = row_nonzero(nw);
-            if (nz) store_row(O.bits + row_off, nw);
-            out_rows += nz;
-            const uint64_t bal = __ballot(nz);
-            const uint32_t pc = wave_sum(row_popc(nw));
-            const Row4 cu = wave_or_row(nw);
-            if (lane == 0) {
-                O.rmask[(size_t)p * 4 + wv] = bal;
-                store_row(colq_of(O, p, wv), cu);
-                if (bal) set_occq(O, wv, I, J);
-                s_sum[wv] = pc;
-            }
+            const MatDev M = mats[uniform(heads[h].target)];
+            const QuarterOut qo = quarter_epilogue(O, M, p, wv, I, J, accb, fuse, out_rows);
+            if (lane == 0) s_sum[wv] = qo.pc;
         }
         __syncthreads();
         if (t == 0) {
@@ -964,37 +966,38 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
 // apply / clear
 // ---------------------------------------------------------------------------
 
-// M[head] |= next-delta tile, for every tile a product kernel wrote.
-__global__ __launch_bounds__(BLOCK) void k_apply(const MatDev *__restrict__ mats,
-                                                 const HeadDev *__restrict__ heads, int parity,
-                                                 StepCounters *__restrict__ c,
-                                                 const uint64_t *__restrict__ touched) {
-    __shared__ uint32_t s_idx;
-    const int r = threadIdx.x, wv = r >> 6, lane = r & 63;
-    const uint32_t n = c->n_touched;
-    for (uint32_t idx = wg_next(&c->apply_next, &s_idx); idx < n; idx = wg_next(&c->apply_next, &s_idx)) {
-        const uint64_t it = touched[idx];
+// M[head] |= next-delta tile, for every tile a product kernel wrote: one
+// wave per (tile, quarter); the quarter-0 wave also merges the tile count.
+__global__ __launch_bounds__(WAVE) void k_apply(const MatDev *__restrict__ mats,
+                                                const HeadDev *__restrict__ heads, int parity,
+                                                StepCounters *__restrict__ c,
+                                                const uint64_t *__restrict__ touched) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = c->n_touched * QUARTERS;
+    for (uint32_t idx = blockIdx.x; idx < n; idx += gridDim.x) {  // static: the units are small and alike
+        const uint64_t it = touched[idx >> 2];
+        const uint32_t q = idx & 3u;
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
         const MatDev M = mats[uniform(heads[h].target)];
         const MatDev O = mats[uniform(heads[h].out[parity])];
         const uint32_t T = uniform(heads[h].T), I = p / T, J = p - (p / T) * T;
-        const uint64_t rm = O.rmask[(size_t)p * 4 + wv];
+        const uint64_t rm = O.rmask[(size_t)p * 4 + q];
         if ((rm >> lane) & 1ull) {
-            const size_t off = (size_t)p * TILE_WORDS + r * TROW_WORDS;
+            const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
             Row4 m = load_row(M.bits + off);
             or_row(m, load_row(O.bits + off));
             store_row(M.bits + off, m);
         }
         if (lane == 0 && rm) {
-            const uint64_t old = M.rmask[(size_t)p * 4 + wv];
-            M.rmask[(size_t)p * 4 + wv] = old | rm;
-            Row4 cu = load_row(colq_of(M, p, wv));
-            or_row(cu, load_row(colq_of(O, p, wv)));
-            store_row(colq_of(M, p, wv), cu);
-            if (!old) set_occq(M, wv, I, J);
+            const uint64_t old = M.rmask[(size_t)p * 4 + q];
+            M.rmask[(size_t)p * 4 + q] = old | rm;
+            Row4 cu = load_row(colq_of(M, p, q));
+            or_row(cu, load_row(colq_of(O, p, q)));
+            store_row(colq_of(M, p, q), cu);
+            if (!old) set_occq(M, q, I, J);
         }
-        if (r == 0) {
+        if (lane == 0 && q == 0) {
             const uint32_t old = M.cnt[p];
             M.cnt[p] = old + O.cnt[p];
             if (!old) set_occ(M, I, J);
@@ -1003,28 +1006,29 @@ __global__ __launch_bounds__(BLOCK) void k_apply(const MatDev *__restrict__ mats
 }
 
 // Zero the delta tiles listed in `list` (written by the step of parity
-// `parity`): rows named by rmask, then rmask and cnt.
-__global__ __launch_bounds__(BLOCK) void k_clear(const MatDev *__restrict__ mats,
-                                                 const HeadDev *__restrict__ heads, int parity,
-                                                 const StepCounters *__restrict__ list_ctr,
-                                                 StepCounters *__restrict__ c,
-                                                 const uint64_t *__restrict__ list) {
-    __shared__ uint32_t s_idx;
-    const int r = threadIdx.x, wv = r >> 6, lane = r & 63;
-    const uint32_t n = list_ctr->n_touched;
-    for (uint32_t idx = wg_next(&c->clear_next, &s_idx); idx < n; idx = wg_next(&c->clear_next, &s_idx)) {
-        const uint64_t it = list[idx];
+// `parity`): one wave per (tile, quarter) -- rows named by the quarter's row
+// mask, then the quarter's metadata; the quarter-0 wave also zeroes the count.
+__global__ __launch_bounds__(WAVE) void k_clear(const MatDev *__restrict__ mats,
+                                                const HeadDev *__restrict__ heads, int parity,
+                                                const StepCounters *__restrict__ list_ctr,
+                                                StepCounters *__restrict__ c,
+                                                const uint64_t *__restrict__ list) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = list_ctr->n_touched * QUARTERS;
+    for (uint32_t idx = blockIdx.x; idx < n; idx += gridDim.x) {  // static: the units are small and alike
+        const uint64_t it = list[idx >> 2];
+        const uint32_t q = idx & 3u;
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
         const MatDev O = mats[uniform(heads[h].out[parity])];
-        const uint64_t rm = O.rmask[(size_t)p * 4 + wv];
-        if ((rm >> lane) & 1ull) store_row(O.bits + (size_t)p * TILE_WORDS + r * TROW_WORDS, Row4{{0, 0, 0, 0}});
-        __syncthreads();
-        if (lane == 0) {
-            O.rmask[(size_t)p * 4 + wv] = 0;
-            store_row(colq_of(O, p, wv), Row4{{0, 0, 0, 0}});
+        const uint64_t rm = O.rmask[(size_t)p * 4 + q];
+        if ((rm >> lane) & 1ull)
+            store_row(O.bits + (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS, Row4{{0, 0, 0, 0}});
+        if (lane == 0 && rm) {
+            O.rmask[(size_t)p * 4 + q] = 0;
+            store_row(colq_of(O, p, q), Row4{{0, 0, 0, 0}});
         }
-        if (r == 0) O.cnt[p] = 0;
+        if (lane == 0 && q == 0) O.cnt[p] = 0;
     }
 }
 

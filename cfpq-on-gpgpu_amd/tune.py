@@ -12,6 +12,7 @@ hipEvents, median of a few runs) while walking the parameters one at a time
   m4r_bits             A-tile popcount from which the Four Russians tables are used
   mfma_bits            A-tile popcount from which a tile goes to the int8 MFMA kernel
   iters_per_sync       fixpoint steps enqueued per host read of the counters
+  fuse_apply           merge new rows into the head matrices inside the products
 
 Every candidate is checked against the first run's fingerprints, so a tuning
 run is also a parity run.  Results are cached in ``tuned.json`` next to this
@@ -38,9 +39,10 @@ SPACE = [
     ("m4r_bits", [8192, 16384, 32768, 0xFFFFFFF0]),
     ("mfma_bits", [8192, 16384, 32768, 0xFFFFFFF0]),
     ("iters_per_sync", [2, 4, 8]),
+    ("fuse_apply", [0, 1]),
 ]
 DEFAULTS = {"sparse_mode": 0, "sparse_batch": 2, "sparse_waves_per_cu": 32, "sparse_chunk": 1, "dense_bits": 2048, "m4r_bits": 16384,
-            "mfma_bits": 32768, "iters_per_sync": 4}
+            "mfma_bits": 32768, "iters_per_sync": 4, "fuse_apply": 0}
 
 
 def workload_key(n: int, grammar_text: str) -> str:

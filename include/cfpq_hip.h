@@ -88,7 +88,9 @@ typedef struct cfpq_opts {
                                    below sched_threshold / 65536 (0 = library default)   */
     int32_t sparse_mode;        /* sparse product: 0 = lane-parallel candidates (default),
                                    1 = row-per-lane with sparse_batch candidates per round */
-    int32_t reserved[2];
+    int32_t fuse_apply;         /* 1: fixpoint steps merge new rows into the head matrices
+                                   inside the product kernels (no separate apply pass)  */
+    int32_t reserved[1];
 } cfpq_opts;
 
 typedef struct cfpq_rule {

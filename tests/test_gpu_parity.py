@@ -279,3 +279,14 @@ def test_fixpoint_graph_replay(pkg, gen, oracle, which):
         for c in init:
             c.free()
         gr.free()
+
+
+@pytest.mark.parametrize("which", ["dense", "dyck", "sg"])
+def test_fixpoint_fused_apply(pkg, gen, oracle, which):
+    """Head matrices updated inside the product kernels: new rows become
+    visible to later units of the same step; the least fixpoint is unchanged."""
+    q = {"dense": lambda: gen.dense_stress(1500), "dyck": lambda: gen.dyck_er(1200, 4000, seed=9),
+         "sg": lambda: gen.sg_forest(8000, seed=10, types_per_vertex=4)}[which]()
+    for kernel in ("auto", "gather", "m4r", "mfma"):
+        _fixpoint_vs_oracle(pkg, gen, oracle, q, kernel=kernel, fuse_apply=True, dense_bits=64, m4r_bits=4096,
+                            mfma_bits=30000)
