@@ -87,15 +87,20 @@ def product_roofline(stats, wname):
                 "frac": round(achieved / MFMA_I8_PEAK_TOPS, 5), "traffic": None, "kernel": "k_product_mfma",
                 "ops": "int8 MACs x2", "launch_ms": round(per_launch_ms, 5),
                 "alg_ops_per_launch": round(ops / launches), "executed_ops_per_launch": round(executed / launches)}
-    kern = "k_product_sparse" if t_dense == 0 else "k_product_sparse+k_product_dense"
-    alg_bytes = sum(32 * (s["a_rows"] + s["b_rows"] + s["out_rows"]) + 8192 * s["tile_products_dense"] for s in stats)
-    per_launch_ms = (t_sparse + t_dense) / launches
+    if t_sparse >= t_dense:
+        kern, t_k = "k_product_sparse", t_sparse
+        alg_bytes = sum(32 * (s["a_rows"] + s["b_rows"] + s["out_rows"]) for s in stats)
+    else:
+        kern, t_k = "k_product_dense", t_dense
+        alg_bytes = sum(32 * (s["a_rows_dense"] + s["b_rows_dense"] + s["out_rows_dense"]) +
+                        8192 * s["tile_products_dense"] for s in stats)
+    per_launch_ms = t_k / launches
     per_launch_bytes = alg_bytes / launches
     achieved = per_launch_bytes / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
     traffic = None
     if os.path.exists(TRAFFIC):
         t = json.load(open(TRAFFIC))
-        if t.get("workload") == wname:
+        if t.get("workload") == wname and t.get("kernel") == kern:
             traffic = t.get("bytes_per_launch")
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kern,
@@ -259,7 +264,7 @@ def main():
         "work": {k: s0[k] for k in ("items", "tile_products", "tile_products_dense", "tile_products_mfma",
                                     "a_rows", "b_rows", "b_rows_lds", "m4r_lookups", "mfma_instr", "out_rows",
                                     "new_bits", "product_launches", "host_syncs", "cand_hits",
-                                    "max_unit_cands")},
+                                    "max_unit_cands", "a_rows_dense", "b_rows_dense", "out_rows_dense")},
         "kernel_ms": {k: round(s0[k], 4) for k in ("sparse_ms", "dense_ms", "mfma_ms")},
         "engine_opts": opts,
         "engines": engines,

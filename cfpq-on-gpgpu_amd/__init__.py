@@ -83,6 +83,9 @@ class Stats(ctypes.Structure):
         ("flush_steps", ctypes.c_uint32),
         ("max_unit_cands", ctypes.c_uint32),
         ("cand_hits", ctypes.c_uint64),
+        ("a_rows_dense", ctypes.c_uint64),
+        ("b_rows_dense", ctypes.c_uint64),
+        ("out_rows_dense", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -496,7 +496,7 @@ struct Plan {
 
 struct StepStats {
     uint64_t new_bits = 0, items = 0, tp = 0, tpd = 0, tpm = 0, arows = 0, brows = 0, orows = 0, brows_lds = 0,
-             lookups = 0, mfma_instr = 0, cand_hits = 0;
+             lookups = 0, mfma_instr = 0, cand_hits = 0, arows_d = 0, brows_d = 0, orows_d = 0;
     uint32_t max_unit_cands = 0;
 };
 
@@ -697,6 +697,9 @@ static void accumulate(StepStats &s, const StepCounters &c) {
     s.items += c.n_units + c.n_dense + c.n_mfma;
     s.mfma_instr += c.mfma_instr;
     s.cand_hits += c.cand_hits;
+    s.arows_d += c.a_rows_dense;
+    s.brows_d += c.b_rows_dense;
+    s.orows_d += c.out_rows_dense;
     s.max_unit_cands = std::max(s.max_unit_cands, c.max_unit_cands);
     s.tp += c.tile_products;
     s.tpd += c.tp_dense;
@@ -923,6 +926,9 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         stats->sched_full = sched_full;
         stats->flush_steps = flush_steps;
         stats->cand_hits = tot.cand_hits;
+        stats->a_rows_dense = tot.arows_d;
+        stats->b_rows_dense = tot.brows_d;
+        stats->out_rows_dense = tot.orows_d;
         stats->max_unit_cands = tot.max_unit_cands;
     }
     return CFPQ_OK;

@@ -753,9 +753,9 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
     const uint32_t ar = wave_sum(arows), br = wave_sum(brows), orow = wave_sum(out_rows);
     const uint32_t brl = wave_sum(brows_lds), lk = wave_sum(lookups);
     if (lane == 0) {
-        if (ar) atomicAdd((unsigned long long *)&c->a_rows, (unsigned long long)ar);
-        if (br) atomicAdd((unsigned long long *)&c->b_rows, (unsigned long long)br);
-        if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
+        if (ar) atomicAdd((unsigned long long *)&c->a_rows_dense, (unsigned long long)ar);
+        if (br) atomicAdd((unsigned long long *)&c->b_rows_dense, (unsigned long long)br);
+        if (orow) atomicAdd((unsigned long long *)&c->out_rows_dense, (unsigned long long)orow);
         if (brl) atomicAdd((unsigned long long *)&c->b_rows_lds, (unsigned long long)brl);
         if (lk) atomicAdd((unsigned long long *)&c->m4r_lookups, (unsigned long long)lk);
     }
@@ -953,7 +953,7 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
     }
     const uint32_t orow = wave_sum(out_rows);
     if (lane == 0) {
-        if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
+        if (orow) atomicAdd((unsigned long long *)&c->out_rows_dense, (unsigned long long)orow);
         if (instr) atomicAdd((unsigned long long *)&c->mfma_instr, (unsigned long long)instr);
     }
     if (t == 0 && tp) {

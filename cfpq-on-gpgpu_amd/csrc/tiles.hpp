@@ -95,6 +95,9 @@ struct StepCounters {
     uint64_t m4r_lookups;     // Four-Russians table lookups
     uint64_t mfma_instr;      // v_mfma_i32_32x32x32_i8 issued (per wave)
     uint64_t cand_hits;       // sparse candidates that passed the row / column filter
+    uint64_t a_rows_dense;    // as a_rows / b_rows / out_rows, for the dense and MFMA kernels
+    uint64_t b_rows_dense;    // (a_rows / b_rows / out_rows count the sparse kernels only)
+    uint64_t out_rows_dense;
 };
 
 constexpr int RING = 64;  // step-counter ring length

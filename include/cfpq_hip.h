@@ -105,11 +105,11 @@ typedef struct cfpq_stats {
     uint64_t tile_products;     /* (rule, C tile, k) tile-products evaluated, all kernels */
     uint64_t tile_products_dense; /* of which on the LDS / M4R paths                     */
     uint64_t tile_products_mfma;  /* of which on the MFMA path                           */
-    uint64_t a_rows;            /* non-empty A-tile rows visited                         */
-    uint64_t b_rows;            /* B rows gathered from HBM/L2 (one per set A bit, sparse path) */
-    uint64_t b_rows_lds;        /* B rows OR-ed from LDS (dense path)                    */
+    uint64_t a_rows;            /* non-empty A-tile rows visited (sparse kernels)        */
+    uint64_t b_rows;            /* B rows gathered from HBM/L2 (sparse kernels)          */
+    uint64_t b_rows_lds;        /* B rows OR-ed from LDS (dense kernel)                  */
     uint64_t m4r_lookups;       /* Four Russians table lookups (32 B each)               */
-    uint64_t out_rows;          /* C-tile rows that received new bits                    */
+    uint64_t out_rows;          /* C-tile rows that received new bits (sparse kernels)   */
     double fixpoint_ms;         /* device time of the whole call (hipEvent)              */
     double product_ms;          /* device time inside the product kernel (hipEvent)      */
     uint32_t product_launches;  /* fixpoint steps whose product kernels were timed       */
@@ -124,6 +124,9 @@ typedef struct cfpq_stats {
     uint32_t flush_steps;       /* steps that evaluated every deferred tile (no deferral) */
     uint32_t max_unit_cands;    /* most candidate k-tiles one sparse work unit visited   */
     uint64_t cand_hits;         /* sparse candidates that passed the row / column filter */
+    uint64_t a_rows_dense;      /* A rows visited by the dense kernel                    */
+    uint64_t b_rows_dense;      /* B rows the dense kernel gathered from HBM/L2          */
+    uint64_t out_rows_dense;    /* C-tile rows written by the dense and MFMA kernels     */
 } cfpq_stats;
 
 /* ---- lifecycle ---------------------------------------------------------- */
