@@ -117,6 +117,7 @@ def main():
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--iters-per-sync", type=int, default=0)
     ap.add_argument("--sparse-batch", type=int, default=0)
+    ap.add_argument("--sparse-waves", type=int, default=0)
     ap.add_argument("--tune", action="store_true", help="autotune on this workload first (cached in tuned.json)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -167,6 +168,8 @@ def main():
         opts["use_graph"] = True
     if args.sparse_batch:
         opts["sparse_batch"] = args.sparse_batch
+    if args.sparse_waves:
+        opts["sparse_waves_per_cu"] = args.sparse_waves
     log(f"[bench] engine options {opts}")
     eng = pkg.Engine(0, **opts)
     gr = cf.Grammar(q.grammar, text=True)

@@ -572,6 +572,11 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                        dim3(256), 0, ctx->stream, (const MatDev *)ctx->d_mats, ids_all, nall,
                        dense_threshold(ctx->opts), mfma_threshold(ctx->opts));
     const uint32_t bx = (uint32_t)((plan.max_tiles_per_head + BLOCK - 1) / BLOCK);
+    // heavy sparse units are split over several waves (lane-parallel kernel,
+    // head update not fused); the units buffer holds 4 entries per head tile
+    // plus as many extra sub-units
+    const int split = (ctx->opts.sparse_mode == 0 && !(fuse && ctx->opts.fuse_apply)) ? 1 : 0;
+    const uint32_t split_budget = (uint32_t)std::min<size_t>(QUARTERS * plan.total_head_tiles, 0xffff0000u);
     const bool use_sched = !full_pass && plan.dirty_words && (sched_on(ctx) || ctx->tr_cap || flush);
     const SchedArgs sa = sched_args(ctx, flush);
     if (use_sched) {
@@ -579,11 +584,12 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
             hipLaunchKernelGGL(k_trace_begin, dim3(1), dim3(64), 0, ctx->stream, c, (const uint32_t *)ctx->tr_count,
                                (uint32_t)ctx->tr_cap);
         hipLaunchKernelGGL(k_build_sched, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                           (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems, sa);
+                           (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems, sa, split,
+                           split_budget);
     } else {
         hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, (int)full_pass, c, ctx->units, ctx->items,
-                           ctx->mitems);
+                           ctx->mitems, split, split_budget);
     }
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
     // product kernels, timed: ev[4i] sparse ev[4i+1] dense ev[4i+2] mfma ev[4i+3]
@@ -657,7 +663,7 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     const size_t nid = plan.all_delta[0].size() + plan.all_delta[1].size() + plan.mats.size();
     CHK(grow(ctx, &ctx->d_ids, &ctx->d_ids_cap, nid));
     CHK(grow(ctx, &ctx->items, &ctx->items_cap, plan.total_head_tiles));
-    CHK(grow(ctx, &ctx->units, &ctx->units_cap, QUARTERS * plan.total_head_tiles));
+    CHK(grow(ctx, &ctx->units, &ctx->units_cap, 2 * QUARTERS * plan.total_head_tiles));
     CHK(grow(ctx, &ctx->mitems, &ctx->mitems_cap, plan.total_head_tiles));
     if (plan.total_head_tiles > ctx->lists_cap || !ctx->lists[0]) {
         size_t cap0 = ctx->lists_cap;
@@ -845,11 +851,14 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             if (step_log)
                 std::fprintf(stderr,
                              "[cfpq step %u] units %u dense %u mfma %u touched %u new_bits %llu cand %llu hits %llu "
-                             "max_unit %u a_rows %llu b_rows %llu | sparse %.3f dense %.3f mfma %.3f ms\n",
+                             "max_unit %u split %u a_rows %llu b_rows %llu busy %.3f | sparse %.3f dense %.3f mfma %.3f ms\n",
                              steps - k + s, c.n_units, c.n_dense, c.n_mfma, c.n_touched,
                              (unsigned long long)c.new_bits, (unsigned long long)c.tile_products,
-                             (unsigned long long)c.cand_hits, c.max_unit_cands, (unsigned long long)c.a_rows,
-                             (unsigned long long)c.b_rows, step_ms[0], step_ms[1], step_ms[2]);
+                             (unsigned long long)c.cand_hits, c.max_unit_cands, c.n_split, (unsigned long long)c.a_rows,
+                             (unsigned long long)c.b_rows,
+                             c.sparse_max ? (double)c.sparse_busy / ((double)c.sparse_max * ctx->num_cu *
+                                                                      ctx->opts.sparse_waves_per_cu) : 0.0,
+                             step_ms[0], step_ms[1], step_ms[2]);
             accumulate(tot, c);
             sched_scored += c.n_scored;
             sched_deferred += c.n_deferred;
@@ -1018,6 +1027,9 @@ extern "C" int cfpq_batch_fixpoint(cfpq_ctx *ctx, const cfpq_query *queries, siz
         }
         if (rc == CFPQ_OK) plan_query(plan, q.mats, D0, D1, q.nmats, q.rules, q.nrules);
     }
+    if (rc == CFPQ_OK && plan.heads.size() >= MAX_HEADS)
+        rc = fail(ctx, CFPQ_E_INVALID, "batch_fixpoint: %zu rule heads in one batch (limit %u); split the batch",
+                  plan.heads.size(), MAX_HEADS - 1);
     if (rc == CFPQ_OK && plan.heads.empty()) {
         for (auto *m : taken) pool_give(ctx, m);
         if (stats) stats->iterations = 0;

@@ -194,6 +194,45 @@ __device__ __forceinline__ QuarterOut quarter_epilogue(const MatDev &O, const Ma
     return QuarterOut{bal, pc};
 }
 
+// Epilogue of one sub-unit of a split quarter: the sibling sub-units of the
+// same quarter write the same rows, so rows and metadata are merged with
+// atomic ORs and the returned count holds only the bits this sub-unit set
+// first.  No in-place head update (splitting is off when that is on).
+__device__ __forceinline__ QuarterOut quarter_epilogue_shared(const MatDev &O, const MatDev &M, uint32_t p,
+                                                              uint32_t q, uint32_t I, uint32_t J, const Row4 &acc,
+                                                              uint32_t &out_rows) {
+    const int lane = threadIdx.x & 63;
+    const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
+    Row4 nw = {{0, 0, 0, 0}};
+    uint32_t mine = 0;
+    if (row_nonzero(acc)) {
+        const Row4 m = load_row(M.bits + off);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            nw.w[w] = acc.w[w] & ~m.w[w];
+            if (nw.w[w]) {
+                const uint64_t old = atomicOr((unsigned long long *)(O.bits + off + w), (unsigned long long)nw.w[w]);
+                mine += (uint32_t)__popcll(nw.w[w] & ~old);
+            }
+        }
+        out_rows += row_nonzero(nw);
+    }
+    const uint64_t bal = __ballot(row_nonzero(nw));
+    const uint32_t pc = wave_sum(mine);
+    if (bal) {
+        const Row4 cu = wave_or_row(nw);
+        if (lane == 0) {
+            atomicOr((unsigned long long *)&O.rmask[(size_t)p * 4 + q], (unsigned long long)bal);
+            uint64_t *cq = colq_of(O, p, q);
+#pragma unroll
+            for (int w = 0; w < 4; ++w)
+                if (cu.w[w]) atomicOr((unsigned long long *)(cq + w), (unsigned long long)cu.w[w]);
+            set_occq(O, q, I, J);
+        }
+    }
+    return QuarterOut{bal, pc};
+}
+
 // ---------------------------------------------------------------------------
 // step bookkeeping
 // ---------------------------------------------------------------------------
@@ -242,7 +281,7 @@ __device__ __forceinline__ void wave_append(bool want, uint64_t val, uint32_t *c
 // candidates, whether some A tile is heavy enough for the dense / MFMA paths.
 __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, const TermDev *__restrict__ terms,
                                            uint32_t t0, uint32_t t1, int parity, uint32_t T, uint32_t I, uint32_t J,
-                                           uint32_t &qmask, bool &dense, bool &mfma) {
+                                           uint32_t &qmask, bool &dense, bool &mfma, uint32_t *qcnt) {
     for (uint32_t t = t0; t < t1; ++t) {
         const MatDev L = mats[terms[t].left[parity]];
         const MatDev R = mats[terms[t].right[parity]];
@@ -252,8 +291,37 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
             if (L.occD[I * L.ow + w] & col) dense = true;
             if (L.occX[I * L.ow + w] & col) mfma = true;
 #pragma unroll
-            for (uint32_t q = 0; q < QUARTERS; ++q)
-                if (L.occQ[(q * T + I) * L.ow + w] & col) qmask |= 1u << q;
+            for (uint32_t q = 0; q < QUARTERS; ++q) {
+                const uint32_t k = L.occQ[(q * T + I) * L.ow + w] & col;
+                if (k) qmask |= 1u << q;
+                qcnt[q] += __popc(k);
+            }
+        }
+    }
+}
+
+// Append the sparse units of one (head, C tile): quarters in qmask, a heavy
+// quarter (more than SPLIT_CANDS candidates) as up to SPLIT_MAX sub-units
+// while the split budget lasts.  Whole wave (wave_append).
+__device__ __forceinline__ void append_units(bool want, uint32_t qmask, const uint32_t *qcnt, uint64_t base,
+                                             int split, uint32_t split_budget, StepCounters *__restrict__ c,
+                                             uint64_t *__restrict__ units) {
+#pragma unroll
+    for (uint32_t q = 0; q < QUARTERS; ++q) {
+        const bool wq = want && ((qmask >> q) & 1u);
+        uint32_t nsub = 1;
+        if (wq && split) {
+            nsub = (qcnt[q] + SPLIT_CANDS - 1) / SPLIT_CANDS;
+            nsub = nsub > SPLIT_MAX ? SPLIT_MAX : (nsub < 1 ? 1 : nsub);
+            if (nsub > 1 && atomicAdd(&c->split_extra, nsub - 1) + (nsub - 1) > split_budget) nsub = 1;
+        }
+        const uint64_t u = base | ((uint64_t)q);
+        wave_append(wq && nsub == 1, u, &c->n_units, units);
+        if (wq && nsub > 1) {
+            const uint32_t at = atomicAdd(&c->n_units, nsub);
+            for (uint32_t sidx = 0; sidx < nsub; ++sidx)
+                units[at + sidx] = u | ((uint64_t)sidx << 55) | ((uint64_t)(nsub - 1) << 59);
+            atomicAdd(&c->n_split, 1u);
         }
     }
 }
@@ -264,7 +332,8 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
                                                  const TermDev *__restrict__ terms, int parity, int full_pass,
                                                  StepCounters *__restrict__ c, uint64_t *__restrict__ units,
                                                  uint64_t *__restrict__ dense_items,
-                                                 uint64_t *__restrict__ mfma_items) {
+                                                 uint64_t *__restrict__ mfma_items, int split,
+                                                 uint32_t split_budget) {
     const uint32_t h = blockIdx.y;
     const uint32_t T = uniform(heads[h].T), target = uniform(heads[h].target);
     const uint32_t t0 = uniform(full_pass ? heads[h].f0 : heads[h].t0);
@@ -273,19 +342,17 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
     const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
     bool dense = false, mfma = false;
     uint32_t qmask = 0;
+    uint32_t qcnt[QUARTERS] = {0, 0, 0, 0};
     if (p < T * T) {
         const uint32_t I = p / T, J = p - I * T;
         if (mats[target].cnt[p] < TILE_BITS) {
-            scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma);
+            scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma, qcnt);
         }
     }
     wave_append(mfma, flag | ((uint64_t)h << 40) | p, &c->n_mfma, mfma_items);
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
-#pragma unroll
-    for (uint32_t q = 0; q < QUARTERS; ++q)
-        wave_append(!dense && ((qmask >> q) & 1u), flag | ((uint64_t)h << 40) | ((uint64_t)p << 2) | q, &c->n_units,
-                    units);
+    append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units);
 }
 
 // ---------------------------------------------------------------------------
@@ -438,9 +505,14 @@ __device__ __forceinline__ int select_bit(uint64_t m, uint32_t j) {
 
 // Lane-parallel variant: the same work unit (head, C tile, 64-row quarter),
 // but the wave's lanes work on different candidate k-tiles at once.
-//   filter  lane l tests k-tile (w + l/32)*32 + l%32 of a pair of occupancy
-//           words: vector loads of the B tile's row mask, the A quarter's
-//           column union and its row mask; a ballot gives the hits;
+//   scan    lane l reads the candidate mask of (term, occupancy word) pair l
+//           (the A quarter's tile-row occupancy AND the B tile-column
+//           occupancy); a prefix sum of their popcounts numbers the
+//           candidates of all terms and words of the unit;
+//   filter  lane g takes candidate g (binary search for its word, bit
+//           selection within it) and tests it with vector loads of the B
+//           tile's row mask and the A quarter's column union and row mask; a
+//           ballot gives the hits;
 //   gather  the hits' (candidate, non-empty A row) pairs are numbered by a
 //           prefix sum over the lanes and dealt out 64 at a time: a lane
 //           finds its pair's candidate by binary search and the row by bit
@@ -460,8 +532,11 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     __shared__ Row4 acc_s[WAVE];
     __shared__ Row4 s_brm[WAVE];
     __shared__ uint64_t s_arm[WAVE];
-    __shared__ uint32_t s_k[WAVE], s_end[WAVE];
+    __shared__ const uint64_t *s_ab[WAVE];
+    __shared__ const uint64_t *s_bb[WAVE];
+    __shared__ uint32_t s_end[WAVE], s_kb[WAVE], s_cend[WAVE];
     const int lane = threadIdx.x & 63;
+    const uint64_t clk0 = __builtin_amdgcn_s_memtime();
     const uint32_t n_units = c->n_units;
     uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0, hits_n = 0, max_cands = 0;
     for (uint32_t u0 = wave_next(&c->unit_next, chunk); u0 < n_units; u0 = wave_next(&c->unit_next, chunk)) {
@@ -472,72 +547,99 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         const uint32_t p = uniform((uint32_t)((it & ITEM_TILE_MASK) >> 2));
         const uint32_t q = uniform((uint32_t)(it & 3u));
         const bool full = (it & ITEM_FULL) != 0;
+        const uint32_t sub = uniform(unit_sub(it)), nsub = uniform(unit_nsub(it));
         const uint32_t T = ld_const(&heads[h].T);
         const uint32_t t0 = full ? ld_const(&heads[h].f0) : ld_const(&heads[h].t0);
         const uint32_t t1 = full ? ld_const(&heads[h].f1) : ld_const(&heads[h].t1);
         const uint32_t I = p / T, J = p - (p / T) * T;
         const uint32_t tp_unit0 = tp;
+        uint32_t round = 0;  // candidate rounds seen; this sub-unit takes rounds = sub (mod nsub)
         acc_s[lane] = Row4{{0, 0, 0, 0}};
         __syncthreads();
-        for (uint32_t t = t0; t < t1; ++t) {
-            const MatDev L = mats[ld_const(&terms[t].left[parity])];
-            const MatDev R = mats[ld_const(&terms[t].right[parity])];
-            for (uint32_t w = 0; w < L.ow; w += 2) {
-                const uint32_t kb0 = ld_const(&L.occQ[(q * T + I) * L.ow + w]) & ld_const(&R.occC[J * R.ow + w]);
-                const uint32_t kb1 = (w + 1 < L.ow)
-                                         ? (ld_const(&L.occQ[(q * T + I) * L.ow + w + 1]) &
-                                            ld_const(&R.occC[J * R.ow + w + 1]))
-                                         : 0u;
-                if (!(kb0 | kb1)) continue;
-                tp += __popc(kb0) + __popc(kb1);
-                const uint32_t K = (w + (lane >> 5)) * 32 + (lane & 31);
-                const bool mine = (((lane < 32) ? kb0 : kb1) >> (lane & 31)) & 1u;
+        const uint32_t ow = (T + 31) / 32;
+        const uint32_t ntw = (t1 - t0) * ow;  // (term, occupancy word) pairs of the unit
+        for (uint32_t tw0 = 0; tw0 < ntw; tw0 += WAVE) {
+            // 1. candidate masks of 64 (term, word) pairs at once, compacted by
+            //    a prefix sum of their popcounts
+            const uint32_t j = tw0 + lane;
+            uint32_t kbw = 0;
+            if (j < ntw) {
+                const uint32_t t = t0 + j / ow, w = j % ow;
+                const uint32_t *oq = mats[terms[t].left[parity]].occQ;
+                const uint32_t *oc = mats[terms[t].right[parity]].occC;
+                kbw = oq[(q * T + I) * ow + w] & oc[J * ow + w];
+            }
+            const uint32_t incw = wave_scan_incl((uint32_t)__popc(kbw));
+            const uint32_t total = __shfl(incw, 63, 64);
+            if (!total) continue;
+            s_kb[lane] = kbw;
+            s_cend[lane] = incw;
+            __syncthreads();
+            for (uint32_t cb = 0; cb < total; cb += WAVE) {
+                if ((round++ % nsub) != sub) continue;
+                tp += (total - cb < WAVE) ? total - cb : WAVE;
+                // 2. filter 64 candidates: lane g takes the g-th candidate
+                const uint32_t g = cb + lane;
                 uint64_t arm = 0;
                 Row4 brm = {{0, 0, 0, 0}};
-                if (mine) {
+                const uint64_t *abase = nullptr, *bbase = nullptr;
+                if (g < total) {
+                    uint32_t lo = 0, hi = WAVE - 1;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s_cend[mid] > g) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    const uint32_t kb = s_kb[lo];
+                    const int bit = select_bit(kb, g - (s_cend[lo] - (uint32_t)__popc(kb)));
+                    const uint32_t jj = tw0 + lo, t = t0 + jj / ow;
+                    const uint32_t K = (jj % ow) * 32 + (uint32_t)bit;
+                    const MatDev &L = mats[terms[t].left[parity]];
+                    const MatDev &R = mats[terms[t].right[parity]];
                     const size_t lt = (size_t)I * T + K, rt = (size_t)K * T + J;
                     brm = load_row(R.rmask + rt * 4);
-                    const Row4 cu = load_row(colq_of(L, (uint32_t)lt, q));
+                    const Row4 cu = load_row(L.colq + (lt * QUARTERS + q) * TROW_WORDS);
                     arm = L.rmask[lt * 4 + q];
                     const bool hit = ((cu.w[0] & brm.w[0]) | (cu.w[1] & brm.w[1]) | (cu.w[2] & brm.w[2]) |
                                       (cu.w[3] & brm.w[3])) != 0;
                     if (!hit) arm = 0;
+                    abase = L.bits + lt * TILE_WORDS + (size_t)q * WAVE * TROW_WORDS;
+                    bbase = R.bits + rt * TILE_WORDS;
                 }
-                // stage the hits in LDS in lane order and spread their
-                // (candidate, A row) pairs over the lanes
+                // 3. stage the hits in lane order and spread their (candidate,
+                //    A row) pairs over the lanes
                 const bool hit = arm != 0;
                 const uint64_t hmask = __ballot(hit);
                 if (!hmask) continue;
                 const uint32_t nh = __popcll(hmask);
                 const uint32_t cnt = hit ? (uint32_t)__popcll(arm) : 0u;
                 const uint32_t incl = wave_scan_incl(cnt);
-                const uint32_t total = __shfl(incl, 63, 64);
+                const uint32_t pairs = __shfl(incl, 63, 64);
                 if (hit) {
                     const uint32_t slot = __popcll(hmask & ((1ull << lane) - 1ull));
                     s_arm[slot] = arm;
-                    s_k[slot] = K;
                     s_end[slot] = incl;
                     s_brm[slot] = brm;
+                    s_ab[slot] = abase;
+                    s_bb[slot] = bbase;
                 }
                 hits_n += hit;
                 arows += cnt;
                 __syncthreads();
-                for (uint32_t base = 0; base < total; base += WAVE) {
-                    const uint32_t g = base + lane;
-                    if (g < total) {
-                        uint32_t lo = 0, hi = nh - 1;  // owner: first slot whose range ends past g
+                for (uint32_t base = 0; base < pairs; base += WAVE) {
+                    const uint32_t gp = base + lane;
+                    if (gp < pairs) {
+                        uint32_t lo = 0, hi = nh - 1;  // owner: first slot whose range ends past gp
                         while (lo < hi) {
                             const uint32_t mid = (lo + hi) >> 1;
-                            if (s_end[mid] > g) hi = mid;
+                            if (s_end[mid] > gp) hi = mid;
                             else lo = mid + 1;
                         }
                         const uint64_t am = s_arm[lo];
-                        const int r = select_bit(am, g - (s_end[lo] - (uint32_t)__popcll(am)));
-                        const uint32_t Ko = s_k[lo];
+                        const int r = select_bit(am, gp - (s_end[lo] - (uint32_t)__popcll(am)));
                         const Row4 bm = s_brm[lo];
-                        const size_t lt = (size_t)I * T + Ko, rt = (size_t)Ko * T + J;
-                        const Row4 a = load_row(L.bits + lt * TILE_WORDS + ((size_t)q * WAVE + r) * TROW_WORDS);
-                        const uint64_t *Bt = R.bits + rt * TILE_WORDS;
+                        const Row4 a = load_row(s_ab[lo] + r * TROW_WORDS);
+                        const uint64_t *Bt = s_bb[lo];
                         Row4 v = {{0, 0, 0, 0}};
 #pragma unroll
                         for (int ww = 0; ww < 4; ++ww) {
@@ -547,19 +649,19 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                 int ks[4];
                                 int nk = 0;
 #pragma unroll
-                                for (int j = 0; j < 4; ++j) {
-                                    ks[j] = ww * 64 + (x ? __builtin_ctzll(x) : 0);
+                                for (int jj = 0; jj < 4; ++jj) {
+                                    ks[jj] = ww * 64 + (x ? __builtin_ctzll(x) : 0);
                                     if (x) {
                                         x &= x - 1;
-                                        nk = j + 1;
+                                        nk = jj + 1;
                                     }
                                 }
                                 Row4 b[4];
 #pragma unroll
-                                for (int j = 0; j < 4; ++j)
-                                    b[j] = j < nk ? load_row(Bt + ks[j] * TROW_WORDS) : Row4{{0, 0, 0, 0}};
+                                for (int jj = 0; jj < 4; ++jj)
+                                    b[jj] = jj < nk ? load_row(Bt + ks[jj] * TROW_WORDS) : Row4{{0, 0, 0, 0}};
 #pragma unroll
-                                for (int j = 0; j < 4; ++j) or_row(v, b[j]);
+                                for (int jj = 0; jj < 4; ++jj) or_row(v, b[jj]);
                                 brows += nk;
                             }
                         }
@@ -568,16 +670,18 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                             if (v.w[i]) atomicOr((unsigned long long *)&acc_s[r].w[i], (unsigned long long)v.w[i]);
                     }
                 }
-                __syncthreads();  // the staging arrays are refilled by the next filter round
+                __syncthreads();  // the hit staging is refilled by the next candidate round
             }
+            __syncthreads();  // the candidate staging is refilled by the next (term, word) chunk
         }
         __syncthreads();
         const Row4 acc = acc_s[lane];
         max_cands = max(max_cands, tp - tp_unit0);
         const MatDev O = mats[uniform(heads[h].out[parity])];
         const MatDev M = mats[uniform(heads[h].target)];
-        const QuarterOut qo = quarter_epilogue(O, M, p, q, I, J, acc, fuse, out_rows);
-        if (qo.bal && lane == 0) {
+        const QuarterOut qo = nsub > 1 ? quarter_epilogue_shared(O, M, p, q, I, J, acc, out_rows)
+                                       : quarter_epilogue(O, M, p, q, I, J, acc, fuse, out_rows);
+        if (qo.pc && lane == 0) {
             const uint32_t old = atomicAdd(&O.cnt[p], qo.pc);
             if (old == 0) {
                 set_occ(O, I, J);
@@ -596,6 +700,9 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
         if (hs) atomicAdd((unsigned long long *)&c->cand_hits, (unsigned long long)hs);
         if (max_cands) atomicMax(&c->max_unit_cands, max_cands);
+        const uint64_t busy = __builtin_amdgcn_s_memtime() - clk0;
+        atomicAdd((unsigned long long *)&c->sparse_busy, (unsigned long long)busy);
+        atomicMax((unsigned long long *)&c->sparse_max, (unsigned long long)busy);
     }
 }
 

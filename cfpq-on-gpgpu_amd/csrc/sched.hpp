@@ -252,7 +252,8 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                                                        const TermDev *__restrict__ terms, int parity,
                                                        StepCounters *__restrict__ c, uint64_t *__restrict__ units,
                                                        uint64_t *__restrict__ dense_items,
-                                                       uint64_t *__restrict__ mfma_items, SchedArgs sa) {
+                                                       uint64_t *__restrict__ mfma_items, SchedArgs sa, int split,
+                                                       uint32_t split_budget) {
     __shared__ float sw[MLP_PARAMS];
     if (sa.w) mlp_load(sa.w, sw);
     const uint32_t h = blockIdx.y;
@@ -262,6 +263,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
     const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
     bool dense = false, mfma = false, full = false;
     uint32_t qmask = 0;
+    uint32_t qcnt[QUARTERS] = {0, 0, 0, 0};
     uint32_t deferred = 0, fulls = 0, scored = 0, newly = 0, cleared = 0;
     if (p < T * T) {
         const uint32_t I = p / T, J = p - I * T;
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
         const bool dirty = (*dw & dbit) != 0;
         const MatDev M = mats[target];
         if (M.cnt[p] < TILE_BITS) {
-            scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma);
+            scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma, qcnt);
             bool evaluate = qmask != 0;
             if (evaluate && !sa.flush && (sa.w || sa.trace_x)) {
                 // score every rule of the head; the item's score is the best one
@@ -315,7 +317,8 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                 full = true;
                 qmask = 0;
                 dense = mfma = false;
-                scan_terms(mats, terms, f0, f1, parity, T, I, J, qmask, dense, mfma);
+                qcnt[0] = qcnt[1] = qcnt[2] = qcnt[3] = 0;
+                scan_terms(mats, terms, f0, f1, parity, T, I, J, qmask, dense, mfma, qcnt);
                 evaluate = qmask != 0;
                 ++fulls;
             }
@@ -332,10 +335,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
     wave_append(mfma, flag | ((uint64_t)h << 40) | p, &c->n_mfma, mfma_items);
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
-#pragma unroll
-    for (uint32_t q = 0; q < QUARTERS; ++q)
-        wave_append(!dense && ((qmask >> q) & 1u), flag | ((uint64_t)h << 40) | ((uint64_t)p << 2) | q, &c->n_units,
-                    units);
+    append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units);
     deferred = wave_sum(deferred);
     fulls = wave_sum(fulls);
     scored = wave_sum(scored);

@@ -84,6 +84,8 @@ struct StepCounters {
     uint32_t dirty_new;    // tiles that became dirty (deferred) in this step
     uint32_t dirty_cleared; // dirty tiles evaluated in full (or found full) in this step
     uint32_t max_unit_cands; // most candidate k-tiles of one sparse unit
+    uint32_t split_extra;    // extra sub-units created by splitting heavy units (budget counter)
+    uint32_t n_split;        // heavy units split
     uint64_t new_bits;
     uint64_t tile_products;   // (term, C tile, k) products evaluated (sparse: per quarter)
     uint64_t tp_dense;        // of which on the LDS / Four Russians paths
@@ -98,6 +100,8 @@ struct StepCounters {
     uint64_t a_rows_dense;    // as a_rows / b_rows / out_rows, for the dense and MFMA kernels
     uint64_t b_rows_dense;    // (a_rows / b_rows / out_rows count the sparse kernels only)
     uint64_t out_rows_dense;
+    uint64_t sparse_busy;     // sum over sparse-product waves of their busy clocks (s_memtime)
+    uint64_t sparse_max;      // longest sparse-product wave (clocks)
 };
 
 constexpr int RING = 64;  // step-counter ring length
@@ -106,10 +110,17 @@ static inline uint32_t tiles_per_side(uint32_t n) { return (n + TILE - 1) / TILE
 static inline uint32_t occ_words(uint32_t T) { return (T + 31) / 32; }
 
 // Item encodings: dense item = (head << 40) | tile; sparse unit =
-// (head << 40) | (tile << 2) | quarter; bit 63 set = evaluate the head's
-// full terms (M_L * M_R) instead of its semi-naive terms.
+// (head << 40) | (tile << 2) | quarter.  A heavy sparse unit is split over
+// nsub waves (sub-units sub = 0 .. nsub-1, each taking every nsub-th round of
+// 64 candidates): (sub << 55) | ((nsub - 1) << 59).  Bit 63 set = evaluate
+// the head's full terms (M_L * M_R) instead of its semi-naive terms.
 constexpr uint64_t ITEM_TILE_MASK = 0xffffffffffull;
 constexpr uint64_t ITEM_FULL = 1ull << 63;
-__host__ __device__ inline uint32_t item_head(uint64_t it) { return (uint32_t)((it >> 40) & 0x7fffffu); }
+constexpr uint32_t MAX_HEADS = 1u << 15;   // heads per plan (15-bit field)
+constexpr uint32_t SPLIT_MAX = 16;         // sub-units per heavy sparse unit
+constexpr uint32_t SPLIT_CANDS = 64;       // candidates per sub-unit round
+__host__ __device__ inline uint32_t item_head(uint64_t it) { return (uint32_t)((it >> 40) & 0x7fffu); }
+__host__ __device__ inline uint32_t unit_sub(uint64_t it) { return (uint32_t)((it >> 55) & 15u); }
+__host__ __device__ inline uint32_t unit_nsub(uint64_t it) { return (uint32_t)((it >> 59) & 15u) + 1u; }
 
 }  // namespace cfpq
