@@ -55,6 +55,10 @@ def lib():
         L.oracle_fixpoint_seminaive.restype = ctypes.c_uint32
         L.oracle_fixpoint_seminaive.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
                                                 ctypes.c_uint32, ctypes.POINTER(_u64p)]
+        L.oracle_fixpoint_seminaive_m4rm.restype = ctypes.c_uint32
+        L.oracle_fixpoint_seminaive_m4rm.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+                                                     ctypes.c_uint32, ctypes.POINTER(_u64p),
+                                                     ctypes.POINTER(ctypes.c_uint64)]
         L.oracle_hellings.restype = ctypes.c_uint64
         L.oracle_hellings.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
                                       ctypes.c_uint32, ctypes.POINTER(_u64p)]
@@ -197,6 +201,14 @@ def fixpoint_seminaive(n, rules, mats) -> int:
     return int(lib().oracle_fixpoint_seminaive(n, len(mats), rp, r.shape[0], _ptrs(mats)))
 
 
+def fixpoint_seminaive_m4rm(n, rules, mats):
+    """Semi-naive fixpoint with dense M4RM products; returns (steps, products)."""
+    r, rp = _rules_ptr(rules)
+    prods = ctypes.c_uint64()
+    steps = int(lib().oracle_fixpoint_seminaive_m4rm(n, len(mats), rp, r.shape[0], _ptrs(mats), ctypes.byref(prods)))
+    return steps, int(prods.value)
+
+
 def hellings(n, rules, mats) -> int:
     r, rp = _rules_ptr(rules)
     return int(lib().oracle_hellings(n, len(mats), rp, r.shape[0], _ptrs(mats)))
@@ -219,6 +231,9 @@ def solve(query, method: str = "seminaive"):
         info = {"iterations": fixpoint_seminaive(query.n, g.rules_array(), mats)}
     elif method in ("m4rm", "rows"):
         info = {"add_mul_calls": fixpoint_worklist(query.n, g.rules_array(), mats, method)}
+    elif method == "seminaive_m4rm":
+        steps, prods = fixpoint_seminaive_m4rm(query.n, g.rules_array(), mats)
+        info = {"iterations": steps, "products": prods}
     elif method == "hellings":
         info = {"facts": hellings(query.n, g.rules_array(), mats)}
     else:

@@ -190,6 +190,62 @@ double oracle_fixpoint_worklist_budget(uint32_t n, uint32_t nnt, const uint32_t 
 // Per iteration, for every rule H -> A B:  new_H |= (dA * A_B) | (M_A * dB),
 // masked by ~M_H;  then M |= new, d := new.  Stops when no bit is new.
 // Returns the number of iterations (including the final empty one).
+// Semi-naive fixpoint (Jacobi steps) whose products are the dense M4RM of
+// the worklist oracle: for every rule, new_H |= (dA*M_B | M_A*dB) & ~M_H; then
+// M |= new, d := new; stops at the first step with no new bit.  For workloads
+// whose matrices saturate, where the row-gather checker is slow.  Returns the
+// number of steps; *products receives the number of M4RM products evaluated.
+uint32_t oracle_fixpoint_seminaive_m4rm(uint32_t n, uint32_t nnt, const uint32_t *rules, uint32_t nrules,
+                                        u64 **mats, uint64_t *products) {
+    const size_t w = wpr_of(n), words = (size_t)n * w;
+    std::vector<std::vector<u64>> D(nnt), NX(nnt);
+    std::vector<u64> T(words);
+    std::vector<uint8_t> dnz(nnt, 0);
+    for (uint32_t x = 0; x < nnt; ++x) {
+        D[x].assign(mats[x], mats[x] + words);
+        NX[x].assign(words, 0);
+        for (size_t j = 0; j < words && !dnz[x]; ++j) dnz[x] = D[x][j] != 0;
+    }
+    uint64_t prods = 0;
+    uint32_t iters = 0;
+    for (;;) {
+        ++iters;
+        for (uint32_t r = 0; r < nrules; ++r) {
+            const uint32_t h = rules[3 * r], a = rules[3 * r + 1], b = rules[3 * r + 2];
+            for (int term = 0; term < 2; ++term) {
+                if (!dnz[term == 0 ? a : b]) continue;
+                const u64 *L = term == 0 ? D[a].data() : mats[a];
+                const u64 *R = term == 0 ? mats[b] : D[b].data();
+                std::fill(T.begin(), T.end(), 0);
+                m4rm_into(T.data(), L, R, n);
+                ++prods;
+                const u64 *mh = mats[h];
+                u64 *nh = NX[h].data();
+#pragma omp parallel for schedule(static)
+                for (int64_t j = 0; j < (int64_t)words; ++j) nh[j] |= T[j] & ~mh[j];
+            }
+        }
+        bool any = false;
+        for (uint32_t x = 0; x < nnt; ++x) {
+            u64 *m = mats[x];
+            u64 *nx = NX[x].data();
+            u64 *d = D[x].data();
+            uint8_t nz = 0;
+            for (size_t j = 0; j < words; ++j) {
+                m[j] |= nx[j];
+                d[j] = nx[j];
+                nz |= nx[j] != 0;
+                nx[j] = 0;
+            }
+            dnz[x] = nz;
+            any = any || nz;
+        }
+        if (!any) break;
+    }
+    if (products) *products = prods;
+    return iters;
+}
+
 uint32_t oracle_fixpoint_seminaive(uint32_t n, uint32_t nnt, const uint32_t *rules,
                                    uint32_t nrules, u64 **mats) {
     const size_t w = wpr_of(n), words = (size_t)n * w;
