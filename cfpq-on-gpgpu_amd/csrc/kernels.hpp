@@ -605,6 +605,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     if (!hit) arm = 0;
                     abase = L.bits + lt * TILE_WORDS + (size_t)q * WAVE * TROW_WORDS;
                     bbase = R.bits + rt * TILE_WORDS;
+                    // a full B tile needs no gathers: bit 0 of the (8-byte aligned) pointer marks it
+                    if (R.cnt[rt] == TILE_BITS) bbase = (const uint64_t *)((uintptr_t)bbase | 1u);
                 }
                 // 3. stage the hits in lane order and spread their (candidate,
                 //    A row) pairs over the lanes
@@ -638,31 +640,34 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                         const uint64_t am = s_arm[lo];
                         const int r = select_bit(am, gp - (s_end[lo] - (uint32_t)__popcll(am)));
                         const Row4 bm = s_brm[lo];
-                        const Row4 a = load_row(s_ab[lo] + r * TROW_WORDS);
                         const uint64_t *Bt = s_bb[lo];
-                        Row4 v = {{0, 0, 0, 0}};
+                        const bool bfull = ((uintptr_t)Bt & 1u) != 0;
+                        const Row4 a = bfull ? Row4{{1, 0, 0, 0}} : load_row(s_ab[lo] + r * TROW_WORDS);
+                        Row4 v = bfull ? Row4{{~0ull, ~0ull, ~0ull, ~0ull}} : Row4{{0, 0, 0, 0}};
+                        if (!bfull) {
 #pragma unroll
-                        for (int ww = 0; ww < 4; ++ww) {
-                            uint64_t x = a.w[ww] & bm.w[ww];
-                            while (x) {
-                                // up to four B rows in flight
-                                int ks[4];
-                                int nk = 0;
+                            for (int ww = 0; ww < 4; ++ww) {
+                                uint64_t x = a.w[ww] & bm.w[ww];
+                                while (x) {
+                                    // up to four B rows in flight
+                                    int ks[4];
+                                    int nk = 0;
 #pragma unroll
-                                for (int jj = 0; jj < 4; ++jj) {
-                                    ks[jj] = ww * 64 + (x ? __builtin_ctzll(x) : 0);
-                                    if (x) {
-                                        x &= x - 1;
-                                        nk = jj + 1;
+                                    for (int jj = 0; jj < 4; ++jj) {
+                                        ks[jj] = ww * 64 + (x ? __builtin_ctzll(x) : 0);
+                                        if (x) {
+                                            x &= x - 1;
+                                            nk = jj + 1;
+                                        }
                                     }
+                                    Row4 b[4];
+#pragma unroll
+                                    for (int jj = 0; jj < 4; ++jj)
+                                        b[jj] = jj < nk ? load_row(Bt + ks[jj] * TROW_WORDS) : Row4{{0, 0, 0, 0}};
+#pragma unroll
+                                    for (int jj = 0; jj < 4; ++jj) or_row(v, b[jj]);
+                                    brows += nk;
                                 }
-                                Row4 b[4];
-#pragma unroll
-                                for (int jj = 0; jj < 4; ++jj)
-                                    b[jj] = jj < nk ? load_row(Bt + ks[jj] * TROW_WORDS) : Row4{{0, 0, 0, 0}};
-#pragma unroll
-                                for (int jj = 0; jj < 4; ++jj) or_row(v, b[jj]);
-                                brows += nk;
                             }
                         }
 #pragma unroll
@@ -829,7 +834,22 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
                     const uint64_t *Bt = R.bits + (size_t)rt * TILE_WORDS;
                     const uint64_t rm = L.rmask[(size_t)lt * 4 + wv];
                     ++tp;
-                    const uint32_t acnt = wg_bcast(L.cnt[lt], s_bc, nbc);
+                    // A-tile count, and bit 31 = "B tile full" (counts are <= 2^16)
+                    const uint32_t ab = wg_bcast(L.cnt[lt] | (R.cnt[rt] == TILE_BITS ? 0x80000000u : 0u), s_bc, nbc);
+                    const uint32_t acnt = ab & 0x7fffffffu;
+                    if (ab >> 31) {
+                        // B all ones: every non-empty A row gives an all-ones row
+                        if ((rm >> (r & 63)) & 1ull) acc = Row4{{~0ull, ~0ull, ~0ull, ~0ull}};
+                        ++tpd;
+                        continue;
+                    }
+                    if (acnt == TILE_BITS) {
+                        // A all ones: every row gets the OR of all B rows
+#pragma unroll
+                        for (int qq = 0; qq < QUARTERS; ++qq) or_row(acc, load_row(colq_of(R, rt, qq)));
+                        ++tpd;
+                        continue;
+                    }
                     if (acnt >= m4r_bits) {
                         ++tpd;
                         tp_m4r(At, Bt, rm, r, Bs, Tab, acc, arows, lookups);

@@ -290,3 +290,26 @@ def test_fixpoint_fused_apply(pkg, gen, oracle, which):
     for kernel in ("auto", "gather", "m4r", "mfma"):
         _fixpoint_vs_oracle(pkg, gen, oracle, q, kernel=kernel, fuse_apply=True, dense_bits=64, m4r_bits=4096,
                             mfma_bits=30000)
+
+
+def test_batch_512_queries_golden(pkg, gen):
+    """The 512-query batch (N log-uniform in [256, 8192], Dyck or SG family)
+    in one batched fixpoint: every query's fingerprints against the oracle
+    fixture."""
+    import json
+    import os
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixpoints.json")))["config3"]
+    qs = gen.config(3)
+    assert [q.digest() for q in qs] == [g["digest"] for g in golden]
+    with pkg.Engine(0) as eng:
+        grs = []
+        for q in qs:
+            gr = cf.Grammar(q.grammar, text=True)
+            gr.load(eng, cf.Graph.from_query(q))
+            grs.append(gr)
+        st = eng.batch_fixpoint([(gr.rule_triples(), gr.matrices) for gr in grs])
+        assert st["iterations"] >= 2
+        for q, gr, g in zip(qs, grs, golden):
+            assert gr.results() == {k: tuple(v) for k, v in g["fingerprints"].items()}, q.name
+            gr.free()
