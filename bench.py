@@ -119,6 +119,7 @@ def main():
     ap.add_argument("--sparse-batch", type=int, default=0)
     ap.add_argument("--sparse-waves", type=int, default=0)
     ap.add_argument("--tune", action="store_true", help="autotune on this workload first (cached in tuned.json)")
+    ap.add_argument("--tune-repeats", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler engine")
@@ -150,13 +151,17 @@ def main():
     golden = json.load(open(GOLDEN))[key]
     q = gen.config(args.config)
     assert q.digest() == golden["digest"], "workload generator drifted from the golden fixture"
-    a_ref = int(golden["add_mul_calls"])
+    # work unit: 2 N^3 bit-ops per add_mul of the reference worklist; where the
+    # fixture has no worklist count (the dense-stress grammar, whose worklist
+    # takes hours on the CPU), per M4RM product of the oracle's semi-naive run
+    work_key = "add_mul_calls" if "add_mul_calls" in golden else "seminaive_m4rm_products"
+    a_ref = int(golden[work_key])
     n = q.n
     log(f"[bench] workload {wname}: n={n} edges={q.n_edges} A_ref={a_ref}")
 
     tune = importlib.import_module("cfpq-on-gpgpu_amd.tune")
     if args.tune:
-        tune.tune(pkg, cf, q, log=log)
+        tune.tune(pkg, cf, q, repeats=args.tune_repeats, log=log)
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
         with open(tune.CACHE) as f, open(os.path.join(ROOT, "gpurun_out", "tuned.json"), "w") as g:
             g.write(f.read())
@@ -256,7 +261,7 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded generator, same inputs as the CPU oracle)",
         "config": {"workload": wname, "n": n, "edges": q.n_edges, "nonterminals": gr.nonterminals_count,
-                   "rules": len(rules), "A_ref": a_ref,
+                   "rules": len(rules), "A_ref": a_ref, "A_ref_kind": work_key,
                    "engine": {"plain": "delta fixpoint", "learned": "delta fixpoint + learned tile scheduler"}[chosen],
                    "kernel": args.kernel, "parallelism": "single GPU"},
         "fixpoint_s": round(statistics.median(fix_ms) / 1e3, 6),
