@@ -33,7 +33,7 @@ CACHE = os.path.join(HERE, "tuned.json")
 SPACE = [
     ("sparse_mode", [0, 1]),
     ("sparse_batch", [1, 2, 4]),
-    ("sparse_waves_per_cu", [8, 16, 32]),
+    ("sparse_waves_per_cu", [2, 4, 8, 12, 16, 32]),
     ("sparse_chunk", [1, 4, 16]),
     ("dense_bits", [512, 2048, 8192, 0xFFFFFFF0]),
     ("m4r_bits", [8192, 16384, 32768, 0xFFFFFFF0]),
