@@ -372,6 +372,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
     const int lane = threadIdx.x & 63;
     const uint32_t n_units = c->n_units;
     uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0, hits = 0, max_cands = 0;
+    uint64_t new_bits = 0;  // lane 0: new bits of this wave's units
     for (uint32_t u0 = wave_next(&c->unit_next, chunk); u0 < n_units; u0 = wave_next(&c->unit_next, chunk)) {
     const uint32_t u1 = (u0 + chunk < n_units) ? u0 + chunk : n_units;
     for (uint32_t u = u0; u < u1; ++u) {
@@ -462,7 +463,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
                 set_occ(O, I, J);
                 touched[atomicAdd(&c->n_touched, 1u)] = ((uint64_t)h << 40) | p;
             }
-            atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)qo.pc);
+            new_bits += qo.pc;
         }
     }
     }
@@ -474,6 +475,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
         if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
         if (hits) atomicAdd((unsigned long long *)&c->cand_hits, (unsigned long long)hits);
         if (max_cands) atomicMax(&c->max_unit_cands, max_cands);
+        if (new_bits) atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)new_bits);
     }
 }
 
@@ -538,10 +540,19 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     const int lane = threadIdx.x & 63;
     const uint64_t clk0 = __builtin_amdgcn_s_memtime();
     const uint32_t n_units = c->n_units;
+    // Claims x are mapped to list entries by reversing the low six bits of x:
+    // neighbours in the list (the quarters of one tile, the sub-units of one
+    // heavy quarter) go to claims far apart within their block of 64, i.e. to
+    // different waves, while blocks -- and the locality of the list order --
+    // are kept.  x runs over whole blocks; entries past the list end are skipped.
+    const uint32_t n_claim = (n_units + 63u) & ~63u;
     uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0, hits_n = 0, max_cands = 0;
-    for (uint32_t u0 = wave_next(&c->unit_next, chunk); u0 < n_units; u0 = wave_next(&c->unit_next, chunk)) {
-    const uint32_t u1 = (u0 + chunk < n_units) ? u0 + chunk : n_units;
-    for (uint32_t u = u0; u < u1; ++u) {
+    uint64_t new_bits = 0;  // lane 0: new bits of this wave's units
+    for (uint32_t u0 = wave_next(&c->unit_next, chunk); u0 < n_claim; u0 = wave_next(&c->unit_next, chunk)) {
+    const uint32_t u1 = (u0 + chunk < n_claim) ? u0 + chunk : n_claim;
+    for (uint32_t x = u0; x < u1; ++x) {
+        const uint32_t u = (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
+        if (u >= n_units) continue;
         const uint64_t it = ld_const(units + u);
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)((it & ITEM_TILE_MASK) >> 2));
@@ -692,7 +703,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                 set_occ(O, I, J);
                 touched[atomicAdd(&c->n_touched, 1u)] = ((uint64_t)h << 40) | p;
             }
-            atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)qo.pc);
+            new_bits += qo.pc;
         }
         __syncthreads();  // the accumulator is reset by the next unit
     }
@@ -705,6 +716,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
         if (hs) atomicAdd((unsigned long long *)&c->cand_hits, (unsigned long long)hs);
         if (max_cands) atomicMax(&c->max_unit_cands, max_cands);
+        if (new_bits) atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)new_bits);
         const uint64_t busy = __builtin_amdgcn_s_memtime() - clk0;
         atomicAdd((unsigned long long *)&c->sparse_busy, (unsigned long long)busy);
         atomicMax((unsigned long long *)&c->sparse_max, (unsigned long long)busy);
@@ -812,6 +824,7 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
     const int r = threadIdx.x, wv = r >> 6, lane = r & 63;
     const uint32_t n_items = c->n_dense;
     uint32_t tp = 0, tpd = 0, arows = 0, brows = 0, brows_lds = 0, out_rows = 0, lookups = 0;
+    uint64_t new_bits = 0;  // thread 0: new bits of this work-group's items
     for (uint32_t idx = wg_next(&c->dense_next, &s_idx); idx < n_items; idx = wg_next(&c->dense_next, &s_idx)) {
         const uint64_t it = items[idx];
         const uint32_t h = uniform(item_head(it));
@@ -873,7 +886,7 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
                 O.cnt[p] = total;
                 set_occ(O, I, J);
                 touched[atomicAdd(&c->n_touched, 1u)] = it;
-                atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)total);
+                new_bits += total;
             }
         }
     }
@@ -890,6 +903,7 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
         atomicAdd((unsigned long long *)&c->tile_products, (unsigned long long)tp);
         atomicAdd((unsigned long long *)&c->tp_dense, (unsigned long long)tpd);
     }
+    if (r == 0 && new_bits) atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)new_bits);
 }
 
 // ---------------------------------------------------------------------------
@@ -972,6 +986,7 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
     const int rs = wv & 3, ch = wv >> 2;
     const uint32_t n_items = c->n_mfma;
     uint32_t tp = 0, instr = 0, out_rows = 0;
+    uint64_t new_bits = 0;  // thread 0: new bits of this work-group's items
     for (uint32_t idx = wg_next(&c->mfma_next, &s_idx); idx < n_items; idx = wg_next(&c->mfma_next, &s_idx)) {
         const uint64_t it = items[idx];
         const uint32_t h = uniform(item_head(it));
@@ -1074,7 +1089,7 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
                 O.cnt[p] = total;
                 set_occ(O, I, J);
                 touched[atomicAdd(&c->n_touched, 1u)] = it;
-                atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)total);
+                new_bits += total;
             }
         }
     }
@@ -1087,6 +1102,7 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
         atomicAdd((unsigned long long *)&c->tile_products, (unsigned long long)tp);
         atomicAdd((unsigned long long *)&c->tp_mfma, (unsigned long long)tp);
     }
+    if (t == 0 && new_bits) atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)new_bits);
 }
 
 // ---------------------------------------------------------------------------
