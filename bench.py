@@ -120,6 +120,7 @@ def main():
     ap.add_argument("--sparse-waves", type=int, default=0)
     ap.add_argument("--tune", action="store_true", help="autotune on this workload first (cached in tuned.json)")
     ap.add_argument("--tune-repeats", type=int, default=3)
+    ap.add_argument("--tune-passes", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler engine")
@@ -161,7 +162,7 @@ def main():
 
     tune = importlib.import_module("cfpq-on-gpgpu_amd.tune")
     if args.tune:
-        tune.tune(pkg, cf, q, repeats=args.tune_repeats, log=log)
+        tune.tune(pkg, cf, q, repeats=args.tune_repeats, log=log, passes=args.tune_passes)
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
         with open(tune.CACHE) as f, open(os.path.join(ROOT, "gpurun_out", "tuned.json"), "w") as g:
             g.write(f.read())

@@ -978,10 +978,10 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
                                                              uint64_t *__restrict__ touched, int fuse) {
     __shared__ __attribute__((aligned(16))) uint32_t BX[TILE * TILE / 4];  // 64 KiB
     __shared__ Row4 Bs[TILE];  // B tile bits; the result bits in the epilogue
+    __shared__ uint32_t s_cand[WAVE * 32];  // candidate k-tiles: (pair index << 16) | K
+    __shared__ uint32_t s_ncand;
     __shared__ uint32_t s_idx;
     __shared__ uint32_t s_sum[4];
-    __shared__ uint32_t s_bc[2];
-    uint32_t nbc = 0;
     const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
     const int rs = wv & 3, ch = wv >> 2;
     const uint32_t n_items = c->n_mfma;
@@ -1000,52 +1000,85 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = i32x16{};
-        for (uint32_t tt = t0; tt < t1; ++tt) {
-            const MatDev L = mats[uniform(terms[tt].left[parity])];
-            const MatDev R = mats[uniform(terms[tt].right[parity])];
-            for (uint32_t w = 0; w < L.ow; ++w) {
-                uint32_t kb = wg_bcast(L.occR[I * L.ow + w] & R.occC[J * R.ow + w], s_bc, nbc);
-                while (kb) {
-                    const uint32_t K = w * 32 + __builtin_ctz(kb);
-                    kb &= kb - 1;
-                    const uint64_t *At = L.bits + (size_t)(I * T + K) * TILE_WORDS;
-                    const uint64_t *Bt = R.bits + (size_t)(K * T + J) * TILE_WORDS;
-                    ++tp;
-                    const Row4 a0 = load_row(At + (rs * 64 + (lane & 31)) * TROW_WORDS);
-                    const Row4 a1 = load_row(At + (rs * 64 + 32 + (lane & 31)) * TROW_WORDS);
-                    __syncthreads();  // the previous k-tile's readers of Bs / BX are done
-                    if (t < TILE) Bs[t] = load_row(Bt + t * TROW_WORDS);
-                    __syncthreads();
-#pragma unroll
-                    for (int i = 0; i < 2; ++i) {
-                        const int blk = 2 * wv + i, kb64 = blk >> 2, cb64 = blk & 3;
-                        const uint64_t col = wave_transpose64(Bs[kb64 * 64 + lane].w[cb64]);
-                        const int n = cb64 * 64 + lane;  // this lane's B column; bits = k in kb64*64 ..
-#pragma unroll
-                        for (int cq = 0; cq < 4; ++cq)
-                            *reinterpret_cast<i32x4 *>(&BX[bx_index(n, kb64 * 4 + cq)]) =
-                                bytes_of_16((uint32_t)(col >> (16 * cq)) & 0xFFFFu);
-                    }
-                    __syncthreads();
-#pragma unroll
-                    for (int k32 = 0; k32 < 8; ++k32) {
-                        const int sh = (k32 & 1) * 32 + (lane >> 5) * 16;
-                        const uint32_t x0 = (uint32_t)(a0.w[k32 >> 1] >> sh) & 0xFFFFu;
-                        const uint32_t x1 = (uint32_t)(a1.w[k32 >> 1] >> sh) & 0xFFFFu;
-                        const bool any0 = __ballot(x0 != 0) != 0, any1 = __ballot(x1 != 0) != 0;
-                        if (!any0 && !any1) continue;
-                        const i32x4 fa0 = bytes_of_16(x0), fa1 = bytes_of_16(x1);
-                        const int chunk = k32 * 2 + (lane >> 5);
-#pragma unroll
-                        for (int cb = 0; cb < 4; ++cb) {
-                            const int n = ch * 128 + cb * 32 + (lane & 31);
-                            const i32x4 fb = *reinterpret_cast<const i32x4 *>(&BX[bx_index(n, chunk)]);
-                            if (any0) acc[0][cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa0, fb, acc[0][cb], 0, 0, 0);
-                            if (any1) acc[1][cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa1, fb, acc[1][cb], 0, 0, 0);
-                        }
-                        instr += 4 * ((uint32_t)any0 + (uint32_t)any1);
-                    }
+        // The item's candidate k-tiles, 64 (term, occupancy word) pairs at a
+        // time: wave 0 reads the masks once and lays the candidates out in LDS,
+        // so every wave walks the same list (the head matrices may be updated
+        // by other work-groups meanwhile), and the next tile's A and B rows can
+        // be fetched while the current one is transposed and multiplied.
+        const uint32_t ow = (T + 31) / 32, ntw = (t1 - t0) * ow;
+        for (uint32_t tw0 = 0; tw0 < ntw; tw0 += WAVE) {
+            __syncthreads();  // the previous chunk's list is no longer read
+            if (wv == 0) {
+                const uint32_t j = tw0 + lane;
+                uint32_t kbw = 0;
+                if (j < ntw) {
+                    const uint32_t tt = t0 + j / ow, w = j % ow;
+                    kbw = mats[terms[tt].left[parity]].occR[I * ow + w] & mats[terms[tt].right[parity]].occC[J * ow + w];
                 }
+                const uint32_t incl = wave_scan_incl((uint32_t)__popc(kbw));
+                uint32_t pos = incl - (uint32_t)__popc(kbw);
+                for (uint32_t x = kbw; x; x &= x - 1)
+                    s_cand[pos++] = ((j - tw0) << 16) | ((j % ow) * 32 + (uint32_t)__builtin_ctz(x));
+                if (lane == 63) s_ncand = incl;
+            }
+            __syncthreads();
+            const uint32_t ncand = s_ncand;
+            if (!ncand) continue;
+            auto tile_ptrs = [&](uint32_t e, const uint64_t *&At, const uint64_t *&Bt) {
+                const uint32_t jj = tw0 + (e >> 16), tt = t0 + jj / ow, K = e & 0xffffu;
+                At = mats[terms[tt].left[parity]].bits + (size_t)(I * T + K) * TILE_WORDS;
+                Bt = mats[terms[tt].right[parity]].bits + (size_t)(K * T + J) * TILE_WORDS;
+            };
+            const uint64_t *At, *Bt;
+            tile_ptrs(s_cand[0], At, Bt);
+            Row4 bcur = t < TILE ? load_row(Bt + t * TROW_WORDS) : Row4{{0, 0, 0, 0}};
+            Row4 a0 = load_row(At + (rs * 64 + (lane & 31)) * TROW_WORDS);
+            Row4 a1 = load_row(At + (rs * 64 + 32 + (lane & 31)) * TROW_WORDS);
+            for (uint32_t ci = 0; ci < ncand; ++ci) {
+                ++tp;
+                __syncthreads();  // the previous k-tile's readers of Bs / BX are done
+                if (t < TILE) Bs[t] = bcur;
+                __syncthreads();
+                // fetch the next tile while this one is transposed and multiplied
+                Row4 bnext = {{0, 0, 0, 0}}, a0n = {{0, 0, 0, 0}}, a1n = {{0, 0, 0, 0}};
+                if (ci + 1 < ncand) {
+                    tile_ptrs(s_cand[ci + 1], At, Bt);
+                    if (t < TILE) bnext = load_row(Bt + t * TROW_WORDS);
+                    a0n = load_row(At + (rs * 64 + (lane & 31)) * TROW_WORDS);
+                    a1n = load_row(At + (rs * 64 + 32 + (lane & 31)) * TROW_WORDS);
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const int blk = 2 * wv + i, kb64 = blk >> 2, cb64 = blk & 3;
+                    const uint64_t col = wave_transpose64(Bs[kb64 * 64 + lane].w[cb64]);
+                    const int n = cb64 * 64 + lane;  // this lane's B column; bits = k in kb64*64 ..
+#pragma unroll
+                    for (int cq = 0; cq < 4; ++cq)
+                        *reinterpret_cast<i32x4 *>(&BX[bx_index(n, kb64 * 4 + cq)]) =
+                            bytes_of_16((uint32_t)(col >> (16 * cq)) & 0xFFFFu);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k32 = 0; k32 < 8; ++k32) {
+                    const int sh = (k32 & 1) * 32 + (lane >> 5) * 16;
+                    const uint32_t x0 = (uint32_t)(a0.w[k32 >> 1] >> sh) & 0xFFFFu;
+                    const uint32_t x1 = (uint32_t)(a1.w[k32 >> 1] >> sh) & 0xFFFFu;
+                    const bool any0 = __ballot(x0 != 0) != 0, any1 = __ballot(x1 != 0) != 0;
+                    if (!any0 && !any1) continue;
+                    const i32x4 fa0 = bytes_of_16(x0), fa1 = bytes_of_16(x1);
+                    const int chunk = k32 * 2 + (lane >> 5);
+#pragma unroll
+                    for (int cb = 0; cb < 4; ++cb) {
+                        const int n = ch * 128 + cb * 32 + (lane & 31);
+                        const i32x4 fb = *reinterpret_cast<const i32x4 *>(&BX[bx_index(n, chunk)]);
+                        if (any0) acc[0][cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa0, fb, acc[0][cb], 0, 0, 0);
+                        if (any1) acc[1][cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa1, fb, acc[1][cb], 0, 0, 0);
+                    }
+                    instr += 4 * ((uint32_t)any0 + (uint32_t)any1);
+                }
+                bcur = bnext;
+                a0 = a0n;
+                a1 = a1n;
             }
         }
         // "> 0" per accumulator register, packed by ballots: register r of block

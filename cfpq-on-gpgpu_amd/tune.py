@@ -63,8 +63,9 @@ def tuned_opts(n: int, grammar_text: str) -> dict:
     return dict(ent["opts"]) if ent else {}
 
 
-def tune(pkg, cf, query, repeats: int = 3, log=print) -> dict:
-    """Tune on `query` (a gen.Query); returns the best option dict and stores it."""
+def tune(pkg, cf, query, repeats: int = 3, log=print, passes: int = 1) -> dict:
+    """Tune on `query` (a gen.Query); returns the best option dict and stores it.
+    `passes` > 1 walks the parameters again from the best point found."""
     eng = pkg.Engine(0)
     gr = cf.Grammar(query.grammar, text=True)
     mats = gr.load(eng, cf.Graph.from_query(query))
@@ -96,17 +97,18 @@ def tune(pkg, cf, query, repeats: int = 3, log=print) -> dict:
     best_ms = run(best)
     log(f"[tune] start {best} -> {best_ms:.3f} ms")
     trace = [{"opts": dict(best), "ms": best_ms}]
-    for name, values in SPACE:
-        for v in values:
-            if v == best[name]:
-                continue
-            cand = dict(best)
-            cand[name] = v
-            ms = run(cand)
-            trace.append({"opts": dict(cand), "ms": ms})
-            log(f"[tune] {name}={v} -> {ms:.3f} ms")
-            if ms < best_ms * 0.98:
-                best, best_ms = cand, ms
+    for _ in range(passes):
+        for name, values in SPACE:
+            for v in values:
+                if v == best[name]:
+                    continue
+                cand = dict(best)
+                cand[name] = v
+                ms = run(cand)
+                trace.append({"opts": dict(cand), "ms": ms})
+                log(f"[tune] {name}={v} -> {ms:.3f} ms")
+                if ms < best_ms * 0.98:
+                    best, best_ms = cand, ms
     for c in init:
         c.free()
     gr.free()
