@@ -298,7 +298,14 @@ def bench_batch(args, pkg, gen, cf):
     golden = allg.get("config3") or allg["config3_first16"]
     for q, g in zip(qs, golden):
         assert q.digest() == g["digest"], "workload generator drifted from the golden fixture"
-    opts = {"kernel": args.kernel}
+    tune = importlib.import_module("cfpq-on-gpgpu_amd.tune")
+    if args.tune:
+        tune.tune_batch(pkg, cf, qs, repeats=1, log=log, passes=args.tune_passes)
+        os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+        with open(tune.CACHE) as f, open(os.path.join(ROOT, "gpurun_out", "tuned.json"), "w") as g:
+            g.write(f.read())
+    opts = tune.tuned_batch_opts(qs)
+    opts["kernel"] = args.kernel
     if args.iters_per_sync:
         opts["iters_per_sync"] = args.iters_per_sync
     if args.graph:
@@ -365,6 +372,7 @@ def bench_batch(args, pkg, gen, cf):
         "data": "synthetic (seeded generator, same inputs as the CPU oracle)",
         "config": {"workload": "batch512_n256_8192_dyck_sg", "queries": len(qs), "engine": "batched delta fixpoint",
                    "graph": bool(args.graph), "parallelism": "single GPU"},
+        "engine_opts": opts,
         "batched_fixpoint_ms_device": round(st["fixpoint_ms"], 4),
         "batched_steps": st["iterations"],
         "sequential_ms": round(seq_ms, 4),

@@ -63,6 +63,40 @@ def tuned_opts(n: int, grammar_text: str) -> dict:
     return dict(ent["opts"]) if ent else {}
 
 
+def batch_key(queries) -> str:
+    h = hashlib.sha1("".join(q.digest() for q in queries).encode()).hexdigest()[:8]
+    return f"batch{len(queries)}:{h}"
+
+
+def _descend(run, log, passes: int):
+    """Coordinate descent from DEFAULTS; returns (best opts, ms, trace)."""
+    best = dict(DEFAULTS)
+    best_ms = run(best)
+    log(f"[tune] start {best} -> {best_ms:.3f} ms")
+    trace = [{"opts": dict(best), "ms": best_ms}]
+    for _ in range(passes):
+        for name, values in SPACE:
+            for v in values:
+                if v == best[name]:
+                    continue
+                cand = dict(best)
+                cand[name] = v
+                ms = run(cand)
+                trace.append({"opts": dict(cand), "ms": ms})
+                log(f"[tune] {name}={v} -> {ms:.3f} ms")
+                if ms < best_ms * 0.98:
+                    best, best_ms = cand, ms
+    return best, best_ms, trace
+
+
+def _store(key, name, best, best_ms, trace, log):
+    cache = load_cache()
+    cache[key] = {"workload": name, "opts": best, "ms": round(best_ms, 4), "trace": trace}
+    with open(CACHE, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+    log(f"[tune] best {best} -> {best_ms:.3f} ms (cached as {key})")
+
+
 def tune(pkg, cf, query, repeats: int = 3, log=print, passes: int = 1) -> dict:
     """Tune on `query` (a gen.Query); returns the best option dict and stores it.
     `passes` > 1 walks the parameters again from the best point found."""
@@ -93,30 +127,58 @@ def tune(pkg, cf, query, repeats: int = 3, log=print, passes: int = 1) -> dict:
             raise AssertionError(f"tuning candidate {opts} changed the result")
         return statistics.median(times)
 
-    best = dict(DEFAULTS)
-    best_ms = run(best)
-    log(f"[tune] start {best} -> {best_ms:.3f} ms")
-    trace = [{"opts": dict(best), "ms": best_ms}]
-    for _ in range(passes):
-        for name, values in SPACE:
-            for v in values:
-                if v == best[name]:
-                    continue
-                cand = dict(best)
-                cand[name] = v
-                ms = run(cand)
-                trace.append({"opts": dict(cand), "ms": ms})
-                log(f"[tune] {name}={v} -> {ms:.3f} ms")
-                if ms < best_ms * 0.98:
-                    best, best_ms = cand, ms
+    best, best_ms, trace = _descend(run, log, passes)
     for c in init:
         c.free()
     gr.free()
     eng.close()
-    key = workload_key(query.n, query.grammar)
-    cache = load_cache()
-    cache[key] = {"workload": query.name, "opts": best, "ms": round(best_ms, 4), "trace": trace}
-    with open(CACHE, "w") as f:
-        json.dump(cache, f, indent=1, sort_keys=True)
-    log(f"[tune] best {best} -> {best_ms:.3f} ms (cached as {key})")
+    _store(workload_key(query.n, query.grammar), query.name, best, best_ms, trace, log)
     return best
+
+
+def tune_batch(pkg, cf, queries, repeats: int = 1, log=print, passes: int = 1) -> dict:
+    """Tune the batched fixpoint of `queries` (gen.Query list); cached under batch_key."""
+    eng = pkg.Engine(0)
+    grs, inits = [], []
+    for q in queries:
+        gr = cf.Grammar(q.grammar, text=True)
+        mats = gr.load(eng, cf.Graph.from_query(q))
+        cp = []
+        for m in mats:
+            c = eng.matrix(q.n)
+            c.copy_from(m)
+            cp.append(c)
+        grs.append(gr)
+        inits.append(cp)
+    work = [(gr.rule_triples(), gr.matrices) for gr in grs]
+    ref = None
+
+    def run(opts) -> float:
+        nonlocal ref
+        eng.set_opts(**opts)
+        times = []
+        for _ in range(repeats):
+            for gr, cp in zip(grs, inits):
+                for m, c in zip(gr.matrices, cp):
+                    m.copy_from(c)
+            times.append(eng.batch_fixpoint(work)["fixpoint_ms"])
+        res = [gr.results() for gr in grs]
+        if ref is None:
+            ref = res
+        elif res != ref:
+            raise AssertionError(f"tuning candidate {opts} changed a result")
+        return statistics.median(times)
+
+    best, best_ms, trace = _descend(run, log, passes)
+    for gr, cp in zip(grs, inits):
+        for c in cp:
+            c.free()
+        gr.free()
+    eng.close()
+    _store(batch_key(queries), f"batch of {len(queries)} queries", best, best_ms, trace, log)
+    return best
+
+
+def tuned_batch_opts(queries) -> dict:
+    ent = load_cache().get(batch_key(queries))
+    return dict(ent["opts"]) if ent else {}
