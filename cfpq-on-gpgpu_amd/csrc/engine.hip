@@ -67,6 +67,8 @@ struct cfpq_ctx {
     size_t d_heads_cap = 0;
     TermDev *d_terms = nullptr;  // [delta terms | full terms]
     size_t d_terms_cap = 0;
+    TermPtr *d_tptr = nullptr;   // [parity][delta terms | full terms]
+    size_t d_tptr_cap = 0;
     uint32_t *d_ids = nullptr;   // id lists for clear/copy kernels
     size_t d_ids_cap = 0;
 
@@ -232,7 +234,7 @@ extern "C" int cfpq_destroy(cfpq_ctx *ctx) {
     for (auto g : ctx->graph_exec)
         if (g) (void)hipGraphExecDestroy(g);
     void *dev[] = {ctx->stage, ctx->ring, ctx->scratch64, ctx->items, ctx->units, ctx->mitems, ctx->lists[0], ctx->lists[1],
-                   ctx->d_mats, ctx->d_heads, ctx->d_terms, ctx->d_ids, ctx->sched_w, ctx->dirty, ctx->dirty_off,
+                   ctx->d_mats, ctx->d_heads, ctx->d_terms, ctx->d_tptr, ctx->d_ids, ctx->sched_w, ctx->dirty, ctx->dirty_off,
                    ctx->tr_x, ctx->tr_y, ctx->tr_rule, ctx->tr_item, ctx->tr_count, ctx->t_master, ctx->t_m1,
                    ctx->t_m2, ctx->t_grad, ctx->t_loss, ctx->t_x, ctx->t_y, ctx->t_rule};
     for (void *p : dev)
@@ -602,7 +604,8 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const int fz = (fuse && ctx->opts.fuse_apply) ? 1 : 0;
     fuse = fz != 0;
     if (ctx->opts.sparse_mode == 0) {
-        hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
+        const TermPtr *tp = ctx->d_tptr + (size_t)q * (plan.terms_delta.size() + plan.terms_full.size());
+        hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un, touched, chunk,
                            fz);
     } else if (ctx->opts.sparse_batch == 1) {
         hipLaunchKernelGGL(k_product_sparse<1>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
@@ -677,6 +680,14 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     std::vector<TermDev> terms(plan.terms_delta);
     terms.insert(terms.end(), plan.terms_full.begin(), plan.terms_full.end());
     HIPCHK(hipMemcpyAsync(ctx->d_terms, terms.data(), terms.size() * sizeof(TermDev), hipMemcpyHostToDevice, ctx->stream));
+    CHK(grow(ctx, &ctx->d_tptr, &ctx->d_tptr_cap, 2 * terms.size()));
+    std::vector<TermPtr> tptr(2 * terms.size());
+    for (int par = 0; par < 2; ++par)
+        for (size_t t = 0; t < terms.size(); ++t) {
+            const MatDev &L = plan.mats[terms[t].left[par]], &R = plan.mats[terms[t].right[par]];
+            tptr[par * terms.size() + t] = TermPtr{L.occQ, R.occC, L.rmask, L.colq, L.bits, R.rmask, R.bits, R.cnt};
+        }
+    HIPCHK(hipMemcpyAsync(ctx->d_tptr, tptr.data(), tptr.size() * sizeof(TermPtr), hipMemcpyHostToDevice, ctx->stream));
     std::vector<uint32_t> ids(plan.all_delta[0]);
     ids.insert(ids.end(), plan.all_delta[1].begin(), plan.all_delta[1].end());
     for (uint32_t x = 0; x < (uint32_t)plan.mats.size(); ++x) ids.push_back(x);  // at plan.ids_all_off
@@ -733,6 +744,7 @@ static void accumulate(StepStats &s, const StepCounters &c) {
 static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &plan) {
     const cfpq_opts &o = ctx->opts;
     return {(uint64_t)(uintptr_t)ctx->d_mats, (uint64_t)(uintptr_t)ctx->d_heads, (uint64_t)(uintptr_t)ctx->d_terms,
+            (uint64_t)(uintptr_t)ctx->d_tptr,
             (uint64_t)(uintptr_t)ctx->d_ids, (uint64_t)(uintptr_t)ctx->units, (uint64_t)(uintptr_t)ctx->items,
             (uint64_t)(uintptr_t)ctx->mitems, (uint64_t)(uintptr_t)ctx->lists[0], (uint64_t)(uintptr_t)ctx->lists[1],
             (uint64_t)(uintptr_t)ctx->ring, plan.heads.size(), plan.mats.size(), plan.terms_delta.size(),

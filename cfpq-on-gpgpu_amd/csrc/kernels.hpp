@@ -27,11 +27,40 @@ struct alignas(32) Row4 {
     uint64_t w[4];
 };
 
+// Tile rows and metadata live in device memory (global address space).  The
+// pointers reach the kernels inside descriptor tables, so the compiler cannot
+// tell their address space and would emit flat instructions, which count
+// against both the vector-memory and the LDS wait counters: every LDS access
+// of a wave (shuffles, the binary searches, the row accumulator) would then
+// also wait for its outstanding loads.  These helpers state the global
+// address space.  Never pass them an LDS pointer.
+#define GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T gld(const T *p) {
+    return *(const GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T *p, T v) {
+    *(GLOBAL T *)p = v;
+}
+__device__ __forceinline__ uint32_t g_or(uint32_t *p, uint32_t v) {
+    return __hip_atomic_fetch_or((GLOBAL uint32_t *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t g_or(uint64_t *p, uint64_t v) {
+    return __hip_atomic_fetch_or((GLOBAL uint64_t *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t g_add(uint32_t *p, uint32_t v) {
+    return __hip_atomic_fetch_add((GLOBAL uint32_t *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// (through a vector type: a struct copy would be lowered as a flat memcpy)
+typedef uint64_t u64x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ Row4 load_row(const uint64_t *p) {
-    return *reinterpret_cast<const Row4 *>(p);
+    const u64x4 v = *(const GLOBAL u64x4 *)p;
+    return Row4{{v.x, v.y, v.z, v.w}};
 }
 __device__ __forceinline__ void store_row(uint64_t *p, const Row4 &v) {
-    *reinterpret_cast<Row4 *>(p) = v;
+    *(GLOBAL u64x4 *)p = u64x4{v.w[0], v.w[1], v.w[2], v.w[3]};
 }
 __device__ __forceinline__ void or_row(Row4 &a, const Row4 &b) {
     a.w[0] |= b.w[0];
@@ -131,11 +160,11 @@ __device__ __forceinline__ Row4 ld_const_row(const uint64_t *p) {
 }
 
 __device__ __forceinline__ void set_occ(const MatDev &M, uint32_t I, uint32_t J) {
-    atomicOr(&M.occR[I * M.ow + (J >> 5)], 1u << (J & 31));
-    atomicOr(&M.occC[J * M.ow + (I >> 5)], 1u << (I & 31));
+    g_or(&M.occR[I * M.ow + (J >> 5)], 1u << (J & 31));
+    g_or(&M.occC[J * M.ow + (I >> 5)], 1u << (I & 31));
 }
 __device__ __forceinline__ void set_occq(const MatDev &M, uint32_t q, uint32_t I, uint32_t J) {
-    atomicOr(&M.occQ[(q * M.T + I) * M.ow + (J >> 5)], 1u << (J & 31));
+    g_or(&M.occQ[(q * M.T + I) * M.ow + (J >> 5)], 1u << (J & 31));
 }
 
 // Epilogue of one 64-row quarter q of C tile p = (I, J) of a head: the new
@@ -177,17 +206,17 @@ __device__ __forceinline__ QuarterOut quarter_epilogue(const MatDev &O, const Ma
     if (bal) {
         const Row4 cu = wave_or_row(nw);
         if (lane == 0) {
-            O.rmask[(size_t)p * 4 + q] = bal;
+            gst(&O.rmask[(size_t)p * 4 + q], bal);
             store_row(colq_of(O, p, q), cu);
             set_occq(O, q, I, J);
             if (fuse) {
-                const uint64_t old = M.rmask[(size_t)p * 4 + q];
-                M.rmask[(size_t)p * 4 + q] = old | bal;
+                const uint64_t old = gld(&M.rmask[(size_t)p * 4 + q]);
+                gst(&M.rmask[(size_t)p * 4 + q], old | bal);
                 Row4 mc = load_row(colq_of(M, p, q));
                 or_row(mc, cu);
                 store_row(colq_of(M, p, q), mc);
                 if (!old) set_occq(M, q, I, J);
-                if (atomicAdd(&M.cnt[p], pc) == 0) set_occ(M, I, J);
+                if (g_add(&M.cnt[p], pc) == 0) set_occ(M, I, J);
             }
         }
     }
@@ -211,7 +240,7 @@ __device__ __forceinline__ QuarterOut quarter_epilogue_shared(const MatDev &O, c
         for (int w = 0; w < 4; ++w) {
             nw.w[w] = acc.w[w] & ~m.w[w];
             if (nw.w[w]) {
-                const uint64_t old = atomicOr((unsigned long long *)(O.bits + off + w), (unsigned long long)nw.w[w]);
+                const uint64_t old = g_or(O.bits + off + w, nw.w[w]);
                 mine += (uint32_t)__popcll(nw.w[w] & ~old);
             }
         }
@@ -222,11 +251,11 @@ __device__ __forceinline__ QuarterOut quarter_epilogue_shared(const MatDev &O, c
     if (bal) {
         const Row4 cu = wave_or_row(nw);
         if (lane == 0) {
-            atomicOr((unsigned long long *)&O.rmask[(size_t)p * 4 + q], (unsigned long long)bal);
+            g_or(&O.rmask[(size_t)p * 4 + q], bal);
             uint64_t *cq = colq_of(O, p, q);
 #pragma unroll
             for (int w = 0; w < 4; ++w)
-                if (cu.w[w]) atomicOr((unsigned long long *)(cq + w), (unsigned long long)cu.w[w]);
+                if (cu.w[w]) g_or(cq + w, cu.w[w]);
             set_occq(O, q, I, J);
         }
     }
@@ -286,13 +315,13 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
         const MatDev L = mats[terms[t].left[parity]];
         const MatDev R = mats[terms[t].right[parity]];
         for (uint32_t w = 0; w < L.ow; ++w) {
-            const uint32_t col = R.occC[J * R.ow + w];
+            const uint32_t col = gld(&R.occC[J * R.ow + w]);
             if (!col) continue;
-            if (L.occD[I * L.ow + w] & col) dense = true;
-            if (L.occX[I * L.ow + w] & col) mfma = true;
+            if (gld(&L.occD[I * L.ow + w]) & col) dense = true;
+            if (gld(&L.occX[I * L.ow + w]) & col) mfma = true;
 #pragma unroll
             for (uint32_t q = 0; q < QUARTERS; ++q) {
-                const uint32_t k = L.occQ[(q * T + I) * L.ow + w] & col;
+                const uint32_t k = gld(&L.occQ[(q * T + I) * L.ow + w]) & col;
                 if (k) qmask |= 1u << q;
                 qcnt[q] += __popc(k);
             }
@@ -345,7 +374,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
     uint32_t qcnt[QUARTERS] = {0, 0, 0, 0};
     if (p < T * T) {
         const uint32_t I = p / T, J = p - I * T;
-        if (mats[target].cnt[p] < TILE_BITS) {
+        if (gld(&mats[target].cnt[p]) < TILE_BITS) {
             scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma, qcnt);
         }
     }
@@ -458,7 +487,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
         const MatDev M = mats[uniform(heads[h].target)];
         const QuarterOut qo = quarter_epilogue(O, M, p, q, I, J, acc, fuse, out_rows);
         if (qo.bal && lane == 0) {
-            const uint32_t old = atomicAdd(&O.cnt[p], qo.pc);
+            const uint32_t old = g_add(&O.cnt[p], qo.pc);
             if (old == 0) {
                 set_occ(O, I, J);
                 touched[atomicAdd(&c->n_touched, 1u)] = ((uint64_t)h << 40) | p;
@@ -505,6 +534,13 @@ __device__ __forceinline__ int select_bit(uint64_t m, uint32_t j) {
     return r;
 }
 
+// The operand arrays of one term, staged per scan lane in LDS for the filter
+// and gather stages (a subset of TermPtr).
+struct LaneOps {
+    const uint64_t *lrmask, *lcolq, *lbits, *rrmask, *rbits;
+    const uint32_t *rcnt;
+};
+
 // Lane-parallel variant: the same work unit (head, C tile, 64-row quarter),
 // but the wave's lanes work on different candidate k-tiles at once.
 //   scan    lane l reads the candidate mask of (term, occupancy word) pair l
@@ -526,7 +562,7 @@ __device__ __forceinline__ int select_bit(uint64_t m, uint32_t j) {
 // wave, however the rows are spread over the candidates.
 __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__restrict__ mats,
                                                             const HeadDev *__restrict__ heads,
-                                                            const TermDev *__restrict__ terms, int parity,
+                                                            const TermPtr *__restrict__ tptr, int parity,
                                                             StepCounters *__restrict__ c,
                                                             const uint64_t *__restrict__ units,
                                                             uint64_t *__restrict__ touched, uint32_t chunk,
@@ -537,6 +573,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     __shared__ const uint64_t *s_ab[WAVE];
     __shared__ const uint64_t *s_bb[WAVE];
     __shared__ uint32_t s_end[WAVE], s_kb[WAVE], s_cend[WAVE];
+    __shared__ LaneOps s_ops[WAVE];
     const int lane = threadIdx.x & 63;
     const uint64_t clk0 = __builtin_amdgcn_s_memtime();
     const uint32_t n_units = c->n_units;
@@ -548,7 +585,12 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     const uint32_t n_claim = (n_units + 63u) & ~63u;
     uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0, hits_n = 0, max_cands = 0;
     uint64_t new_bits = 0;  // lane 0: new bits of this wave's units
-    for (uint32_t u0 = wave_next(&c->unit_next, chunk); u0 < n_claim; u0 = wave_next(&c->unit_next, chunk)) {
+    uint32_t u0 = wave_next(&c->unit_next, chunk);
+    while (u0 < n_claim) {
+    // the next claim is taken now and read after this chunk, so the atomic's
+    // round trip overlaps the chunk's loads instead of preceding them
+    uint32_t u_next = 0;
+    if (lane == 0) u_next = atomicAdd(&c->unit_next, chunk);
     const uint32_t u1 = (u0 + chunk < n_claim) ? u0 + chunk : n_claim;
     for (uint32_t x = u0; x < u1; ++x) {
         const uint32_t u = (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
@@ -563,6 +605,9 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         const uint32_t t0 = full ? ld_const(&heads[h].f0) : ld_const(&heads[h].t0);
         const uint32_t t1 = full ? ld_const(&heads[h].f1) : ld_const(&heads[h].t1);
         const uint32_t I = p / T, J = p - (p / T) * T;
+        // the output descriptors are read now, while the unit's first loads are in flight
+        const MatDev O = mats[uniform(heads[h].out[parity])];
+        const MatDev M = mats[uniform(heads[h].target)];
         const uint32_t tp_unit0 = tp;
         uint32_t round = 0;  // candidate rounds seen; this sub-unit takes rounds = sub (mod nsub)
         acc_s[lane] = Row4{{0, 0, 0, 0}};
@@ -574,17 +619,19 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             //    a prefix sum of their popcounts
             const uint32_t j = tw0 + lane;
             uint32_t kbw = 0;
+            LaneOps ops = {};
             if (j < ntw) {
                 const uint32_t t = t0 + j / ow, w = j % ow;
-                const uint32_t *oq = mats[terms[t].left[parity]].occQ;
-                const uint32_t *oc = mats[terms[t].right[parity]].occC;
-                kbw = oq[(q * T + I) * ow + w] & oc[J * ow + w];
+                const TermPtr e = tptr[t];
+                kbw = gld(&e.occQ[(q * T + I) * ow + w]) & gld(&e.occC[J * ow + w]);
+                ops = LaneOps{e.lrmask, e.lcolq, e.lbits, e.rrmask, e.rbits, e.rcnt};
             }
             const uint32_t incw = wave_scan_incl((uint32_t)__popc(kbw));
             const uint32_t total = __shfl(incw, 63, 64);
             if (!total) continue;
             s_kb[lane] = kbw;
             s_cend[lane] = incw;
+            s_ops[lane] = ops;
             __syncthreads();
             for (uint32_t cb = 0; cb < total; cb += WAVE) {
                 if ((round++ % nsub) != sub) continue;
@@ -603,21 +650,20 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     }
                     const uint32_t kb = s_kb[lo];
                     const int bit = select_bit(kb, g - (s_cend[lo] - (uint32_t)__popc(kb)));
-                    const uint32_t jj = tw0 + lo, t = t0 + jj / ow;
+                    const uint32_t jj = tw0 + lo;
                     const uint32_t K = (jj % ow) * 32 + (uint32_t)bit;
-                    const MatDev &L = mats[terms[t].left[parity]];
-                    const MatDev &R = mats[terms[t].right[parity]];
+                    const LaneOps o = s_ops[lo];
                     const size_t lt = (size_t)I * T + K, rt = (size_t)K * T + J;
-                    brm = load_row(R.rmask + rt * 4);
-                    const Row4 cu = load_row(L.colq + (lt * QUARTERS + q) * TROW_WORDS);
-                    arm = L.rmask[lt * 4 + q];
+                    brm = load_row(o.rrmask + rt * 4);
+                    const Row4 cu = load_row(o.lcolq + (lt * QUARTERS + q) * TROW_WORDS);
+                    arm = gld(&o.lrmask[lt * 4 + q]);
                     const bool hit = ((cu.w[0] & brm.w[0]) | (cu.w[1] & brm.w[1]) | (cu.w[2] & brm.w[2]) |
                                       (cu.w[3] & brm.w[3])) != 0;
                     if (!hit) arm = 0;
-                    abase = L.bits + lt * TILE_WORDS + (size_t)q * WAVE * TROW_WORDS;
-                    bbase = R.bits + rt * TILE_WORDS;
+                    abase = o.lbits + lt * TILE_WORDS + (size_t)q * WAVE * TROW_WORDS;
+                    bbase = o.rbits + rt * TILE_WORDS;
                     // a full B tile needs no gathers: bit 0 of the (8-byte aligned) pointer marks it
-                    if (R.cnt[rt] == TILE_BITS) bbase = (const uint64_t *)((uintptr_t)bbase | 1u);
+                    if (gld(&o.rcnt[rt]) == TILE_BITS) bbase = (const uint64_t *)((uintptr_t)bbase | 1u);
                 }
                 // 3. stage the hits in lane order and spread their (candidate,
                 //    A row) pairs over the lanes
@@ -693,12 +739,10 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         __syncthreads();
         const Row4 acc = acc_s[lane];
         max_cands = max(max_cands, tp - tp_unit0);
-        const MatDev O = mats[uniform(heads[h].out[parity])];
-        const MatDev M = mats[uniform(heads[h].target)];
         const QuarterOut qo = nsub > 1 ? quarter_epilogue_shared(O, M, p, q, I, J, acc, out_rows)
                                        : quarter_epilogue(O, M, p, q, I, J, acc, fuse, out_rows);
         if (qo.pc && lane == 0) {
-            const uint32_t old = atomicAdd(&O.cnt[p], qo.pc);
+            const uint32_t old = g_add(&O.cnt[p], qo.pc);
             if (old == 0) {
                 set_occ(O, I, J);
                 touched[atomicAdd(&c->n_touched, 1u)] = ((uint64_t)h << 40) | p;
@@ -707,6 +751,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         }
         __syncthreads();  // the accumulator is reset by the next unit
     }
+    u0 = uniform(u_next);
     }
     const uint32_t ar = wave_sum(arows), br = wave_sum(brows), orow = wave_sum(out_rows), hs = wave_sum(hits_n);
     if (lane == 0) {
@@ -1158,7 +1203,7 @@ __global__ __launch_bounds__(WAVE) void k_apply(const MatDev *__restrict__ mats,
         const MatDev M = mats[uniform(heads[h].target)];
         const MatDev O = mats[uniform(heads[h].out[parity])];
         const uint32_t T = uniform(heads[h].T), I = p / T, J = p - (p / T) * T;
-        const uint64_t rm = O.rmask[(size_t)p * 4 + q];
+        const uint64_t rm = gld(&O.rmask[(size_t)p * 4 + q]);
         if ((rm >> lane) & 1ull) {
             const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
             Row4 m = load_row(M.bits + off);
@@ -1166,16 +1211,16 @@ __global__ __launch_bounds__(WAVE) void k_apply(const MatDev *__restrict__ mats,
             store_row(M.bits + off, m);
         }
         if (lane == 0 && rm) {
-            const uint64_t old = M.rmask[(size_t)p * 4 + q];
-            M.rmask[(size_t)p * 4 + q] = old | rm;
+            const uint64_t old = gld(&M.rmask[(size_t)p * 4 + q]);
+            gst(&M.rmask[(size_t)p * 4 + q], old | rm);
             Row4 cu = load_row(colq_of(M, p, q));
             or_row(cu, load_row(colq_of(O, p, q)));
             store_row(colq_of(M, p, q), cu);
             if (!old) set_occq(M, q, I, J);
         }
         if (lane == 0 && q == 0) {
-            const uint32_t old = M.cnt[p];
-            M.cnt[p] = old + O.cnt[p];
+            const uint32_t old = gld(&M.cnt[p]);
+            gst(&M.cnt[p], old + gld(&O.cnt[p]));
             if (!old) set_occ(M, I, J);
         }
     }
@@ -1197,14 +1242,14 @@ __global__ __launch_bounds__(WAVE) void k_clear(const MatDev *__restrict__ mats,
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
         const MatDev O = mats[uniform(heads[h].out[parity])];
-        const uint64_t rm = O.rmask[(size_t)p * 4 + q];
+        const uint64_t rm = gld(&O.rmask[(size_t)p * 4 + q]);
         if ((rm >> lane) & 1ull)
             store_row(O.bits + (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS, Row4{{0, 0, 0, 0}});
         if (lane == 0 && rm) {
-            O.rmask[(size_t)p * 4 + q] = 0;
+            gst(&O.rmask[(size_t)p * 4 + q], (uint64_t)0);
             store_row(colq_of(O, p, q), Row4{{0, 0, 0, 0}});
         }
-        if (lane == 0 && q == 0) O.cnt[p] = 0;
+        if (lane == 0 && q == 0) gst(&O.cnt[p], 0u);
     }
 }
 

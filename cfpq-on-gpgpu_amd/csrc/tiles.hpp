@@ -66,6 +66,20 @@ struct TermDev {
     uint32_t rule;
 };
 
+// The operand arrays one term of one step parity reads, resolved on the host
+// so that the sparse kernel reaches them in one dependent load instead of
+// term -> matrix id -> descriptor -> array.  Table layout: [parity][term].
+struct TermPtr {
+    const uint32_t *occQ;   // left operand: per-quarter tile-row occupancy
+    const uint32_t *occC;   // right operand: tile-column occupancy
+    const uint64_t *lrmask; // left operand: row mask per (tile, quarter)
+    const uint64_t *lcolq;  // left operand: column union per (tile, quarter)
+    const uint64_t *lbits;  // left operand tiles
+    const uint64_t *rrmask; // right operand: row mask per tile
+    const uint64_t *rbits;  // right operand tiles
+    const uint32_t *rcnt;   // right operand: bits per tile
+};
+
 // Per-step counters (a ring of them lives in device memory).
 struct StepCounters {
     uint32_t n_units;      // sparse work units (head, tile, quarter) built
