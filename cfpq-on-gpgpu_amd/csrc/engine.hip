@@ -173,6 +173,8 @@ static void normalise(cfpq_opts *o) {
     if (o->sparse_batch != 1 && o->sparse_batch != 2 && o->sparse_batch != 4) o->sparse_batch = 2;
     if (o->sparse_waves_per_cu <= 0) o->sparse_waves_per_cu = 32;
     if (o->sparse_chunk <= 0) o->sparse_chunk = 1;
+    // a divisor of 64 (claims stay inside one shard's block of 64 units)
+    while (64 % o->sparse_chunk) --o->sparse_chunk;
     if (o->iters_per_sync > RING / 2) o->iters_per_sync = RING / 2;
     if (o->sched_threshold == 0) o->sched_threshold = 6554;  // P(active) < 0.1
 }
@@ -848,6 +850,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         pending = 0;
         uint32_t used = 0;
         bool quiet = false;  // a step added no bit
+        for (int s = 0; s < k; ++s) fold_shards(ctx->ring_host[slots[s]]);
         for (int s = 0; s < k; ++s) {
             const StepCounters &c = ctx->ring_host[slots[s]];
             // every executed step counts for the deferred-tile balance (a step
@@ -1105,6 +1108,7 @@ extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const
         if (e != hipSuccess) rc = fail(ctx, CFPQ_E_DEVICE, "add_mul: %s", hipGetErrorString(e));
     }
     if (rc == CFPQ_OK) {
+        fold_shards(ctx->ring_host[0]);
         if (changed) *changed = ctx->ring_host[0].new_bits != 0;
         pool_give(ctx, S);
     } else {

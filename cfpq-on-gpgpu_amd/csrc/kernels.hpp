@@ -146,6 +146,71 @@ __device__ __forceinline__ Row4 wave_or_row(Row4 v) {
     return v;
 }
 
+// Inclusive prefix sum over the lanes of the wave.
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Value of lane l (l wave-uniform), through the scalar unit: no LDS traffic.
+__device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ uint64_t lane_read64(uint64_t v, uint32_t l) {
+    return ((uint64_t)lane_read((uint32_t)(v >> 32), l) << 32) | lane_read((uint32_t)v, l);
+}
+
+// Lanes at or below which a ballot is read lane by lane (v_readlane per set
+// bit) instead of by log-step shuffles through the LDS crossbar: the output
+// rows and filter hits of a sparse unit are usually a handful.
+constexpr int FEW_LANES = 8;
+
+// Column union of the rows v of the lanes in `bal` (a ballot), and the sum
+// of `cnt` over those lanes.  Whole wave, uniform results.
+struct RowsUnion {
+    Row4 cu;
+    uint32_t sum;
+};
+__device__ __forceinline__ RowsUnion rows_union(const Row4 &v, uint32_t cnt, uint64_t bal) {
+    RowsUnion r = {{{0, 0, 0, 0}}, 0};
+    if (__popcll(bal) <= FEW_LANES) {
+        for (uint64_t b = bal; b; b &= b - 1) {
+            const uint32_t l = (uint32_t)__builtin_ctzll(b);
+#pragma unroll
+            for (int w = 0; w < 4; ++w) r.cu.w[w] |= lane_read64(v.w[w], l);
+            r.sum += lane_read(cnt, l);
+        }
+    } else {
+        r.cu = wave_or_row(v);
+        r.sum = wave_sum(cnt);
+    }
+    return r;
+}
+
+// Inclusive prefix sum of v over the lanes, where only lanes in `bal` (a
+// ballot) may hold non-zero values; `total` gets the wave's sum.
+__device__ __forceinline__ uint32_t scan_ballot(uint32_t v, uint64_t bal, uint32_t &total) {
+    const int lane = threadIdx.x & 63;
+    if (__popcll(bal) <= FEW_LANES) {
+        uint32_t run = 0, incl = 0;
+        for (uint64_t b = bal; b; b &= b - 1) {
+            const uint32_t l = (uint32_t)__builtin_ctzll(b);
+            run += lane_read(v, l);
+            if ((uint32_t)lane >= l) incl = run;
+        }
+        total = run;
+        return incl;
+    }
+    const uint32_t incl = wave_scan_incl(v);
+    total = __shfl(incl, 63, 64);
+    return incl;
+}
+
 __device__ __forceinline__ uint64_t *colq_of(const MatDev &M, uint32_t p, uint32_t q) {
     return M.colq + ((size_t)p * QUARTERS + q) * TROW_WORDS;
 }
@@ -202,9 +267,10 @@ __device__ __forceinline__ QuarterOut quarter_epilogue(const MatDev &O, const Ma
         }
     }
     const uint64_t bal = __ballot(row_nonzero(nw));
-    const uint32_t pc = wave_sum(row_popc(nw));
+    const RowsUnion ru = rows_union(nw, row_popc(nw), bal);
+    const uint32_t pc = ru.sum;
     if (bal) {
-        const Row4 cu = wave_or_row(nw);
+        const Row4 cu = ru.cu;
         if (lane == 0) {
             gst(&O.rmask[(size_t)p * 4 + q], bal);
             store_row(colq_of(O, p, q), cu);
@@ -247,9 +313,10 @@ __device__ __forceinline__ QuarterOut quarter_epilogue_shared(const MatDev &O, c
         out_rows += row_nonzero(nw);
     }
     const uint64_t bal = __ballot(row_nonzero(nw));
-    const uint32_t pc = wave_sum(mine);
+    const RowsUnion ru = rows_union(nw, mine, bal);
+    const uint32_t pc = ru.sum;
     if (bal) {
-        const Row4 cu = wave_or_row(nw);
+        const Row4 cu = ru.cu;
         if (lane == 0) {
             g_or(&O.rmask[(size_t)p * 4 + q], bal);
             uint64_t *cq = colq_of(O, p, q);
@@ -267,10 +334,8 @@ __device__ __forceinline__ QuarterOut quarter_epilogue_shared(const MatDev &O, c
 // ---------------------------------------------------------------------------
 
 __global__ void k_begin(StepCounters *ring, uint32_t slot) {
-    if (threadIdx.x == 0) {
-        StepCounters z = {};
-        ring[slot] = z;
-    }
+    uint64_t *w = reinterpret_cast<uint64_t *>(ring + slot);
+    for (uint32_t i = threadIdx.x; i < sizeof(StepCounters) / 8; i += blockDim.x) gst(w + i, (uint64_t)0);
 }
 
 // occD / occX of matrices ids[0..nid): one thread per 32-tile word of a tile row.
@@ -509,15 +574,6 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
 }
 
 // Inclusive prefix sum over the 64 lanes.
-__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(v, o, 64);
-        if (lane >= o) v += u;
-    }
-    return v;
-}
 
 // Position of the j-th (0-based) set bit of m (j < popcount(m)), by halving.
 __device__ __forceinline__ int select_bit(uint64_t m, uint32_t j) {
@@ -582,15 +638,31 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     // heavy quarter) go to claims far apart within their block of 64, i.e. to
     // different waves, while blocks -- and the locality of the list order --
     // are kept.  x runs over whole blocks; entries past the list end are skipped.
+    // Block b belongs to shard b mod SHARDS; a wave claims `chunk` (a divisor
+    // of 64) claims at a time from its own shard's cursor, then from the
+    // other shards' in turn once its own is used up.
     const uint32_t n_claim = (n_units + 63u) & ~63u;
+    const uint32_t nblk = n_claim / 64u;
+    const uint32_t home = blockIdx.x % SHARDS;
+    CounterShard *shards = c->shard;
     uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0, hits_n = 0, max_cands = 0;
     uint64_t new_bits = 0;  // lane 0: new bits of this wave's units
-    uint32_t u0 = wave_next(&c->unit_next, chunk);
-    while (u0 < n_claim) {
-    // the next claim is taken now and read after this chunk, so the atomic's
-    // round trip overlaps the chunk's loads instead of preceding them
-    uint32_t u_next = 0;
-    if (lane == 0) u_next = atomicAdd(&c->unit_next, chunk);
+    uint32_t sh = home, used_up = 0;
+    while (used_up < SHARDS) {
+    uint32_t kc = 0xffffffffu;
+    if (lane == 0) {
+        // a stale (smaller) reading of the cursor only costs a wasted atomic
+        const uint32_t seen = __hip_atomic_load(&shards[sh].unit_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((seen / 64u) * SHARDS + sh < nblk) kc = atomicAdd(&shards[sh].unit_next, chunk);
+    }
+    kc = uniform(kc);
+    const uint32_t blk = kc == 0xffffffffu ? nblk : (kc / 64u) * SHARDS + sh;
+    if (blk >= nblk) {  // this shard's blocks are all claimed (cursors only grow)
+        sh = (sh + 1) % SHARDS;
+        ++used_up;
+        continue;
+    }
+    const uint32_t u0 = blk * 64u + kc % 64u;
     const uint32_t u1 = (u0 + chunk < n_claim) ? u0 + chunk : n_claim;
     for (uint32_t x = u0; x < u1; ++x) {
         const uint32_t u = (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
@@ -626,8 +698,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                 kbw = gld(&e.occQ[(q * T + I) * ow + w]) & gld(&e.occC[J * ow + w]);
                 ops = LaneOps{e.lrmask, e.lcolq, e.lbits, e.rrmask, e.rbits, e.rcnt};
             }
-            const uint32_t incw = wave_scan_incl((uint32_t)__popc(kbw));
-            const uint32_t total = __shfl(incw, 63, 64);
+            uint32_t total;
+            const uint32_t incw = scan_ballot((uint32_t)__popc(kbw), __ballot(kbw != 0), total);
             if (!total) continue;
             s_kb[lane] = kbw;
             s_cend[lane] = incw;
@@ -672,8 +744,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                 if (!hmask) continue;
                 const uint32_t nh = __popcll(hmask);
                 const uint32_t cnt = hit ? (uint32_t)__popcll(arm) : 0u;
-                const uint32_t incl = wave_scan_incl(cnt);
-                const uint32_t pairs = __shfl(incl, 63, 64);
+                uint32_t pairs;
+                const uint32_t incl = scan_ballot(cnt, hmask, pairs);
                 if (hit) {
                     const uint32_t slot = __popcll(hmask & ((1ull << lane) - 1ull));
                     s_arm[slot] = arm;
@@ -751,20 +823,20 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         }
         __syncthreads();  // the accumulator is reset by the next unit
     }
-    u0 = uniform(u_next);
     }
     const uint32_t ar = wave_sum(arows), br = wave_sum(brows), orow = wave_sum(out_rows), hs = wave_sum(hits_n);
     if (lane == 0) {
-        if (tp) atomicAdd((unsigned long long *)&c->tile_products, (unsigned long long)tp);
-        if (ar) atomicAdd((unsigned long long *)&c->a_rows, (unsigned long long)ar);
-        if (br) atomicAdd((unsigned long long *)&c->b_rows, (unsigned long long)br);
-        if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
-        if (hs) atomicAdd((unsigned long long *)&c->cand_hits, (unsigned long long)hs);
-        if (max_cands) atomicMax(&c->max_unit_cands, max_cands);
-        if (new_bits) atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)new_bits);
+        CounterShard &S = shards[home];
+        if (tp) atomicAdd((unsigned long long *)&S.tile_products, (unsigned long long)tp);
+        if (ar) atomicAdd((unsigned long long *)&S.a_rows, (unsigned long long)ar);
+        if (br) atomicAdd((unsigned long long *)&S.b_rows, (unsigned long long)br);
+        if (orow) atomicAdd((unsigned long long *)&S.out_rows, (unsigned long long)orow);
+        if (hs) atomicAdd((unsigned long long *)&S.cand_hits, (unsigned long long)hs);
+        if (max_cands) atomicMax(&S.max_unit_cands, max_cands);
+        if (new_bits) atomicAdd((unsigned long long *)&S.new_bits, (unsigned long long)new_bits);
         const uint64_t busy = __builtin_amdgcn_s_memtime() - clk0;
-        atomicAdd((unsigned long long *)&c->sparse_busy, (unsigned long long)busy);
-        atomicMax((unsigned long long *)&c->sparse_max, (unsigned long long)busy);
+        atomicAdd((unsigned long long *)&S.busy, (unsigned long long)busy);
+        atomicMax((unsigned long long *)&S.busy_max, (unsigned long long)busy);
     }
 }
 

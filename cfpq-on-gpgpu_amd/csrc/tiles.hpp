@@ -80,6 +80,19 @@ struct TermPtr {
     const uint32_t *rcnt;   // right operand: bits per tile
 };
 
+// Statistics and the work cursor of the lane-parallel sparse kernel, one
+// copy per shard of its work-groups.  Shard = blockIdx.x mod SHARDS: the
+// dispatcher deals work-groups round-robin over the 8 XCDs, so a shard is the
+// waves of one XCD, and the same-address atomics of thousands of waves are
+// spread over 8 cache lines instead of queueing on one.  The host folds the
+// shards into the step's totals.
+constexpr int SHARDS = 8;
+struct alignas(128) CounterShard {
+    uint64_t tile_products, a_rows, b_rows, out_rows, cand_hits, new_bits, busy, busy_max;
+    uint32_t unit_next;       // claims taken from this shard's blocks of 64 units
+    uint32_t max_unit_cands;
+};
+
 // Per-step counters (a ring of them lives in device memory).
 struct StepCounters {
     uint32_t n_units;      // sparse work units (head, tile, quarter) built
@@ -116,7 +129,25 @@ struct StepCounters {
     uint64_t out_rows_dense;
     uint64_t sparse_busy;     // sum over sparse-product waves of their busy clocks (s_memtime)
     uint64_t sparse_max;      // longest sparse-product wave (clocks)
+    CounterShard shard[SHARDS];  // lane-parallel sparse kernel (folded into the fields above by the host)
 };
+
+// Host side: add the shards' partial sums into the step's totals.
+inline void fold_shards(StepCounters &c) {
+    for (int s = 0; s < SHARDS; ++s) {
+        CounterShard &h = c.shard[s];
+        c.tile_products += h.tile_products;
+        c.a_rows += h.a_rows;
+        c.b_rows += h.b_rows;
+        c.out_rows += h.out_rows;
+        c.cand_hits += h.cand_hits;
+        c.new_bits += h.new_bits;
+        c.sparse_busy += h.busy;
+        if (h.busy_max > c.sparse_max) c.sparse_max = h.busy_max;
+        if (h.max_unit_cands > c.max_unit_cands) c.max_unit_cands = h.max_unit_cands;
+        h = CounterShard{};
+    }
+}
 
 constexpr int RING = 64;  // step-counter ring length
 
