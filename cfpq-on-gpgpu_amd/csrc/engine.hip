@@ -568,13 +568,18 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const TermDev *terms = ctx->d_terms;
     uint64_t *touched = ctx->lists[slot & 1];
     hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
+    // with the dense and MFMA paths both off, every item is sparse: the
+    // heavy-tile bitsets are neither built nor read, and the dense and MFMA
+    // kernels are not launched
+    const int heavy = (dense_threshold(ctx->opts) <= TILE_BITS || mfma_threshold(ctx->opts) <= TILE_BITS) ? 1 : 0;
     // dense-tile bitsets of every matrix of the plan (operands of this step)
     const uint32_t nall = (uint32_t)plan.mats.size();
     const uint32_t *ids_all = ctx->d_ids + plan.ids_all_off;
-    hipLaunchKernelGGL(k_dense_occ, dim3(std::max<uint32_t>(1, (uint32_t)((plan.max_occ_words + 255) / 256)),
-                                         std::min<uint32_t>(nall, 65535)),
-                       dim3(256), 0, ctx->stream, (const MatDev *)ctx->d_mats, ids_all, nall,
-                       dense_threshold(ctx->opts), mfma_threshold(ctx->opts));
+    if (heavy)
+        hipLaunchKernelGGL(k_dense_occ, dim3(std::max<uint32_t>(1, (uint32_t)((plan.max_occ_words + 255) / 256)),
+                                             std::min<uint32_t>(nall, 65535)),
+                           dim3(256), 0, ctx->stream, (const MatDev *)ctx->d_mats, ids_all, nall,
+                           dense_threshold(ctx->opts), mfma_threshold(ctx->opts));
     const uint32_t bx = (uint32_t)((plan.max_tiles_per_head + BLOCK - 1) / BLOCK);
     // heavy sparse units are split over several waves (lane-parallel kernel,
     // head update not fused); the units buffer holds 4 entries per head tile
@@ -589,11 +594,11 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                                (uint32_t)ctx->tr_cap);
         hipLaunchKernelGGL(k_build_sched, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems, sa, split,
-                           split_budget);
+                           split_budget, heavy);
     } else {
         hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, (int)full_pass, c, ctx->units, ctx->items,
-                           ctx->mitems, split, split_budget);
+                           ctx->mitems, split, split_budget, heavy);
     }
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
     // product kernels, timed: ev[4i] sparse ev[4i+1] dense ev[4i+2] mfma ev[4i+3]
@@ -620,13 +625,15 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                            fz);
     }
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 1], ctx->stream));
-    hipLaunchKernelGGL(k_product_dense, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                       (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
-                       dense_threshold(ctx->opts), m4r_threshold(ctx->opts), fz);
+    if (heavy)
+        hipLaunchKernelGGL(k_product_dense, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                           (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
+                           dense_threshold(ctx->opts), m4r_threshold(ctx->opts), fz);
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 2], ctx->stream));
-    hipLaunchKernelGGL(k_product_mfma, dim3((uint32_t)ctx->num_cu), dim3(MFMA_BLOCK), 0, ctx->stream,
-                       (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, terms, q, c,
-                       (const uint64_t *)ctx->mitems, touched, fz);
+    if (heavy)
+        hipLaunchKernelGGL(k_product_mfma, dim3((uint32_t)ctx->num_cu), dim3(MFMA_BLOCK), 0, ctx->stream,
+                           (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, terms, q, c,
+                           (const uint64_t *)ctx->mitems, touched, fz);
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 3], ctx->stream));
     if (use_sched && ctx->tr_cap)
         hipLaunchKernelGGL(k_trace_label, dim3((uint32_t)ctx->num_cu), dim3(256), 0, ctx->stream,

@@ -372,18 +372,22 @@ __device__ __forceinline__ void wave_append(bool want, uint64_t val, uint32_t *c
 }
 
 // Candidate scan of the terms [t0, t1) for C tile (I, J): which quarters have
-// candidates, whether some A tile is heavy enough for the dense / MFMA paths.
+// candidates, whether some A tile is heavy enough for the dense / MFMA paths
+// (only with `heavy`: the heavy-tile bitsets are not maintained when both
+// paths are off).
 __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, const TermDev *__restrict__ terms,
                                            uint32_t t0, uint32_t t1, int parity, uint32_t T, uint32_t I, uint32_t J,
-                                           uint32_t &qmask, bool &dense, bool &mfma, uint32_t *qcnt) {
+                                           uint32_t &qmask, bool &dense, bool &mfma, uint32_t *qcnt, int heavy) {
     for (uint32_t t = t0; t < t1; ++t) {
         const MatDev L = mats[terms[t].left[parity]];
         const MatDev R = mats[terms[t].right[parity]];
         for (uint32_t w = 0; w < L.ow; ++w) {
             const uint32_t col = gld(&R.occC[J * R.ow + w]);
             if (!col) continue;
-            if (gld(&L.occD[I * L.ow + w]) & col) dense = true;
-            if (gld(&L.occX[I * L.ow + w]) & col) mfma = true;
+            if (heavy) {
+                if (gld(&L.occD[I * L.ow + w]) & col) dense = true;
+                if (gld(&L.occX[I * L.ow + w]) & col) mfma = true;
+            }
 #pragma unroll
             for (uint32_t q = 0; q < QUARTERS; ++q) {
                 const uint32_t k = gld(&L.occQ[(q * T + I) * L.ow + w]) & col;
@@ -396,25 +400,47 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
 
 // Append the sparse units of one (head, C tile): quarters in qmask, a heavy
 // quarter (more than SPLIT_CANDS candidates) as up to SPLIT_MAX sub-units
-// while the split budget lasts.  Whole wave (wave_append).
+// while the split budget lasts.  Whole wave: the single units of all four
+// quarters take one atomic on the list count per wave (the count is one
+// address that every wave of the build hits).
 __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const uint32_t *qcnt, uint64_t base,
                                              int split, uint32_t split_budget, StepCounters *__restrict__ c,
                                              uint64_t *__restrict__ units) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t nsub[QUARTERS];
+    uint64_t single[QUARTERS];
+    uint32_t total = 0;
 #pragma unroll
     for (uint32_t q = 0; q < QUARTERS; ++q) {
         const bool wq = want && ((qmask >> q) & 1u);
-        uint32_t nsub = 1;
+        nsub[q] = wq ? 1u : 0u;
         if (wq && split) {
-            nsub = (qcnt[q] + SPLIT_CANDS - 1) / SPLIT_CANDS;
-            nsub = nsub > SPLIT_MAX ? SPLIT_MAX : (nsub < 1 ? 1 : nsub);
-            if (nsub > 1 && atomicAdd(&c->split_extra, nsub - 1) + (nsub - 1) > split_budget) nsub = 1;
+            uint32_t ns = (qcnt[q] + SPLIT_CANDS - 1) / SPLIT_CANDS;
+            ns = ns > SPLIT_MAX ? SPLIT_MAX : (ns < 1 ? 1 : ns);
+            if (ns > 1 && atomicAdd(&c->split_extra, ns - 1) + (ns - 1) > split_budget) ns = 1;
+            nsub[q] = ns;
         }
-        const uint64_t u = base | ((uint64_t)q);
-        wave_append(wq && nsub == 1, u, &c->n_units, units);
-        if (wq && nsub > 1) {
-            const uint32_t at = atomicAdd(&c->n_units, nsub);
-            for (uint32_t sidx = 0; sidx < nsub; ++sidx)
-                units[at + sidx] = u | ((uint64_t)sidx << 55) | ((uint64_t)(nsub - 1) << 59);
+        single[q] = __ballot(nsub[q] == 1);
+        total += (uint32_t)__popcll(single[q]);
+    }
+    if (total) {
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(&c->n_units, total);
+        at = uniform(at);
+#pragma unroll
+        for (uint32_t q = 0; q < QUARTERS; ++q) {
+            if (nsub[q] == 1) units[at + __popcll(single[q] & below)] = base | (uint64_t)q;
+            at += (uint32_t)__popcll(single[q]);
+        }
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < QUARTERS; ++q) {
+        if (nsub[q] > 1) {
+            const uint64_t u = base | (uint64_t)q;
+            const uint32_t at = atomicAdd(&c->n_units, nsub[q]);
+            for (uint32_t sidx = 0; sidx < nsub[q]; ++sidx)
+                units[at + sidx] = u | ((uint64_t)sidx << 55) | ((uint64_t)(nsub[q] - 1) << 59);
             atomicAdd(&c->n_split, 1u);
         }
     }
@@ -427,7 +453,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
                                                  StepCounters *__restrict__ c, uint64_t *__restrict__ units,
                                                  uint64_t *__restrict__ dense_items,
                                                  uint64_t *__restrict__ mfma_items, int split,
-                                                 uint32_t split_budget) {
+                                                 uint32_t split_budget, int heavy) {
     const uint32_t h = blockIdx.y;
     const uint32_t T = uniform(heads[h].T), target = uniform(heads[h].target);
     const uint32_t t0 = uniform(full_pass ? heads[h].f0 : heads[h].t0);
@@ -440,7 +466,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
     if (p < T * T) {
         const uint32_t I = p / T, J = p - I * T;
         if (gld(&mats[target].cnt[p]) < TILE_BITS) {
-            scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma, qcnt);
+            scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma, qcnt, heavy);
         }
     }
     wave_append(mfma, flag | ((uint64_t)h << 40) | p, &c->n_mfma, mfma_items);
@@ -642,6 +668,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     // of 64) claims at a time from its own shard's cursor, then from the
     // other shards' in turn once its own is used up.
     const uint32_t n_claim = (n_units + 63u) & ~63u;
+    if ((uint64_t)blockIdx.x * chunk >= n_claim) return;  // more waves than claims
     const uint32_t nblk = n_claim / 64u;
     const uint32_t home = blockIdx.x % SHARDS;
     CounterShard *shards = c->shard;
@@ -940,6 +967,7 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
     uint32_t nbc = 0;
     const int r = threadIdx.x, wv = r >> 6, lane = r & 63;
     const uint32_t n_items = c->n_dense;
+    if (blockIdx.x >= n_items) return;  // surplus work-groups would only queue on the cursor
     uint32_t tp = 0, tpd = 0, arows = 0, brows = 0, brows_lds = 0, out_rows = 0, lookups = 0;
     uint64_t new_bits = 0;  // thread 0: new bits of this work-group's items
     for (uint32_t idx = wg_next(&c->dense_next, &s_idx); idx < n_items; idx = wg_next(&c->dense_next, &s_idx)) {
@@ -1102,6 +1130,7 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
     const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
     const int rs = wv & 3, ch = wv >> 2;
     const uint32_t n_items = c->n_mfma;
+    if (blockIdx.x >= n_items) return;  // surplus work-groups would only queue on the cursor
     uint32_t tp = 0, instr = 0, out_rows = 0;
     uint64_t new_bits = 0;  // thread 0: new bits of this work-group's items
     for (uint32_t idx = wg_next(&c->mfma_next, &s_idx); idx < n_items; idx = wg_next(&c->mfma_next, &s_idx)) {

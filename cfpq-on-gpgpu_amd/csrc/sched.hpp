@@ -253,7 +253,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                                                        StepCounters *__restrict__ c, uint64_t *__restrict__ units,
                                                        uint64_t *__restrict__ dense_items,
                                                        uint64_t *__restrict__ mfma_items, SchedArgs sa, int split,
-                                                       uint32_t split_budget) {
+                                                       uint32_t split_budget, int heavy) {
     __shared__ float sw[MLP_PARAMS];
     if (sa.w) mlp_load(sa.w, sw);
     const uint32_t h = blockIdx.y;
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
         const bool dirty = (*dw & dbit) != 0;
         const MatDev M = mats[target];
         if (M.cnt[p] < TILE_BITS) {
-            scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma, qcnt);
+            scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma, qcnt, heavy);
             bool evaluate = qmask != 0;
             if (evaluate && !sa.flush && (sa.w || sa.trace_x)) {
                 // score every rule of the head; the item's score is the best one
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                 qmask = 0;
                 dense = mfma = false;
                 qcnt[0] = qcnt[1] = qcnt[2] = qcnt[3] = 0;
-                scan_terms(mats, terms, f0, f1, parity, T, I, J, qmask, dense, mfma, qcnt);
+                scan_terms(mats, terms, f0, f1, parity, T, I, J, qmask, dense, mfma, qcnt, heavy);
                 evaluate = qmask != 0;
                 ++fulls;
             }
