@@ -371,7 +371,7 @@ def bench_batch(args, pkg, gen, cf):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (seeded generator, same inputs as the CPU oracle)",
         "config": {"workload": "batch512_n256_8192_dyck_sg", "queries": len(qs), "engine": "batched delta fixpoint",
-                   "graph": bool(args.graph), "parallelism": "single GPU"},
+                   "graph": bool(opts.get("use_graph")), "parallelism": "single GPU"},
         "engine_opts": opts,
         "batched_fixpoint_ms_device": round(st["fixpoint_ms"], 4),
         "batched_steps": st["iterations"],

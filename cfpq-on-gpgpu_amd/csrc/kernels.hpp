@@ -382,7 +382,11 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
         const MatDev L = mats[terms[t].left[parity]];
         const MatDev R = mats[terms[t].right[parity]];
         for (uint32_t w = 0; w < L.ow; ++w) {
-            const uint32_t col = gld(&R.occC[J * R.ow + w]);
+            uint32_t col = gld(&R.occC[J * R.ow + w]);
+            if (!col) continue;
+            // the per-quarter and heavy-tile bitsets are subsets of the
+            // tile-row occupancy: one load rules most words out
+            col &= gld(&L.occR[I * L.ow + w]);
             if (!col) continue;
             if (heavy) {
                 if (gld(&L.occD[I * L.ow + w]) & col) dense = true;

@@ -34,14 +34,17 @@ SPACE = [
     ("sparse_mode", [0, 1]),
     ("sparse_batch", [1, 2, 4]),
     ("sparse_waves_per_cu", [2, 4, 8, 12, 16, 32]),
-    ("sparse_chunk", [1, 4, 16]),
+    ("sparse_chunk", [1, 2, 4, 8, 16]),
     ("dense_bits", [512, 2048, 8192, 0xFFFFFFF0]),
     ("m4r_bits", [8192, 16384, 32768, 0xFFFFFFF0]),
     ("mfma_bits", [8192, 16384, 32768, 0xFFFFFFF0]),
     ("iters_per_sync", [2, 4, 8]),
     ("fuse_apply", [0, 1]),
 ]
-DEFAULTS = {"sparse_mode": 0, "sparse_batch": 2, "sparse_waves_per_cu": 32, "sparse_chunk": 1, "dense_bits": 2048, "m4r_bits": 16384,
+# Graph replay (use_graph) is left to the caller: it saves about 7% of an
+# SG-forest fixpoint at N = 65,536 in launch gaps, but per-kernel event
+# timing needs the eager launch path.
+DEFAULTS = {"sparse_mode": 0, "sparse_batch": 2, "sparse_waves_per_cu": 8, "sparse_chunk": 4, "dense_bits": 2048, "m4r_bits": 16384,
             "mfma_bits": 32768, "iters_per_sync": 4, "fuse_apply": 0}
 
 
