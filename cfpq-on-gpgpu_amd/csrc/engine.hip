@@ -567,6 +567,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const uint32_t nh = (uint32_t)plan.heads.size();
     const TermDev *terms = ctx->d_terms;
     uint64_t *touched = ctx->lists[slot & 1];
+    const uint32_t seg_cap = (uint32_t)plan.total_head_tiles;
     hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
     // with the dense and MFMA paths both off, every item is sparse: the
     // heavy-tile bitsets are neither built nor read, and the dense and MFMA
@@ -612,8 +613,8 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     fuse = fz != 0;
     if (ctx->opts.sparse_mode == 0) {
         const TermPtr *tp = ctx->d_tptr + (size_t)q * (plan.terms_delta.size() + plan.terms_full.size());
-        hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un, touched, chunk,
-                           fz);
+        hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
+                           TouchedLists{touched, seg_cap}, chunk, fz);
     } else if (ctx->opts.sparse_batch == 1) {
         hipLaunchKernelGGL(k_product_sparse<1>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
                            fz);
@@ -640,7 +641,8 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, (const StepCounters *)c, sa);
     if (!fuse)
         hipLaunchKernelGGL(k_apply, dim3((uint32_t)ctx->num_cu * 8), dim3(WAVE), 0, ctx->stream,
-                           (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, c, (const uint64_t *)touched);
+                           (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, c,
+                           TouchedLists{touched, seg_cap});
     // retire the delta this step consumed (buffer D_q)
     const uint32_t *ids_q = ctx->d_ids + (q ? plan.all_delta[0].size() : 0);
     const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
@@ -653,7 +655,8 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     } else {
         hipLaunchKernelGGL(k_clear, dim3((uint32_t)ctx->num_cu * 8), dim3(WAVE), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, 1 - q,
-                           (const StepCounters *)(ctx->ring + prev_slot), c, (const uint64_t *)ctx->lists[prev_slot & 1]);
+                           (const StepCounters *)(ctx->ring + prev_slot), c,
+                           TouchedLists{ctx->lists[prev_slot & 1], seg_cap});
         hipLaunchKernelGGL(k_clear_occ, dim3(4, std::min<uint32_t>(nids_q, 65535)), dim3(256), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, ids_q, nids_q);
     }
@@ -677,11 +680,13 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     CHK(grow(ctx, &ctx->items, &ctx->items_cap, plan.total_head_tiles));
     CHK(grow(ctx, &ctx->units, &ctx->units_cap, 2 * QUARTERS * plan.total_head_tiles));
     CHK(grow(ctx, &ctx->mitems, &ctx->mitems_cap, plan.total_head_tiles));
-    if (plan.total_head_tiles > ctx->lists_cap || !ctx->lists[0]) {
+    // touched-tile lists: 1 + SHARDS segments of one entry per head tile
+    const size_t list_need = (size_t)(1 + SHARDS) * plan.total_head_tiles;
+    if (list_need > ctx->lists_cap || !ctx->lists[0]) {
         size_t cap0 = ctx->lists_cap;
-        CHK(grow(ctx, &ctx->lists[0], &cap0, plan.total_head_tiles));
+        CHK(grow(ctx, &ctx->lists[0], &cap0, list_need));
         size_t cap1 = ctx->lists_cap;
-        CHK(grow(ctx, &ctx->lists[1], &cap1, plan.total_head_tiles));
+        CHK(grow(ctx, &ctx->lists[1], &cap1, list_need));
         ctx->lists_cap = std::min(cap0, cap1);
     }
     HIPCHK(hipMemcpyAsync(ctx->d_mats, plan.mats.data(), plan.mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, ctx->stream));
@@ -758,7 +763,7 @@ static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &p
             (uint64_t)(uintptr_t)ctx->mitems, (uint64_t)(uintptr_t)ctx->lists[0], (uint64_t)(uintptr_t)ctx->lists[1],
             (uint64_t)(uintptr_t)ctx->ring, plan.heads.size(), plan.mats.size(), plan.terms_delta.size(),
             plan.all_delta[0].size(), plan.all_delta[1].size(), plan.max_tiles_per_head, plan.max_occ_words,
-            plan.ids_all_off, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
+            plan.ids_all_off, plan.total_head_tiles, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
             (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk, (uint64_t)o.sched, o.sched_threshold,
             (uint64_t)o.sparse_mode, (uint64_t)o.fuse_apply,
             (uint64_t)ctx->sched_have, (uint64_t)(uintptr_t)ctx->sched_w, (uint64_t)(uintptr_t)ctx->dirty,
@@ -922,7 +927,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         hipLaunchKernelGGL(k_clear, dim3((uint32_t)ctx->num_cu * 8), dim3(WAVE), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, 1 - q,
                            (const StepCounters *)(ctx->ring + prev_slot), ctx->ring + slot,
-                           (const uint64_t *)ctx->lists[prev_slot & 1]);
+                           TouchedLists{ctx->lists[prev_slot & 1], (uint32_t)plan.total_head_tiles});
         hipLaunchKernelGGL(k_clear_occ, dim3(4, std::min<uint32_t>(nids_q, 65535)), dim3(256), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, ids_q, nids_q);
         CHK(launch_check(ctx, "final clear"));

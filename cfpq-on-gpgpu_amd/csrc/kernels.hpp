@@ -651,7 +651,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                                             const TermPtr *__restrict__ tptr, int parity,
                                                             StepCounters *__restrict__ c,
                                                             const uint64_t *__restrict__ units,
-                                                            uint64_t *__restrict__ touched, uint32_t chunk,
+                                                            TouchedLists tl, uint32_t chunk,
                                                             int fuse) {
     __shared__ Row4 acc_s[WAVE];
     __shared__ Row4 s_brm[WAVE];
@@ -848,7 +848,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             const uint32_t old = g_add(&O.cnt[p], qo.pc);
             if (old == 0) {
                 set_occ(O, I, J);
-                touched[atomicAdd(&c->n_touched, 1u)] = ((uint64_t)h << 40) | p;
+                const uint32_t at = atomicAdd(&shards[home].n_touched, 1u);
+                gst(&tl.list[(size_t)(1 + home) * tl.seg_cap + at], ((uint64_t)h << 40) | p);
             }
             new_bits += qo.pc;
         }
@@ -1296,12 +1297,11 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
 // wave per (tile, quarter); the quarter-0 wave also merges the tile count.
 __global__ __launch_bounds__(WAVE) void k_apply(const MatDev *__restrict__ mats,
                                                 const HeadDev *__restrict__ heads, int parity,
-                                                StepCounters *__restrict__ c,
-                                                const uint64_t *__restrict__ touched) {
+                                                StepCounters *__restrict__ c, TouchedLists tl) {
     const int lane = threadIdx.x & 63;
-    const uint32_t n = c->n_touched * QUARTERS;
+    const uint32_t n = touched_total(c) * QUARTERS;
     for (uint32_t idx = blockIdx.x; idx < n; idx += gridDim.x) {  // static: the units are small and alike
-        const uint64_t it = touched[idx >> 2];
+        const uint64_t it = touched_entry(c, tl, idx >> 2);
         const uint32_t q = idx & 3u;
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
@@ -1337,12 +1337,11 @@ __global__ __launch_bounds__(WAVE) void k_apply(const MatDev *__restrict__ mats,
 __global__ __launch_bounds__(WAVE) void k_clear(const MatDev *__restrict__ mats,
                                                 const HeadDev *__restrict__ heads, int parity,
                                                 const StepCounters *__restrict__ list_ctr,
-                                                StepCounters *__restrict__ c,
-                                                const uint64_t *__restrict__ list) {
+                                                StepCounters *__restrict__ c, TouchedLists tl) {
     const int lane = threadIdx.x & 63;
-    const uint32_t n = list_ctr->n_touched * QUARTERS;
+    const uint32_t n = touched_total(list_ctr) * QUARTERS;
     for (uint32_t idx = blockIdx.x; idx < n; idx += gridDim.x) {  // static: the units are small and alike
-        const uint64_t it = list[idx >> 2];
+        const uint64_t it = touched_entry(list_ctr, tl, idx >> 2);
         const uint32_t q = idx & 3u;
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));

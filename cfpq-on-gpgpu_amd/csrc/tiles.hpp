@@ -91,6 +91,9 @@ struct alignas(128) CounterShard {
     uint64_t tile_products, a_rows, b_rows, out_rows, cand_hits, new_bits, busy, busy_max;
     uint32_t unit_next;       // claims taken from this shard's blocks of 64 units
     uint32_t max_unit_cands;
+    // entries of this shard's segment of the touched-tile list, on a cache
+    // line of its own (apart from the cursor the same waves claim from)
+    alignas(128) uint32_t n_touched;
 };
 
 // Per-step counters (a ring of them lives in device memory).
@@ -145,11 +148,38 @@ inline void fold_shards(StepCounters &c) {
         c.sparse_busy += h.busy;
         if (h.busy_max > c.sparse_max) c.sparse_max = h.busy_max;
         if (h.max_unit_cands > c.max_unit_cands) c.max_unit_cands = h.max_unit_cands;
+        c.n_touched += h.n_touched;
         h = CounterShard{};
     }
 }
 
 constexpr int RING = 64;  // step-counter ring length
+
+// Touched-tile lists: segment 0 is appended to by the dense, MFMA and
+// row-per-lane sparse kernels (count StepCounters::n_touched), segment 1 + s
+// by shard s of the lane-parallel sparse kernel (count shard[s].n_touched).
+// Each segment holds `seg_cap` entries, one per head tile of the plan.
+struct TouchedLists {
+    uint64_t *list;
+    uint32_t seg_cap;
+};
+// Entry e of the concatenated segments (e below the sum of the counts).
+__device__ __forceinline__ uint64_t touched_entry(const StepCounters *c, TouchedLists tl, uint32_t e) {
+    uint32_t n = c->n_touched;
+    if (e < n) return tl.list[e];
+    e -= n;
+    for (int s = 0; s < SHARDS; ++s) {
+        n = c->shard[s].n_touched;
+        if (e < n) return tl.list[(size_t)(1 + s) * tl.seg_cap + e];
+        e -= n;
+    }
+    return 0;  // not reached for e below the total
+}
+__device__ __forceinline__ uint32_t touched_total(const StepCounters *c) {
+    uint32_t n = c->n_touched;
+    for (int s = 0; s < SHARDS; ++s) n += c->shard[s].n_touched;
+    return n;
+}
 
 static inline uint32_t tiles_per_side(uint32_t n) { return (n + TILE - 1) / TILE; }
 static inline uint32_t occ_words(uint32_t T) { return (T + 31) / 32; }

@@ -250,6 +250,35 @@ def test_fixpoint_sg_forest_65536_golden(pkg, gen):
         gr.free()
 
 
+@pytest.mark.parametrize("opts", [
+    # sparse kernel only (no heavy-tile bitsets, no dense / MFMA launches), 8 waves per CU
+    dict(dense_bits=0xFFFFFFF0, mfma_bits=0xFFFFFFF0, sparse_waves_per_cu=8, sparse_chunk=4),
+    # one claim per atomic, 16 waves per CU: waves drain and steal across all cursor shards
+    dict(dense_bits=0xFFFFFFF0, mfma_bits=0xFFFFFFF0, sparse_waves_per_cu=16, sparse_chunk=1),
+    # 1 wave per CU: most shards are drained by stealing waves
+    dict(sparse_waves_per_cu=1, sparse_chunk=16, use_graph=True),
+])
+def test_fixpoint_sg_forest_65536_options(pkg, gen, opts):
+    """Full-size ontology workload under other launch shapes: fingerprints
+    against the golden fixture, and the step counters (summed over the sparse
+    kernel's per-XCD shards) account for exactly the bits the fixpoint added."""
+    import json
+    import os
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixpoints.json")))["config4"]
+    q = gen.config(4)
+    with pkg.Engine(0, **opts) as eng:
+        gr = cf.Grammar(q.grammar, text=True)
+        mats = gr.load(eng, cf.Graph.from_query(q))
+        before = sum(c for _, c in gr.results().values())
+        stats = eng.fixpoint(gr.rule_triples(), mats)
+        got = gr.results()
+        assert got == {k: tuple(v) for k, v in golden["fingerprints"].items()}
+        assert stats["new_bits"] == sum(c for _, c in got.values()) - before
+        assert stats["cand_hits"] <= stats["tile_products"]
+        gr.free()
+
+
 @pytest.mark.parametrize("which", ["dyck", "dense"])
 def test_fixpoint_graph_replay(pkg, gen, oracle, which):
     """Steps captured as hipGraphs and replayed: several fixpoints in a row on
