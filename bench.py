@@ -107,6 +107,31 @@ def product_roofline(stats, wname):
             "launch_ms": round(per_launch_ms, 5), "alg_bytes_per_launch": round(per_launch_bytes)}
 
 
+def measure_worklist(args, eng, gr, mats, init, want, opts):
+    """The reference driver's structure on the same GPU: its worklist loop
+    (cfpq-cpp/Grammar.h:58-69) with one cfpq_add_mul per popped rule, each
+    returning its changed flag to the host, as the reference's GPU backends do
+    (cuda/naive-gpu/Multiplication.cu:145-166).  Same kernels, no delta."""
+    eng.set_opts(**opts)
+    runs = max(1, min(args.steps, 3))
+    times, stats = [], []
+    for k in range(runs + 1):
+        for m, c in zip(mats, init):
+            m.copy_from(c)
+        eng.sync()
+        t0 = time.perf_counter()
+        calls = gr._worklist(eng)
+        eng.sync()
+        if k:  # the first run warms up
+            times.append((time.perf_counter() - t0) * 1e3)
+            stats.append({"add_mul_calls": calls})
+    ok = gr.results() == want
+    if not ok:
+        log("[bench] worklist MISMATCH vs oracle fixture")
+    log(f"[bench] worklist: {statistics.median(times):.2f} ms per fixpoint, {stats[-1]['add_mul_calls']} add_mul calls")
+    return statistics.median(times), stats, ok
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +149,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler engine")
+    ap.add_argument("--no-worklist", action="store_true",
+                    help="skip the reference-structured engine (worklist of add_mul calls)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (torch), the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -223,7 +250,9 @@ def main():
         if args.sched_threshold:
             lopts["sched_threshold"] = args.sched_threshold
         runs["learned"] = measure("learned", lopts)
-    ok_runs = {k: v for k, v in runs.items() if v[2]}
+    if not args.no_worklist:
+        runs["worklist"] = measure_worklist(args, eng, gr, mats, init, want, opts)
+    ok_runs = {k: v for k, v in runs.items() if v[2] and k != "worklist"}
     chosen = min(ok_runs, key=lambda k: ok_runs[k][0]) if ok_runs else "plain"
     ms_per_step, stats, bit_exact = runs[chosen]
     bit_exact = all(v[2] for v in runs.values())
@@ -232,6 +261,10 @@ def main():
     engines = {}
     for k, (ms, sts, ok) in runs.items():
         last = sts[-1]
+        if k == "worklist":
+            engines[k] = {"ms_per_step": round(ms, 4), "add_mul_calls": last["add_mul_calls"],
+                          "runs": len(sts), "bit_exact": ok}
+            continue
         engines[k] = {"ms_per_step": round(ms, 4), "fixpoint_ms_device_median": round(
             statistics.median(x["fixpoint_ms"] for x in sts), 4), "iterations": last["iterations"],
             "tile_products": last["tile_products"], "bit_exact": ok}
@@ -239,6 +272,8 @@ def main():
             engines[k].update({x: last[x] for x in ("sched_scored", "sched_deferred", "sched_full", "flush_steps")})
     if "learned" in runs:
         engines["learned_vs_plain_speedup"] = round(runs["plain"][0] / runs["learned"][0], 4)
+    if "worklist" in runs:
+        engines["plain_vs_worklist_speedup"] = round(runs["worklist"][0] / runs["plain"][0], 3)
 
     roofline = product_roofline(stats, wname)
 

@@ -878,13 +878,15 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             if (step_log)
                 std::fprintf(stderr,
                              "[cfpq step %u] units %u dense %u mfma %u touched %u new_bits %llu cand %llu hits %llu "
-                             "max_unit %u split %u a_rows %llu b_rows %llu busy %.3f | sparse %.3f dense %.3f mfma %.3f ms\n",
+                             "max_unit %u split %u a_rows %llu b_rows %llu busy %.3f wave_max %llu unit_max %llu clk"
+                             " | sparse %.3f dense %.3f mfma %.3f ms\n",
                              steps - k + s, c.n_units, c.n_dense, c.n_mfma, c.n_touched,
                              (unsigned long long)c.new_bits, (unsigned long long)c.tile_products,
                              (unsigned long long)c.cand_hits, c.max_unit_cands, c.n_split, (unsigned long long)c.a_rows,
                              (unsigned long long)c.b_rows,
                              c.sparse_max ? (double)c.sparse_busy / ((double)c.sparse_max * ctx->num_cu *
                                                                       ctx->opts.sparse_waves_per_cu) : 0.0,
+                             (unsigned long long)c.sparse_max, (unsigned long long)c.sparse_unit_max,
                              step_ms[0], step_ms[1], step_ms[2]);
             accumulate(tot, c);
             sched_scored += c.n_scored;

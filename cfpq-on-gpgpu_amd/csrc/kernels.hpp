@@ -419,8 +419,8 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
     for (uint32_t q = 0; q < QUARTERS; ++q) {
         const bool wq = want && ((qmask >> q) & 1u);
         nsub[q] = wq ? 1u : 0u;
-        if (wq && split) {
-            uint32_t ns = (qcnt[q] + SPLIT_CANDS - 1) / SPLIT_CANDS;
+        if (wq && split && qcnt[q] > SPLIT_CANDS) {
+            uint32_t ns = (qcnt[q] + SPLIT_ROUND - 1) / SPLIT_ROUND;
             ns = ns > SPLIT_MAX ? SPLIT_MAX : (ns < 1 ? 1 : ns);
             if (ns > 1 && atomicAdd(&c->split_extra, ns - 1) + (ns - 1) > split_budget) ns = 1;
             nsub[q] = ns;
@@ -620,6 +620,73 @@ __device__ __forceinline__ int select_bit(uint64_t m, uint32_t j) {
     return r;
 }
 
+// m with its lowest j set bits cleared (j <= popcount(m)).
+__device__ __forceinline__ Row4 bits_from(Row4 m, uint32_t j) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const uint32_t c = (uint32_t)__popcll(m.w[w]);
+        if (j >= c) {
+            j -= c;
+            m.w[w] = 0;
+        } else {
+            if (j) m.w[w] &= ~0ull << select_bit(m.w[w], j);
+            j = 0;
+        }
+    }
+    return m;
+}
+
+// Hub rows: a pair round whose pairs name at most GATHER_HEAVY B rows each
+// is gathered lane by lane; otherwise its rows are dealt out over the wave
+// in chunks of GATHER_CH rows (loads in flight per lane).
+#ifndef CFPQ_GATHER_CH
+#define CFPQ_GATHER_CH 8
+#endif
+#ifndef CFPQ_GATHER_HEAVY
+#define CFPQ_GATHER_HEAVY 32
+#endif
+constexpr int GATHER_CH = CFPQ_GATHER_CH;
+constexpr uint32_t GATHER_HEAVY = CFPQ_GATHER_HEAVY;
+
+// OR the B rows named by the lowest GATHER_CH set bits of x (tile row k is
+// bit k) into the LDS row accumulator *acc; returns the rows loaded.
+__device__ __forceinline__ uint32_t gather_chunk(Row4 x, const uint64_t *Bt, Row4 *acc) {
+    int ks[GATHER_CH];
+    int nk = 0;
+#pragma unroll
+    for (int jj = 0; jj < GATHER_CH; ++jj) {
+        int k = 0;
+        bool have = true;
+        if (x.w[0]) {
+            k = __builtin_ctzll(x.w[0]);
+            x.w[0] &= x.w[0] - 1;
+        } else if (x.w[1]) {
+            k = 64 + __builtin_ctzll(x.w[1]);
+            x.w[1] &= x.w[1] - 1;
+        } else if (x.w[2]) {
+            k = 128 + __builtin_ctzll(x.w[2]);
+            x.w[2] &= x.w[2] - 1;
+        } else if (x.w[3]) {
+            k = 192 + __builtin_ctzll(x.w[3]);
+            x.w[3] &= x.w[3] - 1;
+        } else {
+            have = false;
+        }
+        ks[jj] = k;
+        if (have) nk = jj + 1;
+    }
+    Row4 b[GATHER_CH];
+#pragma unroll
+    for (int jj = 0; jj < GATHER_CH; ++jj) b[jj] = jj < nk ? load_row(Bt + ks[jj] * TROW_WORDS) : Row4{{0, 0, 0, 0}};
+    Row4 v = {{0, 0, 0, 0}};
+#pragma unroll
+    for (int jj = 0; jj < GATHER_CH; ++jj) or_row(v, b[jj]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (v.w[i]) atomicOr((unsigned long long *)&acc->w[i], (unsigned long long)v.w[i]);
+    return (uint32_t)nk;
+}
+
 // The operand arrays of one term, staged per scan lane in LDS for the filter
 // and gather stages (a subset of TermPtr).
 struct LaneOps {
@@ -660,6 +727,9 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     __shared__ const uint64_t *s_bb[WAVE];
     __shared__ uint32_t s_end[WAVE], s_kb[WAVE], s_cend[WAVE];
     __shared__ LaneOps s_ops[WAVE];
+    __shared__ Row4 s_gx[WAVE];
+    __shared__ const uint64_t *s_gb[WAVE];
+    __shared__ uint32_t s_gr[WAVE], s_gend[WAVE];
     const int lane = threadIdx.x & 63;
     const uint64_t clk0 = __builtin_amdgcn_s_memtime();
     const uint32_t n_units = c->n_units;
@@ -678,6 +748,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     CounterShard *shards = c->shard;
     uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0, hits_n = 0, max_cands = 0;
     uint64_t new_bits = 0;  // lane 0: new bits of this wave's units
+    uint64_t unit_max = 0;  // longest unit of this wave (clocks)
     uint32_t sh = home, used_up = 0;
     while (used_up < SHARDS) {
     uint32_t kc = 0xffffffffu;
@@ -698,6 +769,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     for (uint32_t x = u0; x < u1; ++x) {
         const uint32_t u = (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
         if (u >= n_units) continue;
+        const uint64_t tu0 = __builtin_amdgcn_s_memtime();
         const uint64_t it = ld_const(units + u);
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)((it & ITEM_TILE_MASK) >> 2));
@@ -736,15 +808,20 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             s_cend[lane] = incw;
             s_ops[lane] = ops;
             __syncthreads();
-            for (uint32_t cb = 0; cb < total; cb += WAVE) {
+            // a split unit takes rounds of SPLIT_ROUND candidates, the
+            // sub-units in turn, so that a dense quarter's gathers are spread
+            // over several waves
+            const uint32_t step = nsub > 1 ? SPLIT_ROUND : WAVE;
+            for (uint32_t cb = 0; cb < total; cb += step) {
                 if ((round++ % nsub) != sub) continue;
-                tp += (total - cb < WAVE) ? total - cb : WAVE;
-                // 2. filter 64 candidates: lane g takes the g-th candidate
+                const uint32_t nc = (total - cb < step) ? total - cb : step;
+                tp += nc;
+                // 2. filter up to 64 candidates: lane g takes the g-th candidate
                 const uint32_t g = cb + lane;
                 uint64_t arm = 0;
                 Row4 brm = {{0, 0, 0, 0}};
                 const uint64_t *abase = nullptr, *bbase = nullptr;
-                if (g < total) {
+                if ((uint32_t)lane < nc) {
                     uint32_t lo = 0, hi = WAVE - 1;
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi) >> 1;
@@ -789,7 +866,11 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                 arows += cnt;
                 __syncthreads();
                 for (uint32_t base = 0; base < pairs; base += WAVE) {
+                    // 4a. lane gp loads the A row of pair gp; x = the B rows it names
                     const uint32_t gp = base + lane;
+                    Row4 x = {{0, 0, 0, 0}};
+                    uint32_t r = 0;
+                    const uint64_t *Bt = nullptr;
                     if (gp < pairs) {
                         uint32_t lo = 0, hi = nh - 1;  // owner: first slot whose range ends past gp
                         while (lo < hi) {
@@ -798,41 +879,78 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                             else lo = mid + 1;
                         }
                         const uint64_t am = s_arm[lo];
-                        const int r = select_bit(am, gp - (s_end[lo] - (uint32_t)__popcll(am)));
-                        const Row4 bm = s_brm[lo];
-                        const uint64_t *Bt = s_bb[lo];
-                        const bool bfull = ((uintptr_t)Bt & 1u) != 0;
-                        const Row4 a = bfull ? Row4{{1, 0, 0, 0}} : load_row(s_ab[lo] + r * TROW_WORDS);
-                        Row4 v = bfull ? Row4{{~0ull, ~0ull, ~0ull, ~0ull}} : Row4{{0, 0, 0, 0}};
-                        if (!bfull) {
+                        r = (uint32_t)select_bit(am, gp - (s_end[lo] - (uint32_t)__popcll(am)));
+                        Bt = s_bb[lo];
+                        if ((uintptr_t)Bt & 1u) {  // full B tile: the row becomes all ones
 #pragma unroll
-                            for (int ww = 0; ww < 4; ++ww) {
-                                uint64_t x = a.w[ww] & bm.w[ww];
-                                while (x) {
-                                    // up to four B rows in flight
-                                    int ks[4];
-                                    int nk = 0;
+                            for (int i = 0; i < 4; ++i) atomicOr((unsigned long long *)&acc_s[r].w[i], ~0ull);
+                        } else {
+                            const Row4 bm = s_brm[lo];
+                            const Row4 a = load_row(s_ab[lo] + r * TROW_WORDS);
 #pragma unroll
-                                    for (int jj = 0; jj < 4; ++jj) {
-                                        ks[jj] = ww * 64 + (x ? __builtin_ctzll(x) : 0);
-                                        if (x) {
-                                            x &= x - 1;
-                                            nk = jj + 1;
-                                        }
+                            for (int i = 0; i < 4; ++i) x.w[i] = a.w[i] & bm.w[i];
+                        }
+                    }
+                    // 4b. the B-row gathers.  Usually each lane ORs the B rows its
+                    //     own pair names, four loads in flight.  When a pair names
+                    //     more than GATHER_HEAVY rows (a hub row), the rows of all
+                    //     64 pairs are cut into chunks of GATHER_CH, numbered by a
+                    //     prefix sum and dealt out over the lanes, so the hub's
+                    //     gathers are spread over the wave instead of serializing
+                    //     one lane.
+                    if (__ballot(row_popc(x) > GATHER_HEAVY) == 0) {
+                        Row4 v = {{0, 0, 0, 0}};
+#pragma unroll
+                        for (int ww = 0; ww < 4; ++ww) {
+                            uint64_t xw = x.w[ww];
+                            while (xw) {
+                                int ks[4];
+                                int nk = 0;
+#pragma unroll
+                                for (int jj = 0; jj < 4; ++jj) {
+                                    ks[jj] = ww * 64 + (xw ? __builtin_ctzll(xw) : 0);
+                                    if (xw) {
+                                        xw &= xw - 1;
+                                        nk = jj + 1;
                                     }
-                                    Row4 b[4];
-#pragma unroll
-                                    for (int jj = 0; jj < 4; ++jj)
-                                        b[jj] = jj < nk ? load_row(Bt + ks[jj] * TROW_WORDS) : Row4{{0, 0, 0, 0}};
-#pragma unroll
-                                    for (int jj = 0; jj < 4; ++jj) or_row(v, b[jj]);
-                                    brows += nk;
                                 }
+                                Row4 b[4];
+#pragma unroll
+                                for (int jj = 0; jj < 4; ++jj)
+                                    b[jj] = jj < nk ? load_row(Bt + ks[jj] * TROW_WORDS) : Row4{{0, 0, 0, 0}};
+#pragma unroll
+                                for (int jj = 0; jj < 4; ++jj) or_row(v, b[jj]);
+                                brows += nk;
                             }
                         }
 #pragma unroll
                         for (int i = 0; i < 4; ++i)
                             if (v.w[i]) atomicOr((unsigned long long *)&acc_s[r].w[i], (unsigned long long)v.w[i]);
+                    } else {
+                        const uint32_t nch = (row_popc(x) + GATHER_CH - 1) / GATHER_CH;
+                        const uint32_t incl = wave_scan_incl(nch);
+                        const uint32_t tot = lane_read(incl, WAVE - 1);
+                        s_gx[lane] = x;
+                        s_gr[lane] = r;
+                        s_gb[lane] = Bt;
+                        s_gend[lane] = incl;
+                        __syncthreads();
+                        for (uint32_t cb = 0; cb < tot; cb += WAVE) {
+                            const uint32_t g = cb + lane;
+                            if (g < tot) {
+                                uint32_t lo = 0, hi = WAVE - 1;  // owner: first lane whose range ends past g
+                                while (lo < hi) {
+                                    const uint32_t mid = (lo + hi) >> 1;
+                                    if (s_gend[mid] > g) hi = mid;
+                                    else lo = mid + 1;
+                                }
+                                const Row4 xo = s_gx[lo];
+                                const uint32_t first = (g - (s_gend[lo] - (row_popc(xo) + GATHER_CH - 1) / GATHER_CH)) *
+                                                       GATHER_CH;
+                                brows += gather_chunk(bits_from(xo, first), s_gb[lo], &acc_s[s_gr[lo]]);
+                            }
+                        }
+                        __syncthreads();  // the chunk staging is refilled by the next pair round
                     }
                 }
                 __syncthreads();  // the hit staging is refilled by the next candidate round
@@ -853,6 +971,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             }
             new_bits += qo.pc;
         }
+        const uint64_t tu = __builtin_amdgcn_s_memtime() - tu0;
+        unit_max = tu > unit_max ? tu : unit_max;
         __syncthreads();  // the accumulator is reset by the next unit
     }
     }
@@ -869,6 +989,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         const uint64_t busy = __builtin_amdgcn_s_memtime() - clk0;
         atomicAdd((unsigned long long *)&S.busy, (unsigned long long)busy);
         atomicMax((unsigned long long *)&S.busy_max, (unsigned long long)busy);
+        if (unit_max) atomicMax((unsigned long long *)&S.unit_max, (unsigned long long)unit_max);
     }
 }
 

@@ -89,6 +89,7 @@ struct TermPtr {
 constexpr int SHARDS = 8;
 struct alignas(128) CounterShard {
     uint64_t tile_products, a_rows, b_rows, out_rows, cand_hits, new_bits, busy, busy_max;
+    uint64_t unit_max;        // longest single work unit (clocks)
     uint32_t unit_next;       // claims taken from this shard's blocks of 64 units
     uint32_t max_unit_cands;
     // entries of this shard's segment of the touched-tile list, on a cache
@@ -132,6 +133,7 @@ struct StepCounters {
     uint64_t out_rows_dense;
     uint64_t sparse_busy;     // sum over sparse-product waves of their busy clocks (s_memtime)
     uint64_t sparse_max;      // longest sparse-product wave (clocks)
+    uint64_t sparse_unit_max; // longest single sparse work unit (clocks)
     CounterShard shard[SHARDS];  // lane-parallel sparse kernel (folded into the fields above by the host)
 };
 
@@ -147,6 +149,7 @@ inline void fold_shards(StepCounters &c) {
         c.new_bits += h.new_bits;
         c.sparse_busy += h.busy;
         if (h.busy_max > c.sparse_max) c.sparse_max = h.busy_max;
+        if (h.unit_max > c.sparse_unit_max) c.sparse_unit_max = h.unit_max;
         if (h.max_unit_cands > c.max_unit_cands) c.max_unit_cands = h.max_unit_cands;
         c.n_touched += h.n_touched;
         h = CounterShard{};
@@ -193,7 +196,12 @@ constexpr uint64_t ITEM_TILE_MASK = 0xffffffffffull;
 constexpr uint64_t ITEM_FULL = 1ull << 63;
 constexpr uint32_t MAX_HEADS = 1u << 15;   // heads per plan (15-bit field)
 constexpr uint32_t SPLIT_MAX = 16;         // sub-units per heavy sparse unit
-constexpr uint32_t SPLIT_CANDS = 64;       // candidates per sub-unit round
+constexpr uint32_t SPLIT_CANDS = 64;       // a unit with more candidates than this is split
+#ifndef CFPQ_SPLIT_ROUND
+#define CFPQ_SPLIT_ROUND 64
+#endif
+// candidates per round in a split unit; the sub-units take the rounds in turn
+constexpr uint32_t SPLIT_ROUND = CFPQ_SPLIT_ROUND;
 __host__ __device__ inline uint32_t item_head(uint64_t it) { return (uint32_t)((it >> 40) & 0x7fffu); }
 __host__ __device__ inline uint32_t unit_sub(uint64_t it) { return (uint32_t)((it >> 55) & 15u); }
 __host__ __device__ inline uint32_t unit_nsub(uint64_t it) { return (uint32_t)((it >> 59) & 15u) + 1u; }

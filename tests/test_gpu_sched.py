@@ -109,3 +109,42 @@ def test_weights_roundtrip_and_errors(pkg, engine):
     assert np.array_equal(engine.sched_get_weights(), w)
     with pytest.raises(pkg.CfpqError):
         engine.sched_set_weights(w[:10])
+
+
+def _auc(score, y):
+    pos, neg = score[y > 0.5], score[y <= 0.5]
+    order = np.argsort(np.concatenate([pos, neg]), kind="stable")
+    ranks = np.empty(len(order))
+    ranks[order] = np.arange(1, len(order) + 1)
+    return (ranks[:len(pos)].sum() - len(pos) * (len(pos) + 1) / 2) / max(len(pos) * len(neg), 1)
+
+
+def test_trained_network_gains_over_untrained(pkg, gen, engine):
+    """What the trained network gains over the scheduler without it, on
+    workloads it was not trained on (the forest generator at another seed and
+    size, a dense-stress grammar of another seed).  Trace rows come from the
+    plain fixpoint; each network defers its lowest-scored 10% of the items.
+    The trained one must rank far better than an untrained network of the
+    same shape and defer far fewer items that would have added bits (every
+    such item is later re-evaluated over full operands)."""
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    sched = __import__("importlib").import_module("cfpq-on-gpgpu_amd.sched")
+    trained = sched.load()
+    assert trained is not None, "trained scheduler weights are missing"
+    held_out = [gen.sg_forest_dense_types(16384, seed=4242), gen.dense_stress(3000, seed=77)]
+    xs, rs, ys = sched.collect(pkg, cf, held_out, log=lambda *a: None)
+    engine.sched_init(19)
+    untrained = engine.sched_get_weights()
+    res = {}
+    for name, w in (("trained", trained), ("untrained", untrained)):
+        logits, _, _ = engine.sched_eval(w, xs, rs, ys, want_grad=False)
+        cut = np.quantile(logits, 0.10)
+        deferred = logits <= cut
+        res[name] = (_auc(logits, ys), float(ys[deferred].mean()))
+    print("held-out rows", len(ys), "positive rate", float(ys.mean()), res)
+    auc_t, waste_t = res["trained"]
+    auc_u, waste_u = res["untrained"]
+    assert auc_t > 0.85 and auc_t > auc_u + 0.2, res
+    # share of the deferred items that were active: the base rate for a
+    # scorer that knows nothing is the positive rate itself
+    assert waste_t < 0.5 * float(ys.mean()) and waste_t < 0.5 * waste_u, res
