@@ -1,5 +1,6 @@
 import importlib
 import os
+import subprocess
 import sys
 
 import pytest
@@ -40,3 +41,16 @@ def engine(pkg):
     eng = pkg.Engine(0)
     yield eng
     eng.close()
+
+
+@pytest.fixture(scope="session")
+def host_cpu_bin(tmp_path_factory):
+    """The C++ drop-in host (parsers, worklist driver, print_results) over a
+    plain CPU bitset matrix: tests/host/cpu_backend_main.cpp."""
+    d = tmp_path_factory.mktemp("hostbin")
+    exe = str(d / "cfpq_host_cpu")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include", "cfpq"), "-o", exe,
+                    os.path.join(ROOT, "tests", "host", "cpu_backend_main.cpp"),
+                    os.path.join(ROOT, "cfpq-on-gpgpu_amd", "host", "Grammar.cpp"),
+                    os.path.join(ROOT, "cfpq-on-gpgpu_amd", "host", "Graph.cpp")], check=True)
+    return exe

@@ -1,0 +1,156 @@
+"""The test-set harness (cfpq-on-gpgpu_amd/harness.py), the counterpart of the
+reference's test_utils/build_testset.py and testing_system.py.
+
+CPU tests drive it with runners that speak the reference protocol: the C++
+drop-in host over a CPU bitset matrix (tests/host/cpu_backend_main.cpp)
+wrapped to print "<total_ms> <algorithm_ms>", a runner that writes a wrong
+answer and one that fails.  The GPU test runs the engine's own CLI in both of
+its modes on a small data set and checks the answers against the oracle.
+"""
+import importlib
+import os
+import stat
+
+import pytest
+
+harness = importlib.import_module("cfpq-on-gpgpu_amd.harness")
+
+
+def _write_dataset(root, gen, sizes=(("g1", 120, 300), ("g2", 90, 260))):
+    """data/<family>/{Grammars,Matrices}: Dyck and SG grammars over small graphs."""
+    dy = os.path.join(root, "data", "dyck")
+    sg = os.path.join(root, "data", "sg")
+    for fam in (dy, sg):
+        os.makedirs(os.path.join(fam, "Grammars"))
+        os.makedirs(os.path.join(fam, "Matrices"))
+    graphs = {}
+    for name, n, m in sizes:
+        q = gen.dyck_er(n, m, seed=n)
+        open(os.path.join(dy, "Matrices", name + ".txt"), "w").write(q.graph_text())
+        graphs[("dyck", name)] = q
+        s = gen.sg_forest(n, seed=n + 1, types_per_vertex=2)
+        open(os.path.join(sg, "Matrices", name + ".txt"), "w").write(s.graph_text())
+        graphs[("sg", name)] = s
+    open(os.path.join(dy, "Grammars", "dyck.txt"), "w").write(gen.DYCK_GRAMMAR)
+    open(os.path.join(sg, "Grammars", "SG.txt"), "w").write(gen.SG_GRAMMAR)
+    return os.path.join(root, "data"), graphs
+
+
+def _script(path, body):
+    open(path, "w").write("#!/bin/sh\n" + body + "\n")
+    os.chmod(path, os.stat(path).st_mode | stat.S_IEXEC)
+    return path
+
+
+def test_build_testset(tmp_path, gen):
+    data, graphs = _write_dataset(str(tmp_path), gen)
+    out = str(tmp_path / "tests.csv")
+    rows = harness.build_testset(data, out)
+    assert sorted(r[0] for r in rows) == ["SG:g1", "SG:g2", "dyck:g1", "dyck:g2"]
+    lines = open(out).read().splitlines()
+    assert lines[0] == "Name,Grammar,Number of rules,Matrix, Number of vertices, Number of edges"
+    for r in rows:
+        fam = "dyck" if r[0].startswith("dyck") else "sg"
+        q = graphs[(fam, r[0].split(":")[1])]
+        assert r[2] == len(open(r[1]).read().splitlines())
+        # vertices = max id + 1, edges = lines (build_testset.py:21-28)
+        assert (r[4], r[5]) == (int(max(q.src.max(), q.dst.max())) + 1, len(q.src))
+    tests = harness.read_tests(out)
+    assert set(tests) == {r[0] for r in rows} and all(os.path.exists(g) and os.path.exists(m)
+                                                       for g, m in tests.values())
+
+
+def test_strategies():
+    order = harness.strategy(["r1", "r2"], ["a", "b"], "circle", 2)
+    assert order[:3] == [("r1", "a"), ("r2", "a"), (harness.SAVE, "a")] and len(order) == 2 * 2 * 3
+    order = harness.strategy(["r1", "r2"], ["a", "b"], "simple", 2)
+    assert order == [("r1", "a"), ("r1", "a"), ("r1", "b"), ("r1", "b"),
+                     ("r2", "a"), ("r2", "a"), ("r2", "b"), ("r2", "b")]
+    with pytest.raises(ValueError):
+        harness.strategy([], [], "zigzag", 1)
+
+
+def test_run_cross_validation_and_results(tmp_path, gen, oracle, host_cpu_bin):
+    data, graphs = _write_dataset(str(tmp_path), gen)
+    harness.build_testset(data, str(tmp_path / "t.csv"))
+    tests = harness.read_tests(str(tmp_path / "t.csv"))
+    good = _script(str(tmp_path / "good.sh"), f'"{host_cpu_bin}" "$1" "$2" "$3" > /dev/null && echo "25 7"')
+    # a backend that drops the count of every answer line
+    bad = _script(str(tmp_path / "bad.sh"),
+                  f'"{host_cpu_bin}" "$1" "$2" "$3" > /dev/null && sed -i "s/ [0-9]*$/ 0/" "$3" && echo "1 1"')
+    broken = _script(str(tmp_path / "broken.sh"), 'echo "no device" >&2; exit 1')
+    runners = [harness.Runner(good, "c++_good"), harness.Runner(good, "c++_good2"),
+               harness.Runner(broken, "c++_broken")]
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        logs = []
+        res = harness.run_tests(runners, tests, check=["dyck:g1", "SG:g2"], repeat=2, log=logs.append)
+        # the answers the good runner wrote during cross-validation are the oracle's
+        for test, fam in (("dyck:g1", "dyck"), ("SG:g2", "sg")):
+            q = graphs[(fam, test.split(":")[1])]
+            q.n = int(max(q.src.max(), q.dst.max())) + 1
+            g, mats, _ = oracle.solve(q, "seminaive")
+            want = {k: (str(h), str(c)) for k, (h, c) in oracle.fingerprints(g, mats, q.n).items()}
+            assert harness.read_answer(os.path.join("cross_val", f"{test}_c++_good.txt")) == want
+        assert any("c++_broken" in line and "failed" in line for line in logs)
+        rows = [line.split(",") for line in open("result.csv").read().splitlines()]
+        assert rows[0][:7] == ["Test name", "c++_good_time_all_mean", "c++_good_time_all_std",
+                               "c++_good_time_mult_mean", "c++_good_time_mult_std", "c++_good_axiom_hash",
+                               "c++_good_axiom_count"]
+        assert len(rows) == 1 + len(tests)
+        for row in rows[1:]:
+            test = row[0]
+            assert [float(x) for x in row[1:5]] == [0.025, 0.0, 0.007, 0.0]
+            assert row[13:] == ["-"] * 6  # the broken runner
+            per = res[test]["c++_good"]
+            assert len(per) == 2
+            # axiom "s" is absent from these grammars (upper-case S): the last
+            # line of the answer, as the reference reads it
+            cv = os.path.join("cross_val", f"{test}_c++_good.txt")
+            if os.path.exists(cv):
+                ans = harness.read_answer(cv)
+                assert (row[5], row[6]) == ans[list(ans)[-1]]
+        with pytest.raises(harness.Mismatch):
+            harness.run_tests([harness.Runner(good, "c++_good"), harness.Runner(bad, "c++_bad")], tests,
+                              check=["SG:g1"], repeat=1, log=logs.append)
+    finally:
+        os.chdir(cwd)
+
+
+def test_runner_specs_and_threshold(tmp_path):
+    r = harness.parse_runner("/x/m4ri_main=m4ri=--fast,--k=8", 60)
+    assert (r.path, r.name, r.args, r.timeout) == ("/x/m4ri_main", "m4ri", ["--fast", "--k=8"], 60)
+    assert harness.parse_runner("/x/naive_gpu", 5).name == "c++_naive_gpu"
+    slow = _script(str(tmp_path / "slow.sh"), 'printf "s 1 2\\n" > "$3"; echo "700000 1000"')
+    tests = {"t": (str(tmp_path / "g"), str(tmp_path / "m"))}
+    res = harness.run_tests([harness.Runner(slow, "c++_slow")], tests, repeat=5, threshold=600,
+                            out=str(tmp_path / "r.csv"), answer=str(tmp_path / "a.txt"), log=lambda *a: None)
+    # after one run above the 600 s threshold the test is not repeated (testing_system.py:124-129)
+    assert res["t"]["c++_slow"] == [(700.0, 1.0, "1", "2")]
+
+
+@pytest.mark.gpu
+def test_harness_with_engine_cli(tmp_path, gen, oracle):
+    """The engine's CLI as two runners (device fixpoint, worklist over
+    cfpq_add_mul): cross-validated on every test and equal to the oracle."""
+    data, graphs = _write_dataset(str(tmp_path), gen, sizes=(("g1", 700, 2000), ("g2", 300, 900)))
+    csv = str(tmp_path / "t.csv")
+    harness.build_testset(data, csv)
+    tests = harness.read_tests(csv)
+    runners = harness.default_runners(120)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        res = harness.run_tests(runners, tests, check=list(tests), repeat=2, log=print)
+    finally:
+        os.chdir(cwd)
+    for test in tests:
+        fam = "dyck" if test.startswith("dyck") else "sg"
+        q = graphs[(fam, test.split(":")[1])]
+        q.n = int(max(q.src.max(), q.dst.max())) + 1
+        g, mats, _ = oracle.solve(q, "seminaive")
+        want = {k: (str(h), str(c)) for k, (h, c) in oracle.fingerprints(g, mats, q.n).items()}
+        for r in runners:
+            assert harness.read_answer(str(tmp_path / "cross_val" / f"{test}_{r.name}.txt")) == want, (test, r.name)
+            assert len(res[test][r.name]) == 2

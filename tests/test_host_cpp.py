@@ -11,17 +11,6 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-@pytest.fixture(scope="module")
-def host_cpu_bin(tmp_path_factory):
-    d = tmp_path_factory.mktemp("hostbin")
-    exe = str(d / "cfpq_host_cpu")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include", "cfpq"), "-o", exe,
-                    os.path.join(ROOT, "tests", "host", "cpu_backend_main.cpp"),
-                    os.path.join(ROOT, "cfpq-on-gpgpu_amd", "host", "Grammar.cpp"),
-                    os.path.join(ROOT, "cfpq-on-gpgpu_amd", "host", "Graph.cpp")], check=True)
-    return exe
-
-
 def read_results(path):
     out = {}
     for line in open(path):
