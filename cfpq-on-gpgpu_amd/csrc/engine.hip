@@ -568,7 +568,9 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const TermDev *terms = ctx->d_terms;
     uint64_t *touched = ctx->lists[slot & 1];
     const uint32_t seg_cap = (uint32_t)plan.total_head_tiles;
-    hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
+    // a step's counters are zeroed by the previous step's retire launch; the
+    // first step of a run zeroes its own
+    if (prev_slot < 0) hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
     // with the dense and MFMA paths both off, every item is sparse: the
     // heavy-tile bitsets are neither built nor read, and the dense and MFMA
     // kernels are not launched
@@ -626,39 +628,39 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                            fz);
     }
     if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 1], ctx->stream));
+    // (the dense and MFMA launches, and their events, only with a heavy-tile path on)
     if (heavy)
         hipLaunchKernelGGL(k_product_dense, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
                            dense_threshold(ctx->opts), m4r_threshold(ctx->opts), fz);
-    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 2], ctx->stream));
+    if (ev_idx >= 0 && heavy) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 2], ctx->stream));
     if (heavy)
         hipLaunchKernelGGL(k_product_mfma, dim3((uint32_t)ctx->num_cu), dim3(MFMA_BLOCK), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, terms, q, c,
                            (const uint64_t *)ctx->mitems, touched, fz);
-    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 3], ctx->stream));
+    if (ev_idx >= 0 && heavy) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 3], ctx->stream));
     if (use_sched && ctx->tr_cap)
         hipLaunchKernelGGL(k_trace_label, dim3((uint32_t)ctx->num_cu), dim3(256), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, (const StepCounters *)c, sa);
-    if (!fuse)
-        hipLaunchKernelGGL(k_apply, dim3((uint32_t)ctx->num_cu * 8), dim3(WAVE), 0, ctx->stream,
-                           (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, c,
-                           TouchedLists{touched, seg_cap});
-    // retire the delta this step consumed (buffer D_q)
+    // the end of the step in one launch: apply (unless fused into the
+    // products), retire the delta this step consumed (buffer D_q: rows,
+    // metadata, occupancy) and zero the next slot's counters
     const uint32_t *ids_q = ctx->d_ids + (q ? plan.all_delta[0].size() : 0);
     const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
-    if (prev_slot < 0 && full_pass) {
-        // first step of a fixpoint over full terms: its delta buffer was never filled
-    } else if (prev_slot < 0) {
+    const bool retire_delta = prev_slot >= 0;
+    const uint32_t na = fuse ? 0u : (uint32_t)ctx->num_cu * 8;
+    const uint32_t nc = retire_delta ? (uint32_t)ctx->num_cu * 8 : 0u;
+    const uint32_t no = retire_delta && nids_q ? RETIRE_OCC_BLOCKS : 0u;
+    const StepCounters *prev = retire_delta ? ctx->ring + prev_slot : c;
+    const TouchedLists ptl{ctx->lists[retire_delta ? (prev_slot & 1) : (slot & 1)], seg_cap};
+    hipLaunchKernelGGL(k_retire, dim3(na + nc + no + 1), dim3(WAVE), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                       (const HeadDev *)ctx->d_heads, q, (const StepCounters *)c, TouchedLists{touched, seg_cap}, prev,
+                       ptl, ids_q, nids_q, na, nc, no, ctx->ring + (slot + 1) % RING);
+    if (prev_slot < 0 && !full_pass) {
+        // a first step over delta terms: its delta buffers are cleared in full
         hipLaunchKernelGGL(k_clear_full, dim3(std::max<uint32_t>(1, ctx->num_cu * 4 / std::max<uint32_t>(1, nids_q)),
                                               std::min<uint32_t>(nids_q, 65535)),
                            dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats, ids_q, nids_q);
-    } else {
-        hipLaunchKernelGGL(k_clear, dim3((uint32_t)ctx->num_cu * 8), dim3(WAVE), 0, ctx->stream,
-                           (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, 1 - q,
-                           (const StepCounters *)(ctx->ring + prev_slot), c,
-                           TouchedLists{ctx->lists[prev_slot & 1], seg_cap});
-        hipLaunchKernelGGL(k_clear_occ, dim3(4, std::min<uint32_t>(nids_q, 65535)), dim3(256), 0, ctx->stream,
-                           (const MatDev *)ctx->d_mats, ids_q, nids_q);
     }
     return launch_check(ctx, "fixpoint step");
 }
@@ -817,6 +819,8 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
     const int per_sync = single_step_full ? 1 : ctx->opts.iters_per_sync;
     // graph replay needs the parity to follow the slot (RING is even)
     const bool use_graph = ctx->opts.use_graph != 0 && !single_step_full;
+    // the dense and MFMA products (and their timing events) run only with a heavy-tile path on
+    const bool heavy_paths = dense_threshold(ctx->opts) <= TILE_BITS || mfma_threshold(ctx->opts) <= TILE_BITS;
     if (use_graph) {
         std::vector<uint64_t> sig = launch_signature(ctx, plan);
         if (sig != ctx->graph_sig || ctx->graph_exec.size() != GRAPH_VARIANTS * (size_t)RING) {
@@ -870,7 +874,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             dirty_total += (int64_t)c.dirty_new - (int64_t)c.dirty_cleared;
             if (quiet) continue;
             float step_ms[3] = {0, 0, 0};
-            for (int k2 = 0; k2 < 3 && !use_graph; ++k2) {
+            for (int k2 = 0; k2 < (heavy_paths ? 3 : 1) && !use_graph; ++k2) {
                 HIPCHK(hipEventElapsedTime(&step_ms[k2], ctx->ev[4 * s + k2], ctx->ev[4 * s + k2 + 1]));
                 part_ms[k2] += step_ms[k2];
                 product_ms += step_ms[k2];
@@ -923,15 +927,14 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
     {
         const int q = (int)(steps & 1);  // parity the next step would have
         const uint32_t slot = steps % RING;
-        hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
         const uint32_t *ids_q = ctx->d_ids + (q ? nd : 0);
         const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
-        hipLaunchKernelGGL(k_clear, dim3((uint32_t)ctx->num_cu * 8), dim3(WAVE), 0, ctx->stream,
-                           (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, 1 - q,
-                           (const StepCounters *)(ctx->ring + prev_slot), ctx->ring + slot,
-                           TouchedLists{ctx->lists[prev_slot & 1], (uint32_t)plan.total_head_tiles});
-        hipLaunchKernelGGL(k_clear_occ, dim3(4, std::min<uint32_t>(nids_q, 65535)), dim3(256), 0, ctx->stream,
-                           (const MatDev *)ctx->d_mats, ids_q, nids_q);
+        const uint32_t nc = (uint32_t)ctx->num_cu * 8, no = nids_q ? RETIRE_OCC_BLOCKS : 0u;
+        const TouchedLists ptl{ctx->lists[prev_slot & 1], (uint32_t)plan.total_head_tiles};
+        hipLaunchKernelGGL(k_retire, dim3(nc + no + 1), dim3(WAVE), 0, ctx->stream, (const MatDev *)ctx->d_mats,
+                           (const HeadDev *)ctx->d_heads, q, (const StepCounters *)(ctx->ring + slot), ptl,
+                           (const StepCounters *)(ctx->ring + prev_slot), ptl, ids_q, nids_q, 0u, nc, no,
+                           ctx->ring + slot);
         CHK(launch_check(ctx, "final clear"));
     }
     HIPCHK(hipEventRecord(ev_end, ctx->stream));

@@ -1416,12 +1416,14 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
 
 // M[head] |= next-delta tile, for every tile a product kernel wrote: one
 // wave per (tile, quarter); the quarter-0 wave also merges the tile count.
-__global__ __launch_bounds__(WAVE) void k_apply(const MatDev *__restrict__ mats,
-                                                const HeadDev *__restrict__ heads, int parity,
-                                                StepCounters *__restrict__ c, TouchedLists tl) {
+// Merge the new rows of one step's touched tiles into the heads: one wave per
+// (touched tile, quarter), waves blk, blk + nblk, ... of the work.
+__device__ __forceinline__ void apply_waves(const MatDev *__restrict__ mats, const HeadDev *__restrict__ heads,
+                                            int parity, const StepCounters *__restrict__ c, TouchedLists tl,
+                                            uint32_t blk, uint32_t nblk) {
     const int lane = threadIdx.x & 63;
     const uint32_t n = touched_total(c) * QUARTERS;
-    for (uint32_t idx = blockIdx.x; idx < n; idx += gridDim.x) {  // static: the units are small and alike
+    for (uint32_t idx = blk; idx < n; idx += nblk) {  // static: the units are small and alike
         const uint64_t it = touched_entry(c, tl, idx >> 2);
         const uint32_t q = idx & 3u;
         const uint32_t h = uniform(item_head(it));
@@ -1452,16 +1454,16 @@ __global__ __launch_bounds__(WAVE) void k_apply(const MatDev *__restrict__ mats,
     }
 }
 
-// Zero the delta tiles listed in `list` (written by the step of parity
-// `parity`): one wave per (tile, quarter) -- rows named by the quarter's row
-// mask, then the quarter's metadata; the quarter-0 wave also zeroes the count.
-__global__ __launch_bounds__(WAVE) void k_clear(const MatDev *__restrict__ mats,
-                                                const HeadDev *__restrict__ heads, int parity,
-                                                const StepCounters *__restrict__ list_ctr,
-                                                StepCounters *__restrict__ c, TouchedLists tl) {
+// Zero the delta tiles listed in `tl` (written by the step of parity
+// `parity`, counted in `list_ctr`): one wave per (tile, quarter) -- rows named
+// by the quarter's row mask, then the quarter's metadata; the quarter-0 wave
+// also zeroes the count.
+__device__ __forceinline__ void clear_waves(const MatDev *__restrict__ mats, const HeadDev *__restrict__ heads,
+                                            int parity, const StepCounters *__restrict__ list_ctr, TouchedLists tl,
+                                            uint32_t blk, uint32_t nblk) {
     const int lane = threadIdx.x & 63;
     const uint32_t n = touched_total(list_ctr) * QUARTERS;
-    for (uint32_t idx = blockIdx.x; idx < n; idx += gridDim.x) {  // static: the units are small and alike
+    for (uint32_t idx = blk; idx < n; idx += nblk) {  // static: the units are small and alike
         const uint64_t it = touched_entry(list_ctr, tl, idx >> 2);
         const uint32_t q = idx & 3u;
         const uint32_t h = uniform(item_head(it));
@@ -1476,6 +1478,18 @@ __global__ __launch_bounds__(WAVE) void k_clear(const MatDev *__restrict__ mats,
         }
         if (lane == 0 && q == 0) gst(&O.cnt[p], 0u);
     }
+}
+
+__global__ __launch_bounds__(WAVE) void k_apply(const MatDev *__restrict__ mats,
+                                                const HeadDev *__restrict__ heads, int parity,
+                                                StepCounters *__restrict__ c, TouchedLists tl) {
+    apply_waves(mats, heads, parity, c, tl, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(WAVE) void k_clear(const MatDev *__restrict__ mats,
+                                                const HeadDev *__restrict__ heads, int parity,
+                                                const StepCounters *__restrict__ list_ctr, TouchedLists tl) {
+    clear_waves(mats, heads, parity, list_ctr, tl, blockIdx.x, gridDim.x);
 }
 
 // Zero the occupancy bitsets (all four kinds) of one matrix, grid-strided.
@@ -1494,6 +1508,34 @@ __device__ __forceinline__ void zero_occ(const MatDev &O, uint32_t x0, uint32_t 
 __global__ void k_clear_occ(const MatDev *__restrict__ mats, const uint32_t *__restrict__ ids, uint32_t nid) {
     for (uint32_t q = blockIdx.y; q < nid; q += gridDim.y)
         zero_occ(mats[ids[q]], blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+// The end of a step in one launch, its parts on disjoint arrays:
+//   blocks [0, na)             apply this step's new rows to the heads (parity
+//                              `parity`, list `tl`, counts in c);
+//   blocks [na, na + nc)       zero the rows and metadata of the delta the step
+//                              consumed (parity 1 - parity, the previous step's
+//                              list `ptl`, counts in prev);
+//   blocks [.., .. + no)       zero that delta's occupancy bitsets (ids[0..nid));
+//   the last block             zero the counters of the next ring slot, so that
+//                              the next step starts without a launch of its own.
+__global__ __launch_bounds__(WAVE) void k_retire(const MatDev *__restrict__ mats,
+                                                 const HeadDev *__restrict__ heads, int parity,
+                                                 const StepCounters *__restrict__ c, TouchedLists tl,
+                                                 const StepCounters *__restrict__ prev, TouchedLists ptl,
+                                                 const uint32_t *__restrict__ ids, uint32_t nid, uint32_t na,
+                                                 uint32_t nc, uint32_t no, StepCounters *__restrict__ next) {
+    const uint32_t b = blockIdx.x;
+    if (b < na) {
+        apply_waves(mats, heads, parity, c, tl, b, na);
+    } else if (b < na + nc) {
+        clear_waves(mats, heads, 1 - parity, prev, ptl, b - na, nc);
+    } else if (b < na + nc + no) {
+        for (uint32_t q = 0; q < nid; ++q) zero_occ(mats[ids[q]], (b - na - nc) * WAVE + threadIdx.x, no * WAVE);
+    } else if (next) {
+        uint64_t *w = reinterpret_cast<uint64_t *>(next);
+        for (uint32_t i = threadIdx.x; i < sizeof(StepCounters) / 8; i += blockDim.x) gst(w + i, (uint64_t)0);
+    }
 }
 
 // Zero every non-empty tile and the metadata of matrices ids[0..nid).
