@@ -67,7 +67,7 @@ struct cfpq_ctx {
     size_t d_heads_cap = 0;
     TermDev *d_terms = nullptr;  // [delta terms | full terms]
     size_t d_terms_cap = 0;
-    TermPtr *d_tptr = nullptr;   // [parity][delta terms | full terms]
+    TermPtr *d_tptr = nullptr;   // [parity][delta terms | full terms | live delta terms]
     size_t d_tptr_cap = 0;
     uint32_t *d_ids = nullptr;   // id lists for clear/copy kernels
     size_t d_ids_cap = 0;
@@ -489,6 +489,10 @@ struct Plan {
     std::vector<HeadDev> heads;
     std::vector<TermDev> terms_delta;  // semi-naive terms (two per rule)
     std::vector<TermDev> terms_full;   // full terms (one per rule), stored after the delta terms
+    // the semi-naive terms whose delta operand can be non-empty (its
+    // nonterminal heads some rule; a terminal-only one never changes after
+    // the first step, which evaluates full terms), stored after the full terms
+    std::vector<TermDev> terms_live;
     std::vector<uint32_t> all_delta[2];  // ids of every delta buffer of parity q (buffer Dq)
     size_t max_tiles_per_head = 0;     // max T*T over heads
     size_t total_head_tiles = 0;       // sum of T*T over heads
@@ -497,6 +501,11 @@ struct Plan {
     std::vector<uint32_t> dirty_off;   // per head: word offset of its deferred-tile bitmap
     size_t dirty_words = 0;
 };
+
+// terms in the device table (and per parity in the operand-pointer table)
+static size_t plan_terms(const Plan &plan) {
+    return plan.terms_delta.size() + plan.terms_full.size() + plan.terms_live.size();
+}
 
 struct StepStats {
     uint64_t new_bits = 0, items = 0, tp = 0, tpd = 0, tpm = 0, arows = 0, brows = 0, orows = 0, brows_lds = 0,
@@ -614,7 +623,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const int fz = (fuse && ctx->opts.fuse_apply) ? 1 : 0;
     fuse = fz != 0;
     if (ctx->opts.sparse_mode == 0) {
-        const TermPtr *tp = ctx->d_tptr + (size_t)q * (plan.terms_delta.size() + plan.terms_full.size());
+        const TermPtr *tp = ctx->d_tptr + (size_t)q * plan_terms(plan);
         hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
                            TouchedLists{touched, seg_cap}, chunk, fz);
     } else if (ctx->opts.sparse_batch == 1) {
@@ -672,11 +681,15 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     CHK(grow(ctx, &ctx->d_mats, &ctx->d_mats_cap, plan.mats.size()));
     CHK(grow(ctx, &ctx->d_heads, &ctx->d_heads_cap, plan.heads.size()));
     std::vector<HeadDev> heads(plan.heads);
-    for (HeadDev &hd : heads) {  // full terms follow the delta terms in the device table
+    // device table: delta terms, then full terms, then live delta terms
+    const uint32_t nfull_end = (uint32_t)(plan.terms_delta.size() + plan.terms_full.size());
+    for (HeadDev &hd : heads) {
         hd.f0 += (uint32_t)plan.terms_delta.size();
         hd.f1 += (uint32_t)plan.terms_delta.size();
+        hd.t0 += nfull_end;
+        hd.t1 += nfull_end;
     }
-    CHK(grow(ctx, &ctx->d_terms, &ctx->d_terms_cap, plan.terms_delta.size() + plan.terms_full.size()));
+    CHK(grow(ctx, &ctx->d_terms, &ctx->d_terms_cap, plan_terms(plan)));
     const size_t nid = plan.all_delta[0].size() + plan.all_delta[1].size() + plan.mats.size();
     CHK(grow(ctx, &ctx->d_ids, &ctx->d_ids_cap, nid));
     CHK(grow(ctx, &ctx->items, &ctx->items_cap, plan.total_head_tiles));
@@ -695,6 +708,7 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     HIPCHK(hipMemcpyAsync(ctx->d_heads, heads.data(), heads.size() * sizeof(HeadDev), hipMemcpyHostToDevice, ctx->stream));
     std::vector<TermDev> terms(plan.terms_delta);
     terms.insert(terms.end(), plan.terms_full.begin(), plan.terms_full.end());
+    terms.insert(terms.end(), plan.terms_live.begin(), plan.terms_live.end());
     HIPCHK(hipMemcpyAsync(ctx->d_terms, terms.data(), terms.size() * sizeof(TermDev), hipMemcpyHostToDevice, ctx->stream));
     CHK(grow(ctx, &ctx->d_tptr, &ctx->d_tptr_cap, 2 * terms.size()));
     std::vector<TermPtr> tptr(2 * terms.size());
@@ -763,7 +777,7 @@ static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &p
             (uint64_t)(uintptr_t)ctx->d_tptr,
             (uint64_t)(uintptr_t)ctx->d_ids, (uint64_t)(uintptr_t)ctx->units, (uint64_t)(uintptr_t)ctx->items,
             (uint64_t)(uintptr_t)ctx->mitems, (uint64_t)(uintptr_t)ctx->lists[0], (uint64_t)(uintptr_t)ctx->lists[1],
-            (uint64_t)(uintptr_t)ctx->ring, plan.heads.size(), plan.mats.size(), plan.terms_delta.size(),
+            (uint64_t)(uintptr_t)ctx->ring, plan.heads.size(), plan.mats.size(), plan.terms_delta.size(), plan.terms_live.size(),
             plan.all_delta[0].size(), plan.all_delta[1].size(), plan.max_tiles_per_head, plan.max_occ_words,
             plan.ids_all_off, plan.total_head_tiles, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
             (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk, (uint64_t)o.sched, o.sched_threshold,
@@ -998,7 +1012,8 @@ static void plan_query(Plan &plan, cfpq_mat *const *M, const std::vector<cfpq_ma
         hd.target = id_m((uint32_t)h);
         hd.out[0] = id_d(1, (uint32_t)h);  // even step writes D1
         hd.out[1] = id_d(0, (uint32_t)h);  // odd step writes D0
-        hd.t0 = (uint32_t)plan.terms_delta.size();
+        hd.p0 = (uint32_t)plan.terms_delta.size();
+        hd.t0 = (uint32_t)plan.terms_live.size();  // rebased past the delta and full terms in upload_plan
         hd.f0 = (uint32_t)plan.terms_full.size();  // rebased past the delta terms in upload_plan
         hd.T = T;
         for (size_t r : by_head[h]) {
@@ -1011,12 +1026,15 @@ static void plan_query(Plan &plan, cfpq_mat *const *M, const std::vector<cfpq_ma
             t1.rule = t2.rule = (uint32_t)r;
             plan.terms_delta.push_back(t1);
             plan.terms_delta.push_back(t2);
+            if (!by_head[a].empty()) plan.terms_live.push_back(t1);
+            if (!by_head[b].empty()) plan.terms_live.push_back(t2);
             f.left[0] = f.left[1] = id_m(a);
             f.right[0] = f.right[1] = id_m(b);
             f.rule = (uint32_t)r;
             plan.terms_full.push_back(f);
         }
-        hd.t1 = (uint32_t)plan.terms_delta.size();
+        hd.p1 = (uint32_t)plan.terms_delta.size();
+        hd.t1 = (uint32_t)plan.terms_live.size();
         hd.f1 = (uint32_t)plan.terms_full.size();
         plan.heads.push_back(hd);
         plan.max_tiles_per_head = std::max<size_t>(plan.max_tiles_per_head, (size_t)T * T);
@@ -1100,10 +1118,12 @@ extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const
     HeadDev hd;
     hd.target = 0;
     hd.out[0] = hd.out[1] = 3;
-    hd.t0 = 0;
+    hd.t0 = 0;  // rebased to 2 by upload_plan
     hd.t1 = 1;
     hd.f0 = 0;  // rebased to 1 by upload_plan
     hd.f1 = 1;
+    hd.p0 = 0;
+    hd.p1 = 1;
     hd.T = c->T;
     plan.heads.push_back(hd);
     TermDev t;
@@ -1112,6 +1132,7 @@ extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const
     t.rule = 0;
     plan.terms_delta.push_back(t);
     plan.terms_full.push_back(t);
+    plan.terms_live.push_back(t);
     plan.all_delta[0] = {3};
     plan.all_delta[1] = {3};
     plan.max_tiles_per_head = plan.total_head_tiles = (size_t)c->T * c->T;

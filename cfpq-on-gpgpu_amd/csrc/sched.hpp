@@ -260,6 +260,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
     const uint32_t T = uniform(heads[h].T), target = uniform(heads[h].target);
     const uint32_t t0 = uniform(heads[h].t0), t1 = uniform(heads[h].t1);
     const uint32_t f0 = uniform(heads[h].f0), f1 = uniform(heads[h].f1);
+    const uint32_t p0 = uniform(heads[h].p0), p1 = uniform(heads[h].p1);
     const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
     bool dense = false, mfma = false, full = false;
     uint32_t qmask = 0;
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                 const float fill = (float)M.cnt[p] * (1.f / (float)TILE_BITS);
                 const float hd = log2p1(Dh.cnt[p]) * (1.f / 16.f);
                 float best = -1e30f;
-                for (uint32_t t = t0; t + 1 < t1; t += 2) {
+                for (uint32_t t = p0; t + 1 < p1; t += 2) {
                     const TermDev ta = terms[t], tb = terms[t + 1];
                     float x[MLP_F];
                     rule_features(mats[ta.left[parity]], mats[ta.right[parity]], mats[tb.left[parity]],

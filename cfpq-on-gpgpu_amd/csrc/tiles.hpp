@@ -52,8 +52,9 @@ struct MatDev {
 struct HeadDev {
     uint32_t target;
     uint32_t out[2];
-    uint32_t t0, t1;
-    uint32_t f0, f1;
+    uint32_t t0, t1;  // live delta terms: those whose delta operand can be non-empty
+    uint32_t f0, f1;  // full terms, one per rule
+    uint32_t p0, p1;  // every delta term, two per rule in rule order (scheduler features)
     uint32_t T;
 };
 
