@@ -598,6 +598,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     // plus as many extra sub-units
     const int split = (ctx->opts.sparse_mode == 0 && !(fuse && ctx->opts.fuse_apply)) ? 1 : 0;
     const uint32_t split_budget = (uint32_t)std::min<size_t>(QUARTERS * plan.total_head_tiles, 0xffff0000u);
+    const uint32_t heavy_base = (uint32_t)(QUARTERS * plan.total_head_tiles);
     const bool use_sched = !full_pass && plan.dirty_words && (sched_on(ctx) || ctx->tr_cap || flush);
     const SchedArgs sa = sched_args(ctx, flush);
     if (use_sched) {
@@ -606,11 +607,11 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                                (uint32_t)ctx->tr_cap);
         hipLaunchKernelGGL(k_build_sched, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems, sa, split,
-                           split_budget, heavy);
+                           split_budget, heavy, heavy_base);
     } else {
         hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, (int)full_pass, c, ctx->units, ctx->items,
-                           ctx->mitems, split, split_budget, heavy);
+                           ctx->mitems, split, split_budget, heavy, heavy_base);
     }
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
     // product kernels, timed: ev[4i] sparse ev[4i+1] dense ev[4i+2] mfma ev[4i+3]
@@ -624,7 +625,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     fuse = fz != 0;
     if (ctx->opts.sparse_mode == 0) {
         const TermPtr *tp = ctx->d_tptr + (size_t)q * plan_terms(plan);
-        hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
+        hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un, heavy_base,
                            TouchedLists{touched, seg_cap}, chunk, fz);
     } else if (ctx->opts.sparse_batch == 1) {
         hipLaunchKernelGGL(k_product_sparse<1>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
@@ -693,7 +694,9 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     const size_t nid = plan.all_delta[0].size() + plan.all_delta[1].size() + plan.mats.size();
     CHK(grow(ctx, &ctx->d_ids, &ctx->d_ids_cap, nid));
     CHK(grow(ctx, &ctx->items, &ctx->items_cap, plan.total_head_tiles));
-    CHK(grow(ctx, &ctx->units, &ctx->units_cap, 2 * QUARTERS * plan.total_head_tiles));
+    // units: up to QUARTERS per head tile, then the sub-units of split
+    // quarters (at most one list of SPLIT budget plus one per quarter)
+    CHK(grow(ctx, &ctx->units, &ctx->units_cap, 3 * QUARTERS * plan.total_head_tiles));
     CHK(grow(ctx, &ctx->mitems, &ctx->mitems_cap, plan.total_head_tiles));
     // touched-tile lists: 1 + SHARDS segments of one entry per head tile
     const size_t list_need = (size_t)(1 + SHARDS) * plan.total_head_tiles;
@@ -741,7 +744,7 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
 
 static void accumulate(StepStats &s, const StepCounters &c) {
     s.new_bits += c.new_bits;
-    s.items += c.n_units + c.n_dense + c.n_mfma;
+    s.items += c.n_units + c.n_heavy + c.n_dense + c.n_mfma;
     s.mfma_instr += c.mfma_instr;
     s.cand_hits += c.cand_hits;
     s.arows_d += c.a_rows_dense;
@@ -898,7 +901,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
                              "[cfpq step %u] units %u dense %u mfma %u touched %u new_bits %llu cand %llu hits %llu "
                              "max_unit %u split %u a_rows %llu b_rows %llu busy %.3f wave_max %llu unit_max %llu clk"
                              " | sparse %.3f dense %.3f mfma %.3f ms\n",
-                             steps - k + s, c.n_units, c.n_dense, c.n_mfma, c.n_touched,
+                             steps - k + s, c.n_units + c.n_heavy, c.n_dense, c.n_mfma, c.n_touched,
                              (unsigned long long)c.new_bits, (unsigned long long)c.tile_products,
                              (unsigned long long)c.cand_hits, c.max_unit_cands, c.n_split, (unsigned long long)c.a_rows,
                              (unsigned long long)c.b_rows,

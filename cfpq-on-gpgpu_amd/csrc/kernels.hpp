@@ -406,10 +406,12 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
 // quarter (more than SPLIT_CANDS candidates) as up to SPLIT_MAX sub-units
 // while the split budget lasts.  Whole wave: the single units of all four
 // quarters take one atomic on the list count per wave (the count is one
-// address that every wave of the build hits).
+// address that every wave of the build hits).  Sub-units go to their own list
+// at units + heavy_base (count n_heavy), which the sparse kernel claims
+// first: the longest units start early instead of ending the launch late.
 __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const uint32_t *qcnt, uint64_t base,
                                              int split, uint32_t split_budget, StepCounters *__restrict__ c,
-                                             uint64_t *__restrict__ units) {
+                                             uint64_t *__restrict__ units, uint32_t heavy_base) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
     uint32_t nsub[QUARTERS];
@@ -442,7 +444,7 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
     for (uint32_t q = 0; q < QUARTERS; ++q) {
         if (nsub[q] > 1) {
             const uint64_t u = base | (uint64_t)q;
-            const uint32_t at = atomicAdd(&c->n_units, nsub[q]);
+            const uint32_t at = heavy_base + atomicAdd(&c->n_heavy, nsub[q]);
             for (uint32_t sidx = 0; sidx < nsub[q]; ++sidx)
                 units[at + sidx] = u | ((uint64_t)sidx << 55) | ((uint64_t)(nsub[q] - 1) << 59);
             atomicAdd(&c->n_split, 1u);
@@ -457,7 +459,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
                                                  StepCounters *__restrict__ c, uint64_t *__restrict__ units,
                                                  uint64_t *__restrict__ dense_items,
                                                  uint64_t *__restrict__ mfma_items, int split,
-                                                 uint32_t split_budget, int heavy) {
+                                                 uint32_t split_budget, int heavy, uint32_t heavy_base) {
     const uint32_t h = blockIdx.y;
     const uint32_t T = uniform(heads[h].T), target = uniform(heads[h].target);
     const uint32_t t0 = uniform(full_pass ? heads[h].f0 : heads[h].t0);
@@ -476,7 +478,8 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
     wave_append(mfma, flag | ((uint64_t)h << 40) | p, &c->n_mfma, mfma_items);
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
-    append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units);
+    append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units,
+                 heavy_base);
 }
 
 // ---------------------------------------------------------------------------
@@ -718,7 +721,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                                             const TermPtr *__restrict__ tptr, int parity,
                                                             StepCounters *__restrict__ c,
                                                             const uint64_t *__restrict__ units,
-                                                            TouchedLists tl, uint32_t chunk,
+                                                            uint32_t heavy_base, TouchedLists tl, uint32_t chunk,
                                                             int fuse) {
     __shared__ Row4 acc_s[WAVE];
     __shared__ Row4 s_brm[WAVE];
@@ -732,7 +735,9 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     __shared__ uint32_t s_gr[WAVE], s_gend[WAVE];
     const int lane = threadIdx.x & 63;
     const uint64_t clk0 = __builtin_amdgcn_s_memtime();
-    const uint32_t n_units = c->n_units;
+    // list entries [0, n_heavy) are the sub-units of split units, then the others
+    const uint32_t n_heavy = c->n_heavy;
+    const uint32_t n_units = c->n_units + n_heavy;
     // Claims x are mapped to list entries by reversing the low six bits of x:
     // neighbours in the list (the quarters of one tile, the sub-units of one
     // heavy quarter) go to claims far apart within their block of 64, i.e. to
@@ -770,7 +775,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         const uint32_t u = (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
         if (u >= n_units) continue;
         const uint64_t tu0 = __builtin_amdgcn_s_memtime();
-        const uint64_t it = ld_const(units + u);
+        const uint64_t it = ld_const(units + (u < n_heavy ? heavy_base + u : u - n_heavy));
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)((it & ITEM_TILE_MASK) >> 2));
         const uint32_t q = uniform((uint32_t)(it & 3u));

@@ -118,6 +118,7 @@ struct StepCounters {
     uint32_t max_unit_cands; // most candidate k-tiles of one sparse unit
     uint32_t split_extra;    // extra sub-units created by splitting heavy units (budget counter)
     uint32_t n_split;        // heavy units split
+    uint32_t n_heavy;        // sub-units of split units, listed apart so that they are claimed first
     uint64_t new_bits;
     uint64_t tile_products;   // (term, C tile, k) products evaluated (sparse: per quarter)
     uint64_t tp_dense;        // of which on the LDS / Four Russians paths
