@@ -502,6 +502,12 @@ struct Plan {
     size_t dirty_words = 0;
 };
 
+// Waves of the retire launch per matrix whose occupancy bitsets it zeroes:
+// one per 128 words of the largest matrix (2 words per lane), at most 32.
+static uint32_t occ_blocks_per_matrix(const Plan &plan) {
+    return (uint32_t)std::min<size_t>(32, std::max<size_t>(1, (plan.max_occ_words + 127) / 128));
+}
+
 // terms in the device table (and per parity in the operand-pointer table)
 static size_t plan_terms(const Plan &plan) {
     return plan.terms_delta.size() + plan.terms_full.size() + plan.terms_live.size();
@@ -660,12 +666,13 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const bool retire_delta = prev_slot >= 0;
     const uint32_t na = fuse ? 0u : (uint32_t)ctx->num_cu * 8;
     const uint32_t nc = retire_delta ? (uint32_t)ctx->num_cu * 8 : 0u;
-    const uint32_t no = retire_delta && nids_q ? RETIRE_OCC_BLOCKS : 0u;
+    const uint32_t nido = retire_delta ? nids_q : 0u, per = occ_blocks_per_matrix(plan);
     const StepCounters *prev = retire_delta ? ctx->ring + prev_slot : c;
     const TouchedLists ptl{ctx->lists[retire_delta ? (prev_slot & 1) : (slot & 1)], seg_cap};
-    hipLaunchKernelGGL(k_retire, dim3(na + nc + no + 1), dim3(WAVE), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                       (const HeadDev *)ctx->d_heads, q, (const StepCounters *)c, TouchedLists{touched, seg_cap}, prev,
-                       ptl, ids_q, nids_q, na, nc, no, ctx->ring + (slot + 1) % RING);
+    hipLaunchKernelGGL(k_retire, dim3(na + nc + nido * per + 1), dim3(WAVE), 0, ctx->stream,
+                       (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, (const StepCounters *)c,
+                       TouchedLists{touched, seg_cap}, prev, ptl, ids_q, nido, per, na, nc,
+                       ctx->ring + (slot + 1) % RING);
     if (prev_slot < 0 && !full_pass) {
         // a first step over delta terms: its delta buffers are cleared in full
         hipLaunchKernelGGL(k_clear_full, dim3(std::max<uint32_t>(1, ctx->num_cu * 4 / std::max<uint32_t>(1, nids_q)),
@@ -946,12 +953,12 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         const uint32_t slot = steps % RING;
         const uint32_t *ids_q = ctx->d_ids + (q ? nd : 0);
         const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
-        const uint32_t nc = (uint32_t)ctx->num_cu * 8, no = nids_q ? RETIRE_OCC_BLOCKS : 0u;
+        const uint32_t nc = (uint32_t)ctx->num_cu * 8, per = occ_blocks_per_matrix(plan);
         const TouchedLists ptl{ctx->lists[prev_slot & 1], (uint32_t)plan.total_head_tiles};
-        hipLaunchKernelGGL(k_retire, dim3(nc + no + 1), dim3(WAVE), 0, ctx->stream, (const MatDev *)ctx->d_mats,
-                           (const HeadDev *)ctx->d_heads, q, (const StepCounters *)(ctx->ring + slot), ptl,
-                           (const StepCounters *)(ctx->ring + prev_slot), ptl, ids_q, nids_q, 0u, nc, no,
-                           ctx->ring + slot);
+        hipLaunchKernelGGL(k_retire, dim3(nc + nids_q * per + 1), dim3(WAVE), 0, ctx->stream,
+                           (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q,
+                           (const StepCounters *)(ctx->ring + slot), ptl, (const StepCounters *)(ctx->ring + prev_slot),
+                           ptl, ids_q, nids_q, per, 0u, nc, ctx->ring + slot);
         CHK(launch_check(ctx, "final clear"));
     }
     HIPCHK(hipEventRecord(ev_end, ctx->stream));

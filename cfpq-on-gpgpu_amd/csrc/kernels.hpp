@@ -1521,22 +1521,24 @@ __global__ void k_clear_occ(const MatDev *__restrict__ mats, const uint32_t *__r
 //   blocks [na, na + nc)       zero the rows and metadata of the delta the step
 //                              consumed (parity 1 - parity, the previous step's
 //                              list `ptl`, counts in prev);
-//   blocks [.., .. + no)       zero that delta's occupancy bitsets (ids[0..nid));
+//   blocks [.., .. + nid * per) zero that delta's occupancy bitsets (ids[0..nid)),
+//                              `per` blocks per matrix;
 //   the last block             zero the counters of the next ring slot, so that
 //                              the next step starts without a launch of its own.
 __global__ __launch_bounds__(WAVE) void k_retire(const MatDev *__restrict__ mats,
                                                  const HeadDev *__restrict__ heads, int parity,
                                                  const StepCounters *__restrict__ c, TouchedLists tl,
                                                  const StepCounters *__restrict__ prev, TouchedLists ptl,
-                                                 const uint32_t *__restrict__ ids, uint32_t nid, uint32_t na,
-                                                 uint32_t nc, uint32_t no, StepCounters *__restrict__ next) {
+                                                 const uint32_t *__restrict__ ids, uint32_t nid, uint32_t per,
+                                                 uint32_t na, uint32_t nc, StepCounters *__restrict__ next) {
     const uint32_t b = blockIdx.x;
     if (b < na) {
         apply_waves(mats, heads, parity, c, tl, b, na);
     } else if (b < na + nc) {
         clear_waves(mats, heads, 1 - parity, prev, ptl, b - na, nc);
-    } else if (b < na + nc + no) {
-        for (uint32_t q = 0; q < nid; ++q) zero_occ(mats[ids[q]], (b - na - nc) * WAVE + threadIdx.x, no * WAVE);
+    } else if (b < na + nc + nid * per) {
+        const uint32_t o = b - na - nc, m = o / per;
+        zero_occ(mats[ids[m]], (o - m * per) * WAVE + threadIdx.x, per * WAVE);
     } else if (next) {
         uint64_t *w = reinterpret_cast<uint64_t *>(next);
         for (uint32_t i = threadIdx.x; i < sizeof(StepCounters) / 8; i += blockDim.x) gst(w + i, (uint64_t)0);

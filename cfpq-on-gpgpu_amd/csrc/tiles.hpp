@@ -197,7 +197,6 @@ static inline uint32_t occ_words(uint32_t T) { return (T + 31) / 32; }
 constexpr uint64_t ITEM_TILE_MASK = 0xffffffffffull;
 constexpr uint64_t ITEM_FULL = 1ull << 63;
 constexpr uint32_t MAX_HEADS = 1u << 15;   // heads per plan (15-bit field)
-constexpr uint32_t RETIRE_OCC_BLOCKS = 64;  // waves of the retire launch that zero occupancy bitsets
 constexpr uint32_t SPLIT_MAX = 16;         // sub-units per heavy sparse unit
 constexpr uint32_t SPLIT_CANDS = 64;       // a unit with more candidates than this is split
 #ifndef CFPQ_SPLIT_ROUND
