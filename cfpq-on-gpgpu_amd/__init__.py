@@ -20,6 +20,9 @@ LIB_PATH = os.path.join(HERE, "libcfpq_hip.so")
 
 KERNEL_AUTO, KERNEL_GATHER, KERNEL_LDS, KERNEL_M4R, KERNEL_MFMA = range(5)
 KERNELS = {"auto": KERNEL_AUTO, "gather": KERNEL_GATHER, "lds": KERNEL_LDS, "m4r": KERNEL_M4R, "mfma": KERNEL_MFMA}
+#: cfpq_opts.schedule (CFPQ_SCHEDULE_*)
+SCHEDULE_DELTA, SCHEDULE_SWEEP = 0, 1
+SCHEDULES = {"delta": SCHEDULE_DELTA, "sweep": SCHEDULE_SWEEP}
 
 E_INVALID, E_DEVICE, E_NOMEM, E_LIMIT = -1, -2, -3, -4
 
@@ -47,7 +50,7 @@ class Opts(ctypes.Structure):
         ("sched_threshold", ctypes.c_uint32),
         ("sparse_mode", ctypes.c_int32),
         ("fuse_apply", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("schedule", ctypes.c_int32),
     ]
 
 
@@ -167,7 +170,8 @@ def lib():
 
 def make_opts(kernel="auto", dense_bits=0, mfma_bits=0, m4r_bits=0, max_iterations=0, final_full_pass=False,
               iters_per_sync=0, use_graph=False, sparse_batch=0, sparse_waves_per_cu=0,
-              sparse_chunk=0, sched=False, sched_threshold=0, sparse_mode=0, fuse_apply=False) -> Opts:
+              sparse_chunk=0, sched=False, sched_threshold=0, sparse_mode=0, fuse_apply=False,
+              schedule=0) -> Opts:
     o = Opts()
     lib().cfpq_default_opts(ctypes.byref(o))
     o.kernel = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
@@ -183,6 +187,7 @@ def make_opts(kernel="auto", dense_bits=0, mfma_bits=0, m4r_bits=0, max_iteratio
     o.sparse_chunk = sparse_chunk
     o.sparse_mode = sparse_mode
     o.fuse_apply = int(bool(fuse_apply))
+    o.schedule = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
     o.sched = int(bool(sched))
     # a float in (0, 1) is a probability; an int is already in units of 1/65536
     o.sched_threshold = int(round(sched_threshold * 65536)) if isinstance(sched_threshold, float) else sched_threshold

@@ -163,6 +163,7 @@ extern "C" void cfpq_default_opts(cfpq_opts *o) {
     o->final_full_pass = 0;
     o->iters_per_sync = 0;
     o->use_graph = 0;
+    o->schedule = CFPQ_SCHEDULE_DELTA;
 }
 
 static void normalise(cfpq_opts *o) {
@@ -177,6 +178,7 @@ static void normalise(cfpq_opts *o) {
     while (64 % o->sparse_chunk) --o->sparse_chunk;
     if (o->iters_per_sync > RING / 2) o->iters_per_sync = RING / 2;
     if (o->sched_threshold == 0) o->sched_threshold = 6554;  // P(active) < 0.1
+    if (o->schedule != CFPQ_SCHEDULE_SWEEP) o->schedule = CFPQ_SCHEDULE_DELTA;
 }
 
 extern "C" int cfpq_create(int device, const cfpq_opts *opts, cfpq_ctx **out) {
@@ -576,16 +578,20 @@ static SchedArgs sched_args(const cfpq_ctx *ctx, bool flush) {
 // fuse: merge new rows into the head matrices inside the product kernels
 // (fixpoint steps); add_mul keeps a separate apply pass so that aliased
 // operands are read as they were on entry.
+// zero_counters: the step zeroes its own counter block (a run's first step,
+// or a slot the previous step's retire launch did not zero for it).
+// head >= 0: only that head's items are built (a head-sweep step).
 static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, int prev_slot, bool full_pass,
-                        int ev_idx, bool flush = false, bool fuse = true) {
+                        int ev_idx, bool flush, bool fuse, bool zero_counters, int head = -1) {
     StepCounters *c = ctx->ring + slot;
-    const uint32_t nh = (uint32_t)plan.heads.size();
+    const uint32_t nh = head >= 0 ? 1u : (uint32_t)plan.heads.size();
+    const uint32_t h0 = head >= 0 ? (uint32_t)head : 0u;
     const TermDev *terms = ctx->d_terms;
     uint64_t *touched = ctx->lists[slot & 1];
     const uint32_t seg_cap = (uint32_t)plan.total_head_tiles;
-    // a step's counters are zeroed by the previous step's retire launch; the
-    // first step of a run zeroes its own
-    if (prev_slot < 0) hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
+    // a step's counters are zeroed by the previous step's retire launch,
+    // except where the caller says otherwise
+    if (zero_counters) hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ring, slot);
     // with the dense and MFMA paths both off, every item is sparse: the
     // heavy-tile bitsets are neither built nor read, and the dense and MFMA
     // kernels are not launched
@@ -605,7 +611,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const int split = (ctx->opts.sparse_mode == 0 && !(fuse && ctx->opts.fuse_apply)) ? 1 : 0;
     const uint32_t split_budget = (uint32_t)std::min<size_t>(QUARTERS * plan.total_head_tiles, 0xffff0000u);
     const uint32_t heavy_base = (uint32_t)(QUARTERS * plan.total_head_tiles);
-    const bool use_sched = !full_pass && plan.dirty_words && (sched_on(ctx) || ctx->tr_cap || flush);
+    const bool use_sched = head < 0 && !full_pass && plan.dirty_words && (sched_on(ctx) || ctx->tr_cap || flush);
     const SchedArgs sa = sched_args(ctx, flush);
     if (use_sched) {
         if (ctx->tr_cap)
@@ -617,7 +623,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     } else {
         hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, (int)full_pass, c, ctx->units, ctx->items,
-                           ctx->mitems, split, split_budget, heavy, heavy_base);
+                           ctx->mitems, split, split_budget, heavy, heavy_base, h0);
     }
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
     // product kernels, timed: ev[4i] sparse ev[4i+1] dense ev[4i+2] mfma ev[4i+3]
@@ -813,7 +819,10 @@ static int launch_step_graph(cfpq_ctx *ctx, const Plan &plan, uint32_t slot, int
     hipGraphExec_t &ex = ctx->graph_exec[(size_t)variant * RING + slot];
     if (!ex) {
         HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(ctx, plan, (int)(slot & 1), slot, prev_slot, full_pass, -1, flush);
+        // the first step, full passes and flush steps zero their own counters:
+        // they may follow steps whose slots the host reuses
+        int rc = enqueue_step(ctx, plan, (int)(slot & 1), slot, prev_slot, full_pass, -1, flush, true,
+                              prev_slot < 0 || full_pass || flush);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if (rc != CFPQ_OK) {
@@ -830,6 +839,72 @@ static int launch_step_graph(cfpq_ctx *ctx, const Plan &plan, uint32_t slot, int
     }
     HIPCHK(hipGraphLaunch(ex, ctx->stream));
     return CFPQ_OK;
+}
+
+// Head sweeps (CFPQ_SCHEDULE_SWEEP): each step evaluates one head's rules over
+// the full matrices; a sweep visits the heads in order, the first sweep all of
+// them, later sweeps those that read a matrix some head changed in the sweep
+// before.  The first sweep that adds no bit ends the run.  Every head is then
+// stable: a head last evaluated in sweep j saw no operand change after its
+// step (else sweep j + 1 would have revisited it), and none since.
+static int run_sweeps(cfpq_ctx *ctx, const Plan &plan, bool heavy_paths, StepStats &tot, double *part_ms,
+                      double &product_ms, uint32_t &steps, uint32_t &syncs, uint32_t &launches, int &prev_slot) {
+    const size_t nh = plan.heads.size();
+    std::vector<int> head_of(plan.mats.size(), -1);
+    for (size_t h = 0; h < nh; ++h) head_of[plan.heads[h].target] = (int)h;
+    std::vector<std::vector<uint32_t>> readers(nh);  // heads whose rules read head x's matrix
+    for (size_t h = 0; h < nh; ++h)
+        for (uint32_t t = plan.heads[h].f0; t < plan.heads[h].f1; ++t)
+            for (uint32_t m : {plan.terms_full[t].left[0], plan.terms_full[t].right[0]})
+                if (head_of[m] >= 0) readers[head_of[m]].push_back((uint32_t)h);
+    std::vector<char> visit(nh, 1), changed(nh, 0);
+    std::vector<uint32_t> slots, hs;
+    for (;;) {
+        std::fill(changed.begin(), changed.end(), 0);
+        bool any = false;
+        for (size_t h = 0; h <= nh; ++h) {
+            if (h < nh && visit[h]) {
+                const uint32_t slot = steps % RING;
+                CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, true, (int)slots.size(), false, true,
+                                 prev_slot < 0, (int)h));
+                ++launches;
+                slots.push_back(slot);
+                hs.push_back((uint32_t)h);
+                prev_slot = (int)slot;
+                ++steps;
+            }
+            // read the counters when the ring is about to wrap and at the end of the sweep
+            if (slots.empty() || (h < nh && slots.size() < RING - 1)) continue;
+            HIPCHK(hipMemcpyAsync(ctx->ring_host, ctx->ring, sizeof(StepCounters) * RING, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ++syncs;
+            for (size_t s = 0; s < slots.size(); ++s) {
+                StepCounters &c = ctx->ring_host[slots[s]];
+                fold_shards(c);
+                float step_ms[3] = {0, 0, 0};
+                for (int k2 = 0; k2 < (heavy_paths ? 3 : 1); ++k2) {
+                    HIPCHK(hipEventElapsedTime(&step_ms[k2], ctx->ev[4 * s + k2], ctx->ev[4 * s + k2 + 1]));
+                    part_ms[k2] += step_ms[k2];
+                    product_ms += step_ms[k2];
+                }
+                accumulate(tot, c);
+                if (c.new_bits) {
+                    changed[hs[s]] = 1;
+                    any = true;
+                }
+            }
+            slots.clear();
+            hs.clear();
+            if (ctx->opts.max_iterations && steps >= ctx->opts.max_iterations)
+                return fail(ctx, CFPQ_E_LIMIT, "fixpoint: iteration cap %u reached", ctx->opts.max_iterations);
+        }
+        if (!any) return CFPQ_OK;
+        std::fill(visit.begin(), visit.end(), 0);
+        for (size_t x = 0; x < nh; ++x)
+            if (changed[x])
+                for (uint32_t r : readers[x]) visit[r] = 1;
+    }
 }
 
 // Run delta steps to the fixpoint (plus the optional exact full pass).
@@ -867,6 +942,12 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
     // next batch: 0 = delta steps, 1 = one exact full-pass step, 2 = one flush step
     int pending = single_step_full ? 1 : 0;
     const bool want_full = ctx->opts.final_full_pass != 0;
+    if (ctx->opts.schedule == CFPQ_SCHEDULE_SWEEP && !single_step_full) {
+        CHK(run_sweeps(ctx, plan, heavy_paths, tot, part_ms, product_ms, steps, syncs, launches, prev_slot));
+        // the optional exact full pass runs as in the delta schedule
+        if (want_full) pending = 1;
+        else done = true;
+    }
     while (!done) {
         const int kind = pending;
         const int k = kind ? 1 : per_sync;
@@ -877,7 +958,8 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             // (with d = M the two semi-naive terms would repeat the same product)
             const bool full = kind == 1 || steps == 0;
             if (use_graph) CHK(launch_step_graph(ctx, plan, slot, prev_slot, full, kind == 2));
-            else CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, full, s, kind == 2));
+            else CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, full, s, kind == 2, true,
+                                  prev_slot < 0 || (uint32_t)(prev_slot + 1) % RING != slot));
             ++launches;
             slots.push_back(slot);
             prev_slot = (int)slot;
@@ -1147,7 +1229,7 @@ extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const
     plan.all_delta[1] = {3};
     plan.max_tiles_per_head = plan.total_head_tiles = (size_t)c->T * c->T;
     int rc = upload_plan(ctx, plan);
-    if (rc == CFPQ_OK) rc = enqueue_step(ctx, plan, 0, 0, -1, false, -1, false, /*fuse=*/false);
+    if (rc == CFPQ_OK) rc = enqueue_step(ctx, plan, 0, 0, -1, false, -1, false, /*fuse=*/false, true);
     // enqueue_step cleared nothing useful for S yet (prev_slot < 0 clears all
     // non-empty tiles of "D_0" = S *after* apply), which is exactly the reset we need.
     if (rc == CFPQ_OK) {

@@ -452,15 +452,16 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
     }
 }
 
-// One thread per (head, C tile).  Skips full head tiles.
+// One thread per (head h0 + blockIdx.y, C tile).  Skips full head tiles.
 __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats,
                                                  const HeadDev *__restrict__ heads,
                                                  const TermDev *__restrict__ terms, int parity, int full_pass,
                                                  StepCounters *__restrict__ c, uint64_t *__restrict__ units,
                                                  uint64_t *__restrict__ dense_items,
                                                  uint64_t *__restrict__ mfma_items, int split,
-                                                 uint32_t split_budget, int heavy, uint32_t heavy_base) {
-    const uint32_t h = blockIdx.y;
+                                                 uint32_t split_budget, int heavy, uint32_t heavy_base,
+                                                 uint32_t h0) {
+    const uint32_t h = h0 + blockIdx.y;
     const uint32_t T = uniform(heads[h].T), target = uniform(heads[h].target);
     const uint32_t t0 = uniform(full_pass ? heads[h].f0 : heads[h].t0);
     const uint32_t t1 = uniform(full_pass ? heads[h].f1 : heads[h].t1);

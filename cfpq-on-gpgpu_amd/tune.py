@@ -4,6 +4,7 @@ For a workload it times complete fixpoints (device time from the engine's own
 hipEvents, median of a few runs) while walking the parameters one at a time
 (coordinate descent, best value kept before moving to the next parameter):
 
+  schedule             semi-naive delta steps over all rules (0) or head sweeps over full matrices (1)
   sparse_mode          sparse product: lane-parallel candidates (0) or row-per-lane (1)
   sparse_batch         k-tiles whose loads the row-per-lane sparse product issues together
   sparse_waves_per_cu  persistent waves of the sparse product per CU
@@ -31,6 +32,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned.json")
 
 SPACE = [
+    ("schedule", [0, 1]),
     ("sparse_mode", [0, 1]),
     ("sparse_batch", [1, 2, 4]),
     ("sparse_waves_per_cu", [2, 4, 8, 12, 16, 32]),
@@ -44,7 +46,7 @@ SPACE = [
 # Graph replay (use_graph) is left to the caller: it saves about 7% of an
 # SG-forest fixpoint at N = 65,536 in launch gaps, but per-kernel event
 # timing needs the eager launch path.
-DEFAULTS = {"sparse_mode": 0, "sparse_batch": 2, "sparse_waves_per_cu": 8, "sparse_chunk": 4, "dense_bits": 2048, "m4r_bits": 16384,
+DEFAULTS = {"schedule": 0, "sparse_mode": 0, "sparse_batch": 2, "sparse_waves_per_cu": 8, "sparse_chunk": 4, "dense_bits": 2048, "m4r_bits": 16384,
             "mfma_bits": 32768, "iters_per_sync": 4, "fuse_apply": 0}
 
 

@@ -90,8 +90,20 @@ typedef struct cfpq_opts {
                                    1 = row-per-lane with sparse_batch candidates per round */
     int32_t fuse_apply;         /* 1: fixpoint steps merge new rows into the head matrices
                                    inside the product kernels (no separate apply pass)  */
-    int32_t reserved[1];
+    int32_t schedule;           /* CFPQ_SCHEDULE_*: how cfpq_fixpoint orders the rules    */
 } cfpq_opts;
+
+/* cfpq_opts.schedule */
+#define CFPQ_SCHEDULE_DELTA 0   /* semi-naive steps, every rule at once over the previous
+                                   step's delta (default)                                */
+#define CFPQ_SCHEDULE_SWEEP 1   /* head sweeps: one head at a time, its rules over the full
+                                   matrices, each head seeing the bits of the heads before
+                                   it; a sweep revisits only heads one of whose operands
+                                   changed in the previous sweep, and the fixpoint is the
+                                   first sweep that adds no bit.  The reference
+                                   worklist's order of evaluation (cfpq-cpp/Grammar.h:58-69)
+                                   without a host round trip per rule; it wins on
+                                   saturating grammars whose tiles fill early         */
 
 typedef struct cfpq_rule {
     uint32_t head, left, right; /* indices into the matrix array: head -> left right */

@@ -34,7 +34,7 @@ def test_struct_layouts_match_header(pkg):
 #include "cfpq_hip.h"
 int main(void) {
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(cfpq_opts), sizeof(cfpq_rule), sizeof(cfpq_stats),
-         sizeof(cfpq_query), offsetof(cfpq_stats, fixpoint_ms), offsetof(cfpq_opts, reserved));
+         sizeof(cfpq_query), offsetof(cfpq_stats, fixpoint_ms), offsetof(cfpq_opts, schedule));
   return 0;
 }
 '''
@@ -46,7 +46,7 @@ int main(void) {
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     got = [int(x) for x in out]
     want = [ctypes.sizeof(pkg.Opts), ctypes.sizeof(pkg.Rule), ctypes.sizeof(pkg.Stats), ctypes.sizeof(pkg.Query),
-            pkg.Stats.fixpoint_ms.offset, pkg.Opts.reserved.offset]
+            pkg.Stats.fixpoint_ms.offset, pkg.Opts.schedule.offset]
     assert got == want
 
 
