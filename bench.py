@@ -140,7 +140,7 @@ def main():
     ap.add_argument("--config", type=int, default=4, choices=sorted(WORKLOADS) + [3])
     ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs per step")
     ap.add_argument("--kernel", default="auto")
-    ap.add_argument("--schedule", choices=["delta", "sweep"], default=None,
+    ap.add_argument("--schedule", choices=["delta", "sweep", "rules"], default=None,
                     help="rule schedule of the fixpoint (default: the tuned one)")
     ap.add_argument("--iters-per-sync", type=int, default=0)
     ap.add_argument("--sparse-batch", type=int, default=0)
@@ -198,7 +198,7 @@ def main():
     opts = tune.tuned_opts(n, q.grammar)
     opts["kernel"] = args.kernel
     if args.schedule:
-        opts["schedule"] = {"delta": 0, "sweep": 1}[args.schedule]
+        opts["schedule"] = {"delta": 0, "sweep": 1, "rules": 2}[args.schedule]
     if args.iters_per_sync:
         opts["iters_per_sync"] = args.iters_per_sync
     if args.graph:
@@ -302,7 +302,8 @@ def main():
         "data": "synthetic (seeded generator, same inputs as the CPU oracle)",
         "config": {"workload": wname, "n": n, "edges": q.n_edges, "nonterminals": gr.nonterminals_count,
                    "rules": len(rules), "A_ref": a_ref, "A_ref_kind": work_key,
-                   "engine": {"plain": "head-sweep fixpoint" if opts.get("schedule") == 1 else "delta fixpoint",
+                   "engine": {"plain": {1: "head-sweep fixpoint", 2: "rule-sweep fixpoint"}.get(opts.get("schedule"),
+                                                                                              "delta fixpoint"),
                               "learned": "delta fixpoint + learned tile scheduler"}[chosen],
                    "kernel": args.kernel, "parallelism": "single GPU"},
         "fixpoint_s": round(statistics.median(fix_ms) / 1e3, 6),

@@ -21,8 +21,8 @@ LIB_PATH = os.path.join(HERE, "libcfpq_hip.so")
 KERNEL_AUTO, KERNEL_GATHER, KERNEL_LDS, KERNEL_M4R, KERNEL_MFMA = range(5)
 KERNELS = {"auto": KERNEL_AUTO, "gather": KERNEL_GATHER, "lds": KERNEL_LDS, "m4r": KERNEL_M4R, "mfma": KERNEL_MFMA}
 #: cfpq_opts.schedule (CFPQ_SCHEDULE_*)
-SCHEDULE_DELTA, SCHEDULE_SWEEP = 0, 1
-SCHEDULES = {"delta": SCHEDULE_DELTA, "sweep": SCHEDULE_SWEEP}
+SCHEDULE_DELTA, SCHEDULE_SWEEP, SCHEDULE_RULES = 0, 1, 2
+SCHEDULES = {"delta": SCHEDULE_DELTA, "sweep": SCHEDULE_SWEEP, "rules": SCHEDULE_RULES}
 
 E_INVALID, E_DEVICE, E_NOMEM, E_LIMIT = -1, -2, -3, -4
 

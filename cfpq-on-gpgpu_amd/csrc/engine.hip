@@ -178,7 +178,7 @@ static void normalise(cfpq_opts *o) {
     while (64 % o->sparse_chunk) --o->sparse_chunk;
     if (o->iters_per_sync > RING / 2) o->iters_per_sync = RING / 2;
     if (o->sched_threshold == 0) o->sched_threshold = 6554;  // P(active) < 0.1
-    if (o->schedule != CFPQ_SCHEDULE_SWEEP) o->schedule = CFPQ_SCHEDULE_DELTA;
+    if (o->schedule != CFPQ_SCHEDULE_SWEEP && o->schedule != CFPQ_SCHEDULE_RULES) o->schedule = CFPQ_SCHEDULE_DELTA;
 }
 
 extern "C" int cfpq_create(int device, const cfpq_opts *opts, cfpq_ctx **out) {
@@ -502,7 +502,29 @@ struct Plan {
     size_t ids_all_off = 0;            // offset of the "every matrix" id list in d_ids
     std::vector<uint32_t> dirty_off;   // per head: word offset of its deferred-tile bitmap
     size_t dirty_words = 0;
+    bool rule_heads = false;           // one head entry per rule (rule sweeps); entries share targets
 };
+
+// The plan with one head entry per rule, for rule sweeps: entry (h, r) has
+// head h's target and delta buffers and rule r's full term only.  Entries of
+// one head share a target, so they never run in the same step.
+static Plan rule_plan(const Plan &plan) {
+    Plan rp = plan;
+    rp.heads.clear();
+    rp.total_head_tiles = 0;
+    for (const HeadDev &hd : plan.heads)
+        for (uint32_t t = hd.f0; t < hd.f1; ++t) {
+            HeadDev rh = hd;
+            rh.f0 = t;
+            rh.f1 = t + 1;
+            rh.t1 = rh.t0;  // no delta terms: every step of a sweep is a full step
+            rh.p1 = rh.p0;
+            rp.heads.push_back(rh);
+            rp.total_head_tiles += (size_t)hd.T * hd.T;
+        }
+    rp.rule_heads = true;
+    return rp;
+}
 
 // Waves of the retire launch per matrix whose occupancy bitsets it zeroes:
 // one per 128 words of the largest matrix (2 words per lane), at most 32.
@@ -841,24 +863,27 @@ static int launch_step_graph(cfpq_ctx *ctx, const Plan &plan, uint32_t slot, int
     return CFPQ_OK;
 }
 
-// Head sweeps (CFPQ_SCHEDULE_SWEEP): each step evaluates one head's rules over
-// the full matrices; a sweep visits the heads in order, the first sweep all of
-// them, later sweeps those that read a matrix some head changed in the sweep
-// before.  The first sweep that adds no bit ends the run.  Every head is then
-// stable: a head last evaluated in sweep j saw no operand change after its
-// step (else sweep j + 1 would have revisited it), and none since.
-static int run_sweeps(cfpq_ctx *ctx, const Plan &plan, bool heavy_paths, StepStats &tot, double *part_ms,
-                      double &product_ms, uint32_t &steps, uint32_t &syncs, uint32_t &launches, int &prev_slot) {
+// Sweeps (CFPQ_SCHEDULE_SWEEP, CFPQ_SCHEDULE_RULES): each step evaluates one
+// head entry of the plan -- a head's rules, or a single rule in a rule plan --
+// over the full matrices; a sweep visits the entries in order, the first sweep
+// all of them, later sweeps those that read a matrix some entry changed in the
+// sweep before.  The first sweep that adds no bit ends the run.  Every entry
+// is then stable: one last evaluated in sweep j saw no operand change after
+// its step (else sweep j + 1 would have revisited it), and none since.  With
+// `certify`, one more sweep over every entry must add no bit (counted as the
+// exact full pass; were it to add bits, the sweeps would go on).
+static int run_sweeps(cfpq_ctx *ctx, const Plan &plan, bool heavy_paths, bool certify, StepStats &tot,
+                      double *part_ms, double &product_ms, uint32_t &steps, uint32_t &syncs, uint32_t &launches,
+                      int &prev_slot, uint32_t &full_passes) {
     const size_t nh = plan.heads.size();
-    std::vector<int> head_of(plan.mats.size(), -1);
-    for (size_t h = 0; h < nh; ++h) head_of[plan.heads[h].target] = (int)h;
-    std::vector<std::vector<uint32_t>> readers(nh);  // heads whose rules read head x's matrix
+    std::vector<std::vector<uint32_t>> readers(plan.mats.size());  // entries whose rules read matrix m
     for (size_t h = 0; h < nh; ++h)
         for (uint32_t t = plan.heads[h].f0; t < plan.heads[h].f1; ++t)
             for (uint32_t m : {plan.terms_full[t].left[0], plan.terms_full[t].right[0]})
-                if (head_of[m] >= 0) readers[head_of[m]].push_back((uint32_t)h);
+                if (readers[m].empty() || readers[m].back() != (uint32_t)h) readers[m].push_back((uint32_t)h);
     std::vector<char> visit(nh, 1), changed(nh, 0);
     std::vector<uint32_t> slots, hs;
+    bool certifying = false;
     for (;;) {
         std::fill(changed.begin(), changed.end(), 0);
         bool any = false;
@@ -899,11 +924,18 @@ static int run_sweeps(cfpq_ctx *ctx, const Plan &plan, bool heavy_paths, StepSta
             if (ctx->opts.max_iterations && steps >= ctx->opts.max_iterations)
                 return fail(ctx, CFPQ_E_LIMIT, "fixpoint: iteration cap %u reached", ctx->opts.max_iterations);
         }
-        if (!any) return CFPQ_OK;
+        if (certifying) ++full_passes;
+        if (!any) {
+            if (!certify || certifying) return CFPQ_OK;
+            certifying = true;
+            std::fill(visit.begin(), visit.end(), 1);
+            continue;
+        }
+        certifying = false;
         std::fill(visit.begin(), visit.end(), 0);
         for (size_t x = 0; x < nh; ++x)
             if (changed[x])
-                for (uint32_t r : readers[x]) visit[r] = 1;
+                for (uint32_t r : readers[plan.heads[x].target]) visit[r] = 1;
     }
 }
 
@@ -911,6 +943,10 @@ static int run_sweeps(cfpq_ctx *ctx, const Plan &plan, bool heavy_paths, StepSta
 // The caller owns plan buffers; on return all delta buffers are zero again
 // (on success) -- on failure they are released instead of pooled.
 static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool single_step_full) {
+    if (ctx->opts.schedule == CFPQ_SCHEDULE_RULES && !single_step_full && !plan.rule_heads) {
+        Plan rp = rule_plan(plan);
+        return run_fixpoint(ctx, rp, stats, single_step_full);
+    }
     CHK(upload_plan(ctx, plan));
     const uint32_t nd = (uint32_t)plan.all_delta[0].size();
     // (the first step reads the full matrices, so the initial delta buffer
@@ -942,11 +978,10 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
     // next batch: 0 = delta steps, 1 = one exact full-pass step, 2 = one flush step
     int pending = single_step_full ? 1 : 0;
     const bool want_full = ctx->opts.final_full_pass != 0;
-    if (ctx->opts.schedule == CFPQ_SCHEDULE_SWEEP && !single_step_full) {
-        CHK(run_sweeps(ctx, plan, heavy_paths, tot, part_ms, product_ms, steps, syncs, launches, prev_slot));
-        // the optional exact full pass runs as in the delta schedule
-        if (want_full) pending = 1;
-        else done = true;
+    if (ctx->opts.schedule != CFPQ_SCHEDULE_DELTA && !single_step_full) {
+        CHK(run_sweeps(ctx, plan, heavy_paths, want_full, tot, part_ms, product_ms, steps, syncs, launches, prev_slot,
+                       full_passes));
+        done = true;
     }
     while (!done) {
         const int kind = pending;

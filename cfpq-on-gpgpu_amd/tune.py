@@ -4,7 +4,7 @@ For a workload it times complete fixpoints (device time from the engine's own
 hipEvents, median of a few runs) while walking the parameters one at a time
 (coordinate descent, best value kept before moving to the next parameter):
 
-  schedule             semi-naive delta steps over all rules (0) or head sweeps over full matrices (1)
+  schedule             semi-naive delta steps over all rules (0), head sweeps (1) or rule sweeps (2) over full matrices
   sparse_mode          sparse product: lane-parallel candidates (0) or row-per-lane (1)
   sparse_batch         k-tiles whose loads the row-per-lane sparse product issues together
   sparse_waves_per_cu  persistent waves of the sparse product per CU
@@ -32,7 +32,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned.json")
 
 SPACE = [
-    ("schedule", [0, 1]),
+    ("schedule", [0, 1, 2]),
     ("sparse_mode", [0, 1]),
     ("sparse_batch", [1, 2, 4]),
     ("sparse_waves_per_cu", [2, 4, 8, 12, 16, 32]),

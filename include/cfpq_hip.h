@@ -104,6 +104,9 @@ typedef struct cfpq_opts {
                                    worklist's order of evaluation (cfpq-cpp/Grammar.h:58-69)
                                    without a host round trip per rule; it wins on
                                    saturating grammars whose tiles fill early         */
+#define CFPQ_SCHEDULE_RULES 2   /* as CFPQ_SCHEDULE_SWEEP, one rule at a time: a sweep
+                                   revisits only the rules one of whose operands
+                                   changed                                               */
 
 typedef struct cfpq_rule {
     uint32_t head, left, right; /* indices into the matrix array: head -> left right */

@@ -344,29 +344,31 @@ def test_batch_512_queries_golden(pkg, gen):
             gr.free()
 
 
+@pytest.mark.parametrize("schedule", ["sweep", "rules"])
 @pytest.mark.parametrize("which", ["dense", "dyck", "sg", "self"])
-def test_fixpoint_head_sweeps(pkg, gen, oracle, which):
-    """CFPQ_SCHEDULE_SWEEP: heads evaluated one at a time over the full
-    matrices, later heads seeing earlier heads' bits, sweeps until one adds
-    no bit.  Same least fixpoint with every kernel, with and without the head
-    update fused into the products."""
+def test_fixpoint_head_sweeps(pkg, gen, oracle, which, schedule):
+    """CFPQ_SCHEDULE_SWEEP / _RULES: heads (or single rules) evaluated one at a
+    time over the full matrices, later ones seeing earlier ones' bits, sweeps
+    until one adds no bit.  Same least fixpoint with every kernel, with and
+    without the head update fused into the products."""
     q = {"dense": lambda: gen.dense_stress(1500), "dyck": lambda: gen.dyck_er(1200, 4000, seed=9),
          "sg": lambda: gen.sg_forest(8000, seed=10, types_per_vertex=4),
          "self": lambda: gen.dyck_er(300, 800, seed=21)}[which]()
     if which == "self":  # epsilon rule and a rule whose operands alias its head
         q.grammar = "S\nS A S1\nS1 S B\nS S S\nA a\nB b\n"
     for kernel, fuse in (("auto", False), ("auto", True), ("gather", False), ("m4r", True), ("mfma", True)):
-        st = _fixpoint_vs_oracle(pkg, gen, oracle, q, kernel=kernel, fuse_apply=fuse, schedule="sweep", dense_bits=64,
+        st = _fixpoint_vs_oracle(pkg, gen, oracle, q, kernel=kernel, fuse_apply=fuse, schedule=schedule, dense_bits=64,
                                  m4r_bits=4096, mfma_bits=30000)
         assert st["iterations"] >= 2 and st["full_passes"] == 0
 
 
-def test_fixpoint_head_sweeps_final_full_pass_and_cap(pkg, gen, oracle):
+@pytest.mark.parametrize("schedule", ["sweep", "rules"])
+def test_fixpoint_head_sweeps_final_full_pass_and_cap(pkg, gen, oracle, schedule):
     q = gen.dyck_er(700, 2000, seed=3)
-    st = _fixpoint_vs_oracle(pkg, gen, oracle, q, schedule="sweep", final_full_pass=True)
+    st = _fixpoint_vs_oracle(pkg, gen, oracle, q, schedule=schedule, final_full_pass=True)
     assert st["full_passes"] == 1
     cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
-    with pkg.Engine(0, schedule="sweep", max_iterations=3) as eng:
+    with pkg.Engine(0, schedule=schedule, max_iterations=3) as eng:
         gr = cf.Grammar(q.grammar, text=True)
         with pytest.raises(pkg.CfpqError) as ei:
             gr.intersection_with_graph(cf.Graph.from_query(q), eng)
@@ -381,22 +383,24 @@ def test_fixpoint_head_sweeps_dense_4096_golden(pkg, gen):
     golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixpoints.json")))["config2_n4096"]
     q = gen.dense_stress(4096)
     assert q.digest() == golden["digest"]
-    with pkg.Engine(0, schedule="sweep", fuse_apply=True) as eng:
-        gr = cf.Grammar(q.grammar, text=True)
-        gr.intersection_with_graph(cf.Graph.from_query(q), eng)
-        assert gr.results() == {k: tuple(v) for k, v in golden["fingerprints"].items()}
-        gr.free()
+    for schedule in ("sweep", "rules"):
+        with pkg.Engine(0, schedule=schedule, fuse_apply=True) as eng:
+            gr = cf.Grammar(q.grammar, text=True)
+            gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+            assert gr.results() == {k: tuple(v) for k, v in golden["fingerprints"].items()}, schedule
+            gr.free()
 
 
-def test_batch_head_sweeps(pkg, gen, oracle, engine):
-    """A batch under head sweeps: every query's fixpoint as computed alone."""
+@pytest.mark.parametrize("schedule", ["sweep", "rules"])
+def test_batch_head_sweeps(pkg, gen, oracle, engine, schedule):
+    """A batch under head or rule sweeps: every query's fixpoint as computed alone."""
     cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
     qs = gen.batch_queries(6, seed=77, n_lo=200, n_hi=900)
     want = []
     for q in qs:
         g_ref, mats_ref, _ = oracle.solve(q, "seminaive")
         want.append(oracle.fingerprints(g_ref, mats_ref, q.n))
-    with pkg.Engine(0, schedule="sweep") as eng:
+    with pkg.Engine(0, schedule=schedule) as eng:
         grs = []
         for q in qs:
             gr = cf.Grammar(q.grammar, text=True)
