@@ -289,44 +289,37 @@ __device__ __forceinline__ QuarterOut quarter_epilogue(const MatDev &O, const Ma
     return QuarterOut{bal, pc};
 }
 
-// Epilogue of one sub-unit of a split quarter: the sibling sub-units of the
-// same quarter write the same rows, so rows and metadata are merged with
-// atomic ORs and the returned count holds only the bits this sub-unit set
-// first.  No in-place head update (splitting is off when that is on).
-__device__ __forceinline__ QuarterOut quarter_epilogue_shared(const MatDev &O, const MatDev &M, uint32_t p,
-                                                              uint32_t q, uint32_t I, uint32_t J, const Row4 &acc,
-                                                              uint32_t &out_rows) {
+// Epilogue of one sub-unit of a split quarter.  Sub-unit s of n owns the
+// quarter's rows r = s (mod n): its rows are its alone and are written as in
+// quarter_epilogue; the quarter's row mask and column union are shared with
+// the sibling sub-units and merged with atomic ORs.  No in-place head update
+// (splitting is off when that is on).
+__device__ __forceinline__ QuarterOut quarter_epilogue_rows(const MatDev &O, const MatDev &M, uint32_t p, uint32_t q,
+                                                            uint32_t I, uint32_t J, const Row4 &acc,
+                                                            uint32_t &out_rows) {
     const int lane = threadIdx.x & 63;
     const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
     Row4 nw = {{0, 0, 0, 0}};
-    uint32_t mine = 0;
     if (row_nonzero(acc)) {
         const Row4 m = load_row(M.bits + off);
 #pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            nw.w[w] = acc.w[w] & ~m.w[w];
-            if (nw.w[w]) {
-                const uint64_t old = g_or(O.bits + off + w, nw.w[w]);
-                mine += (uint32_t)__popcll(nw.w[w] & ~old);
-            }
+        for (int w = 0; w < 4; ++w) nw.w[w] = acc.w[w] & ~m.w[w];
+        if (row_nonzero(nw)) {
+            store_row(O.bits + off, nw);
+            ++out_rows;
         }
-        out_rows += row_nonzero(nw);
     }
     const uint64_t bal = __ballot(row_nonzero(nw));
-    const RowsUnion ru = rows_union(nw, mine, bal);
-    const uint32_t pc = ru.sum;
-    if (bal) {
-        const Row4 cu = ru.cu;
-        if (lane == 0) {
-            g_or(&O.rmask[(size_t)p * 4 + q], bal);
-            uint64_t *cq = colq_of(O, p, q);
+    const RowsUnion ru = rows_union(nw, row_popc(nw), bal);
+    if (bal && lane == 0) {
+        g_or(&O.rmask[(size_t)p * 4 + q], bal);
+        uint64_t *cq = colq_of(O, p, q);
 #pragma unroll
-            for (int w = 0; w < 4; ++w)
-                if (cu.w[w]) g_or(cq + w, cu.w[w]);
-            set_occq(O, q, I, J);
-        }
+        for (int w = 0; w < 4; ++w)
+            if (ru.cu.w[w]) g_or(cq + w, ru.cu.w[w]);
+        set_occq(O, q, I, J);
     }
-    return QuarterOut{bal, pc};
+    return QuarterOut{bal, ru.sum};
 }
 
 // ---------------------------------------------------------------------------
@@ -790,7 +783,14 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         const MatDev O = mats[uniform(heads[h].out[parity])];
         const MatDev M = mats[uniform(heads[h].target)];
         const uint32_t tp_unit0 = tp;
-        uint32_t round = 0;  // candidate rounds seen; this sub-unit takes rounds = sub (mod nsub)
+        // a split quarter's sub-unit `sub` of `nsub` takes the rows r = sub
+        // (mod nsub) of every candidate, so that a dense quarter's gathers are
+        // spread over several waves and each row keeps a single writer
+        uint64_t rowsel = ~0ull;
+        if (nsub > 1) {
+            rowsel = 0;
+            for (uint32_t r = sub; r < WAVE; r += nsub) rowsel |= 1ull << r;
+        }
         acc_s[lane] = Row4{{0, 0, 0, 0}};
         __syncthreads();
         const uint32_t ow = (T + 31) / 32;
@@ -814,14 +814,9 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             s_cend[lane] = incw;
             s_ops[lane] = ops;
             __syncthreads();
-            // a split unit takes rounds of SPLIT_ROUND candidates, the
-            // sub-units in turn, so that a dense quarter's gathers are spread
-            // over several waves
-            const uint32_t step = nsub > 1 ? SPLIT_ROUND : WAVE;
-            for (uint32_t cb = 0; cb < total; cb += step) {
-                if ((round++ % nsub) != sub) continue;
-                const uint32_t nc = (total - cb < step) ? total - cb : step;
-                tp += nc;
+            for (uint32_t cb = 0; cb < total; cb += WAVE) {
+                const uint32_t nc = (total - cb < WAVE) ? total - cb : WAVE;
+                if (sub == 0) tp += nc;  // a split quarter's candidates count once
                 // 2. filter up to 64 candidates: lane g takes the g-th candidate
                 const uint32_t g = cb + lane;
                 uint64_t arm = 0;
@@ -842,7 +837,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     const size_t lt = (size_t)I * T + K, rt = (size_t)K * T + J;
                     brm = load_row(o.rrmask + rt * 4);
                     const Row4 cu = load_row(o.lcolq + (lt * QUARTERS + q) * TROW_WORDS);
-                    arm = gld(&o.lrmask[lt * 4 + q]);
+                    arm = gld(&o.lrmask[lt * 4 + q]) & rowsel;
                     const bool hit = ((cu.w[0] & brm.w[0]) | (cu.w[1] & brm.w[1]) | (cu.w[2] & brm.w[2]) |
                                       (cu.w[3] & brm.w[3])) != 0;
                     if (!hit) arm = 0;
@@ -966,7 +961,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         __syncthreads();
         const Row4 acc = acc_s[lane];
         max_cands = max(max_cands, tp - tp_unit0);
-        const QuarterOut qo = nsub > 1 ? quarter_epilogue_shared(O, M, p, q, I, J, acc, out_rows)
+        const QuarterOut qo = nsub > 1 ? quarter_epilogue_rows(O, M, p, q, I, J, acc, out_rows)
                                        : quarter_epilogue(O, M, p, q, I, J, acc, fuse, out_rows);
         if (qo.pc && lane == 0) {
             const uint32_t old = g_add(&O.cnt[p], qo.pc);

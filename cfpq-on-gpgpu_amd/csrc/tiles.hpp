@@ -202,7 +202,7 @@ constexpr uint32_t SPLIT_CANDS = 64;       // a unit with more candidates than t
 #ifndef CFPQ_SPLIT_ROUND
 #define CFPQ_SPLIT_ROUND 64
 #endif
-// candidates per round in a split unit; the sub-units take the rounds in turn
+// candidates per sub-unit when a unit is split (sub-units share the rows)
 constexpr uint32_t SPLIT_ROUND = CFPQ_SPLIT_ROUND;
 __host__ __device__ inline uint32_t item_head(uint64_t it) { return (uint32_t)((it >> 40) & 0x7fffu); }
 __host__ __device__ inline uint32_t unit_sub(uint64_t it) { return (uint32_t)((it >> 55) & 15u); }
