@@ -645,7 +645,8 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     } else {
         hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, (int)full_pass, c, ctx->units, ctx->items,
-                           ctx->mitems, split, split_budget, heavy, heavy_base, h0);
+                           ctx->mitems, split, split_budget, heavy, heavy_base, h0,
+                           (const StepCounters *)(prev_slot >= 0 && !full_pass ? ctx->ring + prev_slot : nullptr));
     }
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
     // product kernels, timed: ev[4i] sparse ev[4i+1] dense ev[4i+2] mfma ev[4i+3]

@@ -453,7 +453,15 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
                                                  uint64_t *__restrict__ dense_items,
                                                  uint64_t *__restrict__ mfma_items, int split,
                                                  uint32_t split_budget, int heavy, uint32_t heavy_base,
-                                                 uint32_t h0) {
+                                                 uint32_t h0, const StepCounters *__restrict__ prevc) {
+    // a delta step after one that added no bit has an empty delta: no work
+    // (the steps the host enqueues past convergence end here)
+    if (prevc) {
+        uint64_t nb = prevc->new_bits;
+#pragma unroll
+        for (int s = 0; s < SHARDS; ++s) nb += prevc->shard[s].new_bits;
+        if (uniform((uint32_t)(nb != 0)) == 0) return;
+    }
     const uint32_t h = h0 + blockIdx.y;
     const uint32_t T = uniform(heads[h].T), target = uniform(heads[h].target);
     const uint32_t t0 = uniform(full_pass ? heads[h].f0 : heads[h].t0);
