@@ -415,7 +415,7 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
         const bool wq = want && ((qmask >> q) & 1u);
         nsub[q] = wq ? 1u : 0u;
         if (wq && split && qcnt[q] > SPLIT_CANDS) {
-            uint32_t ns = (qcnt[q] + SPLIT_ROUND - 1) / SPLIT_ROUND;
+            uint32_t ns = (qcnt[q] + SPLIT_CANDS - 1) / SPLIT_CANDS;
             ns = ns > SPLIT_MAX ? SPLIT_MAX : (ns < 1 ? 1 : ns);
             if (ns > 1 && atomicAdd(&c->split_extra, ns - 1) + (ns - 1) > split_budget) ns = 1;
             nsub[q] = ns;
@@ -644,14 +644,8 @@ __device__ __forceinline__ Row4 bits_from(Row4 m, uint32_t j) {
 // Hub rows: a pair round whose pairs name at most GATHER_HEAVY B rows each
 // is gathered lane by lane; otherwise its rows are dealt out over the wave
 // in chunks of GATHER_CH rows (loads in flight per lane).
-#ifndef CFPQ_GATHER_CH
-#define CFPQ_GATHER_CH 8
-#endif
-#ifndef CFPQ_GATHER_HEAVY
-#define CFPQ_GATHER_HEAVY 32
-#endif
-constexpr int GATHER_CH = CFPQ_GATHER_CH;
-constexpr uint32_t GATHER_HEAVY = CFPQ_GATHER_HEAVY;
+constexpr int GATHER_CH = 8;
+constexpr uint32_t GATHER_HEAVY = 32;
 
 // OR the B rows named by the lowest GATHER_CH set bits of x (tile row k is
 // bit k) into the LDS row accumulator *acc; returns the rows loaded.

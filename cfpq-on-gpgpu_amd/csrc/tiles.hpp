@@ -191,19 +191,16 @@ static inline uint32_t occ_words(uint32_t T) { return (T + 31) / 32; }
 
 // Item encodings: dense item = (head << 40) | tile; sparse unit =
 // (head << 40) | (tile << 2) | quarter.  A heavy sparse unit is split over
-// nsub waves (sub-units sub = 0 .. nsub-1, each taking every nsub-th round of
-// 64 candidates): (sub << 55) | ((nsub - 1) << 59).  Bit 63 set = evaluate
-// the head's full terms (M_L * M_R) instead of its semi-naive terms.
+// nsub waves (sub-units sub = 0 .. nsub-1, sub-unit sub taking the quarter's
+// rows r = sub (mod nsub) of every candidate): (sub << 55) | ((nsub - 1) << 59).
+// Bit 63 set = evaluate the head's full terms (M_L * M_R) instead of its
+// semi-naive terms.
 constexpr uint64_t ITEM_TILE_MASK = 0xffffffffffull;
 constexpr uint64_t ITEM_FULL = 1ull << 63;
 constexpr uint32_t MAX_HEADS = 1u << 15;   // heads per plan (15-bit field)
 constexpr uint32_t SPLIT_MAX = 16;         // sub-units per heavy sparse unit
-constexpr uint32_t SPLIT_CANDS = 64;       // a unit with more candidates than this is split
-#ifndef CFPQ_SPLIT_ROUND
-#define CFPQ_SPLIT_ROUND 64
-#endif
-// candidates per sub-unit when a unit is split (sub-units share the rows)
-constexpr uint32_t SPLIT_ROUND = CFPQ_SPLIT_ROUND;
+constexpr uint32_t SPLIT_CANDS = 64;       // a unit with more candidates than this is split,
+                                           // one sub-unit per SPLIT_CANDS candidates
 __host__ __device__ inline uint32_t item_head(uint64_t it) { return (uint32_t)((it >> 40) & 0x7fffu); }
 __host__ __device__ inline uint32_t unit_sub(uint64_t it) { return (uint32_t)((it >> 55) & 15u); }
 __host__ __device__ inline uint32_t unit_nsub(uint64_t it) { return (uint32_t)((it >> 59) & 15u) + 1u; }
