@@ -109,7 +109,7 @@ EXPORTS = (
     "cfpq_default_opts", "cfpq_create", "cfpq_destroy", "cfpq_last_error", "cfpq_set_opts", "cfpq_sync",
     "cfpq_mat_alloc", "cfpq_mat_free", "cfpq_mat_n", "cfpq_mat_upload", "cfpq_mat_download",
     "cfpq_mat_set_bits", "cfpq_mat_popcount", "cfpq_mat_copy", "cfpq_mat_fingerprint", "cfpq_add_mul", "cfpq_fixpoint",
-    "cfpq_batch_fixpoint", "cfpq_sched_set_weights", "cfpq_sched_get_weights", "cfpq_sched_init",
+    "cfpq_batch_fixpoint", "cfpq_nt_closure", "cfpq_sched_set_weights", "cfpq_sched_get_weights", "cfpq_sched_init",
     "cfpq_sched_eval", "cfpq_sched_train", "cfpq_sched_trace_begin", "cfpq_sched_trace_read",
     "cfpq_sched_trace_end",
 )
@@ -151,6 +151,7 @@ def lib():
         "cfpq_add_mul": (i32, [vp, vp, vp, vp, ctypes.POINTER(i32)]),
         "cfpq_fixpoint": (i32, [vp, ctypes.POINTER(Rule), sz, ctypes.POINTER(vp), sz, ctypes.POINTER(Stats)]),
         "cfpq_batch_fixpoint": (i32, [vp, ctypes.POINTER(Query), sz, ctypes.POINTER(Stats)]),
+        "cfpq_nt_closure": (i32, [vp, ctypes.POINTER(Rule), sz, u32p, u32, ctypes.POINTER(Stats)]),
         "cfpq_sched_set_weights": (i32, [vp, ctypes.POINTER(ctypes.c_uint16), sz]),
         "cfpq_sched_get_weights": (i32, [vp, ctypes.POINTER(ctypes.c_uint16), sz]),
         "cfpq_sched_init": (i32, [vp, u64]),
@@ -261,6 +262,19 @@ class Engine:
             qs[i].nrules = len(rules)
         st = Stats()
         self._check(self._L.cfpq_batch_fixpoint(self._h, qs, len(queries), ctypes.byref(st)))
+        return st.as_dict()
+
+    def nt_closure(self, rules, elems: np.ndarray) -> dict:
+        """Single-matrix formulation: elems is an (n, n) uint32 array of
+        nonterminal sets (bit x = nonterminal x), closed in place under the
+        rules (head, left, right) on the device."""
+        assert elems.dtype == np.uint32 and elems.ndim == 2 and elems.shape[0] == elems.shape[1]
+        assert elems.flags.c_contiguous
+        rr = (Rule * max(1, len(rules)))(*[Rule(*map(int, r)) for r in rules])
+        st = Stats()
+        self._check(self._L.cfpq_nt_closure(self._h, rr, len(rules),
+                                            elems.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                            elems.shape[0], ctypes.byref(st)))
         return st.as_dict()
 
 

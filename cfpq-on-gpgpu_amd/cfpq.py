@@ -114,6 +114,31 @@ class Grammar:
                         todo.append(j)
         return calls
 
+    def nt_elements(self, graph: "Graph") -> np.ndarray:
+        """The single matrix of nonterminal sets for `graph` (the CUTLASS
+        backend's input, cfpq-cpp/Graph.cpp:22-48): bit x of element (i, j) for
+        every terminal edge i -a-> j with x -> a, and the diagonal of epsilon
+        nonterminals (cfpq-cpp/Grammar.h:47-51).  At most 32 nonterminals."""
+        if self.nonterminals_count > 32:
+            raise ValueError(f"{self.nonterminals_count} nonterminals: the single-matrix form holds 32")
+        n = graph.vertices_count
+        el = np.zeros((n, n), dtype=np.uint32)
+        for lab, (src, dst) in graph.edges_by_label().items():
+            for nt in self.terminal_to_nonterminals.get(lab, []):
+                np.bitwise_or.at(el, (src.astype(np.int64), dst.astype(np.int64)), np.uint32(1 << nt))
+        for nt in self.epsilon_nonterminals:
+            idx = np.arange(n)
+            el[idx, idx] |= np.uint32(1 << nt)
+        return el
+
+    def nt_results(self, elems: np.ndarray) -> dict[str, tuple[int, int]]:
+        """Fingerprints (name -> (hash, count)) of a closed single matrix, as
+        print_results gives them for the per-nonterminal matrices."""
+        out = {}
+        for name, idx in sorted(self.nonterminal_to_index.items(), key=lambda kv: kv[0].encode()):
+            out[name] = fingerprint_bits(((elems >> np.uint32(idx)) & np.uint32(1)).astype(bool))
+        return out
+
     def results(self) -> dict[str, tuple[int, int]]:
         """name -> (hash, count), in the reference's name order."""
         return {name: self.matrices[idx].fingerprint() for name, idx in sorted(self.nonterminal_to_index.items(),
@@ -128,6 +153,23 @@ class Grammar:
         for m in self.matrices:
             m.free()
         self.matrices = []
+
+
+HASH_MOD = 1_000_000_007
+
+
+def fingerprint_bits(bits: np.ndarray) -> tuple[int, int]:
+    """(hash, count) of an n x n boolean matrix as cfpq-cpp/Grammar.cpp:81-101
+    prints it: row hash = sum_j bit * 11^(n-1-j), matrix hash = sum_i row_i *
+    13^(n-1-i), both mod 1e9+7 (the Horner loops written as sums)."""
+    n = bits.shape[0]
+    p11 = np.ones(n, dtype=np.int64)
+    p13 = np.ones(n, dtype=np.int64)
+    for j in range(n - 2, -1, -1):
+        p11[j] = p11[j + 1] * 11 % HASH_MOD
+        p13[j] = p13[j + 1] * 13 % HASH_MOD
+    line = (bits.astype(np.int64) @ p11) % HASH_MOD  # < n * 2^30 before the reduction
+    return int(((line * p13) % HASH_MOD).sum() % HASH_MOD), int(bits.sum())
 
 
 class Graph:

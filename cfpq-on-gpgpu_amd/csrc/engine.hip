@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "cfpq_hip.h"
+#include "ntmat.hpp"
 #include "sched.hpp"
 
 using namespace cfpq;
@@ -1503,4 +1504,79 @@ extern "C" int cfpq_sched_trace_end(cfpq_ctx *ctx) {
     ctx->tr_rule = ctx->tr_item = ctx->tr_count = nullptr;
     ctx->tr_cap = 0;
     return CFPQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// single-matrix formulation (one matrix of nonterminal sets)
+// ---------------------------------------------------------------------------
+
+extern "C" int cfpq_nt_closure(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nrules, uint32_t *elems, uint32_t n,
+                               cfpq_stats *stats) {
+    if (!ctx || (n && !elems) || (nrules && !rules)) return fail(ctx, CFPQ_E_INVALID, "cfpq_nt_closure: bad argument");
+    if (nrules > (size_t)NT_MAX_RULES)
+        return fail(ctx, CFPQ_E_INVALID, "cfpq_nt_closure: %zu rules (at most %d)", nrules, NT_MAX_RULES);
+    std::vector<uint32_t> packed(nrules);
+    for (size_t r = 0; r < nrules; ++r) {
+        if (rules[r].head >= 32 || rules[r].left >= 32 || rules[r].right >= 32)
+            return fail(ctx, CFPQ_E_INVALID, "cfpq_nt_closure: rule %zu names a nonterminal >= 32", r);
+        packed[r] = rules[r].head | rules[r].left << 8 | rules[r].right << 16;
+    }
+    if (stats) *stats = cfpq_stats{};
+    if (n == 0) return CFPQ_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t total = (size_t)n * n;
+    uint32_t *buf = nullptr, *d_rules = nullptr;
+    if (hipMalloc((void **)&buf, 3 * total * sizeof(uint32_t)) != hipSuccess)
+        return fail(ctx, CFPQ_E_NOMEM, "cfpq_nt_closure: %zu bytes of device memory", 3 * total * sizeof(uint32_t));
+    int rc = CFPQ_OK;
+    uint32_t *T = buf, *D = buf + total, *N = buf + 2 * total;
+    uint32_t steps = 0;
+    uint64_t new_elems = 0;
+    float ms = 0;
+    auto run = [&]() -> int {
+        HIPCHK(hipMalloc((void **)&d_rules, std::max<size_t>(1, nrules) * sizeof(uint32_t)));
+        if (nrules)
+            HIPCHK(hipMemcpyAsync(d_rules, packed.data(), nrules * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(T, elems, total * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        CHK(ensure_events(ctx, 4 * (size_t)RING + 2));
+        HIPCHK(hipEventRecord(ctx->ev[4 * RING], ctx->stream));
+        const uint32_t nb = (n + NT_BLOCK - 1) / NT_BLOCK;
+        const uint32_t ga = (uint32_t)std::min<size_t>((total + 255) / 256, (size_t)ctx->num_cu * 16);
+        for (;;) {
+            HIPCHK(hipMemsetAsync(ctx->scratch64, 0, 8, ctx->stream));
+            hipLaunchKernelGGL(k_nt_product, dim3(nb, nb), dim3(NT_THREADS), 0, ctx->stream, (const uint32_t *)T,
+                               (const uint32_t *)D, N, n, NtRules{d_rules, (uint32_t)nrules}, steps == 0 ? 1 : 0,
+                               ctx->scratch64);
+            CHK(launch_check(ctx, "nt product"));
+            HIPCHK(hipMemcpyAsync(ctx->scratch64_host, ctx->scratch64, 8, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ++steps;
+            const uint64_t fresh = ctx->scratch64_host[0];
+            if (!fresh) break;
+            new_elems += fresh;
+            hipLaunchKernelGGL(k_nt_apply, dim3(ga), dim3(256), 0, ctx->stream, T, (const uint32_t *)N, total);
+            CHK(launch_check(ctx, "nt apply"));
+            std::swap(D, N);  // this step's new elements are the next step's delta
+            if (ctx->opts.max_iterations && steps >= ctx->opts.max_iterations)
+                return fail(ctx, CFPQ_E_LIMIT, "cfpq_nt_closure: iteration cap %u reached", ctx->opts.max_iterations);
+        }
+        HIPCHK(hipEventRecord(ctx->ev[4 * RING + 1], ctx->stream));
+        HIPCHK(hipMemcpyAsync(elems, T, total * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev[4 * RING], ctx->ev[4 * RING + 1]));
+        return CFPQ_OK;
+    };
+    rc = run();
+    (void)hipStreamSynchronize(ctx->stream);
+    if (d_rules) (void)hipFree(d_rules);
+    (void)hipFree(buf);
+    if (rc == CFPQ_OK && stats) {
+        stats->iterations = steps;
+        stats->items = new_elems;
+        stats->fixpoint_ms = ms;
+        stats->product_launches = steps;
+        stats->host_syncs = steps;
+    }
+    return rc;
 }

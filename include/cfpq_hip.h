@@ -189,6 +189,20 @@ typedef struct cfpq_query {
  * launches per step covers every query still changing. */
 int cfpq_batch_fixpoint(cfpq_ctx *ctx, const cfpq_query *queries, size_t nqueries, cfpq_stats *stats);
 
+/* ---- single-matrix formulation --------------------------------------------
+ * Replaces the CUTLASS backend's single matrix of nonterminal sets
+ * (cuda/cutlass/CutlassMatrix.cu:291-316 fixpoint loop, :353-437 entry
+ * MultMatrSquare; grammar packed by cfpq-cpp/Grammar.cpp:63-75, matrix filled
+ * by cfpq-cpp/Graph.cpp:22-48).  elems: n * n uint32 in row-major order, bit x
+ * of element (i, j) set iff nonterminal x derives a path i -> j (terminal
+ * edges, and the diagonal of epsilon nonterminals, set by the caller); closed
+ * under `rules` on the device and written back in place.  Nonterminals < 32,
+ * at most 256 rules.  stats: iterations (products run, the last adds nothing),
+ * items (elements that gained nonterminals), fixpoint_ms.  Device memory:
+ * 3 * n * n * 4 bytes for the call. */
+int cfpq_nt_closure(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nrules, uint32_t *elems, uint32_t n,
+                    cfpq_stats *stats);
+
 /* ---- learned tile scheduler ----------------------------------------------
  * No counterpart in the reference.  A tile-activity network (16 inputs:
  * 8 tile features + 8-dim rule embedding; 64-64 ReLU hidden; 1 logit),
