@@ -72,13 +72,17 @@ __global__ __launch_bounds__(NT_THREADS) void k_nt_product(const uint32_t *__res
         __syncthreads();
         for (uint32_t kk = 0; kk < NT_K; ++kk) {
             uint32_t ta[4], da[4], tb[4], db[4];
+            uint32_t any_a = 0, any_b = 0;
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
                 ta[a] = s_ta[ty + 16 * a][kk];
                 da[a] = s_da[ty + 16 * a][kk];
                 tb[a] = s_tb[kk][tx + 16 * a];
                 db[a] = s_db[kk][tx + 16 * a];
+                any_a |= ta[a] | da[a];
+                any_b |= tb[a] | db[a];
             }
+            if (!any_a || !any_b) continue;  // sparse matrices: most k give no pair
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
