@@ -148,6 +148,8 @@ def main():
     ap.add_argument("--iters-per-sync", type=int, default=0)
     ap.add_argument("--sparse-batch", type=int, default=0)
     ap.add_argument("--sparse-waves", type=int, default=0)
+    ap.add_argument("--sparse-mode", type=int, default=-1, choices=[-1, 0, 1, 2],
+                    help="sparse product: 0 quarter units, 2 tile units, 1 row-per-lane (default: tuned)")
     ap.add_argument("--tune", action="store_true", help="autotune on this workload first (cached in tuned.json)")
     ap.add_argument("--tune-repeats", type=int, default=3)
     ap.add_argument("--tune-passes", type=int, default=1)
@@ -212,6 +214,8 @@ def main():
         opts["sparse_batch"] = args.sparse_batch
     if args.sparse_waves:
         opts["sparse_waves_per_cu"] = args.sparse_waves
+    if args.sparse_mode >= 0:
+        opts["sparse_mode"] = args.sparse_mode
     log(f"[bench] engine options {opts}")
     eng = pkg.Engine(0, **opts)
     gr = cf.Grammar(q.grammar, text=True)

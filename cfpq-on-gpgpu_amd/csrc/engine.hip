@@ -631,7 +631,9 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     // heavy sparse units are split over several waves (lane-parallel kernel,
     // head update not fused); the units buffer holds 4 entries per head tile
     // plus as many extra sub-units
-    const int split = (ctx->opts.sparse_mode == 0 && !(fuse && ctx->opts.fuse_apply)) ? 1 : 0;
+    const int lane_parallel = ctx->opts.sparse_mode != 1;  // modes 0 (quarter units) and 2 (tile units)
+    const int tile_units = ctx->opts.sparse_mode == 2 ? 1 : 0;
+    const int split = (lane_parallel && !(fuse && ctx->opts.fuse_apply)) ? 1 : 0;
     const uint32_t split_budget = (uint32_t)std::min<size_t>(QUARTERS * plan.total_head_tiles, 0xffff0000u);
     const uint32_t heavy_base = (uint32_t)(QUARTERS * plan.total_head_tiles);
     const bool use_sched = head < 0 && !full_pass && plan.dirty_words && (sched_on(ctx) || ctx->tr_cap || flush);
@@ -642,12 +644,13 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                                (uint32_t)ctx->tr_cap);
         hipLaunchKernelGGL(k_build_sched, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems, sa, split,
-                           split_budget, heavy, heavy_base);
+                           split_budget, heavy, heavy_base, tile_units);
     } else {
         hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, (int)full_pass, c, ctx->units, ctx->items,
                            ctx->mitems, split, split_budget, heavy, heavy_base, h0,
-                           (const StepCounters *)(prev_slot >= 0 && !full_pass ? ctx->ring + prev_slot : nullptr));
+                           (const StepCounters *)(prev_slot >= 0 && !full_pass ? ctx->ring + prev_slot : nullptr),
+                           tile_units);
     }
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
     // product kernels, timed: ev[4i] sparse ev[4i+1] dense ev[4i+2] mfma ev[4i+3]
@@ -659,7 +662,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const uint64_t *un = ctx->units;
     const int fz = (fuse && ctx->opts.fuse_apply) ? 1 : 0;
     fuse = fz != 0;
-    if (ctx->opts.sparse_mode == 0) {
+    if (lane_parallel) {
         const TermPtr *tp = ctx->d_tptr + (size_t)q * plan_terms(plan);
         hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un, heavy_base,
                            TouchedLists{touched, seg_cap}, chunk, fz);

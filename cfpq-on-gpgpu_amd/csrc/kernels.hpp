@@ -404,9 +404,19 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
 // first: the longest units start early instead of ending the launch late.
 __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const uint32_t *qcnt, uint64_t base,
                                              int split, uint32_t split_budget, StepCounters *__restrict__ c,
-                                             uint64_t *__restrict__ units, uint32_t heavy_base) {
+                                             uint64_t *__restrict__ units, uint32_t heavy_base, int tile_units) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
+    // tile units: one unit for every quarter of the tile with candidates,
+    // unless a quarter is heavy enough to be split
+    if (tile_units) {
+        bool heavyq = false;
+#pragma unroll
+        for (uint32_t q = 0; q < QUARTERS; ++q) heavyq |= split && ((qmask >> q) & 1u) && qcnt[q] > SPLIT_CANDS;
+        const bool tu = want && qmask && !heavyq;
+        wave_append(tu, base | ITEM_QSET | ((uint64_t)qmask << ITEM_QSET_SHIFT), &c->n_units, units);
+        want = want && !tu;
+    }
     uint32_t nsub[QUARTERS];
     uint64_t single[QUARTERS];
     uint32_t total = 0;
@@ -453,7 +463,8 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
                                                  uint64_t *__restrict__ dense_items,
                                                  uint64_t *__restrict__ mfma_items, int split,
                                                  uint32_t split_budget, int heavy, uint32_t heavy_base,
-                                                 uint32_t h0, const StepCounters *__restrict__ prevc) {
+                                                 uint32_t h0, const StepCounters *__restrict__ prevc,
+                                                 int tile_units) {
     // a delta step after one that added no bit has an empty delta: no work
     // (the steps the host enqueues past convergence end here)
     if (prevc) {
@@ -481,7 +492,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
     append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units,
-                 heavy_base);
+                 heavy_base, tile_units);
 }
 
 // ---------------------------------------------------------------------------
@@ -719,9 +730,11 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                                             const uint64_t *__restrict__ units,
                                                             uint32_t heavy_base, TouchedLists tl, uint32_t chunk,
                                                             int fuse) {
-    __shared__ Row4 acc_s[WAVE];
+    __shared__ Row4 acc_s[TILE];  // rows of the unit's quarters (row 64 q + r)
     __shared__ Row4 s_brm[WAVE];
     __shared__ uint64_t s_arm[WAVE];
+    __shared__ uint32_t s_hq[WAVE];           // quarter of each staged hit
+    __shared__ uint32_t s_kq[WAVE][QUARTERS]; // candidate masks per (term, word) lane and quarter
     __shared__ const uint64_t *s_ab[WAVE];
     __shared__ const uint64_t *s_bb[WAVE];
     __shared__ uint32_t s_end[WAVE], s_kb[WAVE], s_cend[WAVE];
@@ -773,8 +786,9 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         const uint64_t tu0 = __builtin_amdgcn_s_memtime();
         const uint64_t it = ld_const(units + (u < n_heavy ? heavy_base + u : u - n_heavy));
         const uint32_t h = uniform(item_head(it));
-        const uint32_t p = uniform((uint32_t)((it & ITEM_TILE_MASK) >> 2));
-        const uint32_t q = uniform((uint32_t)(it & 3u));
+        const uint32_t p = uniform(unit_tile(it));
+        // the unit's quarters: one, or every quarter of the tile with candidates
+        const uint32_t qset = uniform(unit_quarters(it));
         const bool full = (it & ITEM_FULL) != 0;
         const uint32_t sub = uniform(unit_sub(it)), nsub = uniform(unit_nsub(it));
         const uint32_t T = ld_const(&heads[h].T);
@@ -793,7 +807,9 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             rowsel = 0;
             for (uint32_t r = sub; r < WAVE; r += nsub) rowsel |= 1ull << r;
         }
-        acc_s[lane] = Row4{{0, 0, 0, 0}};
+#pragma unroll
+        for (uint32_t qq = 0; qq < QUARTERS; ++qq)
+            if ((qset >> qq) & 1u) acc_s[qq * WAVE + lane] = Row4{{0, 0, 0, 0}};
         __syncthreads();
         const uint32_t ow = (T + 31) / 32;
         const uint32_t ntw = (t1 - t0) * ow;  // (term, occupancy word) pairs of the unit
@@ -801,17 +817,23 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             // 1. candidate masks of 64 (term, word) pairs at once, compacted by
             //    a prefix sum of their popcounts
             const uint32_t j = tw0 + lane;
-            uint32_t kbw = 0;
+            uint32_t kq[QUARTERS] = {0, 0, 0, 0};
             LaneOps ops = {};
             if (j < ntw) {
                 const uint32_t t = t0 + j / ow, w = j % ow;
                 const TermPtr e = tptr[t];
-                kbw = gld(&e.occQ[(q * T + I) * ow + w]) & gld(&e.occC[J * ow + w]);
+                const uint32_t col = gld(&e.occC[J * ow + w]);
+#pragma unroll
+                for (uint32_t qq = 0; qq < QUARTERS; ++qq)
+                    if ((qset >> qq) & 1u) kq[qq] = gld(&e.occQ[(qq * T + I) * ow + w]) & col;
                 ops = LaneOps{e.lrmask, e.lcolq, e.lbits, e.rrmask, e.rbits, e.rcnt};
             }
+            const uint32_t kbw = (uint32_t)(__popc(kq[0]) + __popc(kq[1]) + __popc(kq[2]) + __popc(kq[3]));
             uint32_t total;
-            const uint32_t incw = scan_ballot((uint32_t)__popc(kbw), __ballot(kbw != 0), total);
+            const uint32_t incw = scan_ballot(kbw, __ballot(kbw != 0), total);
             if (!total) continue;
+#pragma unroll
+            for (uint32_t qq = 0; qq < QUARTERS; ++qq) s_kq[lane][qq] = kq[qq];
             s_kb[lane] = kbw;
             s_cend[lane] = incw;
             s_ops[lane] = ops;
@@ -824,6 +846,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                 uint64_t arm = 0;
                 Row4 brm = {{0, 0, 0, 0}};
                 const uint64_t *abase = nullptr, *bbase = nullptr;
+                uint32_t cq = 0;  // the candidate's quarter
                 if ((uint32_t)lane < nc) {
                     uint32_t lo = 0, hi = WAVE - 1;
                     while (lo < hi) {
@@ -831,8 +854,15 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                         if (s_cend[mid] > g) hi = mid;
                         else lo = mid + 1;
                     }
-                    const uint32_t kb = s_kb[lo];
-                    const int bit = select_bit(kb, g - (s_cend[lo] - (uint32_t)__popc(kb)));
+                    // the owner's candidates, quarter by quarter: find this one's quarter and bit
+                    uint32_t idx = g - (s_cend[lo] - s_kb[lo]);
+                    uint32_t q = 0, kb = s_kq[lo][0];
+                    while ((uint32_t)__popc(kb) <= idx) {
+                        idx -= (uint32_t)__popc(kb);
+                        kb = s_kq[lo][++q];
+                    }
+                    cq = q;
+                    const int bit = select_bit(kb, idx);
                     const uint32_t jj = tw0 + lo;
                     const uint32_t K = (jj % ow) * 32 + (uint32_t)bit;
                     const LaneOps o = s_ops[lo];
@@ -843,7 +873,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     const bool hit = ((cu.w[0] & brm.w[0]) | (cu.w[1] & brm.w[1]) | (cu.w[2] & brm.w[2]) |
                                       (cu.w[3] & brm.w[3])) != 0;
                     if (!hit) arm = 0;
-                    abase = o.lbits + lt * TILE_WORDS + (size_t)q * WAVE * TROW_WORDS;
+                    abase = o.lbits + lt * TILE_WORDS + (size_t)cq * WAVE * TROW_WORDS;
                     bbase = o.rbits + rt * TILE_WORDS;
                     // a full B tile needs no gathers: bit 0 of the (8-byte aligned) pointer marks it
                     if (gld(&o.rcnt[rt]) == TILE_BITS) bbase = (const uint64_t *)((uintptr_t)bbase | 1u);
@@ -861,6 +891,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     const uint32_t slot = __popcll(hmask & ((1ull << lane) - 1ull));
                     s_arm[slot] = arm;
                     s_end[slot] = incl;
+                    s_hq[slot] = cq;
                     s_brm[slot] = brm;
                     s_ab[slot] = abase;
                     s_bb[slot] = bbase;
@@ -872,7 +903,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     // 4a. lane gp loads the A row of pair gp; x = the B rows it names
                     const uint32_t gp = base + lane;
                     Row4 x = {{0, 0, 0, 0}};
-                    uint32_t r = 0;
+                    uint32_t r = 0;  // accumulator row: 64 * quarter + row in the quarter
                     const uint64_t *Bt = nullptr;
                     if (gp < pairs) {
                         uint32_t lo = 0, hi = nh - 1;  // owner: first slot whose range ends past gp
@@ -882,14 +913,15 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                             else lo = mid + 1;
                         }
                         const uint64_t am = s_arm[lo];
-                        r = (uint32_t)select_bit(am, gp - (s_end[lo] - (uint32_t)__popcll(am)));
+                        const uint32_t rq = (uint32_t)select_bit(am, gp - (s_end[lo] - (uint32_t)__popcll(am)));
+                        r = s_hq[lo] * WAVE + rq;
                         Bt = s_bb[lo];
                         if ((uintptr_t)Bt & 1u) {  // full B tile: the row becomes all ones
 #pragma unroll
                             for (int i = 0; i < 4; ++i) atomicOr((unsigned long long *)&acc_s[r].w[i], ~0ull);
                         } else {
                             const Row4 bm = s_brm[lo];
-                            const Row4 a = load_row(s_ab[lo] + r * TROW_WORDS);
+                            const Row4 a = load_row(s_ab[lo] + rq * TROW_WORDS);
 #pragma unroll
                             for (int i = 0; i < 4; ++i) x.w[i] = a.w[i] & bm.w[i];
                         }
@@ -961,18 +993,23 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             __syncthreads();  // the candidate staging is refilled by the next (term, word) chunk
         }
         __syncthreads();
-        const Row4 acc = acc_s[lane];
         max_cands = max(max_cands, tp - tp_unit0);
-        const QuarterOut qo = nsub > 1 ? quarter_epilogue_rows(O, M, p, q, I, J, acc, out_rows)
-                                       : quarter_epilogue(O, M, p, q, I, J, acc, fuse, out_rows);
-        if (qo.pc && lane == 0) {
-            const uint32_t old = g_add(&O.cnt[p], qo.pc);
+        uint32_t pc = 0;  // new bits of the unit's quarters
+#pragma unroll
+        for (uint32_t qq = 0; qq < QUARTERS; ++qq) {
+            if (!((qset >> qq) & 1u)) continue;
+            const Row4 acc = acc_s[qq * WAVE + lane];
+            pc += nsub > 1 ? quarter_epilogue_rows(O, M, p, qq, I, J, acc, out_rows).pc
+                           : quarter_epilogue(O, M, p, qq, I, J, acc, fuse, out_rows).pc;
+        }
+        if (pc && lane == 0) {
+            const uint32_t old = g_add(&O.cnt[p], pc);
             if (old == 0) {
                 set_occ(O, I, J);
                 const uint32_t at = atomicAdd(&shards[home].n_touched, 1u);
                 gst(&tl.list[(size_t)(1 + home) * tl.seg_cap + at], ((uint64_t)h << 40) | p);
             }
-            new_bits += qo.pc;
+            new_bits += pc;
         }
         const uint64_t tu = __builtin_amdgcn_s_memtime() - tu0;
         unit_max = tu > unit_max ? tu : unit_max;

@@ -196,6 +196,11 @@ static inline uint32_t occ_words(uint32_t T) { return (T + 31) / 32; }
 // Bit 63 set = evaluate the head's full terms (M_L * M_R) instead of its
 // semi-naive terms.
 constexpr uint64_t ITEM_TILE_MASK = 0xffffffffffull;
+// A tile unit (lane-parallel sparse kernel, sparse_mode 2) covers every quarter
+// of its tile with candidates: bit ITEM_QSET set, the quarter set in bits
+// 34..37, the tile in bits 2..33 (tiles below 2^32).
+constexpr uint64_t ITEM_QSET = 1ull << 38;
+constexpr int ITEM_QSET_SHIFT = 34;
 constexpr uint64_t ITEM_FULL = 1ull << 63;
 constexpr uint32_t MAX_HEADS = 1u << 15;   // heads per plan (15-bit field)
 constexpr uint32_t SPLIT_MAX = 16;         // sub-units per heavy sparse unit
@@ -203,6 +208,10 @@ constexpr uint32_t SPLIT_CANDS = 64;       // a unit with more candidates than t
                                            // one sub-unit per SPLIT_CANDS candidates
 __host__ __device__ inline uint32_t item_head(uint64_t it) { return (uint32_t)((it >> 40) & 0x7fffu); }
 __host__ __device__ inline uint32_t unit_sub(uint64_t it) { return (uint32_t)((it >> 55) & 15u); }
+__host__ __device__ inline uint32_t unit_tile(uint64_t it) { return (uint32_t)((it >> 2) & 0xffffffffu); }
+__host__ __device__ inline uint32_t unit_quarters(uint64_t it) {
+    return (it & ITEM_QSET) ? (uint32_t)((it >> ITEM_QSET_SHIFT) & 15u) : 1u << (it & 3u);
+}
 __host__ __device__ inline uint32_t unit_nsub(uint64_t it) { return (uint32_t)((it >> 59) & 15u) + 1u; }
 
 }  // namespace cfpq

@@ -5,7 +5,8 @@ hipEvents, median of a few runs) while walking the parameters one at a time
 (coordinate descent, best value kept before moving to the next parameter):
 
   schedule             semi-naive delta steps over all rules (0), head sweeps (1) or rule sweeps (2) over full matrices
-  sparse_mode          sparse product: lane-parallel candidates (0) or row-per-lane (1)
+  sparse_mode          sparse product: lane-parallel candidates over quarter units (0) or tile
+                       units (2), or row-per-lane (1)
   sparse_batch         k-tiles whose loads the row-per-lane sparse product issues together
   sparse_waves_per_cu  persistent waves of the sparse product per CU
   sparse_chunk         work units a sparse-product wave claims per atomic
@@ -33,7 +34,7 @@ CACHE = os.path.join(HERE, "tuned.json")
 
 SPACE = [
     ("schedule", [0, 1, 2]),
-    ("sparse_mode", [0, 1]),
+    ("sparse_mode", [0, 1, 2]),
     ("sparse_batch", [1, 2, 4]),
     ("sparse_waves_per_cu", [2, 4, 8, 12, 16, 32]),
     ("sparse_chunk", [1, 2, 4, 8, 16]),

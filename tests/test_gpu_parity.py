@@ -257,6 +257,8 @@ def test_fixpoint_sg_forest_65536_golden(pkg, gen):
     dict(dense_bits=0xFFFFFFF0, mfma_bits=0xFFFFFFF0, sparse_waves_per_cu=16, sparse_chunk=1),
     # 1 wave per CU: most shards are drained by stealing waves
     dict(sparse_waves_per_cu=1, sparse_chunk=16, use_graph=True),
+    # tile units: one unit per tile for its quarters with candidates (heavy quarters still split)
+    dict(dense_bits=0xFFFFFFF0, mfma_bits=0xFFFFFFF0, sparse_mode=2, sparse_waves_per_cu=8, sparse_chunk=4),
 ])
 def test_fixpoint_sg_forest_65536_options(pkg, gen, opts):
     """Full-size ontology workload under other launch shapes: fingerprints
@@ -410,3 +412,20 @@ def test_batch_head_sweeps(pkg, gen, oracle, engine, schedule):
         for gr, w in zip(grs, want):
             assert gr.results() == w
             gr.free()
+
+
+@pytest.mark.parametrize("which", ["dense", "dyck", "sg", "self"])
+def test_fixpoint_tile_units(pkg, gen, oracle, which):
+    """sparse_mode 2: a work unit covers every quarter of its tile with
+    candidates (rows 64 q + r of one accumulator); same least fixpoint with
+    the other kernels, fused or separate apply, and with delta or rule-sweep
+    schedules."""
+    q = {"dense": lambda: gen.dense_stress(1500), "dyck": lambda: gen.dyck_er(1200, 4000, seed=9),
+         "sg": lambda: gen.sg_forest(8000, seed=10, types_per_vertex=4),
+         "self": lambda: gen.dyck_er(300, 800, seed=21)}[which]()
+    if which == "self":
+        q.grammar = "S\nS A S1\nS1 S B\nS S S\nA a\nB b\n"
+    for kernel, fuse, schedule in (("auto", False, "delta"), ("auto", True, "delta"), ("gather", False, "rules")):
+        st = _fixpoint_vs_oracle(pkg, gen, oracle, q, kernel=kernel, fuse_apply=fuse, schedule=schedule, sparse_mode=2,
+                                 dense_bits=0xFFFFFFF0 if kernel == "auto" else 0, mfma_bits=0xFFFFFFF0)
+        assert st["iterations"] >= 2
