@@ -664,8 +664,12 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     fuse = fz != 0;
     if (lane_parallel) {
         const TermPtr *tp = ctx->d_tptr + (size_t)q * plan_terms(plan);
-        hipLaunchKernelGGL(k_product_sparse_lp, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un, heavy_base,
-                           TouchedLists{touched, seg_cap}, chunk, fz);
+        if (tile_units)
+            hipLaunchKernelGGL(k_product_sparse_lp<true>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
+                               heavy_base, TouchedLists{touched, seg_cap}, chunk, fz);
+        else
+            hipLaunchKernelGGL(k_product_sparse_lp<false>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
+                               heavy_base, TouchedLists{touched, seg_cap}, chunk, fz);
     } else if (ctx->opts.sparse_batch == 1) {
         hipLaunchKernelGGL(k_product_sparse<1>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
                            fz);

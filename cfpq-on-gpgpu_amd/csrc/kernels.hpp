@@ -723,6 +723,10 @@ struct LaneOps {
 //   epilogue  lane r owns row r of the quarter, as in the batched kernel.
 // 64 candidates are filtered per load round and 64 A rows are in flight per
 // wave, however the rows are spread over the candidates.
+// QSET: the units may cover several quarters of a tile (tile units, a
+// 256-row accumulator); otherwise every unit is one quarter and the
+// accumulator has 64 rows (a smaller LDS footprint, more waves per CU).
+template <bool QSET>
 __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__restrict__ mats,
                                                             const HeadDev *__restrict__ heads,
                                                             const TermPtr *__restrict__ tptr, int parity,
@@ -730,11 +734,14 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                                             const uint64_t *__restrict__ units,
                                                             uint32_t heavy_base, TouchedLists tl, uint32_t chunk,
                                                             int fuse) {
-    __shared__ Row4 acc_s[TILE];  // rows of the unit's quarters (row 64 q + r)
+    constexpr uint32_t ACC_ROWS = QSET ? TILE : WAVE;
+    __shared__ Row4 acc_s[ACC_ROWS];  // rows of the unit's quarters (row 64 q + r with QSET, else r)
     __shared__ Row4 s_brm[WAVE];
     __shared__ uint64_t s_arm[WAVE];
-    __shared__ uint32_t s_hq[WAVE];           // quarter of each staged hit
-    __shared__ uint32_t s_kq[WAVE][QUARTERS]; // candidate masks per (term, word) lane and quarter
+    // with QSET: the quarter of each staged hit, and the candidate masks per
+    // (term, word) lane and quarter (without, s_kb holds the one mask)
+    __shared__ uint32_t s_hq[QSET ? WAVE : 1];
+    __shared__ uint32_t s_kq[QSET ? WAVE : 1][QUARTERS];
     __shared__ const uint64_t *s_ab[WAVE];
     __shared__ const uint64_t *s_bb[WAVE];
     __shared__ uint32_t s_end[WAVE], s_kb[WAVE], s_cend[WAVE];
@@ -809,7 +816,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         }
 #pragma unroll
         for (uint32_t qq = 0; qq < QUARTERS; ++qq)
-            if ((qset >> qq) & 1u) acc_s[qq * WAVE + lane] = Row4{{0, 0, 0, 0}};
+            if ((qset >> qq) & 1u) acc_s[(QSET ? qq * WAVE : 0u) + lane] = Row4{{0, 0, 0, 0}};
         __syncthreads();
         const uint32_t ow = (T + 31) / 32;
         const uint32_t ntw = (t1 - t0) * ow;  // (term, occupancy word) pairs of the unit
@@ -833,8 +840,9 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             const uint32_t incw = scan_ballot(kbw, __ballot(kbw != 0), total);
             if (!total) continue;
 #pragma unroll
-            for (uint32_t qq = 0; qq < QUARTERS; ++qq) s_kq[lane][qq] = kq[qq];
-            s_kb[lane] = kbw;
+            for (uint32_t qq = 0; qq < QUARTERS; ++qq)
+                if (QSET) s_kq[lane][qq] = kq[qq];
+            s_kb[lane] = QSET ? kbw : (kq[0] | kq[1] | kq[2] | kq[3]);
             s_cend[lane] = incw;
             s_ops[lane] = ops;
             __syncthreads();
@@ -855,11 +863,19 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                         else lo = mid + 1;
                     }
                     // the owner's candidates, quarter by quarter: find this one's quarter and bit
-                    uint32_t idx = g - (s_cend[lo] - s_kb[lo]);
-                    uint32_t q = 0, kb = s_kq[lo][0];
-                    while ((uint32_t)__popc(kb) <= idx) {
-                        idx -= (uint32_t)__popc(kb);
-                        kb = s_kq[lo][++q];
+                    uint32_t q, kb, idx;
+                    if (QSET) {
+                        idx = g - (s_cend[lo] - s_kb[lo]);
+                        q = 0;
+                        kb = s_kq[lo][0];
+                        while ((uint32_t)__popc(kb) <= idx) {
+                            idx -= (uint32_t)__popc(kb);
+                            kb = s_kq[lo][++q];
+                        }
+                    } else {
+                        q = (uint32_t)__builtin_ctz(qset);
+                        kb = s_kb[lo];
+                        idx = g - (s_cend[lo] - (uint32_t)__popc(kb));
                     }
                     cq = q;
                     const int bit = select_bit(kb, idx);
@@ -891,7 +907,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     const uint32_t slot = __popcll(hmask & ((1ull << lane) - 1ull));
                     s_arm[slot] = arm;
                     s_end[slot] = incl;
-                    s_hq[slot] = cq;
+                    if (QSET) s_hq[slot] = cq;
                     s_brm[slot] = brm;
                     s_ab[slot] = abase;
                     s_bb[slot] = bbase;
@@ -914,7 +930,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                         }
                         const uint64_t am = s_arm[lo];
                         const uint32_t rq = (uint32_t)select_bit(am, gp - (s_end[lo] - (uint32_t)__popcll(am)));
-                        r = s_hq[lo] * WAVE + rq;
+                        r = (QSET ? s_hq[lo] * WAVE : 0u) + rq;
                         Bt = s_bb[lo];
                         if ((uintptr_t)Bt & 1u) {  // full B tile: the row becomes all ones
 #pragma unroll
@@ -998,7 +1014,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
 #pragma unroll
         for (uint32_t qq = 0; qq < QUARTERS; ++qq) {
             if (!((qset >> qq) & 1u)) continue;
-            const Row4 acc = acc_s[qq * WAVE + lane];
+            const Row4 acc = acc_s[(QSET ? qq * WAVE : 0u) + lane];
             pc += nsub > 1 ? quarter_epilogue_rows(O, M, p, qq, I, J, acc, out_rows).pc
                            : quarter_epilogue(O, M, p, qq, I, J, acc, fuse, out_rows).pc;
         }
