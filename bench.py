@@ -406,6 +406,10 @@ def bench_batch(args, pkg, gen, cf):
     opts["kernel"] = args.kernel
     if args.iters_per_sync:
         opts["iters_per_sync"] = args.iters_per_sync
+    if args.sparse_mode >= 0:
+        opts["sparse_mode"] = args.sparse_mode
+    if args.sparse_waves:
+        opts["sparse_waves_per_cu"] = args.sparse_waves
     if args.graph:
         opts["use_graph"] = True
     log(f"[bench] batch of {len(qs)} queries, N in [{min(q.n for q in qs)}, {max(q.n for q in qs)}], "
