@@ -429,3 +429,78 @@ def test_fixpoint_tile_units(pkg, gen, oracle, which):
         st = _fixpoint_vs_oracle(pkg, gen, oracle, q, kernel=kernel, fuse_apply=fuse, schedule=schedule, sparse_mode=2,
                                  dense_bits=0xFFFFFFF0 if kernel == "auto" else 0, mfma_bits=0xFFFFFFF0)
         assert st["iterations"] >= 2
+
+
+def _chain_query(gen, labels, grammar, label_names):
+    """A path 0 -> 1 -> ... -> len(labels) with the given edge labels."""
+    import numpy as np
+    n = len(labels) + 1
+    src = np.arange(n - 1, dtype=np.uint32)
+    dst = np.arange(1, n, dtype=np.uint32)
+    lab = np.asarray(labels, dtype=np.uint8)
+    return gen.Query("chain", n, grammar, src, dst, lab, label_names)
+
+
+@pytest.mark.parametrize("schedule", ["delta", "rules"])
+def test_large_chain_closed_forms(pkg, gen, schedule):
+    """Sizes the oracle cannot run, checked against closed forms:
+    * transitive closure of a 32,768-vertex path under S -> S S | a is the
+      strict upper triangle (every pair i < j), 15 steps of doubling;
+    * the Dyck grammar on the chain a^k b^k (k = 8,192) derives exactly the k
+      nested pairs (k - m, k + m), m = 1..k."""
+    import numpy as np
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    with pkg.Engine(0, schedule=schedule) as eng:
+        n = 32768
+        q = _chain_query(gen, [0] * (n - 1), "S S S\nS a\n", ("a",))
+        gr = cf.Grammar(q.grammar, text=True)
+        gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+        S = gr.matrices[gr.nonterminal_to_index["S"]]
+        assert S.popcount() == n * (n - 1) // 2
+        rows = S.download()
+        # row i holds columns i+1 .. n-1: zero words, one partial word, then all ones
+        first = np.arange(1, n + 1, dtype=np.int64)  # first set column of row i
+        w = np.arange(rows.shape[1], dtype=np.int64)[None, :]
+        fw = (first // 64)[:, None]
+        partial = (~((np.uint64(1) << (first % 64).astype(np.uint64)) - np.uint64(1)))[:, None]
+        want = np.where(w < fw, np.uint64(0), np.where(w == fw, partial, np.uint64(0xFFFFFFFFFFFFFFFF)))
+        assert np.array_equal(rows, want.astype(np.uint64))
+        gr.free()
+        k = 8192
+        q = _chain_query(gen, [0] * k + [1] * k, gen.DYCK_GRAMMAR, gen.DYCK_LABELS)
+        gr = cf.Grammar(q.grammar, text=True)
+        gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+        S = gr.matrices[gr.nonterminal_to_index["S"]].download()
+        idx = np.nonzero(np.unpackbits(S.view(np.uint8), axis=1, bitorder="little")[:, :2 * k + 1])
+        assert sorted(zip(idx[0].tolist(), idx[1].tolist())) == [(k - m, k + m) for m in range(k, 0, -1)]
+        gr.free()
+
+
+def test_empty_inputs(pkg, gen, engine):
+    """No edges, no binary rules, a single vertex: the fixpoint is the initial
+    matrices (epsilon diagonals included)."""
+    import numpy as np
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    empty = gen.Query("empty", 300, gen.DYCK_GRAMMAR, np.zeros(0, np.uint32), np.zeros(0, np.uint32),
+                      np.zeros(0, np.uint8), gen.DYCK_LABELS)
+    gr = cf.Grammar(empty.grammar, text=True)
+    _, _, st = gr.intersection_with_graph(cf.Graph.from_query(empty), engine)
+    assert all(c == 0 for _, c in gr.results().values())
+    gr.free()
+    q = gen.dyck_er(200, 600, seed=31)
+    for grammar, want_counts in (("A a\nB b\n", None), ("S\nA a\n", None)):
+        gr = cf.Grammar(grammar, text=True)
+        graph = cf.Graph.from_query(q)
+        gr.intersection_with_graph(graph, engine)
+        res = gr.results()
+        if "S" in res:
+            assert res["S"][1] == q.n  # epsilon: the diagonal only
+        if "A" in res:
+            assert res["A"][1] == int((np.asarray(q.labels)[q.label] == "a").sum())
+        gr.free()
+    one = gen.Query("one", 1, "S S S\nS a\n", np.zeros(1, np.uint32), np.zeros(1, np.uint32), np.zeros(1, np.uint8),
+                    ("a",))
+    gr = cf.Grammar(one.grammar, text=True)
+    gr.intersection_with_graph(cf.Graph.from_query(one), engine)
+    assert gr.results()["S"][1] == 1
+    gr.free()
