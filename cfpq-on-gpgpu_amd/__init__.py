@@ -104,12 +104,22 @@ class Query(ctypes.Structure):
     ]
 
 
+class Pairs(ctypes.Structure):
+    _fields_ = [
+        ("keys", ctypes.POINTER(ctypes.c_uint64)),
+        ("count", ctypes.c_uint64),
+        ("cap", ctypes.c_uint64),
+    ]
+
+
+E_CAPACITY = -5
+
 #: every symbol include/cfpq_hip.h declares
 EXPORTS = (
     "cfpq_default_opts", "cfpq_create", "cfpq_destroy", "cfpq_last_error", "cfpq_set_opts", "cfpq_sync",
     "cfpq_mat_alloc", "cfpq_mat_free", "cfpq_mat_n", "cfpq_mat_upload", "cfpq_mat_download",
     "cfpq_mat_set_bits", "cfpq_mat_popcount", "cfpq_mat_copy", "cfpq_mat_fingerprint", "cfpq_add_mul", "cfpq_fixpoint",
-    "cfpq_batch_fixpoint", "cfpq_nt_closure", "cfpq_sched_set_weights", "cfpq_sched_get_weights", "cfpq_sched_init",
+    "cfpq_batch_fixpoint", "cfpq_nt_closure", "cfpq_pairs_fixpoint", "cfpq_sched_set_weights", "cfpq_sched_get_weights", "cfpq_sched_init",
     "cfpq_sched_eval", "cfpq_sched_train", "cfpq_sched_trace_begin", "cfpq_sched_trace_read",
     "cfpq_sched_trace_end",
 )
@@ -152,6 +162,8 @@ def lib():
         "cfpq_fixpoint": (i32, [vp, ctypes.POINTER(Rule), sz, ctypes.POINTER(vp), sz, ctypes.POINTER(Stats)]),
         "cfpq_batch_fixpoint": (i32, [vp, ctypes.POINTER(Query), sz, ctypes.POINTER(Stats)]),
         "cfpq_nt_closure": (i32, [vp, ctypes.POINTER(Rule), sz, u32p, u32, ctypes.POINTER(Stats)]),
+        "cfpq_pairs_fixpoint": (i32, [vp, ctypes.POINTER(Rule), sz, ctypes.POINTER(Pairs), u32, u32,
+                                      ctypes.POINTER(Stats)]),
         "cfpq_sched_set_weights": (i32, [vp, ctypes.POINTER(ctypes.c_uint16), sz]),
         "cfpq_sched_get_weights": (i32, [vp, ctypes.POINTER(ctypes.c_uint16), sz]),
         "cfpq_sched_init": (i32, [vp, u64]),
@@ -276,6 +288,32 @@ class Engine:
                                             elems.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                                             elems.shape[0], ctypes.byref(st)))
         return st.as_dict()
+
+    def pairs_fixpoint(self, rules, sets, n: int, cap=None) -> tuple[list, dict]:
+        """Sparse pair-list formulation: sets[x] is a uint64 array of pairs
+        row << 32 | col of nonterminal x (any order, duplicates allowed).
+        Returns (closed sets, each sorted without duplicates, stats).  `cap`
+        bounds the first attempt's output arrays; when a relation outgrows it
+        the call is repeated once with the sizes the library reports."""
+        ins = [np.ascontiguousarray(k, dtype=np.uint64) for k in sets]
+        rr = (Rule * max(1, len(rules)))(*[Rule(*map(int, r)) for r in rules])
+        caps = [max(len(k), cap if cap is not None else 4 * len(k) + 4096) for k in ins]
+        for _ in range(2):
+            bufs = [np.empty(c, dtype=np.uint64) for c in caps]
+            ps = (Pairs * max(1, len(ins)))()
+            for x, (k, b) in enumerate(zip(ins, bufs)):
+                b[: len(k)] = k
+                ps[x].keys = b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+                ps[x].count = len(k)
+                ps[x].cap = len(b)
+            st = Stats()
+            rc = self._L.cfpq_pairs_fixpoint(self._h, rr, len(rules), ps, len(ins), int(n), ctypes.byref(st))
+            if rc == E_CAPACITY:
+                caps = [max(c, int(ps[x].count)) for x, c in enumerate(caps)]
+                continue
+            self._check(rc)
+            return [b[: int(ps[x].count)].copy() for x, b in enumerate(bufs)], st.as_dict()
+        self._check(rc)
 
 
     # ---- learned tile scheduler ------------------------------------------

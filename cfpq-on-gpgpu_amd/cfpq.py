@@ -139,6 +139,26 @@ class Grammar:
             out[name] = fingerprint_bits(((elems >> np.uint32(idx)) & np.uint32(1)).astype(bool))
         return out
 
+    def pair_keys(self, graph: "Graph") -> list[np.ndarray]:
+        """Per nonterminal, the initial pairs row << 32 | col for the pair-list
+        formulation: terminal edges and the epsilon diagonal, as load() sets
+        them (cfpq-cpp/Grammar.h:41-51); duplicates are left in."""
+        n = graph.vertices_count
+        keys = [[] for _ in range(self.nonterminals_count)]
+        for lab, (src, dst) in graph.edges_by_label().items():
+            for nt in self.terminal_to_nonterminals.get(lab, []):
+                keys[nt].append(src.astype(np.uint64) << np.uint64(32) | dst.astype(np.uint64))
+        for nt in self.epsilon_nonterminals:
+            d = np.arange(n, dtype=np.uint64)
+            keys[nt].append(d << np.uint64(32) | d)
+        return [np.concatenate(k) if k else np.zeros(0, np.uint64) for k in keys]
+
+    def pair_results(self, sets: list[np.ndarray], n: int) -> dict[str, tuple[int, int]]:
+        """Fingerprints (name -> (hash, count)) of closed pair sets, as
+        print_results gives them for the matrices."""
+        return {name: fingerprint_pairs(sets[idx], n)
+                for name, idx in sorted(self.nonterminal_to_index.items(), key=lambda kv: kv[0].encode())}
+
     def results(self) -> dict[str, tuple[int, int]]:
         """name -> (hash, count), in the reference's name order."""
         return {name: self.matrices[idx].fingerprint() for name, idx in sorted(self.nonterminal_to_index.items(),
@@ -170,6 +190,33 @@ def fingerprint_bits(bits: np.ndarray) -> tuple[int, int]:
         p13[j] = p13[j + 1] * 13 % HASH_MOD
     line = (bits.astype(np.int64) @ p11) % HASH_MOD  # < n * 2^30 before the reduction
     return int(((line * p13) % HASH_MOD).sum() % HASH_MOD), int(bits.sum())
+
+
+def _pow_mod(base: int, e: np.ndarray) -> np.ndarray:
+    """base ** e mod HASH_MOD elementwise (square and multiply; every product
+    is below 2^60)."""
+    out = np.ones(e.shape, dtype=np.int64)
+    b = np.full(e.shape, base % HASH_MOD, dtype=np.int64)
+    e = e.astype(np.int64).copy()
+    while e.any():
+        odd = (e & 1).astype(bool)
+        out[odd] = out[odd] * b[odd] % HASH_MOD
+        b = b * b % HASH_MOD
+        e >>= 1
+    return out
+
+
+def fingerprint_pairs(keys: np.ndarray, n: int) -> tuple[int, int]:
+    """fingerprint_bits of the matrix whose set bits are the distinct pairs
+    row << 32 | col in keys, without building the n x n matrix: each bit
+    (i, j) adds 11^(n-1-j) * 13^(n-1-i)."""
+    k = np.unique(np.asarray(keys, dtype=np.uint64))
+    if k.size == 0:
+        return 0, 0
+    i = (k >> np.uint64(32)).astype(np.int64)
+    j = (k & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    h = _pow_mod(11, n - 1 - j) * _pow_mod(13, n - 1 - i) % HASH_MOD
+    return int(h.sum() % HASH_MOD), int(k.size)
 
 
 class Graph:

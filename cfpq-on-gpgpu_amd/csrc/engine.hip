@@ -22,6 +22,7 @@
 
 #include "cfpq_hip.h"
 #include "ntmat.hpp"
+#include "pairs.hpp"
 #include "sched.hpp"
 
 using namespace cfpq;
@@ -1584,6 +1585,295 @@ extern "C" int cfpq_nt_closure(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nru
         stats->fixpoint_ms = ms;
         stats->product_launches = steps;
         stats->host_syncs = steps;
+    }
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// sparse pair-list formulation (pairs.hpp)
+// ---------------------------------------------------------------------------
+
+namespace {
+// A device array of uint64 that only grows; its contents are not kept across
+// a growth (callers grow destinations only).
+struct PairBuf {
+    uint64_t *p = nullptr;
+    size_t cap = 0;
+};
+}  // namespace
+
+static int pairs_grow(cfpq_ctx *ctx, PairBuf &b, size_t n) {
+    if (n <= b.cap) return CFPQ_OK;
+    const size_t c = std::max<size_t>({n, b.cap + b.cap / 2, 1024});
+    HIPCHK(hipStreamSynchronize(ctx->stream));  // nothing in flight may still use the old array
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    if (hipMalloc((void **)&b.p, c * sizeof(uint64_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, CFPQ_E_NOMEM, "cfpq_pairs_fixpoint: %zu bytes of device memory", c * sizeof(uint64_t));
+    }
+    b.cap = c;
+    return CFPQ_OK;
+}
+
+extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nrules, cfpq_pairs *sets,
+                                   uint32_t nsets, uint32_t n, cfpq_stats *stats) {
+    if (!ctx || (nsets && !sets) || (nrules && !rules))
+        return fail(ctx, CFPQ_E_INVALID, "cfpq_pairs_fixpoint: bad argument");
+    if (n > (1u << 31)) return fail(ctx, CFPQ_E_INVALID, "cfpq_pairs_fixpoint: n = %u (at most 2^31)", n);
+    for (size_t r = 0; r < nrules; ++r)
+        if (rules[r].head >= nsets || rules[r].left >= nsets || rules[r].right >= nsets)
+            return fail(ctx, CFPQ_E_INVALID, "cfpq_pairs_fixpoint: rule %zu names a set >= %u", r, nsets);
+    for (uint32_t x = 0; x < nsets; ++x) {
+        if (sets[x].count && !sets[x].keys) return fail(ctx, CFPQ_E_INVALID, "cfpq_pairs_fixpoint: set %u has no keys", x);
+        if (sets[x].count > sets[x].cap)
+            return fail(ctx, CFPQ_E_INVALID, "cfpq_pairs_fixpoint: set %u holds %llu keys over its cap %llu", x,
+                        (unsigned long long)sets[x].count, (unsigned long long)sets[x].cap);
+        for (uint64_t e = 0; e < sets[x].count; ++e) {
+            const uint64_t k = sets[x].keys[e];
+            if ((k >> 32) >= n || (k & 0xffffffffull) >= n)
+                return fail(ctx, CFPQ_E_INVALID, "cfpq_pairs_fixpoint: set %u key %llu = (%llu, %llu) outside n = %u", x,
+                            (unsigned long long)e, (unsigned long long)(k >> 32),
+                            (unsigned long long)(k & 0xffffffffull), n);
+        }
+    }
+    if (stats) *stats = cfpq_stats{};
+    if (n == 0 || nsets == 0) return CFPQ_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    uint32_t s = 1;
+    while ((1ull << s) < n) ++s;
+
+    struct Rel {
+        PairBuf m[2], mt[2], d, dn;  // by row (current, spare), by column, delta, next delta
+        uint64_t count = 0, nd = 0, ndn = 0;
+    };
+    std::vector<Rel> rel(nsets);
+    PairBuf cand[2], cnt, start, incl, flags, tmp;
+    std::vector<uint32_t> heads;
+    for (size_t r = 0; r < nrules; ++r)
+        if (std::find(heads.begin(), heads.end(), rules[r].head) == heads.end()) heads.push_back(rules[r].head);
+    uint32_t steps = 0, launches = 0, syncs = 0;
+    uint64_t added = 0, joined = 0;
+    float ms = 0;
+    const hipStream_t st = ctx->stream;
+    auto grid = [&](uint64_t m) {
+        return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((m + PAIRS_THREADS - 1) / PAIRS_THREADS,
+                                                                       (uint64_t)ctx->num_cu * 16)));
+    };
+    auto host_wait = [&]() -> int {
+        HIPCHK(hipStreamSynchronize(st));
+        ++syncs;
+        return CFPQ_OK;
+    };
+    // Sorts cand[0][0, m) (bits below 2s); returns the buffer index holding the result.
+    auto sort_cand = [&](uint64_t m, int &cur) -> int {
+        hipcub::DoubleBuffer<uint64_t> db(cand[0].p, cand[1].p);
+        size_t bytes = 0;
+        HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, db, m, 0, (int)(2 * s), st));
+        CHK(pairs_grow(ctx, tmp, (bytes + 7) / 8));
+        HIPCHK(hipcub::DeviceRadixSort::SortKeys(tmp.p, bytes, db, m, 0, (int)(2 * s), st));
+        cur = db.Current() == cand[0].p ? 0 : 1;
+        return CFPQ_OK;
+    };
+    // Keys of the sorted c[0, m) that are first of their run and not in t[0, nt),
+    // into out (cap >= m); their number into *kept.
+    auto select_new = [&](const uint64_t *c, uint64_t m, const uint64_t *t, uint64_t nt, uint64_t *out,
+                          uint64_t *kept) -> int {
+        CHK(pairs_grow(ctx, flags, (m + 7) / 8));
+        hipLaunchKernelGGL(k_pairs_flag_new, grid(m), dim3(PAIRS_THREADS), 0, st, c, m, t, nt, (uint8_t *)flags.p);
+        CHK(launch_check(ctx, "pairs flag"));
+        size_t bytes = 0;
+        HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, bytes, c, (const uint8_t *)flags.p, out, ctx->scratch64,
+                                             (int64_t)m, st));
+        CHK(pairs_grow(ctx, tmp, (bytes + 7) / 8));
+        HIPCHK(hipcub::DeviceSelect::Flagged(tmp.p, bytes, c, (const uint8_t *)flags.p, out, ctx->scratch64,
+                                             (int64_t)m, st));
+        HIPCHK(hipMemcpyAsync(ctx->scratch64_host, ctx->scratch64, 8, hipMemcpyDeviceToHost, st));
+        CHK(host_wait());
+        *kept = ctx->scratch64_host[0];
+        return CFPQ_OK;
+    };
+    // m[1] (and mt[1]) := merge of the current array with the sorted, disjoint b; swap.
+    auto merge_into = [&](PairBuf (&two)[2], uint64_t na, const uint64_t *b, uint64_t nb) -> int {
+        CHK(pairs_grow(ctx, two[1], na + nb));
+        hipLaunchKernelGGL(k_pairs_merge, grid(na + nb), dim3(PAIRS_THREADS), 0, st, (const uint64_t *)two[0].p, na, b,
+                           nb, two[1].p);
+        CHK(launch_check(ctx, "pairs merge"));
+        std::swap(two[0], two[1]);
+        return CFPQ_OK;
+    };
+    // dst := the column-ordered copy of m row-ordered keys (through cand).
+    auto build_transpose = [&](const uint64_t *keys, uint64_t m, PairBuf &dst) -> int {
+        CHK(pairs_grow(ctx, cand[0], m));
+        CHK(pairs_grow(ctx, cand[1], m));
+        CHK(pairs_grow(ctx, dst, m));
+        hipLaunchKernelGGL(k_pairs_transpose, grid(m), dim3(PAIRS_THREADS), 0, st, keys, cand[0].p, m, s);
+        CHK(launch_check(ctx, "pairs transpose"));
+        int cur = 0;
+        CHK(sort_cand(m, cur));
+        HIPCHK(hipMemcpyAsync(dst.p, cand[cur].p, m * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+        return CFPQ_OK;
+    };
+
+    auto run = [&]() -> int {
+        CHK(ensure_events(ctx, 4 * (size_t)RING + 2));
+        HIPCHK(hipEventRecord(ctx->ev[4 * RING], st));
+        // load: pack, sort, drop duplicates, build the column order
+        for (uint32_t x = 0; x < nsets; ++x) {
+            Rel &r = rel[x];
+            const uint64_t m = sets[x].count;
+            if (!m) continue;
+            CHK(pairs_grow(ctx, cand[0], m));
+            CHK(pairs_grow(ctx, cand[1], m));
+            HIPCHK(hipMemcpyAsync(cand[0].p, sets[x].keys, m * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_pairs_pack, grid(m), dim3(PAIRS_THREADS), 0, st, cand[0].p, m, s);
+            CHK(launch_check(ctx, "pairs pack"));
+            int cur = 0;
+            CHK(sort_cand(m, cur));
+            CHK(pairs_grow(ctx, r.m[0], m));
+            CHK(select_new(cand[cur].p, m, nullptr, 0, r.m[0].p, &r.count));
+            CHK(build_transpose(r.m[0].p, r.count, r.mt[0]));
+        }
+        struct Part {
+            const uint64_t *delta;
+            uint64_t m;
+            const uint64_t *op;
+            uint64_t nop;
+            int side;
+            uint64_t at;  // offset of its cnt/start/incl entries
+        };
+        std::vector<Part> parts;
+        std::vector<uint64_t> totals;
+        for (;;) {
+            bool fresh = false;
+            for (uint32_t h : heads) {
+                Rel &H = rel[h];
+                H.ndn = 0;
+                parts.clear();
+                uint64_t sum_m = 0;
+                for (size_t i = 0; i < nrules; ++i) {
+                    if (rules[i].head != h) continue;
+                    Rel &L = rel[rules[i].left], &R = rel[rules[i].right];
+                    // step 0: every pair is delta, and dL |><| M_R = M_L |><| dR
+                    const uint64_t *dl = steps ? L.d.p : L.m[0].p, *dr = steps ? R.d.p : R.m[0].p;
+                    const uint64_t ndl = steps ? L.nd : L.count, ndr = steps ? R.nd : R.count;
+                    if (ndl && R.count) {
+                        parts.push_back({dl, ndl, R.m[0].p, R.count, 0, sum_m});
+                        sum_m += ndl;
+                    }
+                    if (steps && ndr && L.count) {
+                        parts.push_back({dr, ndr, L.mt[0].p, L.count, 1, sum_m});
+                        sum_m += ndr;
+                    }
+                }
+                if (parts.empty()) continue;
+                CHK(pairs_grow(ctx, cnt, sum_m));
+                CHK(pairs_grow(ctx, start, sum_m));
+                CHK(pairs_grow(ctx, incl, sum_m));
+                CHK(pairs_grow(ctx, cand[0], parts.size()));
+                for (size_t i = 0; i < parts.size(); ++i) {
+                    const Part &p = parts[i];
+                    hipLaunchKernelGGL(k_pairs_count, grid(p.m), dim3(PAIRS_THREADS), 0, st, p.delta, p.m, p.op, p.nop,
+                                       s, p.side, cnt.p + p.at, start.p + p.at);
+                    CHK(launch_check(ctx, "pairs count"));
+                    size_t bytes = 0;
+                    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, cnt.p + p.at, incl.p + p.at, p.m, st));
+                    CHK(pairs_grow(ctx, tmp, (bytes + 7) / 8));
+                    HIPCHK(hipcub::DeviceScan::InclusiveSum(tmp.p, bytes, cnt.p + p.at, incl.p + p.at, p.m, st));
+                    HIPCHK(hipMemcpyAsync(cand[0].p + i, incl.p + p.at + p.m - 1, sizeof(uint64_t),
+                                          hipMemcpyDeviceToDevice, st));
+                }
+                totals.assign(parts.size(), 0);
+                HIPCHK(hipMemcpyAsync(totals.data(), cand[0].p, parts.size() * sizeof(uint64_t),
+                                      hipMemcpyDeviceToHost, st));
+                CHK(host_wait());
+                uint64_t total = 0;
+                for (uint64_t t : totals) total += t;
+                if (!total) continue;
+                joined += total;
+                CHK(pairs_grow(ctx, cand[0], total));
+                CHK(pairs_grow(ctx, cand[1], total));
+                uint64_t off = 0;
+                for (size_t i = 0; i < parts.size(); ++i) {
+                    const Part &p = parts[i];
+                    if (!totals[i]) continue;
+                    hipLaunchKernelGGL(k_pairs_expand, grid(totals[i]), dim3(PAIRS_THREADS), 0, st, p.delta,
+                                       (const uint64_t *)incl.p + p.at, (const uint64_t *)start.p + p.at, p.m, p.op, s,
+                                       p.side, totals[i], cand[0].p + off);
+                    CHK(launch_check(ctx, "pairs expand"));
+                    ++launches;
+                    off += totals[i];
+                }
+                int cur = 0;
+                CHK(sort_cand(total, cur));
+                CHK(select_new(cand[cur].p, total, H.m[0].p, H.count, cand[cur ^ 1].p, &H.ndn));
+                if (!H.ndn) continue;
+                CHK(pairs_grow(ctx, H.dn, H.ndn));
+                HIPCHK(hipMemcpyAsync(H.dn.p, cand[cur ^ 1].p, H.ndn * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+                fresh = true;
+            }
+            ++steps;
+            // every head's new pairs were found against the previous step's
+            // relations; only now are they merged in
+            for (uint32_t h : heads) {
+                Rel &H = rel[h];
+                if (!H.ndn) continue;
+                added += H.ndn;
+                CHK(merge_into(H.m, H.count, H.dn.p, H.ndn));
+                CHK(build_transpose(H.dn.p, H.ndn, H.d));  // H.d is free until the swap below
+                CHK(merge_into(H.mt, H.count, H.d.p, H.ndn));
+                H.count += H.ndn;
+            }
+            for (Rel &r : rel) {
+                std::swap(r.d, r.dn);
+                r.nd = r.ndn;
+                r.ndn = 0;
+            }
+            if (!fresh) break;
+            if (ctx->opts.max_iterations && steps >= ctx->opts.max_iterations)
+                return fail(ctx, CFPQ_E_LIMIT, "cfpq_pairs_fixpoint: iteration cap %u reached", ctx->opts.max_iterations);
+        }
+        HIPCHK(hipEventRecord(ctx->ev[4 * RING + 1], st));
+        CHK(host_wait());
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev[4 * RING], ctx->ev[4 * RING + 1]));
+        bool fits = true;
+        for (uint32_t x = 0; x < nsets; ++x) fits = fits && rel[x].count <= sets[x].cap;
+        if (!fits) {
+            for (uint32_t x = 0; x < nsets; ++x) sets[x].count = rel[x].count;
+            return fail(ctx, CFPQ_E_CAPACITY, "cfpq_pairs_fixpoint: a closed relation exceeds its set's cap");
+        }
+        for (uint32_t x = 0; x < nsets; ++x) {
+            const uint64_t m = rel[x].count;
+            if (m) {
+                CHK(pairs_grow(ctx, cand[0], m));
+                hipLaunchKernelGGL(k_pairs_unpack, grid(m), dim3(PAIRS_THREADS), 0, st, (const uint64_t *)rel[x].m[0].p,
+                                   cand[0].p, m, s);
+                CHK(launch_check(ctx, "pairs unpack"));
+                HIPCHK(hipMemcpyAsync(sets[x].keys, cand[0].p, m * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+            }
+            sets[x].count = m;
+        }
+        return CFPQ_OK;
+    };
+    const int rc = run();
+    (void)hipStreamSynchronize(ctx->stream);
+    auto drop = [](PairBuf &b) {
+        if (b.p) (void)hipFree(b.p);
+        b = PairBuf{};
+    };
+    for (Rel &r : rel) {
+        drop(r.m[0]), drop(r.m[1]), drop(r.mt[0]), drop(r.mt[1]), drop(r.d), drop(r.dn);
+    }
+    drop(cand[0]), drop(cand[1]), drop(cnt), drop(start), drop(incl), drop(flags), drop(tmp);
+    if (rc == CFPQ_OK && stats) {
+        stats->iterations = steps;
+        stats->new_bits = added;
+        stats->items = joined;
+        stats->fixpoint_ms = ms;
+        stats->product_launches = launches;
+        stats->host_syncs = syncs;
     }
     return rc;
 }

@@ -1,0 +1,133 @@
+// Sparse pair-list formulation of the fixpoint, for graphs too large or too
+// sparse for n x n bit tiles: each nonterminal's relation is a sorted array of
+// 64-bit keys i << s | j (s = bits of the vertex count, so key order is row-
+// major order), kept twice, by row (M) and by column (M^T, keys j << s | i).
+// A step evaluates, for every rule H -> L R,
+//     cand_H = dL |><| M_R  (join on L's column = R's row)
+//            | M_L |><| dR  (join on L's column = R's row, through M_L^T)
+// as an expansion of every delta pair into the run of the other operand that
+// shares its join vertex, then sorts the candidates of each head, drops
+// duplicates and pairs already in M_H, and merges what is left into M_H and
+// M_H^T.  Sorting, scanning and selection use hipCUB; the joins, the
+// membership filter and the merge are the kernels below.  The reference's
+// sparse backends are GraphBLAS's GrB_mxm over the LOR semiring
+// (graphBlas/graphblas_matrix.cpp:19-31) and scipy CSR products on the CPU
+// (python/matmul.py:18-28); both rebuild the head matrix on every product, as
+// the merge does here.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <cstdint>
+
+namespace cfpq {
+
+constexpr uint32_t PAIRS_THREADS = 256;
+
+#define PAIRS_FOR(i_, total_)                                                             \
+    for (uint64_t i_ = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i_ < (total_); \
+         i_ += (uint64_t)gridDim.x * blockDim.x)
+
+// First index of a sorted array whose element is >= key.
+__device__ __forceinline__ uint64_t pairs_lower_bound(const uint64_t *__restrict__ a, uint64_t n, uint64_t key) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// ABI keys (row << 32 | col) to internal keys (row << s | col), and back.
+__global__ void k_pairs_pack(uint64_t *__restrict__ k, uint64_t m, uint32_t s) {
+    PAIRS_FOR(e, m) k[e] = (k[e] >> 32) << s | (k[e] & 0xffffffffull);
+}
+__global__ void k_pairs_unpack(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t m, uint32_t s) {
+    const uint64_t mask = (1ull << s) - 1;
+    PAIRS_FOR(e, m) out[e] = (in[e] >> s) << 32 | (in[e] & mask);
+}
+
+// (i, j) -> (j, i)
+__global__ void k_pairs_transpose(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t m, uint32_t s) {
+    const uint64_t mask = (1ull << s) - 1;
+    PAIRS_FOR(e, m) out[e] = (in[e] & mask) << s | in[e] >> s;
+}
+
+// For every delta pair e, the run of the operand holding its join vertex k:
+//   side 0, delta of the left operand (i, k): row k of the right operand M_R;
+//   side 1, delta of the right operand (k, j): column k of the left operand,
+//           i.e. row k of M_L^T.
+// cnt[e] = length of the run, start[e] = its first index in the operand.
+__global__ void k_pairs_count(const uint64_t *__restrict__ delta, uint64_t m, const uint64_t *__restrict__ op,
+                              uint64_t nop, uint32_t s, int side, uint64_t *__restrict__ cnt,
+                              uint64_t *__restrict__ start) {
+    const uint64_t mask = (1ull << s) - 1;
+    PAIRS_FOR(e, m) {
+        const uint64_t key = delta[e];
+        const uint64_t k = side ? key >> s : key & mask;
+        const uint64_t lo = pairs_lower_bound(op, nop, k << s);
+        const uint64_t hi = lo + pairs_lower_bound(op + lo, nop - lo, (k + 1) << s);
+        cnt[e] = hi - lo;
+        start[e] = lo;
+    }
+}
+
+// Output p of the join: delta pair e = the first with incl[e] > p (incl is the
+// inclusive prefix sum of cnt), operand pair start[e] + (p - incl[e - 1]).
+// One thread per output, so a hub vertex's long run is spread over as many
+// threads as it produces pairs.
+__global__ void k_pairs_expand(const uint64_t *__restrict__ delta, const uint64_t *__restrict__ incl,
+                               const uint64_t *__restrict__ start, uint64_t m, const uint64_t *__restrict__ op,
+                               uint32_t s, int side, uint64_t total, uint64_t *__restrict__ out) {
+    const uint64_t mask = (1ull << s) - 1;
+    PAIRS_FOR(p, total) {
+        uint64_t lo = 0, hi = m;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (incl[mid] <= p)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const uint64_t e = lo;
+        const uint64_t before = e ? incl[e - 1] : 0;
+        const uint64_t o = op[start[e] + (p - before)];
+        const uint64_t d = delta[e];
+        // side 0: d = i << s | k, o = k << s | j;  side 1: d = k << s | j, o = k << s | i
+        out[p] = side ? (o & mask) << s | (d & mask) : (d & ~mask) | (o & mask);
+    }
+}
+
+// flag[p] = c[p] is the first of its run in the sorted candidates and is not
+// in the sorted set t.
+__global__ void k_pairs_flag_new(const uint64_t *__restrict__ c, uint64_t nc, const uint64_t *__restrict__ t,
+                                 uint64_t nt, uint8_t *__restrict__ flag) {
+    PAIRS_FOR(p, nc) {
+        const uint64_t x = c[p];
+        bool keep = p == 0 || c[p - 1] != x;
+        if (keep && nt) {
+            const uint64_t at = pairs_lower_bound(t, nt, x);
+            keep = at == nt || t[at] != x;
+        }
+        flag[p] = keep ? 1 : 0;
+    }
+}
+
+// Merge of two sorted arrays with no key in common: an element's place in the
+// output is its index in its own array plus the number of smaller keys in the
+// other one.
+__global__ void k_pairs_merge(const uint64_t *__restrict__ a, uint64_t na, const uint64_t *__restrict__ b, uint64_t nb,
+                              uint64_t *__restrict__ out) {
+    PAIRS_FOR(q, na + nb) {
+        if (q < na)
+            out[q + pairs_lower_bound(b, nb, a[q])] = a[q];
+        else
+            out[(q - na) + pairs_lower_bound(a, na, b[q - na])] = b[q - na];
+    }
+}
+
+#undef PAIRS_FOR
+
+}  // namespace cfpq

@@ -25,6 +25,9 @@
  *   cfpq_batch_fixpoint              no counterpart: the reference runs one query per
  *                                    process (m4ri/main.cpp:7-15, test_utils/runners.py:24)
  *   cfpq_mat_fingerprint             Grammar::print_results (cfpq-cpp/Grammar.cpp:81-101)
+ *   cfpq_pairs_fixpoint              the sparse backends' add_mul, GraphBLAS GrB_mxm
+ *                                    (graphBlas/graphblas_matrix.cpp:19-31) and scipy CSR
+ *                                    (python/matmul.py:18-28), under a device fixpoint
  *   cfpq_mat_copy                    the D2D copy MethodOf4RusMatrix::add_mul makes when
  *                                    the head aliases an operand
  *                                    (cuda/methodOf4RusBooleanSemiringGpu/methodOf4RusBooleanSemiringMatrix.cu:62-67)
@@ -54,6 +57,7 @@ extern "C" {
 #define CFPQ_E_DEVICE (-2)    /* a HIP runtime call failed */
 #define CFPQ_E_NOMEM (-3)     /* device or host allocation failed */
 #define CFPQ_E_LIMIT (-4)     /* iteration cap reached before the fixpoint */
+#define CFPQ_E_CAPACITY (-5)  /* an output array is too small: the sizes needed are written back */
 
 typedef struct cfpq_ctx cfpq_ctx;
 typedef struct cfpq_mat cfpq_mat;
@@ -204,6 +208,30 @@ int cfpq_batch_fixpoint(cfpq_ctx *ctx, const cfpq_query *queries, size_t nquerie
  * 3 * n * n * 4 bytes for the call. */
 int cfpq_nt_closure(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nrules, uint32_t *elems, uint32_t n,
                     cfpq_stats *stats);
+
+/* ---- sparse pair-list formulation ------------------------------------------
+ * Replaces the reference's sparse matrix backends: GraphBLAS GrB_mxm over the
+ * LOR semiring (graphBlas/graphblas_matrix.cpp:19-31, behind the same
+ * Matrix::add_mul of cfpq-cpp/Matrix.h:17) and scipy CSR products
+ * (python/matmul.py:18-28, python/matrix_utils.py:44).  For graphs whose
+ * vertex count is beyond the bit tiles' reach (n up to 2^31) or whose
+ * relations are very sparse: device memory grows with the pairs, not n^2.
+ * One set per nonterminal: keys[0 .. count) hold pairs row << 32 | col, in
+ * any order, duplicates allowed (terminal edges and the epsilon diagonal, set
+ * by the caller); on success they hold the closed relation, sorted ascending
+ * without duplicates, and count its size.  If a closed relation does not fit
+ * its set's cap, CFPQ_E_CAPACITY is returned, no keys are written and every
+ * count holds the size needed. */
+typedef struct cfpq_pairs {
+    uint64_t *keys;
+    uint64_t count;
+    uint64_t cap;
+} cfpq_pairs;
+/* stats: iterations (steps, the last adds nothing), new_bits (pairs added),
+ * items (join outputs before deduplication), fixpoint_ms, product_launches
+ * (join expansions), host_syncs. */
+int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nrules, cfpq_pairs *sets, uint32_t nsets,
+                        uint32_t n, cfpq_stats *stats);
 
 /* ---- learned tile scheduler ----------------------------------------------
  * No counterpart in the reference.  A tile-activity network (16 inputs:

@@ -33,8 +33,9 @@ def test_struct_layouts_match_header(pkg):
 #include <stddef.h>
 #include "cfpq_hip.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(cfpq_opts), sizeof(cfpq_rule), sizeof(cfpq_stats),
-         sizeof(cfpq_query), offsetof(cfpq_stats, fixpoint_ms), offsetof(cfpq_opts, schedule));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cfpq_opts), sizeof(cfpq_rule), sizeof(cfpq_stats),
+         sizeof(cfpq_query), offsetof(cfpq_stats, fixpoint_ms), offsetof(cfpq_opts, schedule),
+         sizeof(cfpq_pairs), offsetof(cfpq_pairs, cap));
   return 0;
 }
 '''
@@ -46,7 +47,7 @@ int main(void) {
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     got = [int(x) for x in out]
     want = [ctypes.sizeof(pkg.Opts), ctypes.sizeof(pkg.Rule), ctypes.sizeof(pkg.Stats), ctypes.sizeof(pkg.Query),
-            pkg.Stats.fixpoint_ms.offset, pkg.Opts.schedule.offset]
+            pkg.Stats.fixpoint_ms.offset, pkg.Opts.schedule.offset, ctypes.sizeof(pkg.Pairs), pkg.Pairs.cap.offset]
     assert got == want
 
 

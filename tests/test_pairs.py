@@ -1,0 +1,140 @@
+"""The sparse pair-list formulation (cfpq_pairs_fixpoint, the counterpart of
+the reference's GraphBLAS and scipy CSR backends): host helpers on the CPU;
+on the GPU, the closed pair sets against the oracle's per-nonterminal
+fixpoint, with vertex ids spread over [0, 2^31) (the least fixpoint does not
+depend on vertex names), and closed forms at sizes the oracle cannot run."""
+import importlib
+
+import numpy as np
+import pytest
+
+cf = importlib.import_module("cfpq-on-gpgpu_amd.cfpq")
+
+
+def _keys_of(dense: np.ndarray) -> np.ndarray:
+    i, j = np.nonzero(dense)
+    return i.astype(np.uint64) << np.uint64(32) | j.astype(np.uint64)
+
+
+def _relabel(keys: np.ndarray, names: np.ndarray) -> np.ndarray:
+    i = names[(keys >> np.uint64(32)).astype(np.int64)].astype(np.uint64)
+    j = names[(keys & np.uint64(0xFFFFFFFF)).astype(np.int64)].astype(np.uint64)
+    return np.sort(i << np.uint64(32) | j)
+
+
+def test_fingerprint_pairs_kats_and_dense():
+    # the hashes of the reference's printed matrices (cfpq-cpp/Grammar.cpp:81-101)
+    assert cf.fingerprint_pairs(np.array([0], np.uint64), 1) == (1, 1)
+    assert cf.fingerprint_pairs(_keys_of(np.eye(2, dtype=bool)), 2) == (144, 2)
+    assert cf.fingerprint_pairs(_keys_of(np.ones((3, 3), bool)), 3) == (24339, 9)
+    assert cf.fingerprint_pairs(np.zeros(0, np.uint64), 5) == (0, 0)
+    rng = np.random.default_rng(7)
+    for n in (2, 17, 200, 1000):
+        d = rng.random((n, n)) < 0.05
+        k = _keys_of(d)
+        assert cf.fingerprint_pairs(np.concatenate([k, k[::3]]), n) == cf.fingerprint_bits(d), n
+
+
+@pytest.mark.parametrize("which", ["dyck", "sg", "eps"])
+def test_pair_keys_match_initial_matrices(oracle, gen, which):
+    q = {"dyck": lambda: gen.dyck_er(150, 500, seed=4), "sg": lambda: gen.sg_forest(300, seed=5, types_per_vertex=2),
+         "eps": lambda: gen.dyck_er(90, 300, seed=6)}[which]()
+    if which == "eps":
+        q.grammar = "S\nS A S1\nS1 S B\nS S S\nA a\nB b\n"
+    gr = cf.Grammar(q.grammar, text=True)
+    keys = gr.pair_keys(cf.Graph.from_query(q))
+    g = oracle.ParsedGrammar(q.grammar)
+    mats = oracle.initial_matrices(g, q.n, q.src, q.dst, q.label, q.labels)
+    for x, k in enumerate(keys):
+        assert np.array_equal(np.unique(k), _keys_of(oracle.to_dense(mats[x], q.n))), x
+
+
+def _cases(gen):
+    eps = gen.dyck_er(100, 300, seed=11)
+    eps.grammar = "S\nS A S1\nS1 S B\nS S S\nA a\nB b\n"
+    return {"dyck": lambda: gen.dyck_er(700, 2000, seed=8), "sg": lambda: gen.sg_forest(1000, seed=9, types_per_vertex=3),
+            "dense": lambda: gen.dense_stress(257, seed=10), "eps": lambda: eps,
+            "tiny": lambda: gen.dyck_er(63, 120, seed=12)}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("spread", [False, True])
+@pytest.mark.parametrize("which", ["dyck", "sg", "dense", "eps", "tiny"])
+def test_pairs_fixpoint_matches_oracle(pkg, oracle, gen, which, spread):
+    q = _cases(gen)[which]()
+    g, mats, _ = oracle.solve(q, "seminaive")
+    gr = cf.Grammar(q.grammar, text=True)
+    keys = gr.pair_keys(cf.Graph.from_query(q))
+    n = q.n
+    names = np.arange(n, dtype=np.int64)
+    if spread:  # distinct vertex ids over [0, 2^31)
+        rng = np.random.default_rng(13)
+        names = np.sort(rng.choice(1 << 31, size=n, replace=False)).astype(np.int64)
+        rng.shuffle(names)
+        keys = [_relabel(k, names) for k in keys]
+        n = 1 << 31
+    with pkg.Engine(0) as eng:
+        out, st = eng.pairs_fixpoint(gr.rule_triples(), keys, n)
+    assert st["iterations"] >= 1
+    for x, k in enumerate(out):
+        want = _keys_of(oracle.to_dense(mats[x], q.n))
+        if spread:
+            want = _relabel(want, names)
+        assert np.array_equal(k, want), (which, x)
+    if not spread:
+        assert gr.pair_results(out, n) == oracle.fingerprints(g, mats, n)
+
+
+@pytest.mark.gpu
+def test_pairs_fixpoint_closed_forms(pkg, gen):
+    """* transitive closure of a 300-vertex path under S -> S S | a: every
+       pair i < j;
+     * the Dyck grammar on the chain a^k b^k (k = 3,000) with vertex ids
+       spread over [0, 2^31): exactly the k nested pairs (k - m, k + m)."""
+    with pkg.Engine(0) as eng:
+        n = 300
+        a = np.arange(n - 1, dtype=np.uint64)
+        out, st = eng.pairs_fixpoint([(0, 0, 0)], [a << np.uint64(32) | (a + np.uint64(1))], n)
+        i, j = np.triu_indices(n, 1)
+        assert np.array_equal(out[0], np.sort(i.astype(np.uint64) << np.uint64(32) | j.astype(np.uint64)))
+        assert st["new_bits"] == n * (n - 1) // 2 - (n - 1)
+        k = 3000
+        gr = cf.Grammar(gen.DYCK_GRAMMAR, text=True)
+        rng = np.random.default_rng(21)
+        names = rng.choice(1 << 31, size=2 * k + 1, replace=False).astype(np.int64)
+        src = names[:-1]
+        dst = names[1:]
+        labels = ["a"] * k + ["b"] * k
+        keys = gr.pair_keys(cf.Graph(src=src, dst=dst, labels=labels, n=1 << 31))
+        out, st = eng.pairs_fixpoint(gr.rule_triples(), keys, 1 << 31)
+        m = np.arange(1, k + 1)
+        want = np.sort(names[k - m].astype(np.uint64) << np.uint64(32) | names[k + m].astype(np.uint64))
+        assert np.array_equal(out[gr.nonterminal_to_index["S"]], want)
+        assert st["iterations"] >= k  # one nesting level per step
+
+
+@pytest.mark.gpu
+def test_pairs_fixpoint_edge_cases(pkg):
+    with pkg.Engine(0) as eng:
+        # nothing at all
+        out, st = eng.pairs_fixpoint([], [], 0)
+        assert out == [] and st["iterations"] == 0
+        # empty sets, rules over them
+        out, _ = eng.pairs_fixpoint([(0, 1, 1)], [np.zeros(0, np.uint64)] * 2, 10)
+        assert all(len(k) == 0 for k in out)
+        # one vertex with a self loop: S -> S S
+        out, _ = eng.pairs_fixpoint([(0, 0, 0)], [np.zeros(1, np.uint64)], 1)
+        assert out[0].tolist() == [0]
+        # duplicates and no rules: sorted, unique
+        k = np.array([5 << 32 | 1, 2, 5 << 32 | 1, 2, 3 << 32], np.uint64)
+        out, _ = eng.pairs_fixpoint([], [k], 6)
+        assert out[0].tolist() == sorted({int(x) for x in k})
+        # a first capacity too small: the binding repeats the call with the reported sizes
+        a = np.arange(199, dtype=np.uint64)
+        path = a << np.uint64(32) | (a + np.uint64(1))
+        out, _ = eng.pairs_fixpoint([(0, 0, 0)], [path], 200, cap=1)
+        assert len(out[0]) == 200 * 199 // 2
+        with pytest.raises(pkg.CfpqError):  # a key outside n
+            eng.pairs_fixpoint([], [np.array([7 << 32], np.uint64)], 7)
+        with pytest.raises(pkg.CfpqError):  # a rule naming a missing set
+            eng.pairs_fixpoint([(0, 0, 1)], [path], 200)
