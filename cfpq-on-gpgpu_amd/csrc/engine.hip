@@ -51,6 +51,9 @@ struct cfpq_ctx {
     StepCounters *ring_host = nullptr;
     unsigned long long *scratch64 = nullptr;  // small reductions
     unsigned long long *scratch64_host = nullptr;
+    // device arrays of the pair-list fixpoint kept between its calls
+    // (pointer, capacity in uint64 elements); freed by cfpq_destroy
+    std::vector<std::pair<uint64_t *, size_t>> pair_pool;
 
     // work lists
     uint64_t *items = nullptr;   // dense work items
@@ -236,6 +239,8 @@ extern "C" int cfpq_destroy(cfpq_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->pool) free_mat(kv.second);
     ctx->pool.clear();
+    for (auto &b : ctx->pair_pool) (void)hipFree(b.first);
+    ctx->pair_pool.clear();
     for (auto e : ctx->ev) (void)hipEventDestroy(e);
     for (auto g : ctx->graph_exec)
         if (g) (void)hipGraphExecDestroy(g);
@@ -1602,16 +1607,42 @@ struct PairBuf {
 };
 }  // namespace
 
+// Hands b back to the context's pool.  Work already enqueued on the stream
+// that reads or writes it runs before any later use, so no wait is needed.
+static void pairs_release(cfpq_ctx *ctx, PairBuf &b) {
+    if (b.p) ctx->pair_pool.emplace_back(b.p, b.cap);
+    b = PairBuf{};
+}
+
 static int pairs_grow(cfpq_ctx *ctx, PairBuf &b, size_t n) {
     if (n <= b.cap) return CFPQ_OK;
-    const size_t c = std::max<size_t>({n, b.cap + b.cap / 2, 1024});
-    HIPCHK(hipStreamSynchronize(ctx->stream));  // nothing in flight may still use the old array
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.cap = 0;
+    pairs_release(ctx, b);
+    // the smallest pooled array that fits
+    size_t best = ctx->pair_pool.size();
+    for (size_t i = 0; i < ctx->pair_pool.size(); ++i)
+        if (ctx->pair_pool[i].second >= n && (best == ctx->pair_pool.size() || ctx->pair_pool[i].second < ctx->pair_pool[best].second))
+            best = i;
+    if (best < ctx->pair_pool.size()) {
+        b.p = ctx->pair_pool[best].first;
+        b.cap = ctx->pair_pool[best].second;
+        ctx->pair_pool.erase(ctx->pair_pool.begin() + best);
+        return CFPQ_OK;
+    }
+    const size_t c = std::max<size_t>(n + n / 2, 1024);
     if (hipMalloc((void **)&b.p, c * sizeof(uint64_t)) != hipSuccess) {
         (void)hipGetLastError();
-        return fail(ctx, CFPQ_E_NOMEM, "cfpq_pairs_fixpoint: %zu bytes of device memory", c * sizeof(uint64_t));
+        // give the pool back to the device and try once more
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (auto &e : ctx->pair_pool) (void)hipFree(e.first);
+        ctx->pair_pool.clear();
+        b.p = nullptr;
+        if (hipMalloc((void **)&b.p, n * sizeof(uint64_t)) != hipSuccess) {
+            (void)hipGetLastError();
+            b.p = nullptr;
+            return fail(ctx, CFPQ_E_NOMEM, "cfpq_pairs_fixpoint: %zu bytes of device memory", n * sizeof(uint64_t));
+        }
+        b.cap = n;
+        return CFPQ_OK;
     }
     b.cap = c;
     return CFPQ_OK;
@@ -1649,7 +1680,7 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
         uint64_t count = 0, nd = 0, ndn = 0;
     };
     std::vector<Rel> rel(nsets);
-    PairBuf cand[2], cnt, start, incl, flags, tmp;
+    PairBuf cand[2], cnt, start, incl, flags, tmp, words;
     std::vector<uint32_t> heads;
     for (size_t r = 0; r < nrules; ++r)
         if (std::find(heads.begin(), heads.end(), rules[r].head) == heads.end()) heads.push_back(rules[r].head);
@@ -1677,22 +1708,25 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
         return CFPQ_OK;
     };
     // Keys of the sorted c[0, m) that are first of their run and not in t[0, nt),
-    // into out (cap >= m); their number into *kept.
+    // into out (cap >= m); their number into the device word *kept, which the
+    // host reads after its next wait.
     auto select_new = [&](const uint64_t *c, uint64_t m, const uint64_t *t, uint64_t nt, uint64_t *out,
                           uint64_t *kept) -> int {
         CHK(pairs_grow(ctx, flags, (m + 7) / 8));
         hipLaunchKernelGGL(k_pairs_flag_new, grid(m), dim3(PAIRS_THREADS), 0, st, c, m, t, nt, (uint8_t *)flags.p);
         CHK(launch_check(ctx, "pairs flag"));
         size_t bytes = 0;
-        HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, bytes, c, (const uint8_t *)flags.p, out, ctx->scratch64,
-                                             (int64_t)m, st));
+        HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, bytes, c, (const uint8_t *)flags.p, out, kept, (int64_t)m, st));
         CHK(pairs_grow(ctx, tmp, (bytes + 7) / 8));
-        HIPCHK(hipcub::DeviceSelect::Flagged(tmp.p, bytes, c, (const uint8_t *)flags.p, out, ctx->scratch64,
-                                             (int64_t)m, st));
-        HIPCHK(hipMemcpyAsync(ctx->scratch64_host, ctx->scratch64, 8, hipMemcpyDeviceToHost, st));
-        CHK(host_wait());
-        *kept = ctx->scratch64_host[0];
+        HIPCHK(hipcub::DeviceSelect::Flagged(tmp.p, bytes, c, (const uint8_t *)flags.p, out, kept, (int64_t)m, st));
         return CFPQ_OK;
+    };
+    // host_words := k device words (one wait).
+    std::vector<uint64_t> host_words;
+    auto read_words = [&](const uint64_t *d, size_t k) -> int {
+        host_words.assign(k, 0);
+        if (k) HIPCHK(hipMemcpyAsync(host_words.data(), d, k * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        return host_wait();
     };
     // m[1] (and mt[1]) := merge of the current array with the sorted, disjoint b; swap.
     auto merge_into = [&](PairBuf (&two)[2], uint64_t na, const uint64_t *b, uint64_t nb) -> int {
@@ -1719,7 +1753,8 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
     auto run = [&]() -> int {
         CHK(ensure_events(ctx, 4 * (size_t)RING + 2));
         HIPCHK(hipEventRecord(ctx->ev[4 * RING], st));
-        // load: pack, sort, drop duplicates, build the column order
+        // load: pack, sort, drop duplicates (one wait for all sets), build the column order
+        CHK(pairs_grow(ctx, words, std::max<size_t>(nsets, heads.size())));
         for (uint32_t x = 0; x < nsets; ++x) {
             Rel &r = rel[x];
             const uint64_t m = sets[x].count;
@@ -1732,8 +1767,12 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
             int cur = 0;
             CHK(sort_cand(m, cur));
             CHK(pairs_grow(ctx, r.m[0], m));
-            CHK(select_new(cand[cur].p, m, nullptr, 0, r.m[0].p, &r.count));
-            CHK(build_transpose(r.m[0].p, r.count, r.mt[0]));
+            CHK(select_new(cand[cur].p, m, nullptr, 0, r.m[0].p, words.p + x));
+        }
+        CHK(read_words(words.p, nsets));
+        for (uint32_t x = 0; x < nsets; ++x) {
+            rel[x].count = sets[x].count ? host_words[x] : 0;
+            if (rel[x].count) CHK(build_transpose(rel[x].m[0].p, rel[x].count, rel[x].mt[0]));
         }
         struct Part {
             const uint64_t *delta;
@@ -1744,16 +1783,17 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
             uint64_t at;  // offset of its cnt/start/incl entries
         };
         std::vector<Part> parts;
-        std::vector<uint64_t> totals;
+        std::vector<size_t> first(heads.size() + 1);
+        std::vector<char> active(heads.size());
         for (;;) {
             bool fresh = false;
-            for (uint32_t h : heads) {
-                Rel &H = rel[h];
-                H.ndn = 0;
-                parts.clear();
-                uint64_t sum_m = 0;
+            // join sizes of every head's parts (one wait)
+            parts.clear();
+            uint64_t sum_m = 0;
+            for (size_t hi = 0; hi < heads.size(); ++hi) {
+                first[hi] = parts.size();
                 for (size_t i = 0; i < nrules; ++i) {
-                    if (rules[i].head != h) continue;
+                    if (rules[i].head != heads[hi]) continue;
                     Rel &L = rel[rules[i].left], &R = rel[rules[i].right];
                     // step 0: every pair is delta, and dL |><| M_R = M_L |><| dR
                     const uint64_t *dl = steps ? L.d.p : L.m[0].p, *dr = steps ? R.d.p : R.m[0].p;
@@ -1767,11 +1807,14 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
                         sum_m += ndr;
                     }
                 }
-                if (parts.empty()) continue;
+            }
+            first[heads.size()] = parts.size();
+            for (uint32_t h : heads) rel[h].ndn = 0;
+            if (!parts.empty()) {
                 CHK(pairs_grow(ctx, cnt, sum_m));
                 CHK(pairs_grow(ctx, start, sum_m));
                 CHK(pairs_grow(ctx, incl, sum_m));
-                CHK(pairs_grow(ctx, cand[0], parts.size()));
+                CHK(pairs_grow(ctx, words, std::max(parts.size(), heads.size())));
                 for (size_t i = 0; i < parts.size(); ++i) {
                     const Part &p = parts[i];
                     hipLaunchKernelGGL(k_pairs_count, grid(p.m), dim3(PAIRS_THREADS), 0, st, p.delta, p.m, p.op, p.nop,
@@ -1781,37 +1824,47 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
                     HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, cnt.p + p.at, incl.p + p.at, p.m, st));
                     CHK(pairs_grow(ctx, tmp, (bytes + 7) / 8));
                     HIPCHK(hipcub::DeviceScan::InclusiveSum(tmp.p, bytes, cnt.p + p.at, incl.p + p.at, p.m, st));
-                    HIPCHK(hipMemcpyAsync(cand[0].p + i, incl.p + p.at + p.m - 1, sizeof(uint64_t),
+                    HIPCHK(hipMemcpyAsync(words.p + i, incl.p + p.at + p.m - 1, sizeof(uint64_t),
                                           hipMemcpyDeviceToDevice, st));
                 }
-                totals.assign(parts.size(), 0);
-                HIPCHK(hipMemcpyAsync(totals.data(), cand[0].p, parts.size() * sizeof(uint64_t),
-                                      hipMemcpyDeviceToHost, st));
-                CHK(host_wait());
-                uint64_t total = 0;
-                for (uint64_t t : totals) total += t;
-                if (!total) continue;
-                joined += total;
-                CHK(pairs_grow(ctx, cand[0], total));
-                CHK(pairs_grow(ctx, cand[1], total));
-                uint64_t off = 0;
-                for (size_t i = 0; i < parts.size(); ++i) {
-                    const Part &p = parts[i];
-                    if (!totals[i]) continue;
-                    hipLaunchKernelGGL(k_pairs_expand, grid(totals[i]), dim3(PAIRS_THREADS), 0, st, p.delta,
-                                       (const uint64_t *)incl.p + p.at, (const uint64_t *)start.p + p.at, p.m, p.op, s,
-                                       p.side, totals[i], cand[0].p + off);
-                    CHK(launch_check(ctx, "pairs expand"));
-                    ++launches;
-                    off += totals[i];
+                CHK(read_words(words.p, parts.size()));
+                const std::vector<uint64_t> totals = host_words;
+                // per head: expand, sort, keep what is new (one wait for all heads)
+                bool any = false;
+                for (size_t hi = 0; hi < heads.size(); ++hi) {
+                    Rel &H = rel[heads[hi]];
+                    active[hi] = 0;
+                    uint64_t total = 0;
+                    for (size_t i = first[hi]; i < first[hi + 1]; ++i) total += totals[i];
+                    if (!total) continue;
+                    joined += total;
+                    CHK(pairs_grow(ctx, cand[0], total));
+                    CHK(pairs_grow(ctx, cand[1], total));
+                    uint64_t off = 0;
+                    for (size_t i = first[hi]; i < first[hi + 1]; ++i) {
+                        const Part &p = parts[i];
+                        if (!totals[i]) continue;
+                        hipLaunchKernelGGL(k_pairs_expand, grid(totals[i]), dim3(PAIRS_THREADS), 0, st, p.delta,
+                                           (const uint64_t *)incl.p + p.at, (const uint64_t *)start.p + p.at, p.m, p.op,
+                                           s, p.side, totals[i], cand[0].p + off);
+                        CHK(launch_check(ctx, "pairs expand"));
+                        ++launches;
+                        off += totals[i];
+                    }
+                    int cur = 0;
+                    CHK(sort_cand(total, cur));
+                    CHK(pairs_grow(ctx, H.dn, total));
+                    CHK(select_new(cand[cur].p, total, H.m[0].p, H.count, H.dn.p, words.p + hi));
+                    active[hi] = 1;
+                    any = true;
                 }
-                int cur = 0;
-                CHK(sort_cand(total, cur));
-                CHK(select_new(cand[cur].p, total, H.m[0].p, H.count, cand[cur ^ 1].p, &H.ndn));
-                if (!H.ndn) continue;
-                CHK(pairs_grow(ctx, H.dn, H.ndn));
-                HIPCHK(hipMemcpyAsync(H.dn.p, cand[cur ^ 1].p, H.ndn * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
-                fresh = true;
+                if (any) {
+                    CHK(read_words(words.p, heads.size()));
+                    for (size_t hi = 0; hi < heads.size(); ++hi) {
+                        rel[heads[hi]].ndn = active[hi] ? host_words[hi] : 0;
+                        fresh = fresh || rel[heads[hi]].ndn;
+                    }
+                }
             }
             ++steps;
             // every head's new pairs were found against the previous step's
@@ -1859,14 +1912,11 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
     };
     const int rc = run();
     (void)hipStreamSynchronize(ctx->stream);
-    auto drop = [](PairBuf &b) {
-        if (b.p) (void)hipFree(b.p);
-        b = PairBuf{};
-    };
+    auto drop = [&](PairBuf &b) { pairs_release(ctx, b); };
     for (Rel &r : rel) {
         drop(r.m[0]), drop(r.m[1]), drop(r.mt[0]), drop(r.mt[1]), drop(r.d), drop(r.dn);
     }
-    drop(cand[0]), drop(cand[1]), drop(cnt), drop(start), drop(incl), drop(flags), drop(tmp);
+    drop(cand[0]), drop(cand[1]), drop(cnt), drop(start), drop(incl), drop(flags), drop(tmp), drop(words);
     if (rc == CFPQ_OK && stats) {
         stats->iterations = steps;
         stats->new_bits = added;

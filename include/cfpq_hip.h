@@ -221,7 +221,10 @@ int cfpq_nt_closure(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nrules, uint32
  * by the caller); on success they hold the closed relation, sorted ascending
  * without duplicates, and count its size.  If a closed relation does not fit
  * its set's cap, CFPQ_E_CAPACITY is returned, no keys are written and every
- * count holds the size needed. */
+ * count holds the size needed.  The device arrays of a call stay with the
+ * context for the next call (no allocation once their sizes are reached);
+ * cfpq_destroy frees them, and a call that runs out of device memory frees
+ * them before it retries. */
 typedef struct cfpq_pairs {
     uint64_t *keys;
     uint64_t count;

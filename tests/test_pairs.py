@@ -86,6 +86,26 @@ def test_pairs_fixpoint_matches_oracle(pkg, oracle, gen, which, spread):
 
 
 @pytest.mark.gpu
+def test_pairs_fixpoint_disjoint_copies(pkg, oracle, gen):
+    """200 disjoint copies of an oracle-checked ontology graph (2,000 vertices
+    each) as one 400,000-vertex graph, copies placed at random offsets in
+    [0, 2^31): the closed sets are the shifted copies of the oracle's."""
+    q = gen.sg_forest(2000, seed=31, types_per_vertex=2)
+    g, mats, _ = oracle.solve(q, "seminaive")
+    gr = cf.Grammar(q.grammar, text=True)
+    keys = gr.pair_keys(cf.Graph.from_query(q))
+    K = 200
+    rng = np.random.default_rng(32)
+    base = np.sort(rng.choice((1 << 31) // q.n, size=K, replace=False)).astype(np.uint64) * np.uint64(q.n)
+    shift = base * np.uint64((1 << 32) + 1)
+    with pkg.Engine(0) as eng:
+        out, _ = eng.pairs_fixpoint(gr.rule_triples(), [np.concatenate([k + d for d in shift]) for k in keys], 1 << 31)
+    for x, k in enumerate(out):
+        want = _keys_of(oracle.to_dense(mats[x], q.n))
+        assert np.array_equal(k, np.concatenate([want + d for d in shift])), x
+
+
+@pytest.mark.gpu
 def test_pairs_fixpoint_closed_forms(pkg, gen):
     """* transitive closure of a 300-vertex path under S -> S S | a: every
        pair i < j;
