@@ -23,9 +23,10 @@ run            cross-validates the runners on the --check tests (every runner's
 A runner is an executable called as ``EXE grammar graph out [ARG...]`` that
 prints "<total_ms> <algorithm_ms>" and writes "name hash count" lines; a
 non-empty stderr or a non-zero exit is a failure (test_utils/runners.py:16-31).
-Without --runner the harness runs cfpq_hip twice: as the device fixpoint
-("c++_cfpq_hip") and as the reference's worklist over cfpq_add_mul
-("c++_cfpq_hip_worklist").
+Without --runner the harness runs cfpq_hip three times: as the device
+fixpoint ("c++_cfpq_hip"), as the reference's worklist over cfpq_add_mul
+("c++_cfpq_hip_worklist") and as the sparse pair-list fixpoint
+("c++_cfpq_hip_pairs").
 """
 from __future__ import annotations
 
@@ -136,7 +137,8 @@ def parse_runner(spec: str, timeout: float) -> Runner:
 def default_runners(timeout: float) -> list[Runner]:
     if not os.path.exists(CLI):
         raise FileNotFoundError(f"{CLI} is missing: build it with `make -C {HERE}`")
-    return [Runner(CLI, "c++_cfpq_hip", [], timeout), Runner(CLI, "c++_cfpq_hip_worklist", ["--worklist"], timeout)]
+    return [Runner(CLI, "c++_cfpq_hip", [], timeout), Runner(CLI, "c++_cfpq_hip_worklist", ["--worklist"], timeout),
+            Runner(CLI, "c++_cfpq_hip_pairs", ["--pairs"], timeout)]
 
 
 def read_answer(path: str) -> dict[str, tuple[str, str]]:

@@ -73,6 +73,8 @@ public:
     const std::unordered_map<std::string, std::vector<int>> &get_nonterminal_from_terminal();
     const std::vector<cfpq_rule_t> &get_rules() const { return rules; }
     unsigned int get_nonterminals_count() const { return nonterminals_count; }
+    const std::vector<unsigned int> &get_epsilon_nonterminals() const { return epsilon_nonterminals; }
+    const std::map<std::string, unsigned int> &get_nonterminal_to_index() const { return nonterminal_to_index; }
     unsigned long long get_add_mul_calls() const { return add_mul_calls; }
     Matrix *matrix(const std::string &nonterminal);
 

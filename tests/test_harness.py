@@ -132,8 +132,9 @@ def test_runner_specs_and_threshold(tmp_path):
 
 @pytest.mark.gpu
 def test_harness_with_engine_cli(tmp_path, gen, oracle):
-    """The engine's CLI as two runners (device fixpoint, worklist over
-    cfpq_add_mul): cross-validated on every test and equal to the oracle."""
+    """The engine's CLI as three runners (device fixpoint, worklist over
+    cfpq_add_mul, pair lists): cross-validated on every test and equal to the
+    oracle."""
     data, graphs = _write_dataset(str(tmp_path), gen, sizes=(("g1", 700, 2000), ("g2", 300, 900)))
     csv = str(tmp_path / "t.csv")
     harness.build_testset(data, csv)
