@@ -66,6 +66,8 @@ def lib():
             getattr(L, f).restype = None
             getattr(L, f).argtypes = [_u64p, ctypes.c_uint32, _u64p, _u64p]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_set_num_threads.restype = None
+        L.oracle_set_num_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -194,6 +196,10 @@ def fixpoint_worklist_budget(n, rules, mats, max_seconds: float, method: str = "
 
 def num_threads() -> int:
     return int(lib().oracle_num_threads())
+
+
+def set_num_threads(k: int) -> None:
+    lib().oracle_set_num_threads(int(k))
 
 
 def fixpoint_seminaive(n, rules, mats) -> int:

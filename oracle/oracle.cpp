@@ -472,4 +472,13 @@ int oracle_num_threads(void) {
 #endif
 }
 
+// Threads of the following OpenMP regions (the baseline's 1-thread figure).
+void oracle_set_num_threads(int k) {
+#ifdef _OPENMP
+    omp_set_num_threads(k > 0 ? k : 1);
+#else
+    (void)k;
+#endif
+}
+
 }  // extern "C"
