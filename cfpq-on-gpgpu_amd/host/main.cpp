@@ -1,14 +1,16 @@
 // Command-line runner with the reference backends' interface
 // (m4ri/main.cpp:7-15, cuda/naive-gpu/main.cpp:7-16):
 //
-//   cfpq_hip <grammar> <graph> <out> [--worklist | --pairs]
+//   cfpq_hip <grammar> <graph> <out> [--worklist | --pairs | --bits]
 //
 // prints "<total_ms> <algorithm_ms>" and writes "name hash count" lines.
 // --worklist runs the reference's per-rule worklist over cfpq_add_mul instead
 // of the device-resident fixpoint.  --pairs runs the sparse pair-list
 // fixpoint (cfpq_pairs_fixpoint, the counterpart of the GraphBLAS backend's
 // runner, graphBlas/main.cpp:11-20): relations as sorted pair keys, no n x n
-// matrices, so graphs far beyond the bit tiles' size run too.
+// matrices, so graphs far beyond the bit tiles' size run too.  Without a
+// flag the bit matrices are used when they fit (kBitsBudget) and pair lists
+// otherwise; --bits forces the matrices.
 #include <chrono>
 #include <cstring>
 #include <fstream>
@@ -24,6 +26,9 @@
 namespace {
 
 constexpr uint64_t kHashMod = 1000000007ull;  // cfpq-cpp/Grammar.cpp:13
+// Device bytes the bit-matrix engine may plan for: a matrix and its delta
+// buffers per nonterminal take about 3 n^2 / 8 bytes (of 288 GB of HBM).
+constexpr double kBitsBudget = 200.0 * (1ull << 30);
 
 uint64_t pow_mod(uint64_t b, uint64_t e) {
     uint64_t r = 1;
@@ -96,14 +101,21 @@ std::pair<unsigned int, unsigned int> run_pairs(Grammar &grammar, Graph &graph, 
 
 int main(int argc, char *argv[]) {
     if (argc < 4) {
-        std::cerr << "usage: cfpq_hip <grammar> <graph> <out> [--worklist | --pairs]\n";
+        std::cerr << "usage: cfpq_hip <grammar> <graph> <out> [--worklist | --pairs | --bits]\n";
         return 2;
     }
-    const bool worklist = argc > 4 && std::strcmp(argv[4], "--worklist") == 0;
-    const bool pairs = argc > 4 && std::strcmp(argv[4], "--pairs") == 0;
+    const char *mode = argc > 4 ? argv[4] : "";
+    if (*mode && std::strcmp(mode, "--worklist") && std::strcmp(mode, "--pairs") && std::strcmp(mode, "--bits")) {
+        std::cerr << "cfpq_hip: unknown option " << mode << "\n";
+        return 2;
+    }
+    const bool worklist = std::strcmp(mode, "--worklist") == 0;
     try {
         Grammar grammar(argv[1]);
         Graph graph(argv[2]);
+        const double n = graph.vertices_count;
+        const bool pairs = std::strcmp(mode, "--pairs") == 0 ||
+                           (!*mode && 3.0 * grammar.get_nonterminals_count() * n * n / 8 > kBitsBudget);
         if (pairs) {
             auto times = run_pairs(grammar, graph, argv[3]);
             std::cout << times.first << ' ' << times.second << std::endl;

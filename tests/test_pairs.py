@@ -158,3 +158,41 @@ def test_pairs_fixpoint_edge_cases(pkg):
             eng.pairs_fixpoint([], [np.array([7 << 32], np.uint64)], 7)
         with pytest.raises(pkg.CfpqError):  # a rule naming a missing set
             eng.pairs_fixpoint([(0, 0, 1)], [path], 200)
+
+
+@pytest.mark.gpu
+def test_cli_switches_to_pairs_beyond_the_bit_matrices(tmp_path, gen):
+    """The CLI on a Dyck chain a^k b^k whose vertex ids reach 2^21: the bit
+    matrices would need terabytes, so the pair lists run without a flag; the
+    output lines are the closed forms' fingerprints."""
+    import subprocess
+    harness = importlib.import_module("cfpq-on-gpgpu_amd.harness")
+    k = 1500
+    rng = np.random.default_rng(41)
+    names = rng.choice((1 << 21) - 1, size=2 * k + 1, replace=False).astype(np.uint64)
+    names[rng.integers(0, 2 * k + 1)] = (1 << 21) - 1  # distinct, and n = 2^21 exactly
+    gpath, mpath, out = tmp_path / "g.txt", tmp_path / "m.txt", tmp_path / "out.txt"
+    gpath.write_text(gen.DYCK_GRAMMAR)
+    with open(mpath, "w") as f:
+        for p in range(2 * k):
+            f.write(f"{names[p]} {'a' if p < k else 'b'} {names[p + 1]}\n")
+    r = subprocess.run([harness.CLI, str(gpath), str(mpath), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and not r.stderr, r.stderr
+    n = 1 << 21
+
+    def keys(pos):
+        pos = np.asarray(pos, dtype=np.int64)
+        return names[pos[:, 0]] << np.uint64(32) | names[pos[:, 1]]
+
+    m = np.arange(1, k + 1)
+    want = {
+        "A": keys([(p, p + 1) for p in range(k)]),
+        "B": keys([(p, p + 1) for p in range(k, 2 * k)]),
+        "S": keys(np.stack([k - m, k + m], 1)),
+        "S1": keys(np.stack([k - m[:-1], k + m[:-1] + 1], 1)),
+    }
+    got = harness.read_answer(str(out))
+    assert set(got) == set(want)
+    for name, kk in want.items():
+        h, c = cf.fingerprint_pairs(kk, n)
+        assert got[name] == (str(h), str(c)), name
