@@ -42,6 +42,7 @@ class Grammar:
                 self.rules.append((head, (b1, b2)))
         self.vertices_count = 0
         self.matrices = []
+        self.pair_sets = None  # closed pair keys after mode="pairs"
 
     def _index(self, name: str) -> int:
         if name not in self.nonterminal_to_index:
@@ -78,7 +79,18 @@ class Grammar:
     def intersection_with_graph(self, graph: "Graph", engine: Engine, mode: str = "fixpoint"):
         """Returns (total_ms, algorithm_ms, stats).  mode "fixpoint" runs the
         device-resident delta fixpoint; "worklist" runs the reference's
-        worklist loop (Grammar.h:58-69) over cfpq_add_mul."""
+        worklist loop (Grammar.h:58-69) over cfpq_add_mul; "pairs" runs the
+        sparse pair-list fixpoint (cfpq_pairs_fixpoint, no n x n matrices),
+        whose closed sets results() then reads."""
+        if mode == "pairs":
+            self.free()
+            self.vertices_count = graph.vertices_count
+            t0 = time.perf_counter()
+            self.pair_sets, stats = engine.pairs_fixpoint(self.rule_triples(), self.pair_keys(graph),
+                                                          graph.vertices_count)
+            total = (time.perf_counter() - t0) * 1e3
+            return total, stats["fixpoint_ms"], stats
+        self.pair_sets = None
         self.load(engine, graph)
         t0 = time.perf_counter()
         engine.sync()
@@ -161,6 +173,8 @@ class Grammar:
 
     def results(self) -> dict[str, tuple[int, int]]:
         """name -> (hash, count), in the reference's name order."""
+        if self.pair_sets is not None:
+            return self.pair_results(self.pair_sets, self.vertices_count)
         return {name: self.matrices[idx].fingerprint() for name, idx in sorted(self.nonterminal_to_index.items(),
                                                                                 key=lambda kv: kv[0].encode())}
 

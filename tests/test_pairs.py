@@ -196,3 +196,19 @@ def test_cli_switches_to_pairs_beyond_the_bit_matrices(tmp_path, gen):
     for name, kk in want.items():
         h, c = cf.fingerprint_pairs(kk, n)
         assert got[name] == (str(h), str(c)), name
+
+
+@pytest.mark.gpu
+def test_grammar_pairs_mode_matches_fixpoint_and_oracle(pkg, oracle, gen):
+    q = gen.sg_forest(3000, seed=51, types_per_vertex=2)
+    g, mats, _ = oracle.solve(q, "seminaive")
+    want = oracle.fingerprints(g, mats, q.n)
+    graph = cf.Graph.from_query(q)
+    with pkg.Engine(0) as eng:
+        gr = cf.Grammar(q.grammar, text=True)
+        _, _, st = gr.intersection_with_graph(graph, eng, mode="pairs")
+        assert gr.matrices == [] and st["iterations"] >= 1
+        assert gr.results() == want
+        gr.intersection_with_graph(graph, eng)  # back to the bit matrices
+        assert gr.pair_sets is None and gr.results() == want
+        gr.free()
