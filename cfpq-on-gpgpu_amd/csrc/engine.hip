@@ -1680,7 +1680,7 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
         uint64_t count = 0, nd = 0, ndn = 0;
     };
     std::vector<Rel> rel(nsets);
-    PairBuf cand[2], cnt, start, incl, flags, tmp, words;
+    PairBuf cand[2], cnt, start, incl, flags, tmp, words, sel;
     std::vector<uint32_t> heads;
     for (size_t r = 0; r < nrules; ++r)
         if (std::find(heads.begin(), heads.end(), rules[r].head) == heads.end()) heads.push_back(rules[r].head);
@@ -1697,24 +1697,27 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
         ++syncs;
         return CFPQ_OK;
     };
-    // Sorts cand[0][0, m) (bits below 2s); returns the buffer index holding the result.
-    auto sort_cand = [&](uint64_t m, int &cur) -> int {
+    const uint32_t key_bits = 2 * s;
+    // heads per shared sort: their tags must fit above the key bits
+    const size_t group = std::min<size_t>(PAIRS_GROUP, (size_t)1 << std::min<uint32_t>(64 - key_bits, 5));
+    auto tag_bits = [](size_t g) {
+        uint32_t b = 0;
+        while (((size_t)1 << b) < g) ++b;
+        return b;
+    };
+    // Sorts cand[0][0, m) on bits [0, end_bit); cur := the buffer index holding the result.
+    auto sort_cand = [&](uint64_t m, uint32_t end_bit, int &cur) -> int {
         hipcub::DoubleBuffer<uint64_t> db(cand[0].p, cand[1].p);
         size_t bytes = 0;
-        HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, db, m, 0, (int)(2 * s), st));
+        HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, db, m, 0, (int)end_bit, st));
         CHK(pairs_grow(ctx, tmp, (bytes + 7) / 8));
-        HIPCHK(hipcub::DeviceRadixSort::SortKeys(tmp.p, bytes, db, m, 0, (int)(2 * s), st));
+        HIPCHK(hipcub::DeviceRadixSort::SortKeys(tmp.p, bytes, db, m, 0, (int)end_bit, st));
         cur = db.Current() == cand[0].p ? 0 : 1;
         return CFPQ_OK;
     };
-    // Keys of the sorted c[0, m) that are first of their run and not in t[0, nt),
-    // into out (cap >= m); their number into the device word *kept, which the
-    // host reads after its next wait.
-    auto select_new = [&](const uint64_t *c, uint64_t m, const uint64_t *t, uint64_t nt, uint64_t *out,
-                          uint64_t *kept) -> int {
-        CHK(pairs_grow(ctx, flags, (m + 7) / 8));
-        hipLaunchKernelGGL(k_pairs_flag_new, grid(m), dim3(PAIRS_THREADS), 0, st, c, m, t, nt, (uint8_t *)flags.p);
-        CHK(launch_check(ctx, "pairs flag"));
+    // The keys of c[0, m) whose flags are set, into out (cap >= m); their
+    // number into the device word *kept, which the host reads after its next wait.
+    auto select_flagged = [&](const uint64_t *c, uint64_t m, uint64_t *out, uint64_t *kept) -> int {
         size_t bytes = 0;
         HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, bytes, c, (const uint8_t *)flags.p, out, kept, (int64_t)m, st));
         CHK(pairs_grow(ctx, tmp, (bytes + 7) / 8));
@@ -1742,10 +1745,10 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
         CHK(pairs_grow(ctx, cand[0], m));
         CHK(pairs_grow(ctx, cand[1], m));
         CHK(pairs_grow(ctx, dst, m));
-        hipLaunchKernelGGL(k_pairs_transpose, grid(m), dim3(PAIRS_THREADS), 0, st, keys, cand[0].p, m, s);
+        hipLaunchKernelGGL(k_pairs_transpose, grid(m), dim3(PAIRS_THREADS), 0, st, keys, cand[0].p, m, s, 0ull);
         CHK(launch_check(ctx, "pairs transpose"));
         int cur = 0;
-        CHK(sort_cand(m, cur));
+        CHK(sort_cand(m, key_bits, cur));
         HIPCHK(hipMemcpyAsync(dst.p, cand[cur].p, m * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
         return CFPQ_OK;
     };
@@ -1754,7 +1757,7 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
         CHK(ensure_events(ctx, 4 * (size_t)RING + 2));
         HIPCHK(hipEventRecord(ctx->ev[4 * RING], st));
         // load: pack, sort, drop duplicates (one wait for all sets), build the column order
-        CHK(pairs_grow(ctx, words, std::max<size_t>(nsets, heads.size())));
+        CHK(pairs_grow(ctx, words, std::max<size_t>(nsets, group + 2)));
         for (uint32_t x = 0; x < nsets; ++x) {
             Rel &r = rel[x];
             const uint64_t m = sets[x].count;
@@ -1765,9 +1768,13 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
             hipLaunchKernelGGL(k_pairs_pack, grid(m), dim3(PAIRS_THREADS), 0, st, cand[0].p, m, s);
             CHK(launch_check(ctx, "pairs pack"));
             int cur = 0;
-            CHK(sort_cand(m, cur));
+            CHK(sort_cand(m, key_bits, cur));
             CHK(pairs_grow(ctx, r.m[0], m));
-            CHK(select_new(cand[cur].p, m, nullptr, 0, r.m[0].p, words.p + x));
+            CHK(pairs_grow(ctx, flags, (m + 7) / 8));
+            hipLaunchKernelGGL(k_pairs_flag_new, grid(m), dim3(PAIRS_THREADS), 0, st, (const uint64_t *)cand[cur].p, m,
+                               (const uint64_t *)nullptr, 0ull, (uint8_t *)flags.p);
+            CHK(launch_check(ctx, "pairs flag"));
+            CHK(select_flagged(cand[cur].p, m, r.m[0].p, words.p + x));
         }
         CHK(read_words(words.p, nsets));
         for (uint32_t x = 0; x < nsets; ++x) {
@@ -1784,7 +1791,6 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
         };
         std::vector<Part> parts;
         std::vector<size_t> first(heads.size() + 1);
-        std::vector<char> active(heads.size());
         for (;;) {
             bool fresh = false;
             // join sizes of every head's parts (one wait)
@@ -1814,7 +1820,7 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
                 CHK(pairs_grow(ctx, cnt, sum_m));
                 CHK(pairs_grow(ctx, start, sum_m));
                 CHK(pairs_grow(ctx, incl, sum_m));
-                CHK(pairs_grow(ctx, words, std::max(parts.size(), heads.size())));
+                CHK(pairs_grow(ctx, words, std::max(parts.size(), group + 2)));
                 for (size_t i = 0; i < parts.size(); ++i) {
                     const Part &p = parts[i];
                     hipLaunchKernelGGL(k_pairs_count, grid(p.m), dim3(PAIRS_THREADS), 0, st, p.delta, p.m, p.op, p.nop,
@@ -1829,54 +1835,95 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
                 }
                 CHK(read_words(words.p, parts.size()));
                 const std::vector<uint64_t> totals = host_words;
-                // per head: expand, sort, keep what is new (one wait for all heads)
-                bool any = false;
-                for (size_t hi = 0; hi < heads.size(); ++hi) {
-                    Rel &H = rel[heads[hi]];
-                    active[hi] = 0;
+                // per group of heads: every part's join outputs tagged with its
+                // head, one sort, what is new kept, split by head (one wait)
+                for (size_t g0 = 0; g0 < heads.size(); g0 += group) {
+                    const size_t g1 = std::min(heads.size(), g0 + group);
                     uint64_t total = 0;
-                    for (size_t i = first[hi]; i < first[hi + 1]; ++i) total += totals[i];
+                    for (size_t i = first[g0]; i < first[g1]; ++i) total += totals[i];
                     if (!total) continue;
                     joined += total;
                     CHK(pairs_grow(ctx, cand[0], total));
                     CHK(pairs_grow(ctx, cand[1], total));
+                    PairHeads tab{};
                     uint64_t off = 0;
-                    for (size_t i = first[hi]; i < first[hi + 1]; ++i) {
-                        const Part &p = parts[i];
-                        if (!totals[i]) continue;
-                        hipLaunchKernelGGL(k_pairs_expand, grid(totals[i]), dim3(PAIRS_THREADS), 0, st, p.delta,
-                                           (const uint64_t *)incl.p + p.at, (const uint64_t *)start.p + p.at, p.m, p.op,
-                                           s, p.side, totals[i], cand[0].p + off);
-                        CHK(launch_check(ctx, "pairs expand"));
-                        ++launches;
-                        off += totals[i];
+                    for (size_t hi = g0; hi < g1; ++hi) {
+                        tab.m[hi - g0] = rel[heads[hi]].m[0].p;
+                        tab.n[hi - g0] = rel[heads[hi]].count;
+                        const uint64_t tag = (uint64_t)(hi - g0) << key_bits;
+                        for (size_t i = first[hi]; i < first[hi + 1]; ++i) {
+                            const Part &p = parts[i];
+                            if (!totals[i]) continue;
+                            hipLaunchKernelGGL(k_pairs_expand, grid(totals[i]), dim3(PAIRS_THREADS), 0, st, p.delta,
+                                               (const uint64_t *)incl.p + p.at, (const uint64_t *)start.p + p.at, p.m,
+                                               p.op, s, p.side, totals[i], tag, cand[0].p + off);
+                            CHK(launch_check(ctx, "pairs expand"));
+                            ++launches;
+                            off += totals[i];
+                        }
                     }
                     int cur = 0;
-                    CHK(sort_cand(total, cur));
-                    CHK(pairs_grow(ctx, H.dn, total));
-                    CHK(select_new(cand[cur].p, total, H.m[0].p, H.count, H.dn.p, words.p + hi));
-                    active[hi] = 1;
-                    any = true;
-                }
-                if (any) {
-                    CHK(read_words(words.p, heads.size()));
-                    for (size_t hi = 0; hi < heads.size(); ++hi) {
-                        rel[heads[hi]].ndn = active[hi] ? host_words[hi] : 0;
-                        fresh = fresh || rel[heads[hi]].ndn;
+                    CHK(sort_cand(total, key_bits + tag_bits(g1 - g0), cur));
+                    CHK(pairs_grow(ctx, flags, (total + 7) / 8));
+                    hipLaunchKernelGGL(k_pairs_flag_new_group, grid(total), dim3(PAIRS_THREADS), 0, st,
+                                       (const uint64_t *)cand[cur].p, total, tab, key_bits, (uint8_t *)flags.p);
+                    CHK(launch_check(ctx, "pairs flag"));
+                    CHK(pairs_grow(ctx, sel, total));
+                    CHK(select_flagged(cand[cur].p, total, sel.p, words.p));
+                    hipLaunchKernelGGL(k_pairs_group_bounds, dim3(1), dim3(64), 0, st, (const uint64_t *)sel.p,
+                                       (const uint64_t *)words.p, key_bits, (uint32_t)(g1 - g0), words.p + 1);
+                    CHK(launch_check(ctx, "pairs bounds"));
+                    CHK(read_words(words.p, g1 - g0 + 2));
+                    for (size_t hi = g0; hi < g1; ++hi) {
+                        const uint64_t b = host_words[1 + hi - g0], e = host_words[2 + hi - g0];
+                        Rel &H = rel[heads[hi]];
+                        H.ndn = e - b;
+                        if (!H.ndn) continue;
+                        CHK(pairs_grow(ctx, H.dn, H.ndn));
+                        hipLaunchKernelGGL(k_pairs_untag, grid(H.ndn), dim3(PAIRS_THREADS), 0, st,
+                                           (const uint64_t *)sel.p + b, H.dn.p, H.ndn, key_bits);
+                        CHK(launch_check(ctx, "pairs untag"));
+                        fresh = true;
                     }
                 }
             }
             ++steps;
             // every head's new pairs were found against the previous step's
-            // relations; only now are they merged in
-            for (uint32_t h : heads) {
-                Rel &H = rel[h];
-                if (!H.ndn) continue;
-                added += H.ndn;
-                CHK(merge_into(H.m, H.count, H.dn.p, H.ndn));
-                CHK(build_transpose(H.dn.p, H.ndn, H.d));  // H.d is free until the swap below
-                CHK(merge_into(H.mt, H.count, H.d.p, H.ndn));
-                H.count += H.ndn;
+            // relations; only now are they merged in, the column copies
+            // through one tagged sort per group
+            for (size_t g0 = 0; g0 < heads.size(); g0 += group) {
+                const size_t g1 = std::min(heads.size(), g0 + group);
+                uint64_t total = 0;
+                for (size_t hi = g0; hi < g1; ++hi) total += rel[heads[hi]].ndn;
+                if (!total) continue;
+                CHK(pairs_grow(ctx, cand[0], total));
+                CHK(pairs_grow(ctx, cand[1], total));
+                uint64_t off = 0;
+                for (size_t hi = g0; hi < g1; ++hi) {
+                    Rel &H = rel[heads[hi]];
+                    if (!H.ndn) continue;
+                    added += H.ndn;
+                    CHK(merge_into(H.m, H.count, H.dn.p, H.ndn));
+                    hipLaunchKernelGGL(k_pairs_transpose, grid(H.ndn), dim3(PAIRS_THREADS), 0, st,
+                                       (const uint64_t *)H.dn.p, cand[0].p + off, H.ndn, s,
+                                       (uint64_t)(hi - g0) << key_bits);
+                    CHK(launch_check(ctx, "pairs transpose"));
+                    off += H.ndn;
+                }
+                int cur = 0;
+                CHK(sort_cand(total, key_bits + tag_bits(g1 - g0), cur));
+                off = 0;
+                for (size_t hi = g0; hi < g1; ++hi) {
+                    Rel &H = rel[heads[hi]];
+                    if (!H.ndn) continue;
+                    CHK(pairs_grow(ctx, H.d, H.ndn));  // H.d is free until the swap below
+                    hipLaunchKernelGGL(k_pairs_untag, grid(H.ndn), dim3(PAIRS_THREADS), 0, st,
+                                       (const uint64_t *)cand[cur].p + off, H.d.p, H.ndn, key_bits);
+                    CHK(launch_check(ctx, "pairs untag"));
+                    CHK(merge_into(H.mt, H.count, H.d.p, H.ndn));
+                    H.count += H.ndn;
+                    off += H.ndn;
+                }
             }
             for (Rel &r : rel) {
                 std::swap(r.d, r.dn);
@@ -1916,7 +1963,7 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
     for (Rel &r : rel) {
         drop(r.m[0]), drop(r.m[1]), drop(r.mt[0]), drop(r.mt[1]), drop(r.d), drop(r.dn);
     }
-    drop(cand[0]), drop(cand[1]), drop(cnt), drop(start), drop(incl), drop(flags), drop(tmp), drop(words);
+    drop(cand[0]), drop(cand[1]), drop(cnt), drop(start), drop(incl), drop(flags), drop(tmp), drop(words), drop(sel);
     if (rc == CFPQ_OK && stats) {
         stats->iterations = steps;
         stats->new_bits = added;

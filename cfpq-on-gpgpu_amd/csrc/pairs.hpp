@@ -23,6 +23,14 @@ namespace cfpq {
 
 constexpr uint32_t PAIRS_THREADS = 256;
 
+// Up to PAIRS_GROUP heads share one sort per step: a key then carries its
+// head's index within the group above its 2s key bits (the tag).
+constexpr int PAIRS_GROUP = 32;
+struct PairHeads {
+    const uint64_t *m[PAIRS_GROUP];  // each head's sorted pairs
+    uint64_t n[PAIRS_GROUP];
+};
+
 #define PAIRS_FOR(i_, total_)                                                             \
     for (uint64_t i_ = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i_ < (total_); \
          i_ += (uint64_t)gridDim.x * blockDim.x)
@@ -49,10 +57,18 @@ __global__ void k_pairs_unpack(const uint64_t *__restrict__ in, uint64_t *__rest
     PAIRS_FOR(e, m) out[e] = (in[e] >> s) << 32 | (in[e] & mask);
 }
 
-// (i, j) -> (j, i)
-__global__ void k_pairs_transpose(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t m, uint32_t s) {
+// (i, j) -> tag | (j, i)
+__global__ void k_pairs_transpose(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t m, uint32_t s,
+                                  uint64_t tag) {
     const uint64_t mask = (1ull << s) - 1;
-    PAIRS_FOR(e, m) out[e] = (in[e] & mask) << s | in[e] >> s;
+    PAIRS_FOR(e, m) out[e] = tag | (in[e] & mask) << s | in[e] >> s;
+}
+
+// Tagged keys back to plain ones.
+__global__ void k_pairs_untag(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t m,
+                              uint32_t key_bits) {
+    const uint64_t mask = (1ull << key_bits) - 1;
+    PAIRS_FOR(e, m) out[e] = in[e] & mask;
 }
 
 // For every delta pair e, the run of the operand holding its join vertex k:
@@ -80,7 +96,7 @@ __global__ void k_pairs_count(const uint64_t *__restrict__ delta, uint64_t m, co
 // threads as it produces pairs.
 __global__ void k_pairs_expand(const uint64_t *__restrict__ delta, const uint64_t *__restrict__ incl,
                                const uint64_t *__restrict__ start, uint64_t m, const uint64_t *__restrict__ op,
-                               uint32_t s, int side, uint64_t total, uint64_t *__restrict__ out) {
+                               uint32_t s, int side, uint64_t total, uint64_t tag, uint64_t *__restrict__ out) {
     const uint64_t mask = (1ull << s) - 1;
     PAIRS_FOR(p, total) {
         uint64_t lo = 0, hi = m;
@@ -96,7 +112,7 @@ __global__ void k_pairs_expand(const uint64_t *__restrict__ delta, const uint64_
         const uint64_t o = op[start[e] + (p - before)];
         const uint64_t d = delta[e];
         // side 0: d = i << s | k, o = k << s | j;  side 1: d = k << s | j, o = k << s | i
-        out[p] = side ? (o & mask) << s | (d & mask) : (d & ~mask) | (o & mask);
+        out[p] = tag | (side ? (o & mask) << s | (d & mask) : (d & ~mask) | (o & mask));
     }
 }
 
@@ -113,6 +129,37 @@ __global__ void k_pairs_flag_new(const uint64_t *__restrict__ c, uint64_t nc, co
         }
         flag[p] = keep ? 1 : 0;
     }
+}
+
+// The same over tagged candidates: the set a candidate is tested against is
+// its head's, named by the tag.
+__global__ void k_pairs_flag_new_group(const uint64_t *__restrict__ c, uint64_t nc, PairHeads heads,
+                                       uint32_t key_bits, uint8_t *__restrict__ flag) {
+    const uint64_t mask = (1ull << key_bits) - 1;
+    PAIRS_FOR(p, nc) {
+        const uint64_t x = c[p];
+        bool keep = p == 0 || c[p - 1] != x;
+        if (keep) {
+            const uint32_t g = (uint32_t)(x >> key_bits);
+            const uint64_t *t = heads.m[g];
+            const uint64_t nt = heads.n[g], key = x & mask;
+            if (nt) {
+                const uint64_t at = pairs_lower_bound(t, nt, key);
+                keep = at == nt || t[at] != key;
+            }
+        }
+        flag[p] = keep ? 1 : 0;
+    }
+}
+
+// bounds[g] = first index of tag g in the sorted tagged keys k[0, *count) for
+// g < groups, bounds[groups] = *count.  One block of at least groups + 1 threads.
+__global__ void k_pairs_group_bounds(const uint64_t *__restrict__ k, const uint64_t *__restrict__ count,
+                                     uint32_t key_bits, uint32_t groups, uint64_t *__restrict__ bounds) {
+    const uint32_t g = threadIdx.x;
+    if (g > groups) return;
+    const uint64_t m = *count;
+    bounds[g] = g == groups ? m : pairs_lower_bound(k, m, (uint64_t)g << key_bits);
 }
 
 // Merge of two sorted arrays with no key in common: an element's place in the
