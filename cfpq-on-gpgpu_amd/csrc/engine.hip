@@ -1680,7 +1680,7 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
         uint64_t count = 0, nd = 0, ndn = 0;
     };
     std::vector<Rel> rel(nsets);
-    PairBuf cand[2], cnt, start, incl, flags, tmp, words, sel;
+    PairBuf cand[2], cnt, start, incl, flags, tmp, words, sel, dparts;
     std::vector<uint32_t> heads;
     for (size_t r = 0; r < nrules; ++r)
         if (std::find(heads.begin(), heads.end(), rules[r].head) == heads.end()) heads.push_back(rules[r].head);
@@ -1705,14 +1705,15 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
         while (((size_t)1 << b) < g) ++b;
         return b;
     };
-    // Sorts cand[0][0, m) on bits [0, end_bit); cur := the buffer index holding the result.
-    auto sort_cand = [&](uint64_t m, uint32_t end_bit, int &cur) -> int {
-        hipcub::DoubleBuffer<uint64_t> db(cand[0].p, cand[1].p);
+    // Sorts cand[0][off, off + m) on bits [0, end_bit); cur := the buffer
+    // index holding the result (at the same offset).
+    auto sort_cand = [&](uint64_t m, uint32_t end_bit, int &cur, uint64_t off = 0) -> int {
+        hipcub::DoubleBuffer<uint64_t> db(cand[0].p + off, cand[1].p + off);
         size_t bytes = 0;
         HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, db, m, 0, (int)end_bit, st));
         CHK(pairs_grow(ctx, tmp, (bytes + 7) / 8));
         HIPCHK(hipcub::DeviceRadixSort::SortKeys(tmp.p, bytes, db, m, 0, (int)end_bit, st));
-        cur = db.Current() == cand[0].p ? 0 : 1;
+        cur = db.Current() == cand[0].p + off ? 0 : 1;
         return CFPQ_OK;
     };
     // The keys of c[0, m) whose flags are set, into out (cap >= m); their
@@ -1781,23 +1782,17 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
             rel[x].count = sets[x].count ? host_words[x] : 0;
             if (rel[x].count) CHK(build_transpose(rel[x].m[0].p, rel[x].count, rel[x].mt[0]));
         }
-        struct Part {
-            const uint64_t *delta;
-            uint64_t m;
-            const uint64_t *op;
-            uint64_t nop;
-            int side;
-            uint64_t at;  // offset of its cnt/start/incl entries
-        };
-        std::vector<Part> parts;
+        std::vector<PairPart> parts;
         std::vector<size_t> first(heads.size() + 1);
         for (;;) {
             bool fresh = false;
-            // join sizes of every head's parts (one wait)
+            // every head's joins, in head order, each output tagged with its
+            // head's index in its group
             parts.clear();
-            uint64_t sum_m = 0;
+            uint64_t sum_m = 0, max_m = 0;
             for (size_t hi = 0; hi < heads.size(); ++hi) {
                 first[hi] = parts.size();
+                const uint64_t tag = (uint64_t)(hi % group) << key_bits;
                 for (size_t i = 0; i < nrules; ++i) {
                     if (rules[i].head != heads[hi]) continue;
                     Rel &L = rel[rules[i].left], &R = rel[rules[i].right];
@@ -1805,71 +1800,77 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
                     const uint64_t *dl = steps ? L.d.p : L.m[0].p, *dr = steps ? R.d.p : R.m[0].p;
                     const uint64_t ndl = steps ? L.nd : L.count, ndr = steps ? R.nd : R.count;
                     if (ndl && R.count) {
-                        parts.push_back({dl, ndl, R.m[0].p, R.count, 0, sum_m});
+                        parts.push_back({dl, R.m[0].p, ndl, R.count, sum_m, tag, 0, 0});
                         sum_m += ndl;
+                        max_m = std::max(max_m, ndl);
                     }
                     if (steps && ndr && L.count) {
-                        parts.push_back({dr, ndr, L.mt[0].p, L.count, 1, sum_m});
+                        parts.push_back({dr, L.mt[0].p, ndr, L.count, sum_m, tag, 1, 0});
                         sum_m += ndr;
+                        max_m = std::max(max_m, ndr);
                     }
                 }
             }
             first[heads.size()] = parts.size();
             for (uint32_t h : heads) rel[h].ndn = 0;
             if (!parts.empty()) {
+                const uint32_t np = (uint32_t)parts.size();
+                if (np > 65535) return fail(ctx, CFPQ_E_INVALID, "cfpq_pairs_fixpoint: %u joins in one step", np);
                 CHK(pairs_grow(ctx, cnt, sum_m));
                 CHK(pairs_grow(ctx, start, sum_m));
                 CHK(pairs_grow(ctx, incl, sum_m));
-                CHK(pairs_grow(ctx, words, std::max(parts.size(), group + 2)));
-                for (size_t i = 0; i < parts.size(); ++i) {
-                    const Part &p = parts[i];
-                    hipLaunchKernelGGL(k_pairs_count, grid(p.m), dim3(PAIRS_THREADS), 0, st, p.delta, p.m, p.op, p.nop,
-                                       s, p.side, cnt.p + p.at, start.p + p.at);
-                    CHK(launch_check(ctx, "pairs count"));
-                    size_t bytes = 0;
-                    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, cnt.p + p.at, incl.p + p.at, p.m, st));
-                    CHK(pairs_grow(ctx, tmp, (bytes + 7) / 8));
-                    HIPCHK(hipcub::DeviceScan::InclusiveSum(tmp.p, bytes, cnt.p + p.at, incl.p + p.at, p.m, st));
-                    HIPCHK(hipMemcpyAsync(words.p + i, incl.p + p.at + p.m - 1, sizeof(uint64_t),
-                                          hipMemcpyDeviceToDevice, st));
-                }
-                CHK(read_words(words.p, parts.size()));
+                CHK(pairs_grow(ctx, words, std::max<size_t>(np, group + 2)));
+                CHK(pairs_grow(ctx, dparts, (np * sizeof(PairPart) + 7) / 8));
+                // `parts` is left alone until the wait below, so the copy may read it late
+                HIPCHK(hipMemcpyAsync(dparts.p, parts.data(), np * sizeof(PairPart), hipMemcpyHostToDevice, st));
+                const PairPart *dp = (const PairPart *)dparts.p;
+                hipLaunchKernelGGL(k_pairs_count_parts, dim3(grid(max_m).x, np), dim3(PAIRS_THREADS), 0, st, dp, s, cnt.p,
+                                   start.p);
+                CHK(launch_check(ctx, "pairs count"));
+                size_t bytes = 0;
+                HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, cnt.p, incl.p, sum_m, st));
+                CHK(pairs_grow(ctx, tmp, (bytes + 7) / 8));
+                HIPCHK(hipcub::DeviceScan::InclusiveSum(tmp.p, bytes, cnt.p, incl.p, sum_m, st));
+                hipLaunchKernelGGL(k_pairs_part_totals, grid(np), dim3(PAIRS_THREADS), 0, st, dp, np,
+                                   (const uint64_t *)incl.p, words.p);
+                CHK(launch_check(ctx, "pairs totals"));
+                CHK(read_words(words.p, np));
                 const std::vector<uint64_t> totals = host_words;
-                // per group of heads: every part's join outputs tagged with its
-                // head, one sort, what is new kept, split by head (one wait)
+                uint64_t total_all = 0;
+                for (uint64_t t : totals) total_all += t;
+                if (total_all) {
+                    joined += total_all;
+                    CHK(pairs_grow(ctx, cand[0], total_all));
+                    CHK(pairs_grow(ctx, cand[1], total_all));
+                    hipLaunchKernelGGL(k_pairs_expand_parts, grid(total_all), dim3(PAIRS_THREADS), 0, st, dp, np,
+                                       (const uint64_t *)incl.p, (const uint64_t *)start.p, sum_m, s, total_all,
+                                       cand[0].p);
+                    CHK(launch_check(ctx, "pairs expand"));
+                    ++launches;
+                }
+                // per group of heads: its outputs are contiguous; one sort, what
+                // is new kept, split by head (one wait)
+                uint64_t off = 0;
                 for (size_t g0 = 0; g0 < heads.size(); g0 += group) {
                     const size_t g1 = std::min(heads.size(), g0 + group);
                     uint64_t total = 0;
                     for (size_t i = first[g0]; i < first[g1]; ++i) total += totals[i];
                     if (!total) continue;
-                    joined += total;
-                    CHK(pairs_grow(ctx, cand[0], total));
-                    CHK(pairs_grow(ctx, cand[1], total));
                     PairHeads tab{};
-                    uint64_t off = 0;
                     for (size_t hi = g0; hi < g1; ++hi) {
                         tab.m[hi - g0] = rel[heads[hi]].m[0].p;
                         tab.n[hi - g0] = rel[heads[hi]].count;
-                        const uint64_t tag = (uint64_t)(hi - g0) << key_bits;
-                        for (size_t i = first[hi]; i < first[hi + 1]; ++i) {
-                            const Part &p = parts[i];
-                            if (!totals[i]) continue;
-                            hipLaunchKernelGGL(k_pairs_expand, grid(totals[i]), dim3(PAIRS_THREADS), 0, st, p.delta,
-                                               (const uint64_t *)incl.p + p.at, (const uint64_t *)start.p + p.at, p.m,
-                                               p.op, s, p.side, totals[i], tag, cand[0].p + off);
-                            CHK(launch_check(ctx, "pairs expand"));
-                            ++launches;
-                            off += totals[i];
-                        }
                     }
                     int cur = 0;
-                    CHK(sort_cand(total, key_bits + tag_bits(g1 - g0), cur));
+                    CHK(sort_cand(total, key_bits + tag_bits(g1 - g0), cur, off));
+                    const uint64_t *sorted = cand[cur].p + off;
+                    off += total;
                     CHK(pairs_grow(ctx, flags, (total + 7) / 8));
-                    hipLaunchKernelGGL(k_pairs_flag_new_group, grid(total), dim3(PAIRS_THREADS), 0, st,
-                                       (const uint64_t *)cand[cur].p, total, tab, key_bits, (uint8_t *)flags.p);
+                    hipLaunchKernelGGL(k_pairs_flag_new_group, grid(total), dim3(PAIRS_THREADS), 0, st, sorted, total,
+                                       tab, key_bits, (uint8_t *)flags.p);
                     CHK(launch_check(ctx, "pairs flag"));
                     CHK(pairs_grow(ctx, sel, total));
-                    CHK(select_flagged(cand[cur].p, total, sel.p, words.p));
+                    CHK(select_flagged(sorted, total, sel.p, words.p));
                     hipLaunchKernelGGL(k_pairs_group_bounds, dim3(1), dim3(64), 0, st, (const uint64_t *)sel.p,
                                        (const uint64_t *)words.p, key_bits, (uint32_t)(g1 - g0), words.p + 1);
                     CHK(launch_check(ctx, "pairs bounds"));
@@ -1963,7 +1964,7 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
     for (Rel &r : rel) {
         drop(r.m[0]), drop(r.m[1]), drop(r.mt[0]), drop(r.mt[1]), drop(r.d), drop(r.dn);
     }
-    drop(cand[0]), drop(cand[1]), drop(cnt), drop(start), drop(incl), drop(flags), drop(tmp), drop(words), drop(sel);
+    drop(cand[0]), drop(cand[1]), drop(cnt), drop(start), drop(incl), drop(flags), drop(tmp), drop(words), drop(sel), drop(dparts);
     if (rc == CFPQ_OK && stats) {
         stats->iterations = steps;
         stats->new_bits = added;

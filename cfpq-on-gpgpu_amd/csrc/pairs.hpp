@@ -64,6 +64,18 @@ __global__ void k_pairs_transpose(const uint64_t *__restrict__ in, uint64_t *__r
     PAIRS_FOR(e, m) out[e] = tag | (in[e] & mask) << s | in[e] >> s;
 }
 
+// One join of a step: the delta pairs of one operand against the other
+// operand's rows.  Every part of a step is evaluated by the same launches;
+// its cnt/start/incl entries sit at [at, at + m) of the step's arrays.
+struct PairPart {
+    const uint64_t *delta;  // side 0: dL (i, k); side 1: dR (k, j)
+    const uint64_t *op;     // side 0: M_R by row; side 1: M_L by column
+    uint64_t m, nop;
+    uint64_t at;
+    uint64_t tag;  // head tag of its outputs
+    uint32_t side, pad;
+};
+
 // Tagged keys back to plain ones.
 __global__ void k_pairs_untag(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t m,
                               uint32_t key_bits) {
@@ -71,35 +83,48 @@ __global__ void k_pairs_untag(const uint64_t *__restrict__ in, uint64_t *__restr
     PAIRS_FOR(e, m) out[e] = in[e] & mask;
 }
 
-// For every delta pair e, the run of the operand holding its join vertex k:
+// For every delta pair e of a part, the run of the operand holding its join
+// vertex k:
 //   side 0, delta of the left operand (i, k): row k of the right operand M_R;
 //   side 1, delta of the right operand (k, j): column k of the left operand,
 //           i.e. row k of M_L^T.
-// cnt[e] = length of the run, start[e] = its first index in the operand.
-__global__ void k_pairs_count(const uint64_t *__restrict__ delta, uint64_t m, const uint64_t *__restrict__ op,
-                              uint64_t nop, uint32_t s, int side, uint64_t *__restrict__ cnt,
-                              uint64_t *__restrict__ start) {
+// cnt = length of the run, start = its first index in the operand.  Every
+// part of a step in one launch: block row blockIdx.y is part y.
+__global__ void k_pairs_count_parts(const PairPart *__restrict__ parts, uint32_t s, uint64_t *__restrict__ cnt,
+                                    uint64_t *__restrict__ start) {
+    const PairPart P = parts[blockIdx.y];
     const uint64_t mask = (1ull << s) - 1;
-    PAIRS_FOR(e, m) {
-        const uint64_t key = delta[e];
-        const uint64_t k = side ? key >> s : key & mask;
-        const uint64_t lo = pairs_lower_bound(op, nop, k << s);
-        const uint64_t hi = lo + pairs_lower_bound(op + lo, nop - lo, (k + 1) << s);
-        cnt[e] = hi - lo;
-        start[e] = lo;
+    PAIRS_FOR(e, P.m) {
+        const uint64_t key = P.delta[e];
+        const uint64_t k = P.side ? key >> s : key & mask;
+        const uint64_t lo = pairs_lower_bound(P.op, P.nop, k << s);
+        const uint64_t hi = lo + pairs_lower_bound(P.op + lo, P.nop - lo, (k + 1) << s);
+        cnt[P.at + e] = hi - lo;
+        start[P.at + e] = lo;
     }
 }
 
-// Output p of the join: delta pair e = the first with incl[e] > p (incl is the
-// inclusive prefix sum of cnt), operand pair start[e] + (p - incl[e - 1]).
+// totals[i] = join outputs of part i; incl is the inclusive scan of the
+// concatenated cnt of every part.
+__global__ void k_pairs_part_totals(const PairPart *__restrict__ parts, uint32_t np, const uint64_t *__restrict__ incl,
+                                    uint64_t *__restrict__ totals) {
+    PAIRS_FOR(i, np) {
+        const uint64_t a = parts[i].at, m = parts[i].m;
+        totals[i] = incl[a + m - 1] - (a ? incl[a - 1] : 0);
+    }
+}
+
+// Output p of all of a step's joins, in part order: delta pair e = the first
+// with incl[e] > p over the concatenated parts, its part the last one
+// starting at or before e, and the operand pair start[e] + (p - incl[e - 1]).
 // One thread per output, so a hub vertex's long run is spread over as many
 // threads as it produces pairs.
-__global__ void k_pairs_expand(const uint64_t *__restrict__ delta, const uint64_t *__restrict__ incl,
-                               const uint64_t *__restrict__ start, uint64_t m, const uint64_t *__restrict__ op,
-                               uint32_t s, int side, uint64_t total, uint64_t tag, uint64_t *__restrict__ out) {
+__global__ void k_pairs_expand_parts(const PairPart *__restrict__ parts, uint32_t np,
+                                     const uint64_t *__restrict__ incl, const uint64_t *__restrict__ start,
+                                     uint64_t sum_m, uint32_t s, uint64_t total, uint64_t *__restrict__ out) {
     const uint64_t mask = (1ull << s) - 1;
     PAIRS_FOR(p, total) {
-        uint64_t lo = 0, hi = m;
+        uint64_t lo = 0, hi = sum_m;
         while (lo < hi) {
             const uint64_t mid = (lo + hi) >> 1;
             if (incl[mid] <= p)
@@ -108,11 +133,19 @@ __global__ void k_pairs_expand(const uint64_t *__restrict__ delta, const uint64_
                 hi = mid;
         }
         const uint64_t e = lo;
+        uint32_t a = 0, b = np;  // parts[a].at <= e < parts[b].at
+        while (b - a > 1) {
+            const uint32_t mid = (a + b) >> 1;
+            if (parts[mid].at <= e)
+                a = mid;
+            else
+                b = mid;
+        }
+        const PairPart &P = parts[a];
         const uint64_t before = e ? incl[e - 1] : 0;
-        const uint64_t o = op[start[e] + (p - before)];
-        const uint64_t d = delta[e];
-        // side 0: d = i << s | k, o = k << s | j;  side 1: d = k << s | j, o = k << s | i
-        out[p] = tag | (side ? (o & mask) << s | (d & mask) : (d & ~mask) | (o & mask));
+        const uint64_t o = P.op[start[e] + (p - before)];
+        const uint64_t d = P.delta[e - P.at];
+        out[p] = P.tag | (P.side ? (o & mask) << s | (d & mask) : (d & ~mask) | (o & mask));
     }
 }
 
