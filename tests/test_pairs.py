@@ -212,3 +212,45 @@ def test_grammar_pairs_mode_matches_fixpoint_and_oracle(pkg, oracle, gen):
         gr.intersection_with_graph(graph, eng)  # back to the bit matrices
         assert gr.pair_sets is None and gr.results() == want
         gr.free()
+
+
+@pytest.mark.gpu
+def test_pairs_fixpoint_abi_capacity_and_iteration_cap(pkg):
+    """CFPQ_E_CAPACITY through the C ABI: no keys written, every count set to
+    the size needed; the iteration cap gives CFPQ_E_LIMIT."""
+    import ctypes
+    n = 64
+    a = np.arange(n - 1, dtype=np.uint64)
+    path = a << np.uint64(32) | (a + np.uint64(1))
+    closed = n * (n - 1) // 2
+    with pkg.Engine(0) as eng:
+        buf = np.full(100, 7, dtype=np.uint64)
+        buf[: n - 1] = path
+        other = np.array([5, 3, 5], dtype=np.uint64)  # a set no rule writes: (0, 5), (0, 3) twice
+        ps = (pkg.Pairs * 2)()
+        ps[0].keys = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        ps[0].count, ps[0].cap = n - 1, len(buf)
+        ps[1].keys = other.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        ps[1].count, ps[1].cap = 3, 3
+        rules = (pkg.Rule * 1)(pkg.Rule(0, 0, 0))
+        st = pkg.Stats()
+        rc = eng._L.cfpq_pairs_fixpoint(eng._h, rules, 1, ps, 2, n, ctypes.byref(st))
+        assert rc == pkg.E_CAPACITY
+        assert ps[0].count == closed and ps[1].count == 2
+        assert np.array_equal(buf[: n - 1], path) and (buf[n - 1:] == 7).all()
+        assert other.tolist() == [5, 3, 5]
+        # the same call with room succeeds
+        big = np.zeros(closed, dtype=np.uint64)
+        big[: n - 1] = path
+        ps[0].keys = big.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        ps[0].count, ps[0].cap = n - 1, closed
+        ps[1].count = 3
+        rc = eng._L.cfpq_pairs_fixpoint(eng._h, rules, 1, ps, 2, n, ctypes.byref(st))
+        assert rc == 0 and ps[0].count == closed and ps[1].count == 2
+        i, j = np.triu_indices(n, 1)
+        assert np.array_equal(big, np.sort(i.astype(np.uint64) << np.uint64(32) | j.astype(np.uint64)))
+        assert other[:2].tolist() == [3, 5]
+    with pkg.Engine(0, max_iterations=2) as eng:
+        with pytest.raises(pkg.CfpqError) as err:
+            eng.pairs_fixpoint([(0, 0, 0)], [path], n)
+        assert err.value.code == pkg.E_LIMIT
