@@ -146,6 +146,31 @@ def measure_worklist(args, eng, gr, mats, init, want, opts):
     return statistics.median(times), stats, ok
 
 
+def measure_pairs(args, eng, gr, q, want):
+    """The sparse pair-list formulation (cfpq_pairs_fixpoint) on the same
+    workload and context: host pair arrays in and out."""
+    keys = gr.pair_keys(cf_module().Graph.from_query(q))
+    rules = gr.rule_triples()
+    out, _ = eng.pairs_fixpoint(rules, keys, q.n)  # warm-up, sizes the output arrays
+    cap = max(len(k) for k in out) if out else None
+    times, stats = [], []
+    for _ in range(max(1, min(args.steps, 3))):
+        t0 = time.perf_counter()
+        out, st = eng.pairs_fixpoint(rules, keys, q.n, cap=cap)
+        times.append((time.perf_counter() - t0) * 1e3)
+        stats.append(st)
+    ok = gr.pair_results(out, q.n) == want
+    if not ok:
+        log("[bench] pairs MISMATCH vs oracle fixture")
+    log(f"[bench] pairs: {statistics.median(times):.2f} ms per call, "
+        f"{statistics.median(s['fixpoint_ms'] for s in stats):.2f} ms device")
+    return statistics.median(times), stats, ok, int(sum(len(k) for k in out))
+
+
+def cf_module():
+    return importlib.import_module("cfpq-on-gpgpu_amd.cfpq")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +198,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler engine")
     ap.add_argument("--no-worklist", action="store_true",
                     help="skip the reference-structured engine (worklist of add_mul calls)")
+    ap.add_argument("--no-pairs", action="store_true", help="skip the pair-list formulation's leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (torch), the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -285,6 +311,7 @@ def main():
         runs["learned"] = measure("learned", lopts)
     if not args.no_worklist:
         runs["worklist"] = measure_worklist(args, eng, gr, mats, init, want, opts)
+    pairs_run = None if args.no_pairs else measure_pairs(args, eng, gr, q, want)
     ok_runs = {k: v for k, v in runs.items() if v[2] and k != "worklist"}
     chosen = min(ok_runs, key=lambda k: ok_runs[k][0]) if ok_runs else "plain"
     ms_per_step, stats, bit_exact = runs[chosen]
@@ -307,6 +334,13 @@ def main():
         engines["learned_vs_plain_speedup"] = round(runs["plain"][0] / runs["learned"][0], 4)
     if "worklist" in runs:
         engines["plain_vs_worklist_speedup"] = round(runs["worklist"][0] / runs["plain"][0], 3)
+    if pairs_run is not None:
+        p_ms, p_stats, p_ok, p_pairs = pairs_run
+        engines["pairs"] = {"ms_per_call": round(p_ms, 4), "fixpoint_ms_device_median": round(
+            statistics.median(x["fixpoint_ms"] for x in p_stats), 4), "iterations": p_stats[-1]["iterations"],
+            "pairs": p_pairs, "runs": len(p_stats), "bit_exact": p_ok,
+            "note": "sparse pair-list formulation; ms_per_call includes host arrays both ways"}
+        bit_exact = bit_exact and p_ok
 
     roofline = product_roofline(stats, wname)
 
