@@ -155,3 +155,14 @@ def test_harness_with_engine_cli(tmp_path, gen, oracle):
         for r in runners:
             assert harness.read_answer(str(tmp_path / "cross_val" / f"{test}_{r.name}.txt")) == want, (test, r.name)
             assert len(res[test][r.name]) == 2
+
+
+def test_cli_rejects_unknown_options(tmp_path):
+    """The CLI's argument check runs before any device call."""
+    import subprocess
+    if not os.path.exists(harness.CLI):
+        pytest.skip("cfpq_hip is not built")
+    r = subprocess.run([harness.CLI, "g", "m", "o", "--bogus"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "unknown option --bogus" in r.stderr
+    r = subprocess.run([harness.CLI, "g"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
