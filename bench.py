@@ -305,7 +305,9 @@ def main():
     weights = None if args.no_learned else sched_mod.load()
     if weights is not None:
         eng.sched_set_weights(weights)
-        lopts = dict(opts, sched=True)
+        # convergence of the learned engine is certified by one exact full
+        # pass over every rule, which must find no new bit
+        lopts = dict(opts, sched=True, final_full_pass=True)
         if args.sched_threshold:
             lopts["sched_threshold"] = args.sched_threshold
         runs["learned"] = measure("learned", lopts)

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "cfpq_hip.h"
@@ -1204,6 +1205,22 @@ extern "C" int cfpq_batch_fixpoint(cfpq_ctx *ctx, const cfpq_query *queries, siz
     if (nqueries == 0) return CFPQ_OK;
     for (size_t qi = 0; qi < nqueries; ++qi)
         CHK(check_query(ctx, queries[qi].rules, queries[qi].nrules, queries[qi].mats, queries[qi].nmats));
+    if (nqueries > 1) {
+        // a matrix that a query writes (a rule head) must belong to that query
+        // alone: another query reading it would never see its new bits in its
+        // own delta buffers.  Read-only operands may be shared.
+        std::unordered_map<const cfpq_mat *, size_t> writer;
+        for (size_t qi = 0; qi < nqueries; ++qi)
+            for (size_t r = 0; r < queries[qi].nrules; ++r) writer.emplace(queries[qi].mats[queries[qi].rules[r].head], qi);
+        for (size_t qi = 0; qi < nqueries; ++qi)
+            for (size_t x = 0; x < queries[qi].nmats; ++x) {
+                auto it = writer.find(queries[qi].mats[x]);
+                if (it != writer.end() && it->second != qi)
+                    return fail(ctx, CFPQ_E_INVALID,
+                                "batch_fixpoint: matrix %zu of query %zu is a rule head of query %zu (a matrix a "
+                                "query writes cannot be shared)", x, qi, it->second);
+            }
+    }
     HIPCHK(hipSetDevice(ctx->device));
     Plan plan;
     std::vector<cfpq_mat *> taken;

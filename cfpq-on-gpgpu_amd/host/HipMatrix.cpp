@@ -113,7 +113,7 @@ void HipMatricesEnv::environment_postprocessing(const std::vector<Matrix *> &mat
     for (Matrix *m : matrices) as_hip(m)->from_device();
 }
 
-bool HipMatricesEnv::run_fixpoint(const std::vector<cfpq_rule_t> &rules, const std::vector<Matrix *> &matrices) {
+bool HipMatricesEnv::run_fixpoint(const std::vector<hip_rule> &rules, const std::vector<Matrix *> &matrices) {
     if (matrices.empty() || as_hip(matrices[0])->size() == 0) return true;  // nothing to compute
     std::vector<cfpq_rule> r(rules.size());
     for (size_t i = 0; i < rules.size(); ++i) r[i] = {rules[i].first, rules[i].second.first, rules[i].second.second};

@@ -12,16 +12,40 @@
 // include/cfpq_hip.h.  All matrices share one process-wide engine context on
 // the device named by $CFPQ_DEVICE (default 0).  Errors throw
 // std::runtime_error carrying cfpq_last_error().
+//
+// Two driver interfaces are accepted.  <Matrix.h> is taken from the include
+// path, so whichever driver the build names first is the one this backend
+// derives from:
+//   * the reference's own cfpq-cpp/Matrix.h (-I cfpq-cpp first): the plain
+//     Matrix / MatricesEnv contract of cfpq-cpp/Matrix.h:7-37.  The driver's
+//     worklist (cfpq-cpp/Grammar.h:58-69) runs and every add_mul is one exact
+//     cfpq_add_mul; fingerprint() and run_fixpoint() are extra members the
+//     reference driver never calls;
+//   * this repository's include/cfpq/Matrix.h, which adds two optional hooks
+//     (CFPQ_DRIVER_HOOKS): the same members then override them, so
+//     HipMatricesEnv hands the whole fixpoint to cfpq_fixpoint and
+//     print_results reads device fingerprints.
 #ifndef CFPQ_HIP_MATRIX_H
 #define CFPQ_HIP_MATRIX_H
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
-#include "Matrix.h"
+#include <Matrix.h>
 #include "cfpq_hip.h"
 
+#ifdef CFPQ_DRIVER_HOOKS
+#define CFPQ_HIP_HOOK override
+#else
+#define CFPQ_HIP_HOOK
+#endif
+
 cfpq_ctx *cfpq_host_context();  // process-wide context (created on first use)
+
+// A binary rule head -> (left, right) as the driver stores it: the element
+// type of the reference's Grammar::rules (cfpq-cpp/Grammar.h:98).
+using hip_rule = std::pair<unsigned int, std::pair<unsigned int, unsigned int>>;
 
 class HipMatrix : public Matrix {
 public:
@@ -31,7 +55,8 @@ public:
     void set_bit(unsigned int row, unsigned col) override;
     unsigned int get_bit(unsigned int row, unsigned col) override;
     bool add_mul(Matrix *left, Matrix *right) override;
-    bool fingerprint(uint64_t *hash, uint64_t *count) override;
+    // (hash, count) of Grammar::print_results, computed on the device
+    bool fingerprint(uint64_t *hash, uint64_t *count) CFPQ_HIP_HOOK;
 
     void to_device();    // allocate + upload the host mirror
     void from_device();  // download into the host mirror, free device memory
@@ -48,12 +73,12 @@ private:
 };
 
 // Uploads before and downloads after the algorithm; run_fixpoint hands the
-// whole fixpoint to the device engine (cfpq_fixpoint).
+// whole fixpoint to the device engine (cfpq_fixpoint) where the driver calls it.
 class HipMatricesEnv : public MatricesEnv {
 public:
     void environment_preprocessing(const std::vector<Matrix *> &matrices) override;
     void environment_postprocessing(const std::vector<Matrix *> &matrices) override;
-    bool run_fixpoint(const std::vector<cfpq_rule_t> &rules, const std::vector<Matrix *> &matrices) override;
+    bool run_fixpoint(const std::vector<hip_rule> &rules, const std::vector<Matrix *> &matrices) CFPQ_HIP_HOOK;
     const cfpq_stats &stats() const { return stats_; }
 
 private:
@@ -63,7 +88,7 @@ private:
 // Same residency, but the driver's worklist runs and calls add_mul per rule.
 class HipWorklistEnv : public HipMatricesEnv {
 public:
-    bool run_fixpoint(const std::vector<cfpq_rule_t> &, const std::vector<Matrix *> &) override { return false; }
+    bool run_fixpoint(const std::vector<hip_rule> &, const std::vector<Matrix *> &) CFPQ_HIP_HOOK { return false; }
 };
 
 #endif  // CFPQ_HIP_MATRIX_H

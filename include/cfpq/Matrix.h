@@ -10,8 +10,13 @@
 //     returning false (the default) leaves the worklist to the driver;
 //   * Matrix::fingerprint -- lets a backend compute the result fingerprint
 //     without N^2 get_bit calls; returning false falls back to get_bit.
-#ifndef CFPQ_MATRIX_H
-#define CFPQ_MATRIX_H
+//
+// The include guard differs from the reference header's, so that a build
+// which sees both interfaces fails loudly instead of dropping the hooks.
+// CFPQ_DRIVER_HOOKS tells backends (HipMatrix.h) that the hooks exist.
+#ifndef CFPQ_HIP_DRIVER_MATRIX_H
+#define CFPQ_HIP_DRIVER_MATRIX_H
+#define CFPQ_DRIVER_HOOKS 1
 
 #include <cstdint>
 #include <utility>
@@ -60,4 +65,4 @@ public:
     }
 };
 
-#endif  // CFPQ_MATRIX_H
+#endif  // CFPQ_HIP_DRIVER_MATRIX_H

@@ -233,6 +233,40 @@ def test_invalid_arguments(pkg, engine):
     b.free()
 
 
+def test_batch_rejects_shared_head(pkg, gen, oracle, engine):
+    """A matrix one query writes (a rule head) cannot appear in another query
+    of the batch: that query's delta buffers would never see its new bits.
+    Read-only operands may be shared, and such a batch gives each query's own
+    fixpoint."""
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    n = 300
+    S1, S2, A, B = (engine.matrix(n) for _ in range(4))
+    rules = [(0, 1, 2)]  # S -> A B
+    with pytest.raises(pkg.CfpqError) as ei:
+        engine.batch_fixpoint([(rules, [S1, A, B]), (rules, [S2, S1, B])])  # S1 written by 0, read by 1
+    assert ei.value.code == pkg.E_INVALID
+    with pytest.raises(pkg.CfpqError) as ei:
+        engine.batch_fixpoint([(rules, [S1, A, B]), (rules, [S1, A, B])])   # S1 written by both
+    assert ei.value.code == pkg.E_INVALID
+    for m in (S1, S2, A, B):
+        m.free()
+    # shared read-only terminal matrices: both queries close as they would alone
+    q = gen.dyck_er(400, 1200, seed=17)
+    g_ref, mats_ref, _ = oracle.solve(q, "seminaive")
+    want = oracle.fingerprints(g_ref, mats_ref, q.n)
+    g1, g2 = cf.Grammar(q.grammar, text=True), cf.Grammar(q.grammar, text=True)
+    m1 = g1.load(engine, cf.Graph.from_query(q))
+    m2 = g2.load(engine, cf.Graph.from_query(q))
+    heads = {h for h, _, _ in g1.rule_triples()}
+    shared = [m1[x] if x not in heads else m2[x] for x in range(len(m2))]
+    engine.batch_fixpoint([(g1.rule_triples(), m1), (g2.rule_triples(), shared)])
+    assert g1.results() == want
+    for x in heads:
+        assert shared[x].fingerprint() == m1[x].fingerprint()
+    g1.free()
+    g2.free()
+
+
 def test_fixpoint_sg_forest_65536_golden(pkg, gen):
     """Full-size ontology workload: fingerprints and counts of every nonterminal
     against the oracle's golden fixture (size-independent check)."""
@@ -247,6 +281,43 @@ def test_fixpoint_sg_forest_65536_golden(pkg, gen):
         _, _, stats = gr.intersection_with_graph(cf.Graph.from_query(q), eng)
         assert gr.results() == {k: tuple(v) for k, v in golden["fingerprints"].items()}
         assert stats["full_passes"] == 1
+        gr.free()
+
+
+@pytest.mark.parametrize("variant", ["tuned", "m4r", "tuned_certified"])
+def test_fixpoint_dense_stress_32768_golden(pkg, gen, variant):
+    """Full-size dense-tile stress workload (24 nonterminals, 64 binary rules,
+    N = 32,768, power-law out-degree 8): every nonterminal's fingerprint and
+    count against the oracle's golden fixture (its semi-naive M4RM run).
+
+    * tuned: the cached options of this workload class (rule sweeps, Four
+      Russians from 8,192 bits per A tile, the LDS path below);
+    * m4r: every tile-product forced onto the Four Russians path;
+    * tuned_certified: the tuned options plus the certification sweep, which
+      must add no bit."""
+    import json
+    import os
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    tune = __import__("importlib").import_module("cfpq-on-gpgpu_amd.tune")
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixpoints.json")))["config2"]
+    q = gen.config(2)
+    assert q.digest() == golden["digest"]
+    opts = tune.tuned_opts(q.n, q.grammar)
+    assert opts, "no tuned options cached for the dense-stress workload class"
+    if variant == "m4r":
+        opts["kernel"] = "m4r"
+    if variant == "tuned_certified":
+        opts["final_full_pass"] = True
+    print(f"[dense32768] {variant}: {opts}", flush=True)
+    with pkg.Engine(0, **opts) as eng:
+        gr = cf.Grammar(q.grammar, text=True)
+        _, _, stats = gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+        assert gr.results() == {k: tuple(v) for k, v in golden["fingerprints"].items()}, variant
+        if variant == "m4r":
+            assert stats["tile_products_dense"] == stats["tile_products"] > 0
+            assert stats["m4r_lookups"] > 0
+        if variant == "tuned_certified":
+            assert stats["full_passes"] == 1
         gr.free()
 
 

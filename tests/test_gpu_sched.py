@@ -148,3 +148,30 @@ def test_trained_network_gains_over_untrained(pkg, gen, engine):
     # share of the deferred items that were active: the base rate for a
     # scorer that knows nothing is the positive rate itself
     assert waste_t < 0.5 * float(ys.mean()) and waste_t < 0.5 * waste_u, res
+
+
+def test_learned_sg_forest_65536_golden(pkg, gen):
+    """The learned engine at full size: the 65,536-vertex ontology query with
+    the trained network ordering and deferring (rule, tile) work, the tuned
+    options of the workload class, and the exact full pass that certifies the
+    fixpoint.  Every nonterminal's fingerprint and count against the oracle's
+    golden fixture; the certification pass runs once and adds no bit."""
+    import json
+    import os
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    sched = __import__("importlib").import_module("cfpq-on-gpgpu_amd.sched")
+    tune = __import__("importlib").import_module("cfpq-on-gpgpu_amd.tune")
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixpoints.json")))["config4"]
+    q = gen.config(4)
+    assert q.digest() == golden["digest"]
+    weights = sched.load()
+    assert weights is not None, "trained scheduler weights are missing"
+    opts = dict(tune.tuned_opts(q.n, q.grammar), sched=True, final_full_pass=True)
+    with pkg.Engine(0, **opts) as eng:
+        eng.sched_set_weights(weights)
+        gr = cf.Grammar(q.grammar, text=True)
+        _, _, st = gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+        assert gr.results() == {k: tuple(v) for k, v in golden["fingerprints"].items()}
+        assert st["full_passes"] == 1
+        assert st["sched_scored"] > 0
+        gr.free()

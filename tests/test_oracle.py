@@ -246,3 +246,33 @@ def test_path_graph_equals_cyk(oracle, gen, which):
             if idx in cell:
                 want[i, j] = True
         assert np.array_equal(got, want), name
+
+
+def test_worklist_budget_matches_worklist(oracle, gen):
+    """The bounded worklist that times the CPU baseline is the reference
+    worklist itself: with a budget it cannot reach, it makes the same add_mul
+    calls in the same order and ends at the same matrices; with a zero budget
+    it stops after exactly one call, which equals one M4RM add_mul of the
+    first popped rule (checked by a numpy product)."""
+    import numpy as np
+    for q in (gen.dyck_er(300, 900, seed=5), gen.dense_stress(400, seed=9)):
+        g = oracle.ParsedGrammar(q.grammar)
+        m_full = oracle.initial_matrices(g, q.n, q.src, q.dst, q.label, q.labels)
+        m_budget = [m.copy() for m in m_full]
+        m_one = [m.copy() for m in m_full]
+        calls = oracle.fixpoint_worklist(q.n, g.rules_array(), m_full)
+        calls_b, secs = oracle.fixpoint_worklist_budget(q.n, g.rules_array(), m_budget, 1e9)
+        assert calls_b == calls and secs >= 0.0
+        for a, b in zip(m_full, m_budget):
+            assert np.array_equal(a, b)
+        calls_1, _ = oracle.fixpoint_worklist_budget(q.n, g.rules_array(), m_one, 0.0)
+        assert calls_1 == 1
+        init = oracle.initial_matrices(g, q.n, q.src, q.dst, q.label, q.labels)
+        changed = [x for x in range(len(init)) if not np.array_equal(init[x], m_one[x])]
+        assert len(changed) <= 1
+        if changed:  # the one call's head now holds head | left * right of the initial matrices
+            x = changed[0]
+            cands = [(l, r) for h, l, r in g.rules_array().tolist() if h == x]
+            dense = [oracle.to_dense(m, q.n).astype(np.uint8) for m in init]
+            assert any(np.array_equal(oracle.to_dense(m_one[x], q.n),
+                                      dense[x].astype(bool) | ((dense[l] @ dense[r]) > 0)) for l, r in cands)
