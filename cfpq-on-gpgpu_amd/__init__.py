@@ -118,12 +118,13 @@ E_CAPACITY = -5
 EXPORTS = (
     "cfpq_default_opts", "cfpq_create", "cfpq_destroy", "cfpq_last_error", "cfpq_set_opts", "cfpq_sync",
     "cfpq_mat_alloc", "cfpq_mat_free", "cfpq_mat_n", "cfpq_mat_upload", "cfpq_mat_download",
-    "cfpq_mat_set_bits", "cfpq_mat_popcount", "cfpq_mat_copy", "cfpq_mat_fingerprint", "cfpq_add_mul", "cfpq_fixpoint",
+    "cfpq_mat_set_bits", "cfpq_mat_popcount", "cfpq_mat_copy", "cfpq_mat_fingerprint", "cfpq_add_mul", "cfpq_add_mul_noalloc", "cfpq_fixpoint",
     "cfpq_batch_fixpoint", "cfpq_nt_closure", "cfpq_pairs_fixpoint", "cfpq_sched_set_weights", "cfpq_sched_get_weights", "cfpq_sched_init",
     "cfpq_sched_eval", "cfpq_sched_train", "cfpq_sched_trace_begin", "cfpq_sched_trace_read",
     "cfpq_sched_trace_end",
 )
 
+NOALLOC_MAX_N = 4096
 SCHED_PARAMS = 5825
 SCHED_FEATURES = 8
 
@@ -159,6 +160,7 @@ def lib():
         "cfpq_mat_copy": (i32, [vp, vp, vp]),
         "cfpq_mat_fingerprint": (i32, [vp, vp, u64p, u64p]),
         "cfpq_add_mul": (i32, [vp, vp, vp, vp, ctypes.POINTER(i32)]),
+        "cfpq_add_mul_noalloc": (i32, [vp, vp, vp, vp, ctypes.POINTER(i32)]),
         "cfpq_fixpoint": (i32, [vp, ctypes.POINTER(Rule), sz, ctypes.POINTER(vp), sz, ctypes.POINTER(Stats)]),
         "cfpq_batch_fixpoint": (i32, [vp, ctypes.POINTER(Query), sz, ctypes.POINTER(Stats)]),
         "cfpq_nt_closure": (i32, [vp, ctypes.POINTER(Rule), sz, u32p, u32, ctypes.POINTER(Stats)]),
@@ -251,6 +253,13 @@ class Engine:
     def add_mul(self, c: "DeviceMatrix", a: "DeviceMatrix", b: "DeviceMatrix") -> bool:
         ch = ctypes.c_int(0)
         self._check(self._L.cfpq_add_mul(self._h, c._h, a._h, b._h, ctypes.byref(ch)))
+        return bool(ch.value)
+
+    def add_mul_noalloc(self, c: "DeviceMatrix", a: "DeviceMatrix", b: "DeviceMatrix") -> bool:
+        """cfpq_add_mul_noalloc: the same product without device allocation
+        (n <= NOALLOC_MAX_N)."""
+        ch = ctypes.c_int(0)
+        self._check(self._L.cfpq_add_mul_noalloc(self._h, c._h, a._h, b._h, ctypes.byref(ch)))
         return bool(ch.value)
 
     def fixpoint(self, rules, mats) -> dict:

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -100,12 +101,84 @@ struct cfpq_ctx {
     float *tr_x = nullptr, *tr_y = nullptr;
     uint32_t *tr_rule = nullptr, *tr_item = nullptr, *tr_count = nullptr;
     size_t tr_cap = 0;
+    // cfpq_add_mul_noalloc: this device's copy of the scratch array (its
+    // device address), and the flag that turns every allocation into an error
+    char *noalloc_base = nullptr;
+    bool no_alloc = false;
+
     // training state
     float *t_master = nullptr, *t_m1 = nullptr, *t_m2 = nullptr, *t_grad = nullptr, *t_loss = nullptr;
     float *t_x = nullptr, *t_y = nullptr;
     uint32_t *t_rule = nullptr;
     size_t t_cap = 0, t_x_cap = 0, t_y_cap = 0, t_rule_cap = 0;
 };
+
+static constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Bytes of a matrix with T tiles per side (the layout of mat_layout).
+static constexpr size_t mat_bytes(size_t T) {
+    return align256(T * T * TILE_WORDS * 8) + align256(T * T * 4) + align256(T * T * 4 * 8) +
+           align256(T * T * QUARTERS * TROW_WORDS * 8) + align256(OCC_ARRAYS * T * ((T + 31) / 32) * 4);
+}
+
+// ---------------------------------------------------------------------------
+// scratch of cfpq_add_mul_noalloc
+// ---------------------------------------------------------------------------
+//
+// Everything beyond its inputs and output that one add_mul step needs -- the
+// delta matrix of the new bits, the descriptor tables, the work lists, the
+// touched-tile lists, the counter ring and the scheduler's dirty bitmap --
+// sized for n <= CFPQ_NOALLOC_MAX_N, lives in one file-scope device array.
+// Each GPU holds its own copy for the life of the process; in host code the
+// name `scratch` is not a device address, so the array is only ever reached
+// through the address hipGetSymbolAddress returns for the current device
+// (kept per device by noalloc_init).
+constexpr uint32_t NOALLOC_T = (CFPQ_NOALLOC_MAX_N + TILE - 1) / TILE;
+constexpr size_t NOALLOC_TILES = (size_t)NOALLOC_T * NOALLOC_T;
+constexpr size_t NA_MATS_CAP = 4, NA_HEADS_CAP = 1, NA_TERMS_CAP = 3, NA_TPTR_CAP = 6, NA_IDS_CAP = 8;
+constexpr size_t NA_UNITS_CAP = 3 * QUARTERS * NOALLOC_TILES, NA_LIST_CAP = (1 + SHARDS) * NOALLOC_TILES;
+constexpr size_t NA_DIRTY_CAP = (NOALLOC_TILES + 31) / 32;
+constexpr size_t NA_RING = align256(mat_bytes(NOALLOC_T));
+constexpr size_t NA_MATS = NA_RING + align256(sizeof(StepCounters) * RING);
+constexpr size_t NA_HEADS = NA_MATS + align256(sizeof(MatDev) * NA_MATS_CAP);
+constexpr size_t NA_TERMS = NA_HEADS + align256(sizeof(HeadDev) * NA_HEADS_CAP);
+constexpr size_t NA_TPTR = NA_TERMS + align256(sizeof(TermDev) * NA_TERMS_CAP);
+constexpr size_t NA_IDS = NA_TPTR + align256(sizeof(TermPtr) * NA_TPTR_CAP);
+constexpr size_t NA_ITEMS = NA_IDS + align256(4 * NA_IDS_CAP);
+constexpr size_t NA_MITEMS = NA_ITEMS + align256(8 * NOALLOC_TILES);
+constexpr size_t NA_UNITS = NA_MITEMS + align256(8 * NOALLOC_TILES);
+constexpr size_t NA_LIST0 = NA_UNITS + align256(8 * NA_UNITS_CAP);
+constexpr size_t NA_LIST1 = NA_LIST0 + align256(8 * NA_LIST_CAP);
+constexpr size_t NA_DIRTY = NA_LIST1 + align256(8 * NA_LIST_CAP);
+constexpr size_t NA_DIRTY_OFF = NA_DIRTY + align256(4 * NA_DIRTY_CAP);
+constexpr size_t NOALLOC_BYTES = NA_DIRTY_OFF + 256;
+static_assert(NOALLOC_BYTES <= (3u << 20), "the no-allocation scratch stays below 3 MiB");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_DEVICES = 64;
+static std::mutex g_noalloc_init_mu;
+static char *g_noalloc_ptr[NOALLOC_DEVICES];
+static std::string g_noalloc_err[NOALLOC_DEVICES];
+// calls on one GPU share its copy of the array: they are serialised per device
+static std::mutex g_noalloc_call_mu[NOALLOC_DEVICES];
+
+// Resolve this device's copy of the array once (the device must be current).
+// A failure does not fail context creation: cfpq_add_mul_noalloc then
+// reports it.
+static void noalloc_init(cfpq_ctx *ctx, int device) {
+    if (device < 0 || device >= NOALLOC_DEVICES) return;
+    std::lock_guard<std::mutex> lock(g_noalloc_init_mu);
+    if (!g_noalloc_ptr[device] && g_noalloc_err[device].empty()) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e == hipSuccess && p)
+            g_noalloc_ptr[device] = (char *)p;
+        else
+            g_noalloc_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    }
+    ctx->noalloc_base = g_noalloc_ptr[device];
+}
 
 // ---------------------------------------------------------------------------
 // error helpers
@@ -140,6 +213,7 @@ static int fail(cfpq_ctx *ctx, int code, const char *fmt, ...) {
 template <class T>
 static int grow(cfpq_ctx *ctx, T **ptr, size_t *cap, size_t need) {
     if (need <= *cap && *ptr) return CFPQ_OK;
+    if (ctx->no_alloc) return fail(ctx, CFPQ_E_NOMEM, "a call that allocates nothing needed a larger device buffer");
     if (*ptr) HIPCHK(hipFree(*ptr));
     *ptr = nullptr;
     size_t n = std::max<size_t>(need, 16);
@@ -211,6 +285,7 @@ extern "C" int cfpq_create(int device, const cfpq_opts *opts, cfpq_ctx **out) {
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, device));
         ctx->num_cu = prop.multiProcessorCount;
+        noalloc_init(ctx, device);
         HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         HIPCHK(hipMalloc((void **)&ctx->ring, sizeof(StepCounters) * RING));
         HIPCHK(hipMemset(ctx->ring, 0, sizeof(StepCounters) * RING));
@@ -279,20 +354,38 @@ extern "C" int cfpq_sync(cfpq_ctx *ctx) {
 // matrices
 // ---------------------------------------------------------------------------
 
-static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
-
-static int mat_create(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
-    cfpq_mat *m = new (std::nothrow) cfpq_mat();
-    if (!m) return fail(ctx, CFPQ_E_NOMEM, "host allocation failed");
-    m->n = n;
-    m->T = tiles_per_side(n);
-    const size_t T = m->T, ow = occ_words(m->T);
+// Bytes of one matrix with T tiles per side in its single allocation, and
+// (with base) the descriptor of a matrix laid out at base.
+static size_t mat_layout(uint32_t Tn, char *base, MatDev *d) {
+    const size_t T = Tn, ow = occ_words(Tn);
     const size_t b_bits = align256(T * T * TILE_WORDS * sizeof(uint64_t));
     const size_t b_cnt = align256(T * T * sizeof(uint32_t));
     const size_t b_rm = align256(T * T * 4 * sizeof(uint64_t));
     const size_t b_cq = align256(T * T * QUARTERS * TROW_WORDS * sizeof(uint64_t));
     const size_t b_occ = align256(OCC_ARRAYS * T * ow * sizeof(uint32_t));  // occR, occC, occQ, occD, occX
-    m->bytes = b_bits + b_cnt + b_rm + b_cq + b_occ;
+    if (base && d) {
+        d->bits = (uint64_t *)base;
+        d->cnt = (uint32_t *)(base + b_bits);
+        d->rmask = (uint64_t *)(base + b_bits + b_cnt);
+        d->colq = (uint64_t *)(base + b_bits + b_cnt + b_rm);
+        d->occR = (uint32_t *)(base + b_bits + b_cnt + b_rm + b_cq);
+        d->occC = d->occR + T * ow;
+        d->occQ = d->occC + T * ow;
+        d->occD = d->occQ + QUARTERS * T * ow;
+        d->occX = d->occD + T * ow;
+        d->T = Tn;
+        d->ow = (uint32_t)ow;
+    }
+    return b_bits + b_cnt + b_rm + b_cq + b_occ;
+}
+
+static int mat_create(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
+    if (ctx->no_alloc) return fail(ctx, CFPQ_E_NOMEM, "a call that allocates nothing needed a device matrix");
+    cfpq_mat *m = new (std::nothrow) cfpq_mat();
+    if (!m) return fail(ctx, CFPQ_E_NOMEM, "host allocation failed");
+    m->n = n;
+    m->T = tiles_per_side(n);
+    m->bytes = mat_layout(m->T, nullptr, nullptr);
     hipError_t e = hipMalloc(&m->base, m->bytes);
     if (e != hipSuccess) {
         delete m;
@@ -304,18 +397,7 @@ static int mat_create(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
         free_mat(m);
         return fail(ctx, CFPQ_E_DEVICE, "hipMemsetAsync: %s", hipGetErrorString(e));
     }
-    char *b = (char *)m->base;
-    m->d.bits = (uint64_t *)b;
-    m->d.cnt = (uint32_t *)(b + b_bits);
-    m->d.rmask = (uint64_t *)(b + b_bits + b_cnt);
-    m->d.colq = (uint64_t *)(b + b_bits + b_cnt + b_rm);
-    m->d.occR = (uint32_t *)(b + b_bits + b_cnt + b_rm + b_cq);
-    m->d.occC = m->d.occR + T * ow;
-    m->d.occQ = m->d.occC + T * ow;
-    m->d.occD = m->d.occQ + QUARTERS * T * ow;
-    m->d.occX = m->d.occD + T * ow;
-    m->d.T = m->T;
-    m->d.ow = (uint32_t)ow;
+    mat_layout(m->T, (char *)m->base, &m->d);
     *out = m;
     return CFPQ_OK;
 }
@@ -337,6 +419,7 @@ extern "C" uint32_t cfpq_mat_n(const cfpq_mat *mat) { return mat ? mat->n : 0; }
 
 static int ensure_stage(cfpq_ctx *ctx, size_t bytes) {
     if (bytes <= ctx->stage_bytes) return CFPQ_OK;
+    if (ctx->no_alloc) return fail(ctx, CFPQ_E_NOMEM, "a call that allocates nothing needed a staging buffer");
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctx->stage) HIPCHK(hipFree(ctx->stage));
     ctx->stage = nullptr;
@@ -485,6 +568,7 @@ static int pool_take(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
 static void pool_give(cfpq_ctx *ctx, cfpq_mat *m) { ctx->pool.emplace(m->n, m); }
 
 static int ensure_events(cfpq_ctx *ctx, size_t n) {
+    if (ctx->no_alloc && ctx->ev.size() < n) return fail(ctx, CFPQ_E_NOMEM, "a call that allocates nothing needed events");
     while (ctx->ev.size() < n) {
         hipEvent_t e;
         HIPCHK(hipEventCreate(&e));
@@ -1265,15 +1349,10 @@ extern "C" int cfpq_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nrule
     return cfpq_batch_fixpoint(ctx, &q, 1, stats);
 }
 
-extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const cfpq_mat *b, int *changed) {
-    if (!ctx || !c || !a || !b) return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul: bad argument");
-    if (a->n != c->n || b->n != c->n) return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul: size mismatch");
-    HIPCHK(hipSetDevice(ctx->device));
-    cfpq_mat *S = nullptr;
-    CHK(pool_take(ctx, c->n, &S));
-    // one full step:  S = (a*b) & ~c ;  c |= S ;  S cleared again
-    Plan plan;
-    plan.mats = {c->d, a->d, b->d, S->d};
+// The plan of one add_mul: a single full step  S = (a*b) & ~c ;  c |= S ;
+// S cleared again.  Descriptor ids: 0 = c, 1 = a, 2 = b, 3 = S.
+static void add_mul_plan(Plan &plan, const cfpq_mat *c, const cfpq_mat *a, const cfpq_mat *b, const MatDev &S) {
+    plan.mats = {c->d, a->d, b->d, S};
     HeadDev hd;
     hd.target = 0;
     hd.out[0] = hd.out[1] = 3;
@@ -1295,6 +1374,11 @@ extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const
     plan.all_delta[0] = {3};
     plan.all_delta[1] = {3};
     plan.max_tiles_per_head = plan.total_head_tiles = (size_t)c->T * c->T;
+}
+
+// Upload the plan, run its step and read the changed flag back.  On success
+// S is all-zero again (the step's first-step clear).
+static int add_mul_run(cfpq_ctx *ctx, Plan &plan, int *changed) {
     int rc = upload_plan(ctx, plan);
     if (rc == CFPQ_OK) rc = enqueue_step(ctx, plan, 0, 0, -1, false, -1, false, /*fuse=*/false, true);
     // enqueue_step cleared nothing useful for S yet (prev_slot < 0 clears all
@@ -1307,10 +1391,99 @@ extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const
     if (rc == CFPQ_OK) {
         fold_shards(ctx->ring_host[0]);
         if (changed) *changed = ctx->ring_host[0].new_bits != 0;
+    }
+    return rc;
+}
+
+extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const cfpq_mat *b, int *changed) {
+    if (!ctx || !c || !a || !b) return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul: bad argument");
+    if (a->n != c->n || b->n != c->n) return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul: size mismatch");
+    HIPCHK(hipSetDevice(ctx->device));
+    cfpq_mat *S = nullptr;
+    CHK(pool_take(ctx, c->n, &S));
+    Plan plan;
+    add_mul_plan(plan, c, a, b, S->d);
+    const int rc = add_mul_run(ctx, plan, changed);
+    if (rc == CFPQ_OK) {
         pool_give(ctx, S);
     } else {
         (void)hipStreamSynchronize(ctx->stream);
         free_mat(S);
+    }
+    return rc;
+}
+
+// While alive, the context's work buffers are the regions of this device's
+// scratch array (capacities set so that nothing grows) and every allocation
+// path fails; the destructor puts the context's own buffers back.
+struct NoallocBuffers {
+    cfpq_ctx *ctx;
+    MatDev *d_mats; HeadDev *d_heads; TermDev *d_terms; TermPtr *d_tptr; uint32_t *d_ids;
+    uint64_t *items, *units, *mitems, *lists[2];
+    StepCounters *ring;
+    uint32_t *dirty, *dirty_off;
+    size_t caps[11];
+    explicit NoallocBuffers(cfpq_ctx *c) : ctx(c) {
+        d_mats = c->d_mats; d_heads = c->d_heads; d_terms = c->d_terms; d_tptr = c->d_tptr; d_ids = c->d_ids;
+        items = c->items; units = c->units; mitems = c->mitems; lists[0] = c->lists[0]; lists[1] = c->lists[1];
+        ring = c->ring; dirty = c->dirty; dirty_off = c->dirty_off;
+        const size_t cv[11] = {c->d_mats_cap, c->d_heads_cap, c->d_terms_cap, c->d_tptr_cap, c->d_ids_cap, c->items_cap,
+                               c->units_cap, c->mitems_cap, c->lists_cap, c->dirty_cap, c->dirty_off_cap};
+        std::copy(cv, cv + 11, caps);
+        char *base = c->noalloc_base;
+        c->d_mats = (MatDev *)(base + NA_MATS);       c->d_mats_cap = NA_MATS_CAP;
+        c->d_heads = (HeadDev *)(base + NA_HEADS);    c->d_heads_cap = NA_HEADS_CAP;
+        c->d_terms = (TermDev *)(base + NA_TERMS);    c->d_terms_cap = NA_TERMS_CAP;
+        c->d_tptr = (TermPtr *)(base + NA_TPTR);      c->d_tptr_cap = NA_TPTR_CAP;
+        c->d_ids = (uint32_t *)(base + NA_IDS);       c->d_ids_cap = NA_IDS_CAP;
+        c->items = (uint64_t *)(base + NA_ITEMS);     c->items_cap = NOALLOC_TILES;
+        c->mitems = (uint64_t *)(base + NA_MITEMS);   c->mitems_cap = NOALLOC_TILES;
+        c->units = (uint64_t *)(base + NA_UNITS);     c->units_cap = NA_UNITS_CAP;
+        c->lists[0] = (uint64_t *)(base + NA_LIST0);
+        c->lists[1] = (uint64_t *)(base + NA_LIST1);  c->lists_cap = NA_LIST_CAP;
+        c->ring = (StepCounters *)(base + NA_RING);
+        c->dirty = (uint32_t *)(base + NA_DIRTY);     c->dirty_cap = NA_DIRTY_CAP;
+        c->dirty_off = (uint32_t *)(base + NA_DIRTY_OFF); c->dirty_off_cap = 1;
+        c->no_alloc = true;
+    }
+    ~NoallocBuffers() {
+        cfpq_ctx *c = ctx;
+        c->d_mats = d_mats; c->d_heads = d_heads; c->d_terms = d_terms; c->d_tptr = d_tptr; c->d_ids = d_ids;
+        c->items = items; c->units = units; c->mitems = mitems; c->lists[0] = lists[0]; c->lists[1] = lists[1];
+        c->ring = ring; c->dirty = dirty; c->dirty_off = dirty_off;
+        c->d_mats_cap = caps[0]; c->d_heads_cap = caps[1]; c->d_terms_cap = caps[2]; c->d_tptr_cap = caps[3];
+        c->d_ids_cap = caps[4]; c->items_cap = caps[5]; c->units_cap = caps[6]; c->mitems_cap = caps[7];
+        c->lists_cap = caps[8]; c->dirty_cap = caps[9]; c->dirty_off_cap = caps[10];
+        c->no_alloc = false;
+    }
+};
+
+extern "C" int cfpq_add_mul_noalloc(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const cfpq_mat *b, int *changed) {
+    if (!ctx || !c || !a || !b) return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul_noalloc: bad argument");
+    if (a->n != c->n || b->n != c->n) return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul_noalloc: size mismatch");
+    if (c->n > CFPQ_NOALLOC_MAX_N)
+        return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul_noalloc: n = %u is above the limit of %u of its scratch array "
+                    "(use cfpq_add_mul)", c->n, (unsigned)CFPQ_NOALLOC_MAX_N);
+    if (!ctx->noalloc_base) {
+        const bool dev_ok = ctx->device >= 0 && ctx->device < NOALLOC_DEVICES;
+        return fail(ctx, CFPQ_E_DEVICE, "cfpq_add_mul_noalloc: no scratch array on device %d: %s", ctx->device,
+                    dev_ok ? g_noalloc_err[ctx->device].c_str() : "device index out of range");
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_call_mu[ctx->device]);
+    HIPCHK(hipSetDevice(ctx->device));
+    // the delta matrix S at the start of the array, cleared here: the array
+    // keeps whatever the last call on this device left in it
+    MatDev S = {};
+    const size_t sbytes = mat_layout(c->T, ctx->noalloc_base, &S);
+    if (sbytes > NA_RING) return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul_noalloc: delta matrix exceeds its region");
+    HIPCHK(hipMemsetAsync(ctx->noalloc_base, 0, sbytes, ctx->stream));
+    Plan plan;
+    add_mul_plan(plan, c, a, b, S);
+    int rc;
+    {
+        NoallocBuffers guard(ctx);
+        rc = add_mul_run(ctx, plan, changed);
+        if (rc != CFPQ_OK) (void)hipStreamSynchronize(ctx->stream);
     }
     return rc;
 }

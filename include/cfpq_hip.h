@@ -177,6 +177,20 @@ int cfpq_mat_fingerprint(cfpq_ctx *ctx, const cfpq_mat *mat, uint64_t *hash, uin
  * the operands as they were on entry.  *changed = 1 if c gained a bit. */
 int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const cfpq_mat *b, int *changed);
 
+/* cfpq_add_mul without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (serving loops with a
+ * fixed memory budget, harnesses that forbid hipMalloc during a run).  Same
+ * result, aliasing rules and changed flag as cfpq_add_mul, for matrices of
+ * n <= CFPQ_NOALLOC_MAX_N (larger n: CFPQ_E_INVALID).  Everything the step
+ * needs beyond a, b and c (the delta matrix, work lists, descriptor tables,
+ * counters) lives in a file-scope device array of the library, about
+ * 2.4 MiB, which each GPU it runs on holds until the process ends; no call
+ * to hipMalloc/hipFree is made.  Calls on one GPU share that array: the
+ * library serialises them within a process (a per-device lock), so calls
+ * from several threads on one GPU run one after another. */
+#define CFPQ_NOALLOC_MAX_N 4096u
+int cfpq_add_mul_noalloc(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const cfpq_mat *b, int *changed);
+
 /* Least fixpoint of  M[head] |= M[left] * M[right]  for all rules, in place on
  * mats[0..nmats) (all of one size n).  stats may be NULL. */
 int cfpq_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nrules, cfpq_mat *const *mats,
