@@ -103,7 +103,7 @@ def product_roofline(stats, wname):
                 "alg_ops_per_launch": round(ops / launches), "executed_ops_per_launch": round(executed / launches)}
     if t_sparse >= t_dense:
         kern, t_k = "k_product_sparse", t_sparse
-        alg_bytes = sum(32 * (s["a_rows"] + s["b_rows"] + s["out_rows"]) for s in stats)
+        alg_bytes = sum(32 * (s["a_rows"] + s["b_rows"] + s["out_rows"]) + 8 * s["a_cols"] for s in stats)
     else:
         kern, t_k = "k_product_dense", t_dense
         alg_bytes = sum(32 * (s["a_rows_dense"] + s["b_rows_dense"] + s["out_rows_dense"]) +
@@ -377,7 +377,7 @@ def main():
         "iterations": s0["iterations"],
         "tile_products": s0["tile_products"],
         "work": {k: s0[k] for k in ("items", "tile_products", "tile_products_dense", "tile_products_mfma",
-                                    "a_rows", "b_rows", "b_rows_lds", "m4r_lookups", "mfma_instr", "out_rows",
+                                    "a_rows", "a_cols", "b_rows", "b_rows_lds", "m4r_lookups", "mfma_instr", "out_rows",
                                     "new_bits", "product_launches", "host_syncs", "cand_hits",
                                     "max_unit_cands", "a_rows_dense", "b_rows_dense", "out_rows_dense")},
         "kernel_ms": {k: round(s0[k], 4) for k in ("sparse_ms", "dense_ms", "mfma_ms")},

@@ -89,6 +89,7 @@ class Stats(ctypes.Structure):
         ("a_rows_dense", ctypes.c_uint64),
         ("b_rows_dense", ctypes.c_uint64),
         ("out_rows_dense", ctypes.c_uint64),
+        ("a_cols", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

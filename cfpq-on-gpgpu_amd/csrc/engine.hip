@@ -32,6 +32,10 @@ using namespace cfpq;
 struct cfpq_mat {
     uint32_t n = 0;
     uint32_t T = 0;
+    // the column-major form (MatDev::colT) is out of date: the matrix was the
+    // head of a fixpoint that did not read it as a left operand; rebuilt
+    // before any call reads it as one (colT_fresh)
+    bool colT_stale = false;
     void *base = nullptr;  // single device allocation
     size_t bytes = 0;
     MatDev d = {};
@@ -66,6 +70,16 @@ struct cfpq_ctx {
     size_t mitems_cap = 0;
     uint64_t *lists[2] = {nullptr, nullptr};
     size_t lists_cap = 0;
+    // split quarters of the sparse product: partial rows (64 rows per slot),
+    // their masks, and the list of split quarters (2 words each); at most
+    // split_slots sub-units per step
+    uint64_t *partials = nullptr;
+    size_t partials_cap = 0;
+    uint64_t *pmask = nullptr;
+    size_t pmask_cap = 0;
+    uint64_t *splits = nullptr;
+    size_t splits_cap = 0;
+    uint32_t split_slots = 32768;
 
     // descriptor tables
     MatDev *d_mats = nullptr;
@@ -117,7 +131,7 @@ static constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Bytes of a matrix with T tiles per side (the layout of mat_layout).
 static constexpr size_t mat_bytes(size_t T) {
-    return align256(T * T * TILE_WORDS * 8) + align256(T * T * 4) + align256(T * T * 4 * 8) +
+    return 2 * align256(T * T * TILE_WORDS * 8) + align256(T * T * 4) + align256(T * T * 4 * 8) +
            align256(T * T * QUARTERS * TROW_WORDS * 8) + align256(OCC_ARRAYS * T * ((T + 31) / 32) * 4);
 }
 
@@ -151,8 +165,12 @@ constexpr size_t NA_LIST0 = NA_UNITS + align256(8 * NA_UNITS_CAP);
 constexpr size_t NA_LIST1 = NA_LIST0 + align256(8 * NA_LIST_CAP);
 constexpr size_t NA_DIRTY = NA_LIST1 + align256(8 * NA_LIST_CAP);
 constexpr size_t NA_DIRTY_OFF = NA_DIRTY + align256(4 * NA_DIRTY_CAP);
-constexpr size_t NOALLOC_BYTES = NA_DIRTY_OFF + 256;
-static_assert(NOALLOC_BYTES <= (3u << 20), "the no-allocation scratch stays below 3 MiB");
+constexpr uint32_t NA_SPLIT_SLOTS = 32;  // split sub-units per call
+constexpr size_t NA_PARTIALS = NA_DIRTY_OFF + 256;
+constexpr size_t NA_PMASK = NA_PARTIALS + align256(8 * NA_SPLIT_SLOTS * WAVE * TROW_WORDS);
+constexpr size_t NA_SPLITS = NA_PMASK + align256(8 * NA_SPLIT_SLOTS);
+constexpr size_t NOALLOC_BYTES = NA_SPLITS + align256(8 * 2 * (NA_SPLIT_SLOTS / 2 + 1));
+static_assert(NOALLOC_BYTES <= (5u << 20), "the no-allocation scratch stays below 5 MiB");
 
 static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 
@@ -321,6 +339,7 @@ extern "C" int cfpq_destroy(cfpq_ctx *ctx) {
     for (auto g : ctx->graph_exec)
         if (g) (void)hipGraphExecDestroy(g);
     void *dev[] = {ctx->stage, ctx->ring, ctx->scratch64, ctx->items, ctx->units, ctx->mitems, ctx->lists[0], ctx->lists[1],
+                   ctx->partials, ctx->pmask, ctx->splits,
                    ctx->d_mats, ctx->d_heads, ctx->d_terms, ctx->d_tptr, ctx->d_ids, ctx->sched_w, ctx->dirty, ctx->dirty_off,
                    ctx->tr_x, ctx->tr_y, ctx->tr_rule, ctx->tr_item, ctx->tr_count, ctx->t_master, ctx->t_m1,
                    ctx->t_m2, ctx->t_grad, ctx->t_loss, ctx->t_x, ctx->t_y, ctx->t_rule};
@@ -363,11 +382,13 @@ static size_t mat_layout(uint32_t Tn, char *base, MatDev *d) {
     const size_t b_rm = align256(T * T * 4 * sizeof(uint64_t));
     const size_t b_cq = align256(T * T * QUARTERS * TROW_WORDS * sizeof(uint64_t));
     const size_t b_occ = align256(OCC_ARRAYS * T * ow * sizeof(uint32_t));  // occR, occC, occQ, occD, occX
+    const size_t b_ct = b_bits;  // the quarters in column-major form: as many words as the tiles
     if (base && d) {
         d->bits = (uint64_t *)base;
         d->cnt = (uint32_t *)(base + b_bits);
         d->rmask = (uint64_t *)(base + b_bits + b_cnt);
         d->colq = (uint64_t *)(base + b_bits + b_cnt + b_rm);
+        d->colT = (uint64_t *)(base + b_bits + b_cnt + b_rm + b_cq + b_occ);
         d->occR = (uint32_t *)(base + b_bits + b_cnt + b_rm + b_cq);
         d->occC = d->occR + T * ow;
         d->occQ = d->occC + T * ow;
@@ -376,7 +397,7 @@ static size_t mat_layout(uint32_t Tn, char *base, MatDev *d) {
         d->T = Tn;
         d->ow = (uint32_t)ow;
     }
-    return b_bits + b_cnt + b_rm + b_cq + b_occ;
+    return b_bits + b_cnt + b_rm + b_cq + b_occ + b_ct;
 }
 
 static int mat_create(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
@@ -434,6 +455,17 @@ static uint32_t grid_for_tiles(cfpq_ctx *ctx, size_t tiles) {
     return (uint32_t)std::max<size_t>(g, 1);
 }
 
+// Before a call reads m as a left operand: rebuild its column-major form if a
+// fixpoint that did not keep it left it out of date.
+static int colT_fresh(cfpq_ctx *ctx, const cfpq_mat *cm) {
+    cfpq_mat *m = const_cast<cfpq_mat *>(cm);
+    if (!m->colT_stale) return CFPQ_OK;
+    hipLaunchKernelGGL(k_rebuild_colT, dim3(grid_for_tiles(ctx, (size_t)m->T * m->T)), dim3(BLOCK), 0, ctx->stream, m->d);
+    CHK(launch_check(ctx, "k_rebuild_colT"));
+    m->colT_stale = false;
+    return CFPQ_OK;
+}
+
 extern "C" int cfpq_mat_upload(cfpq_ctx *ctx, cfpq_mat *mat, const uint64_t *rows, size_t words_per_row) {
     if (!ctx || !mat || !rows) return fail(ctx, CFPQ_E_INVALID, "cfpq_mat_upload: bad argument");
     const size_t wpr = (mat->n + 63) / 64;
@@ -447,6 +479,7 @@ extern "C" int cfpq_mat_upload(cfpq_ctx *ctx, cfpq_mat *mat, const uint64_t *row
     hipLaunchKernelGGL(k_rows_to_tiles, dim3(grid_for_tiles(ctx, (size_t)mat->T * mat->T)), dim3(BLOCK), 0, ctx->stream,
                        mat->d, (const uint64_t *)ctx->stage, mat->n, (uint32_t)wpr);
     CHK(launch_check(ctx, "k_rows_to_tiles"));
+    mat->colT_stale = false;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return CFPQ_OK;
 }
@@ -486,6 +519,7 @@ extern "C" int cfpq_mat_set_bits(cfpq_ctx *ctx, cfpq_mat *mat, const uint32_t *r
     HIPCHK(hipMemsetAsync(mat->d.occR, 0, occ_bytes, ctx->stream));
     hipLaunchKernelGGL(k_recount, dim3(grid_for_tiles(ctx, (size_t)mat->T * mat->T)), dim3(BLOCK), 0, ctx->stream, mat->d);
     CHK(launch_check(ctx, "k_recount"));
+    mat->colT_stale = false;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return CFPQ_OK;
 }
@@ -547,6 +581,7 @@ extern "C" int cfpq_mat_copy(cfpq_ctx *ctx, cfpq_mat *dst, const cfpq_mat *src) 
     hipLaunchKernelGGL(k_copy, dim3(g, 1), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                        (const uint32_t *)ctx->d_ids, (const uint32_t *)ctx->d_ids + 1, 1u);
     CHK(launch_check(ctx, "cfpq_mat_copy"));
+    dst->colT_stale = src->colT_stale;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return CFPQ_OK;
 }
@@ -631,7 +666,7 @@ static size_t plan_terms(const Plan &plan) {
 
 struct StepStats {
     uint64_t new_bits = 0, items = 0, tp = 0, tpd = 0, tpm = 0, arows = 0, brows = 0, orows = 0, brows_lds = 0,
-             lookups = 0, mfma_instr = 0, cand_hits = 0, arows_d = 0, brows_d = 0, orows_d = 0;
+             lookups = 0, mfma_instr = 0, cand_hits = 0, arows_d = 0, brows_d = 0, orows_d = 0, acols = 0;
     uint32_t max_unit_cands = 0;
 };
 
@@ -668,6 +703,11 @@ static uint32_t m4r_threshold(const cfpq_opts &o) {
 // prev_slot < 0: the delta consumed by this step is the initial one
 // (cleared in full); otherwise it was written by the step in prev_slot.
 static bool sched_on(const cfpq_ctx *ctx) { return ctx->opts.sched != 0 && ctx->sched_have; }
+
+// Sub-units (partial-row slots) the build may create per step.
+static uint32_t split_slots_of(const cfpq_ctx *ctx, const Plan &plan) {
+    return (uint32_t)std::min<size_t>(QUARTERS * plan.total_head_tiles, ctx->split_slots);
+}
 
 static SchedArgs sched_args(const cfpq_ctx *ctx, bool flush) {
     SchedArgs a = {};
@@ -725,7 +765,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const int lane_parallel = ctx->opts.sparse_mode != 1;  // modes 0 (quarter units) and 2 (tile units)
     const int tile_units = ctx->opts.sparse_mode == 2 ? 1 : 0;
     const int split = (lane_parallel && !(fuse && ctx->opts.fuse_apply)) ? 1 : 0;
-    const uint32_t split_budget = (uint32_t)std::min<size_t>(QUARTERS * plan.total_head_tiles, 0xffff0000u);
+    const uint32_t split_budget = split_slots_of(ctx, plan);
     const uint32_t heavy_base = (uint32_t)(QUARTERS * plan.total_head_tiles);
     const bool use_sched = head < 0 && !full_pass && plan.dirty_words && (sched_on(ctx) || ctx->tr_cap || flush);
     const SchedArgs sa = sched_args(ctx, flush);
@@ -735,13 +775,13 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                                (uint32_t)ctx->tr_cap);
         hipLaunchKernelGGL(k_build_sched, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems, sa, split,
-                           split_budget, heavy, heavy_base, tile_units);
+                           split_budget, heavy, heavy_base, tile_units, ctx->splits);
     } else {
         hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, (int)full_pass, c, ctx->units, ctx->items,
                            ctx->mitems, split, split_budget, heavy, heavy_base, h0,
                            (const StepCounters *)(prev_slot >= 0 && !full_pass ? ctx->ring + prev_slot : nullptr),
-                           tile_units);
+                           tile_units, ctx->splits);
     }
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
     // product kernels, timed: ev[4i] sparse ev[4i+1] dense ev[4i+2] mfma ev[4i+3]
@@ -757,10 +797,17 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
         const TermPtr *tp = ctx->d_tptr + (size_t)q * plan_terms(plan);
         if (tile_units)
             hipLaunchKernelGGL(k_product_sparse_lp<true>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
-                               heavy_base, TouchedLists{touched, seg_cap}, chunk, fz);
+                               heavy_base, TouchedLists{touched, seg_cap}, chunk, fz, ctx->partials, ctx->pmask);
         else
             hipLaunchKernelGGL(k_product_sparse_lp<false>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
-                               heavy_base, TouchedLists{touched, seg_cap}, chunk, fz);
+                               heavy_base, TouchedLists{touched, seg_cap}, chunk, fz, ctx->partials, ctx->pmask);
+        // the split quarters, from their sub-units' partial rows
+        if (split)
+            // (a grid of 4 waves per CU measured best: larger grids cost more
+            // in dispatch than the merge work they spread out)
+            hipLaunchKernelGGL(k_merge_split, dim3((uint32_t)ctx->num_cu * 4), dim3(WAVE), 0, ctx->stream, dm, dh, q, c,
+                               (const uint64_t *)ctx->splits, (const uint64_t *)ctx->partials,
+                               (const uint64_t *)ctx->pmask, touched);
     } else if (ctx->opts.sparse_batch == 1) {
         hipLaunchKernelGGL(k_product_sparse<1>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
                            fz);
@@ -792,7 +839,10 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const uint32_t *ids_q = ctx->d_ids + (q ? plan.all_delta[0].size() : 0);
     const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
     const bool retire_delta = prev_slot >= 0;
-    const uint32_t na = fuse ? 0u : (uint32_t)ctx->num_cu * 8;
+    // the apply waves run in both modes: without fused apply they merge the
+    // new rows into the heads, and either way they build the new quarters'
+    // column-major form
+    const uint32_t na = (uint32_t)ctx->num_cu * 32;
     const uint32_t nc = retire_delta ? (uint32_t)ctx->num_cu * 8 : 0u;
     const uint32_t nido = retire_delta ? nids_q : 0u, per = occ_blocks_per_matrix(plan);
     const StepCounters *prev = retire_delta ? ctx->ring + prev_slot : c;
@@ -800,7 +850,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     hipLaunchKernelGGL(k_retire, dim3(na + nc + nido * per + 1), dim3(WAVE), 0, ctx->stream,
                        (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, (const StepCounters *)c,
                        TouchedLists{touched, seg_cap}, prev, ptl, ids_q, nido, per, na, nc,
-                       ctx->ring + (slot + 1) % RING);
+                       ctx->ring + (slot + 1) % RING, fuse ? 0 : 1);
     if (prev_slot < 0 && !full_pass) {
         // a first step over delta terms: its delta buffers are cleared in full
         hipLaunchKernelGGL(k_clear_full, dim3(std::max<uint32_t>(1, ctx->num_cu * 4 / std::max<uint32_t>(1, nids_q)),
@@ -830,9 +880,15 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     CHK(grow(ctx, &ctx->d_ids, &ctx->d_ids_cap, nid));
     CHK(grow(ctx, &ctx->items, &ctx->items_cap, plan.total_head_tiles));
     // units: up to QUARTERS per head tile, then the sub-units of split
-    // quarters (at most one list of SPLIT budget plus one per quarter)
+    // quarters (at most the split budget plus one per quarter)
     CHK(grow(ctx, &ctx->units, &ctx->units_cap, 3 * QUARTERS * plan.total_head_tiles));
     CHK(grow(ctx, &ctx->mitems, &ctx->mitems_cap, plan.total_head_tiles));
+    {
+        const size_t slots = split_slots_of(ctx, plan);
+        CHK(grow(ctx, &ctx->partials, &ctx->partials_cap, slots * WAVE * TROW_WORDS));
+        CHK(grow(ctx, &ctx->pmask, &ctx->pmask_cap, slots));
+        CHK(grow(ctx, &ctx->splits, &ctx->splits_cap, 2 * (slots / 2 + 1)));
+    }
     // touched-tile lists: 1 + SHARDS segments of one entry per head tile
     const size_t list_need = (size_t)(1 + SHARDS) * plan.total_head_tiles;
     if (list_need > ctx->lists_cap || !ctx->lists[0]) {
@@ -853,7 +909,7 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     for (int par = 0; par < 2; ++par)
         for (size_t t = 0; t < terms.size(); ++t) {
             const MatDev &L = plan.mats[terms[t].left[par]], &R = plan.mats[terms[t].right[par]];
-            tptr[par * terms.size() + t] = TermPtr{L.occQ, R.occC, L.rmask, L.colq, L.bits, R.rmask, R.bits, R.cnt};
+            tptr[par * terms.size() + t] = TermPtr{L.occQ, R.occC, L.rmask, L.colq, L.colT, L.bits, R.rmask, R.bits, R.cnt};
         }
     HIPCHK(hipMemcpyAsync(ctx->d_tptr, tptr.data(), tptr.size() * sizeof(TermPtr), hipMemcpyHostToDevice, ctx->stream));
     std::vector<uint32_t> ids(plan.all_delta[0]);
@@ -890,6 +946,7 @@ static void accumulate(StepStats &s, const StepCounters &c) {
     s.tpd += c.tp_dense;
     s.tpm += c.tp_mfma;
     s.arows += c.a_rows;
+    s.acols += c.a_cols;
     s.brows += c.b_rows;
     s.orows += c.out_rows;
     s.brows_lds += c.b_rows_lds;
@@ -915,7 +972,8 @@ static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &p
             (uint64_t)(uintptr_t)ctx->d_tptr,
             (uint64_t)(uintptr_t)ctx->d_ids, (uint64_t)(uintptr_t)ctx->units, (uint64_t)(uintptr_t)ctx->items,
             (uint64_t)(uintptr_t)ctx->mitems, (uint64_t)(uintptr_t)ctx->lists[0], (uint64_t)(uintptr_t)ctx->lists[1],
-            (uint64_t)(uintptr_t)ctx->ring, plan.heads.size(), plan.mats.size(), plan.terms_delta.size(), plan.terms_live.size(),
+            (uint64_t)(uintptr_t)ctx->ring, (uint64_t)(uintptr_t)ctx->partials, (uint64_t)(uintptr_t)ctx->pmask,
+            (uint64_t)(uintptr_t)ctx->splits, (uint64_t)ctx->split_slots, plan.heads.size(), plan.mats.size(), plan.terms_delta.size(), plan.terms_live.size(),
             plan.all_delta[0].size(), plan.all_delta[1].size(), plan.max_tiles_per_head, plan.max_occ_words,
             plan.ids_all_off, plan.total_head_tiles, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
             (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk, (uint64_t)o.sched, o.sched_threshold,
@@ -1123,8 +1181,10 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             if (step_log)
                 std::fprintf(stderr,
                              "[cfpq step %u] units %u dense %u mfma %u touched %u new_bits %llu cand %llu hits %llu "
-                             "max_unit %u split %u a_rows %llu b_rows %llu busy %.3f wave_max %llu unit_max %llu clk"
-                             " | sparse %.3f dense %.3f mfma %.3f ms\n",
+                             "max_unit %u split %u a_rows %llu b_rows %llu busy %.3f "
+                             "wave_max %llu unit_max %llu clk (nsub %u cands %u pairs %u gathers %u; clk scan %u filter %u "
+                             "pairs %u epilogue %u) | sparse %.3f "
+                             "dense %.3f mfma %.3f ms\n",
                              steps - k + s, c.n_units + c.n_heavy, c.n_dense, c.n_mfma, c.n_touched,
                              (unsigned long long)c.new_bits, (unsigned long long)c.tile_products,
                              (unsigned long long)c.cand_hits, c.max_unit_cands, c.n_split, (unsigned long long)c.a_rows,
@@ -1132,6 +1192,12 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
                              c.sparse_max ? (double)c.sparse_busy / ((double)c.sparse_max * ctx->num_cu *
                                                                       ctx->opts.sparse_waves_per_cu) : 0.0,
                              (unsigned long long)c.sparse_max, (unsigned long long)c.sparse_unit_max,
+                             (unsigned)(c.sparse_unit_max_a & 31u), (unsigned)((c.sparse_unit_max_a >> 5) & 511u),
+                             (unsigned)((c.sparse_unit_max_a >> 14) & 1023u) * 16u,
+                             (unsigned)(c.sparse_unit_max_b & 0xffffffu),
+                             (unsigned)((c.sparse_unit_max_d >> 12) & 4095u) * 256u,
+                             (unsigned)((c.sparse_unit_max_c >> 12) & 4095u) * 256u,
+                             (unsigned)(c.sparse_unit_max_c & 4095u) * 256u, (unsigned)(c.sparse_unit_max_d & 4095u) * 256u,
                              step_ms[0], step_ms[1], step_ms[2]);
             accumulate(tot, c);
             sched_scored += c.n_scored;
@@ -1175,7 +1241,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         hipLaunchKernelGGL(k_retire, dim3(nc + nids_q * per + 1), dim3(WAVE), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q,
                            (const StepCounters *)(ctx->ring + slot), ptl, (const StepCounters *)(ctx->ring + prev_slot),
-                           ptl, ids_q, nids_q, per, 0u, nc, ctx->ring + slot);
+                           ptl, ids_q, nids_q, per, 0u, nc, ctx->ring + slot, 1);
         CHK(launch_check(ctx, "final clear"));
     }
     HIPCHK(hipEventRecord(ev_end, ctx->stream));
@@ -1209,6 +1275,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         stats->flush_steps = flush_steps;
         stats->cand_hits = tot.cand_hits;
         stats->a_rows_dense = tot.arows_d;
+        stats->a_cols = tot.acols;
         stats->b_rows_dense = tot.brows_d;
         stats->out_rows_dense = tot.orows_d;
         stats->max_unit_cands = tot.max_unit_cands;
@@ -1263,6 +1330,8 @@ static void plan_query(Plan &plan, cfpq_mat *const *M, const std::vector<cfpq_ma
         hd.p1 = (uint32_t)plan.terms_delta.size();
         hd.t1 = (uint32_t)plan.terms_live.size();
         hd.f1 = (uint32_t)plan.terms_full.size();
+        hd.colT = 0;
+        for (size_t r = 0; r < nrules; ++r) hd.colT |= rules[r].left == h ? 1u : 0u;
         plan.heads.push_back(hd);
         plan.max_tiles_per_head = std::max<size_t>(plan.max_tiles_per_head, (size_t)T * T);
         plan.total_head_tiles += (size_t)T * T;
@@ -1306,6 +1375,17 @@ extern "C" int cfpq_batch_fixpoint(cfpq_ctx *ctx, const cfpq_query *queries, siz
             }
     }
     HIPCHK(hipSetDevice(ctx->device));
+    // column-major forms: exact for every matrix read as a left operand; the
+    // heads no rule reads as one are not kept up to date by this call
+    for (size_t qi = 0; qi < nqueries; ++qi)
+        for (size_t r = 0; r < queries[qi].nrules; ++r) CHK(colT_fresh(ctx, queries[qi].mats[queries[qi].rules[r].left]));
+    for (size_t qi = 0; qi < nqueries; ++qi)
+        for (size_t r = 0; r < queries[qi].nrules; ++r) {
+            const cfpq_query &q = queries[qi];
+            bool left = false;
+            for (size_t r2 = 0; r2 < q.nrules; ++r2) left |= q.rules[r2].left == q.rules[r].head;
+            if (!left) q.mats[q.rules[r].head]->colT_stale = true;
+        }
     Plan plan;
     std::vector<cfpq_mat *> taken;
     int rc = CFPQ_OK;
@@ -1363,6 +1443,7 @@ static void add_mul_plan(Plan &plan, const cfpq_mat *c, const cfpq_mat *a, const
     hd.p0 = 0;
     hd.p1 = 1;
     hd.T = c->T;
+    hd.colT = 1;
     plan.heads.push_back(hd);
     TermDev t;
     t.left[0] = t.left[1] = 1;
@@ -1399,6 +1480,8 @@ extern "C" int cfpq_add_mul(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *a, const
     if (!ctx || !c || !a || !b) return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul: bad argument");
     if (a->n != c->n || b->n != c->n) return fail(ctx, CFPQ_E_INVALID, "cfpq_add_mul: size mismatch");
     HIPCHK(hipSetDevice(ctx->device));
+    CHK(colT_fresh(ctx, a));
+    CHK(colT_fresh(ctx, c));  // the head's form is kept up to date from here
     cfpq_mat *S = nullptr;
     CHK(pool_take(ctx, c->n, &S));
     Plan plan;
@@ -1422,14 +1505,18 @@ struct NoallocBuffers {
     uint64_t *items, *units, *mitems, *lists[2];
     StepCounters *ring;
     uint32_t *dirty, *dirty_off;
-    size_t caps[11];
+    uint64_t *partials, *pmask, *splits;
+    uint32_t split_slots;
+    size_t caps[14];
     explicit NoallocBuffers(cfpq_ctx *c) : ctx(c) {
         d_mats = c->d_mats; d_heads = c->d_heads; d_terms = c->d_terms; d_tptr = c->d_tptr; d_ids = c->d_ids;
         items = c->items; units = c->units; mitems = c->mitems; lists[0] = c->lists[0]; lists[1] = c->lists[1];
         ring = c->ring; dirty = c->dirty; dirty_off = c->dirty_off;
-        const size_t cv[11] = {c->d_mats_cap, c->d_heads_cap, c->d_terms_cap, c->d_tptr_cap, c->d_ids_cap, c->items_cap,
-                               c->units_cap, c->mitems_cap, c->lists_cap, c->dirty_cap, c->dirty_off_cap};
-        std::copy(cv, cv + 11, caps);
+        partials = c->partials; pmask = c->pmask; splits = c->splits; split_slots = c->split_slots;
+        const size_t cv[14] = {c->d_mats_cap, c->d_heads_cap, c->d_terms_cap, c->d_tptr_cap, c->d_ids_cap, c->items_cap,
+                               c->units_cap, c->mitems_cap, c->lists_cap, c->dirty_cap, c->dirty_off_cap,
+                               c->partials_cap, c->pmask_cap, c->splits_cap};
+        std::copy(cv, cv + 14, caps);
         char *base = c->noalloc_base;
         c->d_mats = (MatDev *)(base + NA_MATS);       c->d_mats_cap = NA_MATS_CAP;
         c->d_heads = (HeadDev *)(base + NA_HEADS);    c->d_heads_cap = NA_HEADS_CAP;
@@ -1444,6 +1531,10 @@ struct NoallocBuffers {
         c->ring = (StepCounters *)(base + NA_RING);
         c->dirty = (uint32_t *)(base + NA_DIRTY);     c->dirty_cap = NA_DIRTY_CAP;
         c->dirty_off = (uint32_t *)(base + NA_DIRTY_OFF); c->dirty_off_cap = 1;
+        c->partials = (uint64_t *)(base + NA_PARTIALS); c->partials_cap = (size_t)NA_SPLIT_SLOTS * WAVE * TROW_WORDS;
+        c->pmask = (uint64_t *)(base + NA_PMASK);     c->pmask_cap = NA_SPLIT_SLOTS;
+        c->splits = (uint64_t *)(base + NA_SPLITS);   c->splits_cap = 2 * (NA_SPLIT_SLOTS / 2 + 1);
+        c->split_slots = NA_SPLIT_SLOTS;
         c->no_alloc = true;
     }
     ~NoallocBuffers() {
@@ -1451,6 +1542,8 @@ struct NoallocBuffers {
         c->d_mats = d_mats; c->d_heads = d_heads; c->d_terms = d_terms; c->d_tptr = d_tptr; c->d_ids = d_ids;
         c->items = items; c->units = units; c->mitems = mitems; c->lists[0] = lists[0]; c->lists[1] = lists[1];
         c->ring = ring; c->dirty = dirty; c->dirty_off = dirty_off;
+        c->partials = partials; c->pmask = pmask; c->splits = splits; c->split_slots = split_slots;
+        c->partials_cap = caps[11]; c->pmask_cap = caps[12]; c->splits_cap = caps[13];
         c->d_mats_cap = caps[0]; c->d_heads_cap = caps[1]; c->d_terms_cap = caps[2]; c->d_tptr_cap = caps[3];
         c->d_ids_cap = caps[4]; c->items_cap = caps[5]; c->units_cap = caps[6]; c->mitems_cap = caps[7];
         c->lists_cap = caps[8]; c->dirty_cap = caps[9]; c->dirty_off_cap = caps[10];
@@ -1471,6 +1564,8 @@ extern "C" int cfpq_add_mul_noalloc(cfpq_ctx *ctx, cfpq_mat *c, const cfpq_mat *
     }
     std::lock_guard<std::mutex> lock(g_noalloc_call_mu[ctx->device]);
     HIPCHK(hipSetDevice(ctx->device));
+    CHK(colT_fresh(ctx, a));
+    CHK(colT_fresh(ctx, c));
     // the delta matrix S at the start of the array, cleared here: the array
     // keeps whatever the last call on this device left in it
     MatDev S = {};

@@ -82,6 +82,9 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    return ((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | uniform((uint32_t)v);
+}
 
 // Wave-uniform read of memory that no thread writes during the current
 // kernel (operand metadata, work lists): going through the constant address
@@ -214,6 +217,49 @@ __device__ __forceinline__ uint32_t scan_ballot(uint32_t v, uint64_t bal, uint32
 __device__ __forceinline__ uint64_t *colq_of(const MatDev &M, uint32_t p, uint32_t q) {
     return M.colq + ((size_t)p * QUARTERS + q) * TROW_WORDS;
 }
+__device__ __forceinline__ uint64_t *colT_of(const MatDev &M, uint32_t p, uint32_t q) {
+    return M.colT + ((size_t)p * QUARTERS + q) * TILE;
+}
+
+// Transpose of a 64 x 64 bit block held one row per lane (bit j of lane i =
+// element (i, j)): afterwards lane i holds column i (bit j = element (j, i)).
+// Six butterfly stages; stage s swaps the off-diagonal s x s sub-blocks of
+// every 2s x 2s block between lanes i and i ^ s.
+__device__ __forceinline__ uint64_t transpose64(uint64_t x) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                               0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const int sft = 32 >> t;
+        const uint64_t m = masks[t];
+        const uint32_t lo = __shfl_xor((uint32_t)x, sft, 64), hi = __shfl_xor((uint32_t)(x >> 32), sft, 64);
+        const uint64_t y = ((uint64_t)hi << 32) | lo;
+        x = (lane & sft) ? ((x & ~m) | ((y & ~m) >> sft)) : ((x & m) | ((y & m) << sft));
+    }
+    return x;
+}
+
+// Column-major form of one 64-row quarter, whole wave: lane r holds row r in
+// v, cu is the OR of the rows.  Word k of dst becomes the mask of the rows
+// holding column k, for every column k of cu; with orinto, word k of orinto
+// is OR-ed with it as well (the caller is the only writer of both).  Words of
+// columns outside cu are left as they are: no reader reads them.  Lane l
+// computes the words of columns l, 64 + l, 128 + l and 192 + l.
+__device__ __forceinline__ void quarter_transpose(const Row4 &v, const Row4 &cu, uint64_t *dst, uint64_t *orinto) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        if (uniform64(cu.w[w]) == 0) continue;
+        const uint64_t col = transpose64(v.w[w]);
+        if ((cu.w[w] >> lane) & 1ull) {
+            const uint32_t k = (uint32_t)w * 64u + (uint32_t)lane;
+            gst(dst + k, col);
+            if (orinto) gst(orinto + k, gld(orinto + k) | col);
+        }
+    }
+}
+
 __device__ __forceinline__ Row4 ld_const_row(const uint64_t *p) {
     const __attribute__((address_space(4))) uint64_t *c = (const __attribute__((address_space(4))) uint64_t *)p;
     Row4 r;
@@ -287,39 +333,6 @@ __device__ __forceinline__ QuarterOut quarter_epilogue(const MatDev &O, const Ma
         }
     }
     return QuarterOut{bal, pc};
-}
-
-// Epilogue of one sub-unit of a split quarter.  Sub-unit s of n owns the
-// quarter's rows r = s (mod n): its rows are its alone and are written as in
-// quarter_epilogue; the quarter's row mask and column union are shared with
-// the sibling sub-units and merged with atomic ORs.  No in-place head update
-// (splitting is off when that is on).
-__device__ __forceinline__ QuarterOut quarter_epilogue_rows(const MatDev &O, const MatDev &M, uint32_t p, uint32_t q,
-                                                            uint32_t I, uint32_t J, const Row4 &acc,
-                                                            uint32_t &out_rows) {
-    const int lane = threadIdx.x & 63;
-    const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
-    Row4 nw = {{0, 0, 0, 0}};
-    if (row_nonzero(acc)) {
-        const Row4 m = load_row(M.bits + off);
-#pragma unroll
-        for (int w = 0; w < 4; ++w) nw.w[w] = acc.w[w] & ~m.w[w];
-        if (row_nonzero(nw)) {
-            store_row(O.bits + off, nw);
-            ++out_rows;
-        }
-    }
-    const uint64_t bal = __ballot(row_nonzero(nw));
-    const RowsUnion ru = rows_union(nw, row_popc(nw), bal);
-    if (bal && lane == 0) {
-        g_or(&O.rmask[(size_t)p * 4 + q], bal);
-        uint64_t *cq = colq_of(O, p, q);
-#pragma unroll
-        for (int w = 0; w < 4; ++w)
-            if (ru.cu.w[w]) g_or(cq + w, ru.cu.w[w]);
-        set_occq(O, q, I, J);
-    }
-    return QuarterOut{bal, ru.sum};
 }
 
 // ---------------------------------------------------------------------------
@@ -397,14 +410,20 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
 
 // Append the sparse units of one (head, C tile): quarters in qmask, a heavy
 // quarter (more than SPLIT_CANDS candidates) as up to SPLIT_MAX sub-units
-// while the split budget lasts.  Whole wave: the single units of all four
+// while the split budget lasts.  Sub-unit s of a split quarter takes the
+// candidates g = s (mod nsub) of the quarter and leaves its rows in a slot
+// of the partial-row buffer (slot = its index in the sub-unit list); the
+// split quarter is listed in `splits` (the base item with nsub, and the
+// first slot) for k_merge_split, which ORs the partial rows and writes the
+// quarter.  Whole wave: the single units of all four
 // quarters take one atomic on the list count per wave (the count is one
 // address that every wave of the build hits).  Sub-units go to their own list
 // at units + heavy_base (count n_heavy), which the sparse kernel claims
 // first: the longest units start early instead of ending the launch late.
 __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const uint32_t *qcnt, uint64_t base,
                                              int split, uint32_t split_budget, StepCounters *__restrict__ c,
-                                             uint64_t *__restrict__ units, uint32_t heavy_base, int tile_units) {
+                                             uint64_t *__restrict__ units, uint32_t heavy_base, int tile_units,
+                                             uint64_t *__restrict__ splits) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
     // tile units: one unit for every quarter of the tile with candidates,
@@ -427,7 +446,9 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
         if (wq && split && qcnt[q] > SPLIT_CANDS) {
             uint32_t ns = (qcnt[q] + SPLIT_CANDS - 1) / SPLIT_CANDS;
             ns = ns > SPLIT_MAX ? SPLIT_MAX : (ns < 1 ? 1 : ns);
-            if (ns > 1 && atomicAdd(&c->split_extra, ns - 1) + (ns - 1) > split_budget) ns = 1;
+            // the budget counts sub-units (partial-row slots); a refused
+            // reservation only leaves budget unused
+            if (ns > 1 && atomicAdd(&c->split_extra, ns) + ns > split_budget) ns = 1;
             nsub[q] = ns;
         }
         single[q] = __ballot(nsub[q] == 1);
@@ -446,11 +467,12 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
 #pragma unroll
     for (uint32_t q = 0; q < QUARTERS; ++q) {
         if (nsub[q] > 1) {
-            const uint64_t u = base | (uint64_t)q;
-            const uint32_t at = heavy_base + atomicAdd(&c->n_heavy, nsub[q]);
-            for (uint32_t sidx = 0; sidx < nsub[q]; ++sidx)
-                units[at + sidx] = u | ((uint64_t)sidx << 55) | ((uint64_t)(nsub[q] - 1) << 59);
-            atomicAdd(&c->n_split, 1u);
+            const uint64_t u = base | (uint64_t)q | ((uint64_t)(nsub[q] - 1) << 59);
+            const uint32_t at = atomicAdd(&c->n_heavy, nsub[q]);
+            for (uint32_t sidx = 0; sidx < nsub[q]; ++sidx) units[heavy_base + at + sidx] = u | ((uint64_t)sidx << 55);
+            const uint32_t sq = atomicAdd(&c->n_split, 1u);
+            splits[2 * (size_t)sq] = u;
+            splits[2 * (size_t)sq + 1] = at;
         }
     }
 }
@@ -464,7 +486,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
                                                  uint64_t *__restrict__ mfma_items, int split,
                                                  uint32_t split_budget, int heavy, uint32_t heavy_base,
                                                  uint32_t h0, const StepCounters *__restrict__ prevc,
-                                                 int tile_units) {
+                                                 int tile_units, uint64_t *__restrict__ splits) {
     // a delta step after one that added no bit has an empty delta: no work
     // (the steps the host enqueues past convergence end here)
     if (prevc) {
@@ -492,7 +514,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
     append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units,
-                 heavy_base, tile_units);
+                 heavy_base, tile_units, splits);
 }
 
 // ---------------------------------------------------------------------------
@@ -636,6 +658,27 @@ __device__ __forceinline__ int select_bit(uint64_t m, uint32_t j) {
     return r;
 }
 
+// (hit, shared column) items a lane of the lane-parallel sparse kernel takes
+// per round in its column-major gathers: their loads are in flight together.
+constexpr int COL_ITEMS = 4;
+
+// Column of the j-th (0-based) set bit of a 256-bit row mask (j < popcount).
+__device__ __forceinline__ uint32_t select_bit256(const Row4 &m, uint32_t j) {
+    uint32_t base = 0, k = 0;
+    bool found = false;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const uint32_t c = (uint32_t)__popcll(m.w[w]);
+        if (!found && j < c) {
+            k = base + (uint32_t)select_bit(m.w[w], j);
+            found = true;
+        }
+        if (!found) j -= c;
+        base += 64;
+    }
+    return k;
+}
+
 // m with its lowest j set bits cleared (j <= popcount(m)).
 __device__ __forceinline__ Row4 bits_from(Row4 m, uint32_t j) {
 #pragma unroll
@@ -703,7 +746,7 @@ __device__ __forceinline__ uint32_t gather_chunk(Row4 x, const uint64_t *Bt, Row
 // The operand arrays of one term, staged per scan lane in LDS for the filter
 // and gather stages (a subset of TermPtr).
 struct LaneOps {
-    const uint64_t *lrmask, *lcolq, *lbits, *rrmask, *rbits;
+    const uint64_t *lrmask, *lcolq, *lcolT, *lbits, *rrmask, *rbits;
     const uint32_t *rcnt;
 };
 
@@ -736,7 +779,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                                             StepCounters *__restrict__ c,
                                                             const uint64_t *__restrict__ units,
                                                             uint32_t heavy_base, TouchedLists tl, uint32_t chunk,
-                                                            int fuse) {
+                                                            int fuse, uint64_t *__restrict__ partials,
+                                                            uint64_t *__restrict__ pmask) {
     constexpr uint32_t ACC_ROWS = QSET ? TILE : WAVE;
     __shared__ Row4 acc_s[ACC_ROWS];  // rows of the unit's quarters (row 64 q + r with QSET, else r)
     __shared__ Row4 s_brm[WAVE];
@@ -770,9 +814,10 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     const uint32_t nblk = n_claim / 64u;
     const uint32_t home = blockIdx.x % SHARDS;
     CounterShard *shards = c->shard;
-    uint32_t tp = 0, arows = 0, brows = 0, out_rows = 0, hits_n = 0, max_cands = 0;
+    uint32_t tp = 0, arows = 0, acols = 0, brows = 0, out_rows = 0, hits_n = 0, max_cands = 0;
     uint64_t new_bits = 0;  // lane 0: new bits of this wave's units
     uint64_t unit_max = 0;  // longest unit of this wave (clocks)
+    uint64_t unit_max_a = 0, unit_max_b = 0, unit_max_c = 0, unit_max_d = 0;  // its shape (CounterShard)
     uint32_t sh = home, used_up = 0;
     while (used_up < SHARDS) {
     uint32_t kc = 0xffffffffu;
@@ -808,24 +853,22 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         // the output descriptors are read now, while the unit's first loads are in flight
         const MatDev O = mats[uniform(heads[h].out[parity])];
         const MatDev M = mats[uniform(heads[h].target)];
-        const uint32_t tp_unit0 = tp;
-        // a split quarter's sub-unit `sub` of `nsub` takes the rows r = sub
-        // (mod nsub) of every candidate, so that a dense quarter's gathers are
-        // spread over several waves and each row keeps a single writer
-        uint64_t rowsel = ~0ull;
-        if (nsub > 1) {
-            rowsel = 0;
-            for (uint32_t r = sub; r < WAVE; r += nsub) rowsel |= 1ull << r;
-        }
+        const uint32_t tp_unit0 = tp, br_unit0 = brows;
+        uint32_t unit_pairs = 0;  // (hit, shared column) items of the unit (its shape, for the step log)
+        // a split quarter's sub-unit `sub` of `nsub` takes the candidates
+        // g = sub (mod nsub), so that a heavy quarter's gathers are spread over
+        // several waves; its rows go to a partial-row slot (k_merge_split)
 #pragma unroll
         for (uint32_t qq = 0; qq < QUARTERS; ++qq)
             if ((qset >> qq) & 1u) acc_s[(QSET ? qq * WAVE : 0u) + lane] = Row4{{0, 0, 0, 0}};
         __syncthreads();
         const uint32_t ow = (T + 31) / 32;
         const uint32_t ntw = (t1 - t0) * ow;  // (term, occupancy word) pairs of the unit
+        uint64_t c_scan = 0, c_filter = 0, c_pairs = 0;  // stage clocks of the unit (step log)
         for (uint32_t tw0 = 0; tw0 < ntw; tw0 += WAVE) {
             // 1. candidate masks of 64 (term, word) pairs at once, compacted by
             //    a prefix sum of their popcounts
+            const uint64_t t_scan = __builtin_amdgcn_s_memtime();
             const uint32_t j = tw0 + lane;
             uint32_t kq[QUARTERS] = {0, 0, 0, 0};
             LaneOps ops = {};
@@ -836,11 +879,12 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
 #pragma unroll
                 for (uint32_t qq = 0; qq < QUARTERS; ++qq)
                     if ((qset >> qq) & 1u) kq[qq] = gld(&e.occQ[(qq * T + I) * ow + w]) & col;
-                ops = LaneOps{e.lrmask, e.lcolq, e.lbits, e.rrmask, e.rbits, e.rcnt};
+                ops = LaneOps{e.lrmask, e.lcolq, e.lcolT, e.lbits, e.rrmask, e.rbits, e.rcnt};
             }
             const uint32_t kbw = (uint32_t)(__popc(kq[0]) + __popc(kq[1]) + __popc(kq[2]) + __popc(kq[3]));
             uint32_t total;
             const uint32_t incw = scan_ballot(kbw, __ballot(kbw != 0), total);
+            c_scan += __builtin_amdgcn_s_memtime() - t_scan;
             if (!total) continue;
 #pragma unroll
             for (uint32_t qq = 0; qq < QUARTERS; ++qq)
@@ -851,14 +895,17 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             __syncthreads();
             for (uint32_t cb = 0; cb < total; cb += WAVE) {
                 const uint32_t nc = (total - cb < WAVE) ? total - cb : WAVE;
-                if (sub == 0) tp += nc;  // a split quarter's candidates count once
+                const uint64_t t_filter = __builtin_amdgcn_s_memtime();
                 // 2. filter up to 64 candidates: lane g takes the g-th candidate
+                //    (of a sub-unit: the candidates of its residue class)
                 const uint32_t g = cb + lane;
+                const bool mine = (uint32_t)lane < nc && (nsub == 1 || g % nsub == sub);
+                tp += (uint32_t)__popcll(__ballot(mine));
                 uint64_t arm = 0;
                 Row4 brm = {{0, 0, 0, 0}};
                 const uint64_t *abase = nullptr, *bbase = nullptr;
                 uint32_t cq = 0;  // the candidate's quarter
-                if ((uint32_t)lane < nc) {
+                if (mine) {
                     uint32_t lo = 0, hi = WAVE - 1;
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi) >> 1;
@@ -888,11 +935,14 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     const size_t lt = (size_t)I * T + K, rt = (size_t)K * T + J;
                     brm = load_row(o.rrmask + rt * 4);
                     const Row4 cu = load_row(o.lcolq + (lt * QUARTERS + q) * TROW_WORDS);
-                    arm = gld(&o.lrmask[lt * 4 + q]) & rowsel;
+                    arm = gld(&o.lrmask[lt * 4 + q]);
                     const bool hit = ((cu.w[0] & brm.w[0]) | (cu.w[1] & brm.w[1]) | (cu.w[2] & brm.w[2]) |
                                       (cu.w[3] & brm.w[3])) != 0;
                     if (!hit) arm = 0;
-                    abase = o.lbits + lt * TILE_WORDS + (size_t)cq * WAVE * TROW_WORDS;
+                    // column-major gathers: the columns the A quarter and the B
+                    // tile's non-empty rows share, and the quarter's column words
+                    brm = Row4{{cu.w[0] & brm.w[0], cu.w[1] & brm.w[1], cu.w[2] & brm.w[2], cu.w[3] & brm.w[3]}};
+                    abase = o.lcolT + (lt * QUARTERS + q) * TILE;
                     bbase = o.rbits + rt * TILE_WORDS;
                     // a full B tile needs no gathers: bit 0 of the (8-byte aligned) pointer marks it
                     if (gld(&o.rcnt[rt]) == TILE_BITS) bbase = (const uint64_t *)((uintptr_t)bbase | 1u);
@@ -903,9 +953,12 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                 const uint64_t hmask = __ballot(hit);
                 if (!hmask) continue;
                 const uint32_t nh = __popcll(hmask);
-                const uint32_t cnt = hit ? (uint32_t)__popcll(arm) : 0u;
+                // work items of a hit: the columns it shares with the B tile
+                // (one item for a full B tile, whose rows are all ones)
+                const uint32_t cnt = !hit ? 0u : ((uintptr_t)bbase & 1u) ? 1u : row_popc(brm);
                 uint32_t pairs;
                 const uint32_t incl = scan_ballot(cnt, hmask, pairs);
+                unit_pairs += pairs;
                 if (hit) {
                     const uint32_t slot = __popcll(hmask & ((1ull << lane) - 1ull));
                     s_arm[slot] = arm;
@@ -916,123 +969,88 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     s_bb[slot] = bbase;
                 }
                 hits_n += hit;
-                arows += cnt;
+                acols += ((uintptr_t)bbase & 1u) ? 0u : cnt;
                 __syncthreads();
-                for (uint32_t base = 0; base < pairs; base += WAVE) {
-                    // 4a. lane gp loads the A row of pair gp; x = the B rows it names
-                    const uint32_t gp = base + lane;
-                    Row4 x = {{0, 0, 0, 0}};
-                    uint32_t r = 0;  // accumulator row: 64 * quarter + row in the quarter
-                    const uint64_t *Bt = nullptr;
-                    if (gp < pairs) {
-                        uint32_t lo = 0, hi = nh - 1;  // owner: first slot whose range ends past gp
-                        while (lo < hi) {
-                            const uint32_t mid = (lo + hi) >> 1;
-                            if (s_end[mid] > gp) hi = mid;
-                            else lo = mid + 1;
-                        }
-                        const uint64_t am = s_arm[lo];
-                        const uint32_t rq = (uint32_t)select_bit(am, gp - (s_end[lo] - (uint32_t)__popcll(am)));
-                        r = (QSET ? s_hq[lo] * WAVE : 0u) + rq;
-                        Bt = s_bb[lo];
-                        if ((uintptr_t)Bt & 1u) {  // full B tile: the row becomes all ones
+                const uint64_t t_pairs = __builtin_amdgcn_s_memtime();
+                c_filter += t_pairs - t_filter;
+                {
+                    // 4. column-major gathers: work item gp is (hit, shared
+                    //     column k).  The quarter's column word k (the rows
+                    //     holding k) and B row k are loaded together, and B row
+                    //     k is OR-ed into each of those rows.  No A row is
+                    //     loaded for nothing, and no B row twice per hit.
+                    for (uint32_t base = 0; base < pairs; base += WAVE * COL_ITEMS) {
+                        // COL_ITEMS items per lane: all their loads first
+                        uint64_t rows[COL_ITEMS];
+                        Row4 b[COL_ITEMS];
+                        uint32_t r0[COL_ITEMS];
 #pragma unroll
-                            for (int i = 0; i < 4; ++i) atomicOr((unsigned long long *)&acc_s[r].w[i], ~0ull);
-                        } else {
-                            const Row4 bm = s_brm[lo];
-                            const Row4 a = load_row(s_ab[lo] + rq * TROW_WORDS);
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) x.w[i] = a.w[i] & bm.w[i];
-                        }
-                    }
-                    // 4b. the B-row gathers.  Usually each lane ORs the B rows its
-                    //     own pair names, four loads in flight.  When a pair names
-                    //     more than GATHER_HEAVY rows (a hub row), the rows of all
-                    //     64 pairs are cut into chunks of GATHER_CH, numbered by a
-                    //     prefix sum and dealt out over the lanes, so the hub's
-                    //     gathers are spread over the wave instead of serializing
-                    //     one lane.
-                    if (__ballot(row_popc(x) > GATHER_HEAVY) == 0) {
-                        Row4 v = {{0, 0, 0, 0}};
-                        // the set bits of all four words in one stream, GATHER_ROUND
-                        // loads in flight per round (a pair's few rows often span words)
-                        uint64_t w0 = x.w[0], w1 = x.w[1], w2 = x.w[2], w3 = x.w[3];
-                        while (w0 | w1 | w2 | w3) {
-                            int ks[GATHER_ROUND];
-                            int nk = 0;
-#pragma unroll
-                            for (int jj = 0; jj < GATHER_ROUND; ++jj) {
-                                int k = 0;
-                                if (w0) {
-                                    k = __builtin_ctzll(w0);
-                                    w0 &= w0 - 1;
-                                    nk = jj + 1;
-                                } else if (w1) {
-                                    k = 64 + __builtin_ctzll(w1);
-                                    w1 &= w1 - 1;
-                                    nk = jj + 1;
-                                } else if (w2) {
-                                    k = 128 + __builtin_ctzll(w2);
-                                    w2 &= w2 - 1;
-                                    nk = jj + 1;
-                                } else if (w3) {
-                                    k = 192 + __builtin_ctzll(w3);
-                                    w3 &= w3 - 1;
-                                    nk = jj + 1;
-                                }
-                                ks[jj] = k;
-                            }
-                            Row4 b[GATHER_ROUND];
-#pragma unroll
-                            for (int jj = 0; jj < GATHER_ROUND; ++jj)
-                                b[jj] = jj < nk ? load_row(Bt + ks[jj] * TROW_WORDS) : Row4{{0, 0, 0, 0}};
-#pragma unroll
-                            for (int jj = 0; jj < GATHER_ROUND; ++jj) or_row(v, b[jj]);
-                            brows += nk;
-                        }
-#pragma unroll
-                        for (int i = 0; i < 4; ++i)
-                            if (v.w[i]) atomicOr((unsigned long long *)&acc_s[r].w[i], (unsigned long long)v.w[i]);
-                    } else {
-                        const uint32_t nch = (row_popc(x) + GATHER_CH - 1) / GATHER_CH;
-                        const uint32_t incl = wave_scan_incl(nch);
-                        const uint32_t tot = lane_read(incl, WAVE - 1);
-                        s_gx[lane] = x;
-                        s_gr[lane] = r;
-                        s_gb[lane] = Bt;
-                        s_gend[lane] = incl;
-                        __syncthreads();
-                        for (uint32_t cb = 0; cb < tot; cb += WAVE) {
-                            const uint32_t g = cb + lane;
-                            if (g < tot) {
-                                uint32_t lo = 0, hi = WAVE - 1;  // owner: first lane whose range ends past g
+                        for (int j = 0; j < COL_ITEMS; ++j) {
+                            const uint32_t gp = base + (uint32_t)j * WAVE + lane;
+                            rows[j] = 0;
+                            b[j] = Row4{{0, 0, 0, 0}};
+                            r0[j] = 0;
+                            if (gp < pairs) {
+                                uint32_t lo = 0, hi = nh - 1;  // owner: first slot whose range ends past gp
                                 while (lo < hi) {
                                     const uint32_t mid = (lo + hi) >> 1;
-                                    if (s_gend[mid] > g) hi = mid;
+                                    if (s_end[mid] > gp) hi = mid;
                                     else lo = mid + 1;
                                 }
-                                const Row4 xo = s_gx[lo];
-                                const uint32_t first = (g - (s_gend[lo] - (row_popc(xo) + GATHER_CH - 1) / GATHER_CH)) *
-                                                       GATHER_CH;
-                                brows += gather_chunk(bits_from(xo, first), s_gb[lo], &acc_s[s_gr[lo]]);
+                                r0[j] = QSET ? s_hq[lo] * WAVE : 0u;
+                                const uint64_t *Bt = s_bb[lo];
+                                if ((uintptr_t)Bt & 1u) {  // full B tile: every row of the quarter becomes all ones
+                                    for (uint64_t rr = s_arm[lo]; rr; rr &= rr - 1) {
+                                        const uint32_t r = r0[j] + (uint32_t)__builtin_ctzll(rr);
+#pragma unroll
+                                        for (int i = 0; i < 4; ++i) atomicOr((unsigned long long *)&acc_s[r].w[i], ~0ull);
+                                    }
+                                } else {
+                                    const Row4 rel = s_brm[lo];
+                                    const uint32_t k = select_bit256(rel, gp - (s_end[lo] - row_popc(rel)));
+                                    rows[j] = gld(s_ab[lo] + k);
+                                    b[j] = load_row(Bt + k * TROW_WORDS);
+                                    ++brows;
+                                }
                             }
                         }
-                        __syncthreads();  // the chunk staging is refilled by the next pair round
+#pragma unroll
+                        for (int j = 0; j < COL_ITEMS; ++j) {
+                            for (uint64_t rr = rows[j]; rr; rr &= rr - 1) {
+                                const uint32_t r = r0[j] + (uint32_t)__builtin_ctzll(rr);
+#pragma unroll
+                                for (int i = 0; i < 4; ++i)
+                                    if (b[j].w[i]) atomicOr((unsigned long long *)&acc_s[r].w[i], (unsigned long long)b[j].w[i]);
+                            }
+                        }
                     }
                 }
+                c_pairs += __builtin_amdgcn_s_memtime() - t_pairs;
                 __syncthreads();  // the hit staging is refilled by the next candidate round
             }
             __syncthreads();  // the candidate staging is refilled by the next (term, word) chunk
         }
         __syncthreads();
+        const uint64_t t_epi = __builtin_amdgcn_s_memtime();
         max_cands = max(max_cands, tp - tp_unit0);
         uint32_t pc = 0;  // new bits of the unit's quarters
-#pragma unroll
-        for (uint32_t qq = 0; qq < QUARTERS; ++qq) {
-            if (!((qset >> qq) & 1u)) continue;
+        if (nsub > 1) {
+            // a sub-unit of a split quarter: its rows go to its partial-row
+            // slot (its index u in the sub-unit list), non-empty ones only,
+            // with their mask; k_merge_split finishes the quarter
+            const uint32_t qq = (uint32_t)__builtin_ctz(qset);
             const Row4 acc = acc_s[(QSET ? qq * WAVE : 0u) + lane];
-            pc += nsub > 1 ? quarter_epilogue_rows(O, M, p, qq, I, J, acc, out_rows).pc
-                           : quarter_epilogue(O, M, p, qq, I, J, acc, fuse, out_rows).pc;
+            const bool nz = row_nonzero(acc);
+            if (nz) store_row(partials + ((size_t)u * WAVE + lane) * TROW_WORDS, acc);
+            const uint64_t bal = __ballot(nz);
+            if (lane == 0) gst(&pmask[u], bal);
+        } else {
+#pragma unroll
+            for (uint32_t qq = 0; qq < QUARTERS; ++qq) {
+                if (!((qset >> qq) & 1u)) continue;
+                const Row4 acc = acc_s[(QSET ? qq * WAVE : 0u) + lane];
+                pc += quarter_epilogue(O, M, p, qq, I, J, acc, fuse, out_rows).pc;
+            }
         }
         if (pc && lane == 0) {
             const uint32_t old = g_add(&O.cnt[p], pc);
@@ -1043,16 +1061,28 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             }
             new_bits += pc;
         }
-        const uint64_t tu = __builtin_amdgcn_s_memtime() - tu0;
-        unit_max = tu > unit_max ? tu : unit_max;
+        const uint64_t t_end = __builtin_amdgcn_s_memtime();
+        const uint64_t tu = t_end - tu0;
+        const uint32_t ucand = min(tp - tp_unit0, 511u), upairs = min(unit_pairs / 16u, 1023u);
+        const uint32_t ugath = min(wave_sum(brows - br_unit0), 0xffffffu);
+        if (tu > unit_max) {
+            unit_max = tu;
+            unit_max_a = (tu << 24) | min(nsub, 31u) | (ucand << 5) | (upairs << 14);
+            unit_max_b = (tu << 24) | ugath;
+            auto cap12 = [](uint64_t v) -> uint64_t { return (v >> 8) < 4095u ? (v >> 8) : 4095u; };
+            unit_max_c = (tu << 24) | (cap12(c_filter) << 12) | cap12(c_pairs);
+            unit_max_d = (tu << 24) | (cap12(c_scan) << 12) | cap12(t_end - t_epi);
+        }
         __syncthreads();  // the accumulator is reset by the next unit
     }
     }
     const uint32_t ar = wave_sum(arows), br = wave_sum(brows), orow = wave_sum(out_rows), hs = wave_sum(hits_n);
+    const uint32_t ac = wave_sum(acols);
     if (lane == 0) {
         CounterShard &S = shards[home];
         if (tp) atomicAdd((unsigned long long *)&S.tile_products, (unsigned long long)tp);
         if (ar) atomicAdd((unsigned long long *)&S.a_rows, (unsigned long long)ar);
+        if (ac) atomicAdd((unsigned long long *)&S.a_cols, (unsigned long long)ac);
         if (br) atomicAdd((unsigned long long *)&S.b_rows, (unsigned long long)br);
         if (orow) atomicAdd((unsigned long long *)&S.out_rows, (unsigned long long)orow);
         if (hs) atomicAdd((unsigned long long *)&S.cand_hits, (unsigned long long)hs);
@@ -1062,6 +1092,63 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         atomicAdd((unsigned long long *)&S.busy, (unsigned long long)busy);
         atomicMax((unsigned long long *)&S.busy_max, (unsigned long long)busy);
         if (unit_max) atomicMax((unsigned long long *)&S.unit_max, (unsigned long long)unit_max);
+        if (unit_max_a) atomicMax((unsigned long long *)&S.unit_max_a, (unsigned long long)unit_max_a);
+        if (unit_max_b) atomicMax((unsigned long long *)&S.unit_max_b, (unsigned long long)unit_max_b);
+        if (unit_max_c) atomicMax((unsigned long long *)&S.unit_max_c, (unsigned long long)unit_max_c);
+        if (unit_max_d) atomicMax((unsigned long long *)&S.unit_max_d, (unsigned long long)unit_max_d);
+    }
+}
+
+// The split quarters of a step (k_build's list `splits`: base item with
+// nsub, first partial-row slot): one wave per quarter ORs the partial rows of
+// its sub-units (written by the sparse product, launched before) and writes
+// the quarter as a single owner, like the unsplit units' epilogue.  Split
+// units exist only without the fused head update.
+__global__ __launch_bounds__(WAVE) void k_merge_split(const MatDev *__restrict__ mats,
+                                                      const HeadDev *__restrict__ heads, int parity,
+                                                      StepCounters *__restrict__ c, const uint64_t *__restrict__ splits,
+                                                      const uint64_t *__restrict__ partials,
+                                                      const uint64_t *__restrict__ pmask, uint64_t *__restrict__ touched) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t n_split = c->n_split;
+    uint32_t out_rows = 0;
+    uint64_t new_bits = 0;
+    for (uint32_t sq = blockIdx.x; sq < n_split; sq += gridDim.x) {
+        const uint64_t it = ld_const(splits + 2 * (size_t)sq);
+        const uint32_t at = (uint32_t)ld_const(splits + 2 * (size_t)sq + 1);
+        const uint32_t h = uniform(item_head(it)), p = uniform(unit_tile(it)), q = uniform((uint32_t)(it & 3u));
+        const uint32_t nsub = uniform(unit_nsub(it));
+        // the sub-units' partial rows, four sub-units' loads in flight at a time
+        Row4 acc = {{0, 0, 0, 0}};
+        for (uint32_t s0 = 0; s0 < nsub; s0 += 4) {
+            Row4 part[4];
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                const uint32_t sidx = s0 + j;
+                const uint64_t m = sidx < nsub ? ld_const(pmask + at + sidx) : 0ull;
+                part[j] = ((m >> lane) & 1ull) ? load_row(partials + ((size_t)(at + sidx) * WAVE + lane) * TROW_WORDS)
+                                               : Row4{{0, 0, 0, 0}};
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) or_row(acc, part[j]);
+        }
+        const uint32_t T = ld_const(&heads[h].T), I = p / T, J = p - (p / T) * T;
+        const MatDev O = mats[uniform(heads[h].out[parity])];
+        const MatDev M = mats[uniform(heads[h].target)];
+        const QuarterOut qo = quarter_epilogue(O, M, p, q, I, J, acc, 0, out_rows);
+        if (qo.bal && lane == 0) {
+            const uint32_t old = g_add(&O.cnt[p], qo.pc);
+            if (old == 0) {
+                set_occ(O, I, J);
+                touched[atomicAdd(&c->n_touched, 1u)] = ((uint64_t)h << 40) | p;
+            }
+            new_bits += qo.pc;
+        }
+    }
+    const uint32_t orow = wave_sum(out_rows);
+    if (lane == 0) {
+        if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
+        if (new_bits) atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)new_bits);
     }
 }
 
@@ -1490,9 +1577,12 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
 // wave per (tile, quarter); the quarter-0 wave also merges the tile count.
 // Merge the new rows of one step's touched tiles into the heads: one wave per
 // (touched tile, quarter), waves blk, blk + nblk, ... of the work.
+// The quarter's column-major form is computed here for the delta (from its
+// new rows) and OR-ed into the head's, with or without `merge` (merge = 0: the
+// product kernels already merged rows and metadata into the head).
 __device__ __forceinline__ void apply_waves(const MatDev *__restrict__ mats, const HeadDev *__restrict__ heads,
                                             int parity, const StepCounters *__restrict__ c, TouchedLists tl,
-                                            uint32_t blk, uint32_t nblk) {
+                                            uint32_t blk, uint32_t nblk, int merge) {
     const int lane = threadIdx.x & 63;
     const uint32_t n = touched_total(c) * QUARTERS;
     for (uint32_t idx = blk; idx < n; idx += nblk) {  // static: the units are small and alike
@@ -1503,25 +1593,32 @@ __device__ __forceinline__ void apply_waves(const MatDev *__restrict__ mats, con
         const MatDev M = mats[uniform(heads[h].target)];
         const MatDev O = mats[uniform(heads[h].out[parity])];
         const uint32_t T = uniform(heads[h].T), I = p / T, J = p - (p / T) * T;
-        const uint64_t rm = gld(&O.rmask[(size_t)p * 4 + q]);
-        if ((rm >> lane) & 1ull) {
-            const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
-            Row4 m = load_row(M.bits + off);
-            or_row(m, load_row(O.bits + off));
-            store_row(M.bits + off, m);
-        }
-        if (lane == 0 && rm) {
-            const uint64_t old = gld(&M.rmask[(size_t)p * 4 + q]);
-            gst(&M.rmask[(size_t)p * 4 + q], old | rm);
-            Row4 cu = load_row(colq_of(M, p, q));
-            or_row(cu, load_row(colq_of(O, p, q)));
-            store_row(colq_of(M, p, q), cu);
-            if (!old) set_occq(M, q, I, J);
-        }
-        if (lane == 0 && q == 0) {
+        if (merge && lane == 0 && q == 0) {
             const uint32_t old = gld(&M.cnt[p]);
             gst(&M.cnt[p], old + gld(&O.cnt[p]));
             if (!old) set_occ(M, I, J);
+        }
+        const uint64_t rm = uniform64(gld(&O.rmask[(size_t)p * 4 + q]));
+        if (!rm) continue;
+        Row4 nw = {{0, 0, 0, 0}};
+        if ((rm >> lane) & 1ull) {
+            const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
+            nw = load_row(O.bits + off);
+            if (merge) {
+                Row4 m = load_row(M.bits + off);
+                or_row(m, nw);
+                store_row(M.bits + off, m);
+            }
+        }
+        const Row4 ocu = load_row(colq_of(O, p, q));
+        if (uniform(heads[h].colT)) quarter_transpose(nw, ocu, colT_of(O, p, q), colT_of(M, p, q));
+        if (merge && lane == 0) {
+            const uint64_t old = gld(&M.rmask[(size_t)p * 4 + q]);
+            gst(&M.rmask[(size_t)p * 4 + q], old | rm);
+            Row4 cu = load_row(colq_of(M, p, q));
+            or_row(cu, ocu);
+            store_row(colq_of(M, p, q), cu);
+            if (!old) set_occq(M, q, I, J);
         }
     }
 }
@@ -1555,7 +1652,7 @@ __device__ __forceinline__ void clear_waves(const MatDev *__restrict__ mats, con
 __global__ __launch_bounds__(WAVE) void k_apply(const MatDev *__restrict__ mats,
                                                 const HeadDev *__restrict__ heads, int parity,
                                                 StepCounters *__restrict__ c, TouchedLists tl) {
-    apply_waves(mats, heads, parity, c, tl, blockIdx.x, gridDim.x);
+    apply_waves(mats, heads, parity, c, tl, blockIdx.x, gridDim.x, 1);
 }
 
 __global__ __launch_bounds__(WAVE) void k_clear(const MatDev *__restrict__ mats,
@@ -1584,7 +1681,9 @@ __global__ void k_clear_occ(const MatDev *__restrict__ mats, const uint32_t *__r
 
 // The end of a step in one launch, its parts on disjoint arrays:
 //   blocks [0, na)             apply this step's new rows to the heads (parity
-//                              `parity`, list `tl`, counts in c);
+//                              `parity`, list `tl`, counts in c) unless the
+//                              products merged them already (merge = 0), and
+//                              the column-major form of every new quarter;
 //   blocks [na, na + nc)       zero the rows and metadata of the delta the step
 //                              consumed (parity 1 - parity, the previous step's
 //                              list `ptl`, counts in prev);
@@ -1597,10 +1696,11 @@ __global__ __launch_bounds__(WAVE) void k_retire(const MatDev *__restrict__ mats
                                                  const StepCounters *__restrict__ c, TouchedLists tl,
                                                  const StepCounters *__restrict__ prev, TouchedLists ptl,
                                                  const uint32_t *__restrict__ ids, uint32_t nid, uint32_t per,
-                                                 uint32_t na, uint32_t nc, StepCounters *__restrict__ next) {
+                                                 uint32_t na, uint32_t nc, StepCounters *__restrict__ next,
+                                                 int merge) {
     const uint32_t b = blockIdx.x;
     if (b < na) {
-        apply_waves(mats, heads, parity, c, tl, b, na);
+        apply_waves(mats, heads, parity, c, tl, b, na, merge);
     } else if (b < na + nc) {
         clear_waves(mats, heads, 1 - parity, prev, ptl, b - na, nc);
     } else if (b < na + nc + nid * per) {
@@ -1653,6 +1753,11 @@ __global__ __launch_bounds__(BLOCK) void k_copy(const MatDev *__restrict__ mats,
             if (r < 4) D.rmask[(size_t)p * 4 + r] = S.rmask[(size_t)p * 4 + r];
             if (r < QUARTERS * TROW_WORDS) D.colq[(size_t)p * QUARTERS * TROW_WORDS + r] = S.colq[(size_t)p * QUARTERS * TROW_WORDS + r];
             if (r == 0) D.cnt[p] = cn;
+            // the column-major words of the columns each quarter holds (thread r: column r)
+#pragma unroll
+            for (uint32_t qq = 0; qq < QUARTERS; ++qq)
+                if ((S.colq[((size_t)p * QUARTERS + qq) * TROW_WORDS + (r >> 6)] >> (r & 63)) & 1ull)
+                    colT_of(D, p, qq)[r] = colT_of(S, p, qq)[r];
         }
         const uint32_t words = S.T * S.ow;
         for (uint32_t x = blockIdx.x * BLOCK + r; x < words; x += gridDim.x * BLOCK) {
@@ -1683,6 +1788,7 @@ __device__ __forceinline__ void tile_meta(const MatDev &M, uint32_t p, const Row
         if (bal) set_occq(M, wv, I, J);
         s_sum[wv] = pc;
     }
+    if (bal) quarter_transpose(v, cu, colT_of(M, p, wv), nullptr);
     __syncthreads();
     if (r == 0) {
         const uint32_t total = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
@@ -1745,6 +1851,19 @@ __global__ void k_set_bits(MatDev M, const uint32_t *__restrict__ ri, const uint
         const uint32_t p = (i / TILE) * M.T + (j / TILE);
         const size_t off = (size_t)p * TILE_WORDS + (i % TILE) * TROW_WORDS + ((j % TILE) >> 6);
         atomicOr((unsigned long long *)(M.bits + off), 1ull << (j & 63));
+    }
+}
+
+// Rebuild the column-major form of every non-empty quarter of M (one wave per
+// quarter, four per work-group).
+__global__ __launch_bounds__(BLOCK) void k_rebuild_colT(MatDev M) {
+    const int r = threadIdx.x, wv = r >> 6;
+    const uint32_t tiles = M.T * M.T;
+    for (uint32_t p = blockIdx.x; p < tiles; p += gridDim.x) {
+        if (!uniform(M.cnt[p])) continue;
+        if (!uniform64(M.rmask[(size_t)p * 4 + wv])) continue;
+        const Row4 v = load_row(M.bits + (size_t)p * TILE_WORDS + r * TROW_WORDS);
+        quarter_transpose(v, load_row(colq_of(M, p, wv)), colT_of(M, p, wv), nullptr);
     }
 }
 

@@ -254,7 +254,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                                                        uint64_t *__restrict__ dense_items,
                                                        uint64_t *__restrict__ mfma_items, SchedArgs sa, int split,
                                                        uint32_t split_budget, int heavy, uint32_t heavy_base,
-                                                       int tile_units) {
+                                                       int tile_units, uint64_t *__restrict__ splits) {
     __shared__ float sw[MLP_PARAMS];
     if (sa.w) mlp_load(sa.w, sw);
     const uint32_t h = blockIdx.y;
@@ -338,7 +338,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
     append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units,
-                 heavy_base, tile_units);
+                 heavy_base, tile_units, splits);
     deferred = wave_sum(deferred);
     fulls = wave_sum(fulls);
     scored = wave_sum(scored);

@@ -9,6 +9,10 @@
 //   cnt[I*T+J]          popcount of the tile,
 //   rmask[(I*T+J)*4+q]  bit r%64 of word q = r/64 set iff tile row r != 0,
 //   colq[((I*T+J)*4+q)*4 .. +4]  OR of the rows of quarter q (256 column bits),
+//   colT[((I*T+J)*4+q)*256 + k]  the quarter in column-major form: bit r set iff
+//                       row 64q + r holds column k.  Exact for every column k
+//                       in the quarter's colq; other words are undefined and
+//                       never read (colq is what tells a reader which to read),
 // and occupancy bitsets over tile rows / columns (ow = ceil(T/32) words each):
 //   occR[I*ow + K/32]           bit K%32 <=> tile (I, K) non-empty,
 //   occC[J*ow + I/32]           bit I%32 <=> tile (I, J) non-empty,
@@ -36,6 +40,7 @@ struct MatDev {
     uint32_t *cnt;
     uint64_t *rmask;
     uint64_t *colq;
+    uint64_t *colT;
     uint32_t *occR;
     uint32_t *occC;
     uint32_t *occQ;
@@ -56,6 +61,8 @@ struct HeadDev {
     uint32_t f0, f1;  // full terms, one per rule
     uint32_t p0, p1;  // every delta term, two per rule in rule order (scheduler features)
     uint32_t T;
+    uint32_t colT;    // 1: the plan reads the head as a left operand, so the column-major
+                      // form of the head and of its deltas is kept up to date
 };
 
 // One product term  left * right  contributing to a head; operands are
@@ -75,6 +82,7 @@ struct TermPtr {
     const uint32_t *occC;   // right operand: tile-column occupancy
     const uint64_t *lrmask; // left operand: row mask per (tile, quarter)
     const uint64_t *lcolq;  // left operand: column union per (tile, quarter)
+    const uint64_t *lcolT;  // left operand: quarters in column-major form
     const uint64_t *lbits;  // left operand tiles
     const uint64_t *rrmask; // right operand: row mask per tile
     const uint64_t *rbits;  // right operand tiles
@@ -90,7 +98,15 @@ struct TermPtr {
 constexpr int SHARDS = 8;
 struct alignas(128) CounterShard {
     uint64_t tile_products, a_rows, b_rows, out_rows, cand_hits, new_bits, busy, busy_max;
+    uint64_t a_cols;          // column words of A quarters read (column-major gathers)
     uint64_t unit_max;        // longest single work unit (clocks)
+    // the longest unit's shape, as (clocks << 24) | info so that one 64-bit
+    // max keeps the info of the longest unit: a = sub-units of its quarter
+    // (bits 0-4), candidates (5-13, capped), pairs (14-23, capped);
+    // b = B-row gathers (bits 0-23, capped); c = clocks / 256 in the filter
+    // (bits 12-23) and pair (0-11) stages; d = the same for the candidate scan
+    // (12-23) and the epilogue (0-11)
+    uint64_t unit_max_a, unit_max_b, unit_max_c, unit_max_d;
     uint32_t unit_next;       // claims taken from this shard's blocks of 64 units
     uint32_t max_unit_cands;
     // entries of this shard's segment of the touched-tile list, on a cache
@@ -124,6 +140,7 @@ struct StepCounters {
     uint64_t tp_dense;        // of which on the LDS / Four Russians paths
     uint64_t tp_mfma;         // of which on the MFMA path
     uint64_t a_rows;          // non-empty A rows visited
+    uint64_t a_cols;          // A-quarter column words read (column-major gathers, 8 B each)
     uint64_t b_rows;          // B rows gathered from HBM/L2
     uint64_t out_rows;        // C-tile rows that received new bits
     uint64_t b_rows_lds;      // B rows OR-ed from LDS
@@ -136,6 +153,8 @@ struct StepCounters {
     uint64_t sparse_busy;     // sum over sparse-product waves of their busy clocks (s_memtime)
     uint64_t sparse_max;      // longest sparse-product wave (clocks)
     uint64_t sparse_unit_max; // longest single sparse work unit (clocks)
+    uint64_t sparse_unit_max_a, sparse_unit_max_b, sparse_unit_max_c, sparse_unit_max_d;  // its shape
+                                                                                     // (CounterShard)
     CounterShard shard[SHARDS];  // lane-parallel sparse kernel (folded into the fields above by the host)
 };
 
@@ -145,6 +164,7 @@ inline void fold_shards(StepCounters &c) {
         CounterShard &h = c.shard[s];
         c.tile_products += h.tile_products;
         c.a_rows += h.a_rows;
+        c.a_cols += h.a_cols;
         c.b_rows += h.b_rows;
         c.out_rows += h.out_rows;
         c.cand_hits += h.cand_hits;
@@ -152,6 +172,10 @@ inline void fold_shards(StepCounters &c) {
         c.sparse_busy += h.busy;
         if (h.busy_max > c.sparse_max) c.sparse_max = h.busy_max;
         if (h.unit_max > c.sparse_unit_max) c.sparse_unit_max = h.unit_max;
+        if (h.unit_max_a > c.sparse_unit_max_a) c.sparse_unit_max_a = h.unit_max_a;
+        if (h.unit_max_b > c.sparse_unit_max_b) c.sparse_unit_max_b = h.unit_max_b;
+        if (h.unit_max_c > c.sparse_unit_max_c) c.sparse_unit_max_c = h.unit_max_c;
+        if (h.unit_max_d > c.sparse_unit_max_d) c.sparse_unit_max_d = h.unit_max_d;
         if (h.max_unit_cands > c.max_unit_cands) c.max_unit_cands = h.max_unit_cands;
         c.n_touched += h.n_touched;
         h = CounterShard{};

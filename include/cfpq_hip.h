@@ -148,6 +148,8 @@ typedef struct cfpq_stats {
     uint64_t a_rows_dense;      /* A rows visited by the dense kernel                    */
     uint64_t b_rows_dense;      /* B rows the dense kernel gathered from HBM/L2          */
     uint64_t out_rows_dense;    /* C-tile rows written by the dense and MFMA kernels     */
+    uint64_t a_cols;            /* A-quarter column words read by the sparse kernel's
+                                   column-major gathers (8 B each)                       */
 } cfpq_stats;
 
 /* ---- lifecycle ---------------------------------------------------------- */
