@@ -277,6 +277,7 @@ static void normalise(cfpq_opts *o) {
     if (o->iters_per_sync > RING / 2) o->iters_per_sync = RING / 2;
     if (o->sched_threshold == 0) o->sched_threshold = 6554;  // P(active) < 0.1
     if (o->schedule != CFPQ_SCHEDULE_SWEEP && o->schedule != CFPQ_SCHEDULE_RULES) o->schedule = CFPQ_SCHEDULE_DELTA;
+    if (o->m4r_keys != 8) o->m4r_keys = 4;
 }
 
 extern "C" int cfpq_create(int device, const cfpq_opts *opts, cfpq_ctx **out) {
@@ -823,7 +824,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     if (heavy)
         hipLaunchKernelGGL(k_product_dense, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
-                           dense_threshold(ctx->opts), m4r_threshold(ctx->opts), fz);
+                           dense_threshold(ctx->opts), m4r_threshold(ctx->opts), fz, ctx->opts.m4r_keys == 8 ? 1 : 0);
     if (ev_idx >= 0 && heavy) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 2], ctx->stream));
     if (heavy)
         hipLaunchKernelGGL(k_product_mfma, dim3((uint32_t)ctx->num_cu), dim3(MFMA_BLOCK), 0, ctx->stream,
@@ -977,7 +978,7 @@ static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &p
             plan.all_delta[0].size(), plan.all_delta[1].size(), plan.max_tiles_per_head, plan.max_occ_words,
             plan.ids_all_off, plan.total_head_tiles, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
             (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk, (uint64_t)o.sched, o.sched_threshold,
-            (uint64_t)o.sparse_mode, (uint64_t)o.fuse_apply,
+            (uint64_t)o.sparse_mode, (uint64_t)o.fuse_apply, (uint64_t)o.m4r_keys,
             (uint64_t)ctx->sched_have, (uint64_t)(uintptr_t)ctx->sched_w, (uint64_t)(uintptr_t)ctx->dirty,
             (uint64_t)(uintptr_t)ctx->dirty_off, (uint64_t)ctx->tr_cap, (uint64_t)(uintptr_t)ctx->tr_x};
 }

@@ -221,23 +221,26 @@ __device__ __forceinline__ uint64_t *colT_of(const MatDev &M, uint32_t p, uint32
     return M.colT + ((size_t)p * QUARTERS + q) * TILE;
 }
 
-// Transpose of a 64 x 64 bit block held one row per lane (bit j of lane i =
-// element (i, j)): afterwards lane i holds column i (bit j = element (j, i)).
-// Six butterfly stages; stage s swaps the off-diagonal s x s sub-blocks of
-// every 2s x 2s block between lanes i and i ^ s.
-__device__ __forceinline__ uint64_t transpose64(uint64_t x) {
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// 64 x 64 bit transpose across the wave: on entry lane r holds row r (bit c =
+// element (r, c)), on exit lane c holds column c (bit r = element (r, c)).
+// Stage j swaps the off-diagonal j x j sub-blocks of every 2j x 2j block.
+__device__ __forceinline__ uint64_t wave_transpose64(uint64_t v) {
     const int lane = threadIdx.x & 63;
-    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
-                               0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+    const uint64_t keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                              0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
 #pragma unroll
-    for (int t = 0; t < 6; ++t) {
-        const int sft = 32 >> t;
-        const uint64_t m = masks[t];
-        const uint32_t lo = __shfl_xor((uint32_t)x, sft, 64), hi = __shfl_xor((uint32_t)(x >> 32), sft, 64);
-        const uint64_t y = ((uint64_t)hi << 32) | lo;
-        x = (lane & sft) ? ((x & ~m) | ((y & ~m) >> sft)) : ((x & m) | ((y & m) << sft));
+    for (int s = 0; s < 6; ++s) {
+        const int j = 32 >> s;
+        const uint64_t m0 = keep[s];
+        const uint64_t p = shfl_xor64(v, j);
+        v = (lane & j) ? ((v & ~m0) | ((p & ~m0) >> j)) : ((v & m0) | ((p & m0) << j));
     }
-    return x;
+    return v;
 }
 
 // Column-major form of one 64-row quarter, whole wave: lane r holds row r in
@@ -251,7 +254,7 @@ __device__ __forceinline__ void quarter_transpose(const Row4 &v, const Row4 &cu,
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
         if (uniform64(cu.w[w]) == 0) continue;
-        const uint64_t col = transpose64(v.w[w]);
+        const uint64_t col = wave_transpose64(v.w[w]);
         if ((cu.w[w] >> lane) & 1ull) {
             const uint32_t k = (uint32_t)w * 64u + (uint32_t)lane;
             gst(dst + k, col);
@@ -1237,13 +1240,72 @@ __device__ __forceinline__ void tp_m4r(const uint64_t *__restrict__ At, const ui
     }
 }
 
+// Very dense A tile, 8-bit keys, one table per wavefront: wave w owns the
+// 64-column slice w of the C tile for all 256 rows (lane l: rows l + 64 j,
+// accumulated in acc8[j]).  The B tile is staged in LDS; then, four groups of
+// 8 B rows at a time, each wave builds the 256 OR-combinations of its slice of
+// those rows (2 KiB per group, 8 KiB per wave, in the shared table space) and
+// every row costs one 8-byte lookup per non-zero byte of A.  Must be reached
+// by the whole work-group.  rmp: the A tile's four quarter row masks.
+__device__ __forceinline__ void tp_m4r8(const uint64_t *__restrict__ At, const uint64_t *__restrict__ Bt,
+                                        const uint64_t *__restrict__ rmp, int r, Row4 *Bs, uint64_t *T8,
+                                        uint64_t *acc8, uint32_t &arows, uint32_t &lookups) {
+    const int wv = r >> 6, lane = r & 63;
+    __syncthreads();
+    Bs[r] = load_row(Bt + r * TROW_WORDS);
+    Row4 a[QUARTERS];
+#pragma unroll
+    for (int j = 0; j < QUARTERS; ++j) {
+        a[j] = Row4{{0, 0, 0, 0}};
+        if ((rmp[j] >> lane) & 1ull) a[j] = load_row(At + (j * WAVE + lane) * TROW_WORDS);
+    }
+    if (wv == 0) {
+#pragma unroll
+        for (int j = 0; j < QUARTERS; ++j) arows += (rmp[j] >> lane) & 1ull;
+    }
+    uint64_t *tw = T8 + (size_t)wv * 4 * 256;  // this wave's four 256-entry tables
+    __syncthreads();
+    for (int g0 = 0; g0 < TILE / 8; g0 += 4) {
+#pragma unroll
+        for (int gi = 0; gi < 4; ++gi) {
+            const int g = g0 + gi;
+            uint64_t b[8];
+#pragma unroll
+            for (int bb = 0; bb < 8; ++bb) b[bb] = Bs[8 * g + bb].w[wv];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int key = e * WAVE + lane;
+                uint64_t v = 0;
+#pragma unroll
+                for (int bb = 0; bb < 8; ++bb)
+                    if ((key >> bb) & 1) v |= b[bb];
+                tw[gi * 256 + key] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < QUARTERS; ++j) {
+#pragma unroll
+            for (int gi = 0; gi < 4; ++gi) {
+                const int g = g0 + gi;
+                const uint32_t key = (uint32_t)(a[j].w[g >> 3] >> ((g & 7) * 8)) & 255u;
+                if (key) {
+                    acc8[j] |= tw[gi * 256 + key];
+                    ++lookups;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restrict__ mats,
                                                          const HeadDev *__restrict__ heads,
                                                          const TermDev *__restrict__ terms, int parity,
                                                          StepCounters *__restrict__ c,
                                                          const uint64_t *__restrict__ items,
                                                          uint64_t *__restrict__ touched, uint32_t dense_bits,
-                                                         uint32_t m4r_bits, int fuse) {
+                                                         uint32_t m4r_bits, int fuse, int m4r8) {
     __shared__ Row4 Bs[TILE];
     __shared__ Row4 Tab[M4R_GROUPS * 16];
     __shared__ uint32_t s_idx;
@@ -1264,6 +1326,8 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
         const uint32_t t0 = uniform(full ? heads[h].f0 : heads[h].t0), t1 = uniform(full ? heads[h].f1 : heads[h].t1);
         const uint32_t I = p / T, J = p - (p / T) * T;
         Row4 acc = {{0, 0, 0, 0}};
+        uint64_t acc8[QUARTERS] = {0, 0, 0, 0};  // 8-bit Four Russians: slice wv of rows lane + 64 j
+        bool used8 = false;
         for (uint32_t t = t0; t < t1; ++t) {
             const MatDev L = mats[uniform(terms[t].left[parity])];
             const MatDev R = mats[uniform(terms[t].right[parity])];
@@ -1293,7 +1357,11 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
                         ++tpd;
                         continue;
                     }
-                    if (acnt >= m4r_bits) {
+                    if (acnt >= m4r_bits && m4r8) {
+                        ++tpd;
+                        used8 = true;
+                        tp_m4r8(At, Bt, L.rmask + (size_t)lt * 4, r, Bs, (uint64_t *)Tab, acc8, arows, lookups);
+                    } else if (acnt >= m4r_bits) {
                         ++tpd;
                         tp_m4r(At, Bt, rm, r, Bs, Tab, acc, arows, lookups);
                     } else if (acnt >= dense_bits) {
@@ -1304,6 +1372,14 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
                     }
                 }
             }
+        }
+        if (used8) {  // (uniform: every thread of the group takes the same paths)
+            // the slice accumulators back into rows: wave w holds word w of rows lane + 64 j
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < QUARTERS; ++j) Bs[j * WAVE + lane].w[wv] = acc8[j];
+            __syncthreads();
+            or_row(acc, Bs[r]);
         }
         const MatDev O = mats[uniform(heads[h].out[parity])];
         const MatDev M = mats[uniform(heads[h].target)];
@@ -1373,28 +1449,6 @@ __device__ __forceinline__ i32x4 bytes_of_16(uint32_t x) {
     r[2] = bytes_of_nibble(x >> 8);
     r[3] = bytes_of_nibble(x >> 12);
     return r;
-}
-
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
-    const uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// 64 x 64 bit transpose across the wave: on entry lane r holds row r (bit c =
-// element (r, c)), on exit lane c holds column c (bit r = element (r, c)).
-// Stage j swaps the off-diagonal j x j sub-blocks of every 2j x 2j block.
-__device__ __forceinline__ uint64_t wave_transpose64(uint64_t v) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
-                              0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        const int j = 32 >> s;
-        const uint64_t m0 = keep[s];
-        const uint64_t p = shfl_xor64(v, j);
-        v = (lane & j) ? ((v & ~m0) | ((p & ~m0) >> j)) : ((v & m0) | ((p & m0) << j));
-    }
-    return v;
 }
 
 // word index of the 16-byte chunk `chunk` (0..15, along k) of column n in BX

@@ -97,6 +97,9 @@ typedef struct cfpq_opts {
     int32_t fuse_apply;         /* 1: fixpoint steps merge new rows into the head matrices
                                    inside the product kernels (no separate apply pass)  */
     int32_t schedule;           /* CFPQ_SCHEDULE_*: how cfpq_fixpoint orders the rules    */
+    int32_t m4r_keys;           /* Four Russians key width: 4 = 16-entry tables over whole
+                                   tile rows, shared by the work-group (default); 8 =
+                                   256-entry tables per wavefront over a 64-column slice */
 } cfpq_opts;
 
 /* cfpq_opts.schedule */

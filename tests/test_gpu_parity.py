@@ -609,3 +609,51 @@ def test_left_operand_after_fixpoint_that_did_not_keep_it(pkg, gen, oracle, engi
         if left == "copy":
             src.free()
     gr.free()
+
+
+@pytest.mark.parametrize("n", [1, 63, 256, 257, 600])
+def test_add_mul_m4r_8bit_keys(pkg, oracle, n):
+    """The Four Russians path with 8-bit keys and per-wavefront tables over
+    64-column slices, forced for every tile-product: bit-exact against the
+    oracle's M4RM over densities up to all ones, and with aliasing."""
+    rng = np.random.default_rng(900 + n)
+    with pkg.Engine(0, kernel="m4r", m4r_keys=8) as eng:
+        for density in (0.003, 0.05, 0.3, 0.7, 1.0):
+            a, b, c = rand_mat(rng, n, density), rand_mat(rng, n, density), rand_mat(rng, n, 0.01)
+            A, B, C = oracle.from_dense(a), oracle.from_dense(b), oracle.from_dense(c)
+            want = C.copy()
+            want_changed = oracle.addmul_m4rm(want, A, B, n)
+            dA, dB, dC = upload(eng, oracle, a), upload(eng, oracle, b), upload(eng, oracle, c)
+            assert eng.add_mul(dC, dA, dB) == want_changed
+            assert np.array_equal(dC.download(), want), (n, density)
+            assert dC.fingerprint() == oracle.fingerprint(want, n)
+            for m in (dA, dB, dC):
+                m.free()
+        x = rand_mat(rng, n, 0.2)
+        X = oracle.from_dense(x)
+        want = X.copy()
+        oracle.addmul_m4rm(want, X.copy(), X.copy(), n)
+        dX = upload(eng, oracle, x)
+        eng.add_mul(dX, dX, dX)
+        assert np.array_equal(dX.download(), want)
+        dX.free()
+
+
+@pytest.mark.parametrize("schedule", ["delta", "rules"])
+def test_fixpoint_m4r_8bit_keys(pkg, gen, oracle, schedule):
+    stats = _fixpoint_vs_oracle(pkg, gen, oracle, gen.dense_stress(1500), kernel="m4r", m4r_keys=8, schedule=schedule)
+    assert stats["m4r_lookups"] > 0 and stats["tile_products_dense"] == stats["tile_products"]
+
+
+def test_fixpoint_dense_stress_4096_golden_m4r_8bit(pkg, gen):
+    import json
+    import os
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixpoints.json")))["config2_n4096"]
+    q = gen.dense_stress(4096)
+    with pkg.Engine(0, schedule="rules", m4r_bits=4096, dense_bits=64, m4r_keys=8) as eng:
+        gr = cf.Grammar(q.grammar, text=True)
+        _, _, st = gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+        assert gr.results() == {k: tuple(v) for k, v in golden["fingerprints"].items()}
+        assert st["m4r_lookups"] > 0
+        gr.free()
