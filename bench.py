@@ -332,6 +332,15 @@ def main():
             "tile_products": last["tile_products"], "bit_exact": ok}
         if k == "learned":
             engines[k].update({x: last[x] for x in ("sched_scored", "sched_deferred", "sched_full", "flush_steps")})
+        if last["unit_clocks"]:
+            # what a scheduler that skipped exactly the sparse units adding no
+            # bit could save (measured per unit with s_memtime in the kernel)
+            cs = last["unit_clocks_idle"] / last["unit_clocks"]
+            engines[k]["skip_bound"] = {
+                "units_idle_share": round(last["units_idle"] / max(last["units_done"], 1), 4),
+                "unit_clock_share": round(cs, 4),
+                "b_rows_share": round(last["b_rows_idle"] / max(last["b_rows"], 1), 4),
+                "sparse_speedup_bound": round(1.0 / max(1.0 - cs, 1e-9), 3)}
     if "learned" in runs:
         engines["learned_vs_plain_speedup"] = round(runs["plain"][0] / runs["learned"][0], 4)
     if "worklist" in runs:

@@ -91,6 +91,11 @@ class Stats(ctypes.Structure):
         ("b_rows_dense", ctypes.c_uint64),
         ("out_rows_dense", ctypes.c_uint64),
         ("a_cols", ctypes.c_uint64),
+        ("units_done", ctypes.c_uint64),
+        ("units_idle", ctypes.c_uint64),
+        ("unit_clocks", ctypes.c_uint64),
+        ("unit_clocks_idle", ctypes.c_uint64),
+        ("b_rows_idle", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -668,6 +668,7 @@ static size_t plan_terms(const Plan &plan) {
 struct StepStats {
     uint64_t new_bits = 0, items = 0, tp = 0, tpd = 0, tpm = 0, arows = 0, brows = 0, orows = 0, brows_lds = 0,
              lookups = 0, mfma_instr = 0, cand_hits = 0, arows_d = 0, brows_d = 0, orows_d = 0, acols = 0;
+    uint64_t units_done = 0, units_idle = 0, unit_clk = 0, idle_clk = 0, idle_brows = 0;
     uint32_t max_unit_cands = 0;
 };
 
@@ -948,6 +949,11 @@ static void accumulate(StepStats &s, const StepCounters &c) {
     s.tpm += c.tp_mfma;
     s.arows += c.a_rows;
     s.acols += c.a_cols;
+    s.units_done += c.units_done;
+    s.units_idle += c.idle_units;
+    s.unit_clk += c.unit_clk;
+    s.idle_clk += c.idle_clk;
+    s.idle_brows += c.idle_brows;
     s.brows += c.b_rows;
     s.orows += c.out_rows;
     s.brows_lds += c.b_rows_lds;
@@ -1277,6 +1283,11 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         stats->cand_hits = tot.cand_hits;
         stats->a_rows_dense = tot.arows_d;
         stats->a_cols = tot.acols;
+        stats->units_done = tot.units_done;
+        stats->units_idle = tot.units_idle;
+        stats->unit_clocks = tot.unit_clk;
+        stats->unit_clocks_idle = tot.idle_clk;
+        stats->b_rows_idle = tot.idle_brows;
         stats->b_rows_dense = tot.brows_d;
         stats->out_rows_dense = tot.orows_d;
         stats->max_unit_cands = tot.max_unit_cands;

@@ -244,22 +244,17 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t v) {
 }
 
 // Column-major form of one 64-row quarter, whole wave: lane r holds row r in
-// v, cu is the OR of the rows.  Word k of dst becomes the mask of the rows
-// holding column k, for every column k of cu; with orinto, word k of orinto
-// is OR-ed with it as well (the caller is the only writer of both).  Words of
-// columns outside cu are left as they are: no reader reads them.  Lane l
+// v.  Word k of dst becomes the mask of the rows holding column k, for every
+// column k of cu (the caller's choice of columns: those of the quarter, or
+// those that changed).  Words of other columns are left as they are.  Lane l
 // computes the words of columns l, 64 + l, 128 + l and 192 + l.
-__device__ __forceinline__ void quarter_transpose(const Row4 &v, const Row4 &cu, uint64_t *dst, uint64_t *orinto) {
+__device__ __forceinline__ void quarter_transpose(const Row4 &v, const Row4 &cu, uint64_t *dst) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
         if (uniform64(cu.w[w]) == 0) continue;
         const uint64_t col = wave_transpose64(v.w[w]);
-        if ((cu.w[w] >> lane) & 1ull) {
-            const uint32_t k = (uint32_t)w * 64u + (uint32_t)lane;
-            gst(dst + k, col);
-            if (orinto) gst(orinto + k, gld(orinto + k) | col);
-        }
+        if ((cu.w[w] >> lane) & 1ull) gst(dst + (uint32_t)w * 64u + (uint32_t)lane, col);
     }
 }
 
@@ -821,6 +816,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     uint64_t new_bits = 0;  // lane 0: new bits of this wave's units
     uint64_t unit_max = 0;  // longest unit of this wave (clocks)
     uint64_t unit_max_a = 0, unit_max_b = 0, unit_max_c = 0, unit_max_d = 0;  // its shape (CounterShard)
+    uint64_t unit_clk = 0, idle_clk = 0;  // lane 0: clocks of all units / of units that added no bit
+    uint32_t idle_units = 0, idle_brows = 0, units_done = 0;
     uint32_t sh = home, used_up = 0;
     while (used_up < SHARDS) {
     uint32_t kc = 0xffffffffu;
@@ -1067,7 +1064,15 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         const uint64_t t_end = __builtin_amdgcn_s_memtime();
         const uint64_t tu = t_end - tu0;
         const uint32_t ucand = min(tp - tp_unit0, 511u), upairs = min(unit_pairs / 16u, 1023u);
-        const uint32_t ugath = min(wave_sum(brows - br_unit0), 0xffffffu);
+        const uint32_t ugath_all = wave_sum(brows - br_unit0);
+        const uint32_t ugath = min(ugath_all, 0xffffffu);
+        unit_clk += tu;
+        ++units_done;
+        if (nsub == 1 && pc == 0) {  // (a split quarter's sub-units count as productive)
+            idle_clk += tu;
+            ++idle_units;
+            idle_brows += ugath_all;
+        }
         if (tu > unit_max) {
             unit_max = tu;
             unit_max_a = (tu << 24) | min(nsub, 31u) | (ucand << 5) | (upairs << 14);
@@ -1099,6 +1104,11 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         if (unit_max_b) atomicMax((unsigned long long *)&S.unit_max_b, (unsigned long long)unit_max_b);
         if (unit_max_c) atomicMax((unsigned long long *)&S.unit_max_c, (unsigned long long)unit_max_c);
         if (unit_max_d) atomicMax((unsigned long long *)&S.unit_max_d, (unsigned long long)unit_max_d);
+        if (unit_clk) atomicAdd((unsigned long long *)&S.unit_clk, (unsigned long long)unit_clk);
+        if (idle_clk) atomicAdd((unsigned long long *)&S.idle_clk, (unsigned long long)idle_clk);
+        if (idle_units) atomicAdd((unsigned long long *)&S.idle_units, (unsigned long long)idle_units);
+        if (idle_brows) atomicAdd((unsigned long long *)&S.idle_brows, (unsigned long long)idle_brows);
+        if (units_done) atomicAdd((unsigned long long *)&S.units_done, (unsigned long long)units_done);
     }
 }
 
@@ -1654,18 +1664,28 @@ __device__ __forceinline__ void apply_waves(const MatDev *__restrict__ mats, con
         }
         const uint64_t rm = uniform64(gld(&O.rmask[(size_t)p * 4 + q]));
         if (!rm) continue;
-        Row4 nw = {{0, 0, 0, 0}};
-        if ((rm >> lane) & 1ull) {
-            const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
-            nw = load_row(O.bits + off);
-            if (merge) {
-                Row4 m = load_row(M.bits + off);
-                or_row(m, nw);
-                store_row(M.bits + off, m);
-            }
+        const bool keepT = uniform(heads[h].colT) != 0;
+        // the head's rows: with merge, its non-empty rows before the merge and
+        // the new ones; without, the product kernels merged them already
+        const uint64_t mrm = uniform64(gld(&M.rmask[(size_t)p * 4 + q]));
+        const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
+        Row4 nw = {{0, 0, 0, 0}}, m = {{0, 0, 0, 0}};
+        if ((rm >> lane) & 1ull) nw = load_row(O.bits + off);
+        const uint64_t need = (keepT ? (mrm | rm) : 0ull) | (merge ? rm : 0ull);
+        if ((need >> lane) & 1ull) m = load_row(M.bits + off);
+        if (merge && ((rm >> lane) & 1ull)) {
+            or_row(m, nw);
+            store_row(M.bits + off, m);
         }
         const Row4 ocu = load_row(colq_of(O, p, q));
-        if (uniform(heads[h].colT)) quarter_transpose(nw, ocu, colT_of(O, p, q), colT_of(M, p, q));
+        if (keepT) {
+            // column words of the columns that gained bits: the delta's from its
+            // rows, the head's recomputed from all of its rows (a head that was
+            // cleared and filled again may hold stale words for columns that
+            // left its colq, so they are rewritten, never OR-ed into)
+            quarter_transpose(nw, ocu, colT_of(O, p, q));
+            quarter_transpose(m, ocu, colT_of(M, p, q));
+        }
         if (merge && lane == 0) {
             const uint64_t old = gld(&M.rmask[(size_t)p * 4 + q]);
             gst(&M.rmask[(size_t)p * 4 + q], old | rm);
@@ -1777,6 +1797,12 @@ __global__ __launch_bounds__(BLOCK) void k_clear_full(const MatDev *__restrict__
             if (uniform(O.cnt[p])) {
                 const uint64_t rm = O.rmask[(size_t)p * 4 + wv];
                 if ((rm >> lane) & 1ull) store_row(O.bits + (size_t)p * TILE_WORDS + r * TROW_WORDS, Row4{{0, 0, 0, 0}});
+                // the column words of the quarter's columns go too: a word is
+                // never more than its column's rows, also after a clear
+                const Row4 cq = load_row(colq_of(O, p, wv));
+#pragma unroll
+                for (int w = 0; w < 4; ++w)
+                    if ((cq.w[w] >> lane) & 1ull) colT_of(O, p, wv)[w * 64 + lane] = 0;
                 __syncthreads();
                 if (lane == 0) {
                     O.rmask[(size_t)p * 4 + wv] = 0;
@@ -1842,7 +1868,9 @@ __device__ __forceinline__ void tile_meta(const MatDev &M, uint32_t p, const Row
         if (bal) set_occq(M, wv, I, J);
         s_sum[wv] = pc;
     }
-    if (bal) quarter_transpose(v, cu, colT_of(M, p, wv), nullptr);
+    // every column word of the quarter, zeros included (the tile may have held
+    // other bits before)
+    quarter_transpose(v, Row4{{~0ull, ~0ull, ~0ull, ~0ull}}, colT_of(M, p, wv));
     __syncthreads();
     if (r == 0) {
         const uint32_t total = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
@@ -1908,16 +1936,15 @@ __global__ void k_set_bits(MatDev M, const uint32_t *__restrict__ ri, const uint
     }
 }
 
-// Rebuild the column-major form of every non-empty quarter of M (one wave per
-// quarter, four per work-group).
+// Rebuild the column-major form of every quarter of M, every word (one wave
+// per quarter, four per work-group).
 __global__ __launch_bounds__(BLOCK) void k_rebuild_colT(MatDev M) {
     const int r = threadIdx.x, wv = r >> 6;
     const uint32_t tiles = M.T * M.T;
     for (uint32_t p = blockIdx.x; p < tiles; p += gridDim.x) {
-        if (!uniform(M.cnt[p])) continue;
-        if (!uniform64(M.rmask[(size_t)p * 4 + wv])) continue;
-        const Row4 v = load_row(M.bits + (size_t)p * TILE_WORDS + r * TROW_WORDS);
-        quarter_transpose(v, load_row(colq_of(M, p, wv)), colT_of(M, p, wv), nullptr);
+        const Row4 v = uniform64(M.rmask[(size_t)p * 4 + wv]) ? load_row(M.bits + (size_t)p * TILE_WORDS + r * TROW_WORDS)
+                                                               : Row4{{0, 0, 0, 0}};
+        quarter_transpose(v, Row4{{~0ull, ~0ull, ~0ull, ~0ull}}, colT_of(M, p, wv));
     }
 }
 

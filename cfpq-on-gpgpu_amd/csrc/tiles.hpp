@@ -10,9 +10,10 @@
 //   rmask[(I*T+J)*4+q]  bit r%64 of word q = r/64 set iff tile row r != 0,
 //   colq[((I*T+J)*4+q)*4 .. +4]  OR of the rows of quarter q (256 column bits),
 //   colT[((I*T+J)*4+q)*256 + k]  the quarter in column-major form: bit r set iff
-//                       row 64q + r holds column k.  Exact for every column k
-//                       in the quarter's colq; other words are undefined and
-//                       never read (colq is what tells a reader which to read),
+//                       row 64q + r holds column k.  A word never holds more
+//                       than its column's rows (a cleared quarter's words are
+//                       zeroed), and between fixpoint steps the words of the
+//                       columns in colq are exact; readers read only those,
 // and occupancy bitsets over tile rows / columns (ow = ceil(T/32) words each):
 //   occR[I*ow + K/32]           bit K%32 <=> tile (I, K) non-empty,
 //   occC[J*ow + I/32]           bit I%32 <=> tile (I, J) non-empty,
@@ -99,6 +100,9 @@ constexpr int SHARDS = 8;
 struct alignas(128) CounterShard {
     uint64_t tile_products, a_rows, b_rows, out_rows, cand_hits, new_bits, busy, busy_max;
     uint64_t a_cols;          // column words of A quarters read (column-major gathers)
+    // work units and their clocks, and those of the units that added no bit
+    // (what a scheduler that skipped exactly the unproductive units would save)
+    uint64_t unit_clk, idle_clk, idle_units, idle_brows, units_done;
     uint64_t unit_max;        // longest single work unit (clocks)
     // the longest unit's shape, as (clocks << 24) | info so that one 64-bit
     // max keeps the info of the longest unit: a = sub-units of its quarter
@@ -141,6 +145,7 @@ struct StepCounters {
     uint64_t tp_mfma;         // of which on the MFMA path
     uint64_t a_rows;          // non-empty A rows visited
     uint64_t a_cols;          // A-quarter column words read (column-major gathers, 8 B each)
+    uint64_t unit_clk, idle_clk, idle_units, idle_brows, units_done;  // (CounterShard)
     uint64_t b_rows;          // B rows gathered from HBM/L2
     uint64_t out_rows;        // C-tile rows that received new bits
     uint64_t b_rows_lds;      // B rows OR-ed from LDS
@@ -165,6 +170,11 @@ inline void fold_shards(StepCounters &c) {
         c.tile_products += h.tile_products;
         c.a_rows += h.a_rows;
         c.a_cols += h.a_cols;
+        c.unit_clk += h.unit_clk;
+        c.idle_clk += h.idle_clk;
+        c.idle_units += h.idle_units;
+        c.idle_brows += h.idle_brows;
+        c.units_done += h.units_done;
         c.b_rows += h.b_rows;
         c.out_rows += h.out_rows;
         c.cand_hits += h.cand_hits;

@@ -153,6 +153,12 @@ typedef struct cfpq_stats {
     uint64_t out_rows_dense;    /* C-tile rows written by the dense and MFMA kernels     */
     uint64_t a_cols;            /* A-quarter column words read by the sparse kernel's
                                    column-major gathers (8 B each)                       */
+    uint64_t units_done;        /* work units the lane-parallel sparse kernel evaluated  */
+    uint64_t units_idle;        /* of which added no bit (split sub-units not counted)   */
+    uint64_t unit_clocks;       /* summed clocks of those units (s_memtime)              */
+    uint64_t unit_clocks_idle;  /* summed clocks of the units that added no bit: what a
+                                   scheduler skipping exactly those would save         */
+    uint64_t b_rows_idle;       /* B rows gathered by the units that added no bit        */
 } cfpq_stats;
 
 /* ---- lifecycle ---------------------------------------------------------- */
