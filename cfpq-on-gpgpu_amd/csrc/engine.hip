@@ -764,8 +764,9 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     // heavy sparse units are split over several waves (lane-parallel kernel,
     // head update not fused); the units buffer holds 4 entries per head tile
     // plus as many extra sub-units
-    const int lane_parallel = ctx->opts.sparse_mode != 1;  // modes 0 (quarter units) and 2 (tile units)
-    const int tile_units = ctx->opts.sparse_mode == 2 ? 1 : 0;
+    const int lane_parallel = ctx->opts.sparse_mode != 1;  // modes 0, 3, 2: quarter, half-tile, tile units
+    // work-unit shape of the lane-parallel kernel: 2 = whole tiles, 3 = halves
+    const int tile_units = ctx->opts.sparse_mode == 2 ? 1 : ctx->opts.sparse_mode == 3 ? 2 : 0;
     const int split = (lane_parallel && !(fuse && ctx->opts.fuse_apply)) ? 1 : 0;
     const uint32_t split_budget = split_slots_of(ctx, plan);
     const uint32_t heavy_base = (uint32_t)(QUARTERS * plan.total_head_tiles);

@@ -424,22 +424,30 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
                                              uint64_t *__restrict__ splits) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
-    // tile units: one unit for every quarter of the tile with candidates,
-    // unless a quarter is heavy enough to be split
+    // the work-unit shape: tile_units 1 = one unit for every quarter of the
+    // tile with candidates (256 output rows), 2 = one per half (quarters 0-1
+    // and 2-3, 128 rows), 0 = one per quarter (64 rows).  A group holding a
+    // quarter heavy enough to be split is left to the quarter units.
+    uint32_t rest = want ? qmask : 0u;
     if (tile_units) {
-        bool heavyq = false;
+        const uint32_t groups = tile_units == 1 ? 1u : 2u;
+        for (uint32_t gi = 0; gi < groups; ++gi) {
+            const uint32_t gm = tile_units == 1 ? 0xFu : (3u << (2 * gi));
+            const uint32_t m = rest & gm;
+            bool heavyq = false;
 #pragma unroll
-        for (uint32_t q = 0; q < QUARTERS; ++q) heavyq |= split && ((qmask >> q) & 1u) && qcnt[q] > SPLIT_CANDS;
-        const bool tu = want && qmask && !heavyq;
-        wave_append(tu, base | ITEM_QSET | ((uint64_t)qmask << ITEM_QSET_SHIFT), &c->n_units, units);
-        want = want && !tu;
+            for (uint32_t q = 0; q < QUARTERS; ++q) heavyq |= split && ((m >> q) & 1u) && qcnt[q] > SPLIT_CANDS;
+            const bool tu = m && !heavyq;
+            wave_append(tu, base | ITEM_QSET | ((uint64_t)m << ITEM_QSET_SHIFT), &c->n_units, units);
+            if (tu) rest &= ~gm;
+        }
     }
     uint32_t nsub[QUARTERS];
     uint64_t single[QUARTERS];
     uint32_t total = 0;
 #pragma unroll
     for (uint32_t q = 0; q < QUARTERS; ++q) {
-        const bool wq = want && ((qmask >> q) & 1u);
+        const bool wq = ((rest >> q) & 1u) != 0;
         nsub[q] = wq ? 1u : 0u;
         if (wq && split && qcnt[q] > SPLIT_CANDS) {
             uint32_t ns = (qcnt[q] + SPLIT_CANDS - 1) / SPLIT_CANDS;

@@ -5,8 +5,9 @@ hipEvents, median of a few runs) while walking the parameters one at a time
 (coordinate descent, best value kept before moving to the next parameter):
 
   schedule             semi-naive delta steps over all rules (0), head sweeps (1) or rule sweeps (2) over full matrices
-  sparse_mode          sparse product: lane-parallel candidates over quarter units (0) or tile
-                       units (2), or row-per-lane (1)
+  sparse_mode          sparse product and its work-tile shape: lane-parallel candidates over
+                       work units of 64 (0), 128 (3) or 256 (2) output rows of a 256-column
+                       tile, or row-per-lane (1)
   sparse_batch         k-tiles whose loads the row-per-lane sparse product issues together
   sparse_waves_per_cu  persistent waves of the sparse product per CU
   sparse_chunk         work units a sparse-product wave claims per atomic
@@ -34,7 +35,7 @@ CACHE = os.path.join(HERE, "tuned.json")
 
 SPACE = [
     ("schedule", [0, 1, 2]),
-    ("sparse_mode", [0, 1, 2]),
+    ("sparse_mode", [0, 1, 2, 3]),
     ("sparse_batch", [1, 2, 4]),
     ("sparse_waves_per_cu", [2, 4, 8, 12, 16, 32]),
     ("sparse_chunk", [1, 2, 4, 8, 16]),
@@ -44,9 +45,9 @@ SPACE = [
     ("iters_per_sync", [2, 4, 8]),
     ("fuse_apply", [0, 1]),
 ]
-# Graph replay (use_graph) is left to the caller: it saves about 7% of an
-# SG-forest fixpoint at N = 65,536 in launch gaps, but per-kernel event
-# timing needs the eager launch path.
+# Graph replay (use_graph) is a dimension for batches (tune_batch), where the
+# steps are short and launch-bound; for single queries the caller decides, as
+# per-kernel event timing needs the eager launch path.
 DEFAULTS = {"schedule": 0, "sparse_mode": 0, "sparse_batch": 2, "sparse_waves_per_cu": 8, "sparse_chunk": 4, "dense_bits": 2048, "m4r_bits": 16384,
             "mfma_bits": 32768, "iters_per_sync": 4, "fuse_apply": 0}
 
@@ -74,14 +75,16 @@ def batch_key(queries) -> str:
     return f"batch{len(queries)}:{h}"
 
 
-def _descend(run, log, passes: int):
-    """Coordinate descent from DEFAULTS; returns (best opts, ms, trace)."""
-    best = dict(DEFAULTS)
+def _descend(run, log, passes: int, space=None, start=None):
+    """Coordinate descent from DEFAULTS (or `start`) over `space` (default
+    SPACE); returns (best opts, ms, trace)."""
+    space = SPACE if space is None else space
+    best = dict(DEFAULTS if start is None else start)
     best_ms = run(best)
     log(f"[tune] start {best} -> {best_ms:.3f} ms")
     trace = [{"opts": dict(best), "ms": best_ms}]
     for _ in range(passes):
-        for name, values in SPACE:
+        for name, values in space:
             for v in values:
                 if v == best[name]:
                     continue
@@ -175,7 +178,8 @@ def tune_batch(pkg, cf, queries, repeats: int = 1, log=print, passes: int = 1) -
             raise AssertionError(f"tuning candidate {opts} changed a result")
         return statistics.median(times)
 
-    best, best_ms, trace = _descend(run, log, passes)
+    best, best_ms, trace = _descend(run, log, passes, space=SPACE + [("use_graph", [0, 1])],
+                                    start=dict(DEFAULTS, use_graph=0))
     for gr, cp in zip(grs, inits):
         for c in cp:
             c.free()

@@ -90,10 +90,12 @@ typedef struct cfpq_opts {
                                    (needs weights: cfpq_sched_set_weights / _train)     */
     uint32_t sched_threshold;   /* defer an item whose predicted activity probability is
                                    below sched_threshold / 65536 (0 = library default)   */
-    int32_t sparse_mode;        /* sparse product: 0 = lane-parallel candidates, one work
-                                   unit per 64-row quarter (default); 2 = the same with one
-                                   unit per tile covering its quarters with candidates;
-                                   1 = row-per-lane with sparse_batch candidates per round */
+    int32_t sparse_mode;        /* sparse product and its work-tile shape: 0 = lane-parallel
+                                   candidates, one work unit per 64-row quarter (default);
+                                   3 = the same with one unit per 128-row half tile; 2 = one
+                                   unit per 256-row tile (each covering its quarters with
+                                   candidates); 1 = row-per-lane with sparse_batch
+                                   candidates per round                                  */
     int32_t fuse_apply;         /* 1: fixpoint steps merge new rows into the head matrices
                                    inside the product kernels (no separate apply pass)  */
     int32_t schedule;           /* CFPQ_SCHEDULE_*: how cfpq_fixpoint orders the rules    */

@@ -394,27 +394,45 @@ def test_fixpoint_fused_apply(pkg, gen, oracle, which):
                             mfma_bits=30000)
 
 
-def test_batch_512_queries_golden(pkg, gen):
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_512_queries_golden(pkg, gen, graph):
     """The 512-query batch (N log-uniform in [256, 8192], Dyck or SG family)
-    in one batched fixpoint: every query's fingerprints against the oracle
-    fixture."""
+    in one batched fixpoint, with its steps launched eagerly or replayed from
+    captured hipGraphs (twice, so that the second run replays the graphs the
+    first captured): every query's fingerprints against the oracle fixture."""
     import json
     import os
     cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
     golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixpoints.json")))["config3"]
     qs = gen.config(3)
     assert [q.digest() for q in qs] == [g["digest"] for g in golden]
-    with pkg.Engine(0) as eng:
-        grs = []
+    with pkg.Engine(0, use_graph=graph) as eng:
+        grs, init = [], []
         for q in qs:
             gr = cf.Grammar(q.grammar, text=True)
-            gr.load(eng, cf.Graph.from_query(q))
+            mats = gr.load(eng, cf.Graph.from_query(q))
             grs.append(gr)
-        st = eng.batch_fixpoint([(gr.rule_triples(), gr.matrices) for gr in grs])
-        assert st["iterations"] >= 2
-        for q, gr, g in zip(qs, grs, golden):
-            assert gr.results() == {k: tuple(v) for k, v in g["fingerprints"].items()}, q.name
+            if graph:
+                cp = []
+                for m in mats:
+                    c = eng.matrix(q.n)
+                    c.copy_from(m)
+                    cp.append(c)
+                init.append(cp)
+        for rep in range(2 if graph else 1):
+            if rep:
+                for gr, cp in zip(grs, init):
+                    for m, c in zip(gr.matrices, cp):
+                        m.copy_from(c)
+            st = eng.batch_fixpoint([(gr.rule_triples(), gr.matrices) for gr in grs])
+            assert st["iterations"] >= 2
+            for q, gr, g in zip(qs, grs, golden):
+                assert gr.results() == {k: tuple(v) for k, v in g["fingerprints"].items()}, (q.name, rep)
+        for gr in grs:
             gr.free()
+        for cp in init:
+            for c in cp:
+                c.free()
 
 
 @pytest.mark.parametrize("schedule", ["sweep", "rules"])
@@ -496,9 +514,12 @@ def test_fixpoint_tile_units(pkg, gen, oracle, which):
          "self": lambda: gen.dyck_er(300, 800, seed=21)}[which]()
     if which == "self":
         q.grammar = "S\nS A S1\nS1 S B\nS S S\nA a\nB b\n"
-    for kernel, fuse, schedule in (("auto", False, "delta"), ("auto", True, "delta"), ("gather", False, "rules")):
-        st = _fixpoint_vs_oracle(pkg, gen, oracle, q, kernel=kernel, fuse_apply=fuse, schedule=schedule, sparse_mode=2,
-                                 dense_bits=0xFFFFFFF0 if kernel == "auto" else 0, mfma_bits=0xFFFFFFF0)
+    for kernel, fuse, schedule, mode in (("auto", False, "delta", 2), ("auto", True, "delta", 2),
+                                         ("gather", False, "rules", 2), ("auto", False, "delta", 3),
+                                         ("auto", True, "rules", 3)):
+        st = _fixpoint_vs_oracle(pkg, gen, oracle, q, kernel=kernel, fuse_apply=fuse, schedule=schedule,
+                                 sparse_mode=mode, dense_bits=0xFFFFFFF0 if kernel == "auto" else 0,
+                                 mfma_bits=0xFFFFFFF0)
         assert st["iterations"] >= 2
 
 
