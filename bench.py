@@ -192,6 +192,26 @@ def measure_pairs(args, eng, gr, q, want):
     return statistics.median(times), stats, ok, int(sum(len(k) for k in out))
 
 
+def measure_mfma(eng, gr, mats, init, rules, opts, want, step):
+    """The same fixpoint with every dense tile-product on the int8 matrix cores
+    (v_mfma_i32_32x32x32_i8, 0/1 operands expanded from the bit tiles): its
+    time and its rate against the 5.0 POP/s dense int8 peak, beside the
+    tuned (Four Russians) run.  One run: it is the slow leg."""
+    eng.set_opts(**dict(opts, kernel="mfma"))
+    st = step()
+    ok = gr.results() == want
+    eng.set_opts(**opts)
+    ops = st["tile_products_mfma"] * 2.0 * 256 ** 3
+    executed = st["mfma_instr"] * 2.0 * 32 ** 3
+    t = st["mfma_ms"] / 1e3
+    log(f"[bench] mfma leg: {st['fixpoint_ms']:.1f} ms, {st['tile_products_mfma']} MFMA tile-products")
+    return {"fixpoint_ms": round(st["fixpoint_ms"], 3), "mfma_kernel_ms": round(st["mfma_ms"], 3),
+            "tile_products_mfma": st["tile_products_mfma"],
+            "achieved_TOPs": round(ops / t / 1e12, 2) if t > 0 else None,
+            "frac_of_int8_peak": round(ops / t / 1e12 / MFMA_I8_PEAK_TOPS, 4) if t > 0 else None,
+            "executed_TOPs": round(executed / t / 1e12, 2) if t > 0 else None, "bit_exact": ok}
+
+
 def measure_total(args, eng, gr, mats, init, rules, opts, want):
     """The reference's "total" window (cfpq-cpp/Grammar.h:53-80): upload of
     every nonterminal matrix from host buffers (environment_preprocessing), the
@@ -255,6 +275,8 @@ def main():
     ap.add_argument("--no-worklist", action="store_true",
                     help="skip the reference-structured engine (worklist of add_mul calls)")
     ap.add_argument("--no-pairs", action="store_true", help="skip the pair-list formulation's leg")
+    ap.add_argument("--no-mfma", action="store_true", help="--config 2: skip the forced int8 MFMA leg")
+    ap.add_argument("--no-graph-leg", action="store_true", help="skip the graph-replay leg")
     ap.add_argument("--no-total", action="store_true",
                     help="skip the total window (upload + fixpoint + download from host buffers)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -359,6 +381,10 @@ def main():
         return wall * 1e3 / args.steps, stats, ok
 
     runs = {"plain": measure("plain", opts)}
+    if not args.no_graph_leg and not opts.get("use_graph"):
+        # the same engine with every step replayed from captured hipGraphs
+        # (fewer launch gaps; per-kernel timing needs the eager plain leg)
+        runs["graph"] = measure("graph", dict(opts, use_graph=True, iters_per_sync=8))
     sched_mod = importlib.import_module("cfpq-on-gpgpu_amd.sched")
     weights = None if args.no_learned else sched_mod.load()
     if weights is not None:
@@ -373,10 +399,12 @@ def main():
         runs["worklist"] = measure_worklist(args, eng, gr, mats, init, want, opts)
     pairs_run = None if args.no_pairs else measure_pairs(args, eng, gr, q, want)
     total_window = None if args.no_total else measure_total(args, eng, gr, mats, init, rules, opts, want)
+    mfma_leg = measure_mfma(eng, gr, mats, init, rules, opts, want, step) if (args.config == 2 and not args.no_mfma) else None
     ok_runs = {k: v for k, v in runs.items() if v[2] and k != "worklist"}
     chosen = min(ok_runs, key=lambda k: ok_runs[k][0]) if ok_runs else "plain"
     ms_per_step, stats, bit_exact = runs[chosen]
-    bit_exact = all(v[2] for v in runs.values()) and (total_window is None or total_window["bit_exact"])
+    bit_exact = all(v[2] for v in runs.values()) and (total_window is None or total_window["bit_exact"]) and \
+        (mfma_leg is None or mfma_leg["bit_exact"])
     fix_ms = [s["fixpoint_ms"] for s in stats]
     value = 2.0 * n ** 3 * a_ref / (ms_per_step / 1e3) / 1e12
     engines = {}
@@ -412,14 +440,15 @@ def main():
             "note": "sparse pair-list formulation; ms_per_call includes host arrays both ways"}
         bit_exact = bit_exact and p_ok
 
-    roofline = product_roofline(stats, wname)
+    # the roofline needs per-kernel event timing: the eager plain leg's stats
+    roofline = product_roofline(runs["plain"][1], wname)
 
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline (budget {args.cpu_budget:.0f} s)")
         cpu = cpu_baseline(q, a_ref, args.cpu_budget)
 
-    s0 = stats[-1]
+    s0 = runs["plain"][1][-1]  # work counts and per-kernel times: the eager plain leg
     line = {
         "metric": "CFPQ fixpoint time (s) & eff. Tbit-op/s; batched/learned vs plain, bit-exact",
         "value": round(value, 3),
@@ -437,6 +466,9 @@ def main():
                    "rules": len(rules), "A_ref": a_ref, "A_ref_kind": work_key,
                    "engine": {"plain": {1: "head-sweep fixpoint", 2: "rule-sweep fixpoint"}.get(opts.get("schedule"),
                                                                                               "delta fixpoint"),
+                              "graph": {1: "head-sweep fixpoint", 2: "rule-sweep fixpoint"}.get(opts.get("schedule"),
+                                                                                              "delta fixpoint") +
+                                       ", steps replayed from captured hipGraphs",
                               "learned": "delta fixpoint + learned tile scheduler"}[chosen],
                    "kernel": args.kernel, "parallelism": "single GPU"},
         "fixpoint_s": round(statistics.median(fix_ms) / 1e3, 6),
@@ -452,6 +484,7 @@ def main():
         "engine_opts": opts,
         "engines": engines,
         "window_total": total_window,
+        "mfma_leg": mfma_leg,
         "bit_exact": bit_exact,
         "roofline": roofline,
         "cpu_baseline": cpu,
