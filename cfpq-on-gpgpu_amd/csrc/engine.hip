@@ -278,6 +278,7 @@ static void normalise(cfpq_opts *o) {
     if (o->sched_threshold == 0) o->sched_threshold = 6554;  // P(active) < 0.1
     if (o->schedule != CFPQ_SCHEDULE_SWEEP && o->schedule != CFPQ_SCHEDULE_RULES) o->schedule = CFPQ_SCHEDULE_DELTA;
     if (o->m4r_keys != 8) o->m4r_keys = 4;
+    if (o->order_threshold == 0) o->order_threshold = 32768;  // P(active) >= 1/2
 }
 
 extern "C" int cfpq_create(int device, const cfpq_opts *opts, cfpq_ctx **out) {
@@ -705,6 +706,7 @@ static uint32_t m4r_threshold(const cfpq_opts &o) {
 // prev_slot < 0: the delta consumed by this step is the initial one
 // (cleared in full); otherwise it was written by the step in prev_slot.
 static bool sched_on(const cfpq_ctx *ctx) { return ctx->opts.sched != 0 && ctx->sched_have; }
+static bool order_on(const cfpq_ctx *ctx) { return ctx->opts.sched_order != 0 && ctx->sched_have; }
 
 // Sub-units (partial-row slots) the build may create per step.
 static uint32_t split_slots_of(const cfpq_ctx *ctx, const Plan &plan) {
@@ -713,7 +715,10 @@ static uint32_t split_slots_of(const cfpq_ctx *ctx, const Plan &plan) {
 
 static SchedArgs sched_args(const cfpq_ctx *ctx, bool flush) {
     SchedArgs a = {};
-    a.w = sched_on(ctx) ? ctx->sched_w : nullptr;
+    a.w = (sched_on(ctx) || order_on(ctx)) ? ctx->sched_w : nullptr;
+    a.order = order_on(ctx) && !flush;
+    const double po = std::min(0.999999, std::max(1e-6, ctx->opts.order_threshold / 65536.0));
+    a.order_logit = (float)std::log(po / (1.0 - po));
     a.dirty = ctx->dirty;
     a.dirty_off = ctx->dirty_off;
     const double pth = std::min(0.999999, std::max(1e-6, ctx->opts.sched_threshold / 65536.0));
@@ -770,7 +775,11 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const int split = (lane_parallel && !(fuse && ctx->opts.fuse_apply)) ? 1 : 0;
     const uint32_t split_budget = split_slots_of(ctx, plan);
     const uint32_t heavy_base = (uint32_t)(QUARTERS * plan.total_head_tiles);
-    const bool use_sched = head < 0 && !full_pass && plan.dirty_words && (sched_on(ctx) || ctx->tr_cap || flush);
+    const bool use_sched =
+        head < 0 && !full_pass && plan.dirty_words && (sched_on(ctx) || order_on(ctx) || ctx->tr_cap || flush);
+    // the scheduler's first-phase list (its ranked units), after the build's lists
+    uint64_t *prio_list = ctx->units + (size_t)2 * QUARTERS * plan.total_head_tiles;
+    const bool ordered = use_sched && order_on(ctx) && !flush;
     const SchedArgs sa = sched_args(ctx, flush);
     if (use_sched) {
         if (ctx->tr_cap)
@@ -778,7 +787,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                                (uint32_t)ctx->tr_cap);
         hipLaunchKernelGGL(k_build_sched, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems, sa, split,
-                           split_budget, heavy, heavy_base, tile_units, ctx->splits);
+                           split_budget, heavy, heavy_base, tile_units, ctx->splits, prio_list);
     } else {
         hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, (int)full_pass, c, ctx->units, ctx->items,
@@ -798,12 +807,19 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     fuse = fz != 0;
     if (lane_parallel) {
         const TermPtr *tp = ctx->d_tptr + (size_t)q * plan_terms(plan);
-        if (tile_units)
-            hipLaunchKernelGGL(k_product_sparse_lp<true>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
-                               heavy_base, TouchedLists{touched, seg_cap}, chunk, fz, ctx->partials, ctx->pmask);
-        else
-            hipLaunchKernelGGL(k_product_sparse_lp<false>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
-                               heavy_base, TouchedLists{touched, seg_cap}, chunk, fz, ctx->partials, ctx->pmask);
+        // with the scheduler's ranking, phase 1 evaluates its ranked units and
+        // phase 0 the rest; the launch boundary makes phase 1's head updates
+        // (fused apply) visible to every unit of phase 0
+        for (int phase = ordered ? 1 : 0; phase >= 0; --phase) {
+            if (tile_units)
+                hipLaunchKernelGGL(k_product_sparse_lp<true>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
+                                   heavy_base, TouchedLists{touched, seg_cap}, chunk, fz, ctx->partials, ctx->pmask,
+                                   phase, (const uint64_t *)prio_list);
+            else
+                hipLaunchKernelGGL(k_product_sparse_lp<false>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
+                                   heavy_base, TouchedLists{touched, seg_cap}, chunk, fz, ctx->partials, ctx->pmask,
+                                   phase, (const uint64_t *)prio_list);
+        }
         // the split quarters, from their sub-units' partial rows
         if (split)
             // (a grid of 4 waves per CU measured best: larger grids cost more
@@ -985,7 +1001,7 @@ static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &p
             plan.all_delta[0].size(), plan.all_delta[1].size(), plan.max_tiles_per_head, plan.max_occ_words,
             plan.ids_all_off, plan.total_head_tiles, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
             (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk, (uint64_t)o.sched, o.sched_threshold,
-            (uint64_t)o.sparse_mode, (uint64_t)o.fuse_apply, (uint64_t)o.m4r_keys,
+            (uint64_t)o.sparse_mode, (uint64_t)o.fuse_apply, (uint64_t)o.m4r_keys, (uint64_t)o.sched_order, (uint64_t)o.order_threshold,
             (uint64_t)ctx->sched_have, (uint64_t)(uintptr_t)ctx->sched_w, (uint64_t)(uintptr_t)ctx->dirty,
             (uint64_t)(uintptr_t)ctx->dirty_off, (uint64_t)ctx->tr_cap, (uint64_t)(uintptr_t)ctx->tr_x};
 }

@@ -421,9 +421,22 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
 __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const uint32_t *qcnt, uint64_t base,
                                              int split, uint32_t split_budget, StepCounters *__restrict__ c,
                                              uint64_t *__restrict__ units, uint32_t heavy_base, int tile_units,
-                                             uint64_t *__restrict__ splits) {
+                                             uint64_t *__restrict__ splits, bool prio = false,
+                                             uint64_t *__restrict__ prio_list = nullptr) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
+    if (prio_list) {
+        // items the scheduler ranks first: their units (whole, unsplit) go to
+        // the list the first phase of the step evaluates
+        const bool pr = want && prio && qmask;
+        if (tile_units) {
+            wave_append(pr, base | ITEM_QSET | ((uint64_t)qmask << ITEM_QSET_SHIFT), &c->n_prio, prio_list);
+        } else {
+#pragma unroll
+            for (uint32_t q = 0; q < QUARTERS; ++q) wave_append(pr && ((qmask >> q) & 1u), base | (uint64_t)q, &c->n_prio, prio_list);
+        }
+        want = want && !pr;
+    }
     // the work-unit shape: tile_units 1 = one unit for every quarter of the
     // tile with candidates (256 output rows), 2 = one per half (quarters 0-1
     // and 2-3, 128 rows), 0 = one per quarter (64 rows).  A group holding a
@@ -786,7 +799,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                                             const uint64_t *__restrict__ units,
                                                             uint32_t heavy_base, TouchedLists tl, uint32_t chunk,
                                                             int fuse, uint64_t *__restrict__ partials,
-                                                            uint64_t *__restrict__ pmask) {
+                                                            uint64_t *__restrict__ pmask, int phase,
+                                                            const uint64_t *__restrict__ prio_list) {
     constexpr uint32_t ACC_ROWS = QSET ? TILE : WAVE;
     __shared__ Row4 acc_s[ACC_ROWS];  // rows of the unit's quarters (row 64 q + r with QSET, else r)
     __shared__ Row4 s_brm[WAVE];
@@ -804,9 +818,10 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     __shared__ uint32_t s_gr[WAVE], s_gend[WAVE];
     const int lane = threadIdx.x & 63;
     const uint64_t clk0 = __builtin_amdgcn_s_memtime();
+    // phase 1: the units the scheduler ranked first (prio_list); otherwise
     // list entries [0, n_heavy) are the sub-units of split units, then the others
-    const uint32_t n_heavy = c->n_heavy;
-    const uint32_t n_units = c->n_units + n_heavy;
+    const uint32_t n_heavy = phase == 1 ? 0u : c->n_heavy;
+    const uint32_t n_units = phase == 1 ? c->n_prio : c->n_units + n_heavy;
     // Claims x are mapped to list entries by reversing the low six bits of x:
     // neighbours in the list (the quarters of one tile, the sub-units of one
     // heavy quarter) go to claims far apart within their block of 64, i.e. to
@@ -831,8 +846,9 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     uint32_t kc = 0xffffffffu;
     if (lane == 0) {
         // a stale (smaller) reading of the cursor only costs a wasted atomic
-        const uint32_t seen = __hip_atomic_load(&shards[sh].unit_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((seen / 64u) * SHARDS + sh < nblk) kc = atomicAdd(&shards[sh].unit_next, chunk);
+        uint32_t *cur = phase == 1 ? &shards[sh].prio_next : &shards[sh].unit_next;
+        const uint32_t seen = __hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((seen / 64u) * SHARDS + sh < nblk) kc = atomicAdd(cur, chunk);
     }
     kc = uniform(kc);
     const uint32_t blk = kc == 0xffffffffu ? nblk : (kc / 64u) * SHARDS + sh;
@@ -847,7 +863,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         const uint32_t u = (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
         if (u >= n_units) continue;
         const uint64_t tu0 = __builtin_amdgcn_s_memtime();
-        const uint64_t it = ld_const(units + (u < n_heavy ? heavy_base + u : u - n_heavy));
+        const uint64_t it = phase == 1 ? ld_const(prio_list + u) : ld_const(units + (u < n_heavy ? heavy_base + u : u - n_heavy));
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform(unit_tile(it));
         // the unit's quarters: one, or every quarter of the tile with candidates

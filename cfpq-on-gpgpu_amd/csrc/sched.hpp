@@ -238,6 +238,9 @@ struct SchedArgs {
     float logit_threshold; // defer an item whose best logit is below this
     int defer;             // 1: may defer this step
     int flush;             // 1: evaluate every dirty tile in full, defer nothing
+    int order;             // 1: rank items: predicted active (logit >= order_logit) go to the
+                           //    first phase of the step
+    float order_logit;
     float *trace_x;        // trace rows (nullptr: no trace)
     uint32_t *trace_rule;
     uint32_t *trace_item;  // (head << 40 | tile) per row, for the labels
@@ -254,7 +257,8 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                                                        uint64_t *__restrict__ dense_items,
                                                        uint64_t *__restrict__ mfma_items, SchedArgs sa, int split,
                                                        uint32_t split_budget, int heavy, uint32_t heavy_base,
-                                                       int tile_units, uint64_t *__restrict__ splits) {
+                                                       int tile_units, uint64_t *__restrict__ splits,
+                                                       uint64_t *__restrict__ prio_list) {
     __shared__ float sw[MLP_PARAMS];
     if (sa.w) mlp_load(sa.w, sw);
     const uint32_t h = blockIdx.y;
@@ -267,6 +271,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
     uint32_t qmask = 0;
     uint32_t qcnt[QUARTERS] = {0, 0, 0, 0};
     uint32_t deferred = 0, fulls = 0, scored = 0, newly = 0, cleared = 0;
+    bool prio = false;  // ranked into the step's first phase
     if (p < T * T) {
         const uint32_t I = p / T, J = p - I * T;
         uint32_t *dw = sa.dirty + sa.dirty_off[h] + (p >> 5);
@@ -302,6 +307,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                         }
                     }
                 }
+                prio = sa.w && sa.order && best >= sa.order_logit;
                 if (sa.w && sa.defer && best < sa.logit_threshold) {
                     evaluate = false;
                     if (!dirty) {
@@ -317,6 +323,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                 atomicAnd(dw, ~dbit);
                 ++cleared;
                 full = true;
+                prio = false;
                 qmask = 0;
                 dense = mfma = false;
                 qcnt[0] = qcnt[1] = qcnt[2] = qcnt[3] = 0;
@@ -338,7 +345,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
     append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units,
-                 heavy_base, tile_units, splits);
+                 heavy_base, tile_units, splits, prio, sa.order ? prio_list : nullptr);
     deferred = wave_sum(deferred);
     fulls = wave_sum(fulls);
     scored = wave_sum(scored);

@@ -112,6 +112,7 @@ struct alignas(128) CounterShard {
     // (12-23) and the epilogue (0-11)
     uint64_t unit_max_a, unit_max_b, unit_max_c, unit_max_d;
     uint32_t unit_next;       // claims taken from this shard's blocks of 64 units
+    uint32_t prio_next;       // the same, over the first phase's list (scheduler ordering)
     uint32_t max_unit_cands;
     // entries of this shard's segment of the touched-tile list, on a cache
     // line of its own (apart from the cursor the same waves claim from)
@@ -139,6 +140,7 @@ struct StepCounters {
     uint32_t split_extra;    // extra sub-units created by splitting heavy units (budget counter)
     uint32_t n_split;        // heavy units split
     uint32_t n_heavy;        // sub-units of split units, listed apart so that they are claimed first
+    uint32_t n_prio;         // units the scheduler ranked first (evaluated by the step's first phase)
     uint64_t new_bits;
     uint64_t tile_products;   // (term, C tile, k) products evaluated (sparse: per quarter)
     uint64_t tp_dense;        // of which on the LDS / Four Russians paths

@@ -102,6 +102,14 @@ typedef struct cfpq_opts {
     int32_t m4r_keys;           /* Four Russians key width: 4 = 16-entry tables over whole
                                    tile rows, shared by the work-group (default); 8 =
                                    256-entry tables per wavefront over a 64-column slice */
+    int32_t sched_order;        /* 1: the learned tile scheduler ranks the work of each delta
+                                   step: the (head, tile) items it predicts active are
+                                   evaluated in a first phase, the rest in a second one, which
+                                   with fuse_apply reads the heads the first phase updated
+                                   (needs weights; independent of deferral)              */
+    uint32_t order_threshold;   /* sched_order: an item goes to the first phase when its
+                                   predicted activity probability is at least
+                                   order_threshold / 65536 (0 = library default, 1/2)    */
 } cfpq_opts;
 
 /* cfpq_opts.schedule */

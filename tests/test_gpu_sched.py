@@ -175,3 +175,37 @@ def test_learned_sg_forest_65536_golden(pkg, gen):
         assert st["full_passes"] == 1
         assert st["sched_scored"] > 0
         gr.free()
+
+
+@pytest.mark.parametrize("which", ["dense", "dyck", "sg", "closure"])
+def test_ordered_fixpoint_is_bit_exact(pkg, gen, oracle, which):
+    """The scheduler's ranking (its predicted-active items evaluated in a
+    first phase of each delta step, the rest in a second) with trained and
+    untrained weights, with and without the fused head update, alone and
+    with deferral: the least fixpoint is the same."""
+    import numpy as np
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    sched = __import__("importlib").import_module("cfpq-on-gpgpu_amd.sched")
+    if which == "closure":
+        rng = np.random.default_rng(81)
+        n, m = 1500, 2200
+        src, dst = rng.integers(0, n, m).astype(np.uint32), rng.integers(0, n, m).astype(np.uint32)
+        q = gen.Query("closure", n, "S S S\nS a\n", src, dst, np.zeros(m, np.uint8), ("a",))
+    else:
+        q = {"dense": lambda: gen.dense_stress(1200), "dyck": lambda: gen.dyck_er(1500, 5000, seed=6),
+             "sg": lambda: gen.sg_forest(6000, seed=7, types_per_vertex=3)}[which]()
+    g_ref, mats_ref, _ = oracle.solve(q, "seminaive")
+    want = oracle.fingerprints(g_ref, mats_ref, q.n)
+    trained = sched.load()
+    for weights in ("trained", "untrained"):
+        for fuse, defer in ((True, False), (False, False), (True, True)):
+            with pkg.Engine(0, sched_order=True, fuse_apply=fuse, sched=defer, sched_threshold=0.3) as eng:
+                if weights == "trained":
+                    eng.sched_set_weights(trained)
+                else:
+                    eng.sched_init(5)
+                gr = cf.Grammar(q.grammar, text=True)
+                _, _, st = gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+                assert gr.results() == want, (which, weights, fuse, defer)
+                assert st["sched_scored"] > 0
+                gr.free()
