@@ -480,7 +480,7 @@ def main():
                                     "a_rows", "a_cols", "b_rows", "b_rows_lds", "m4r_lookups", "mfma_instr", "out_rows",
                                     "new_bits", "product_launches", "host_syncs", "cand_hits",
                                     "max_unit_cands", "a_rows_dense", "b_rows_dense", "out_rows_dense")},
-        "kernel_ms": {k: round(s0[k], 4) for k in ("sparse_ms", "dense_ms", "mfma_ms")},
+        "kernel_ms": {k: round(s0[k], 4) for k in ("sparse_ms", "merge_ms", "dense_ms", "mfma_ms")},
         "engine_opts": opts,
         "engines": engines,
         "window_total": total_window,

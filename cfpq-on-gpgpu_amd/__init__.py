@@ -98,6 +98,7 @@ class Stats(ctypes.Structure):
         ("unit_clocks", ctypes.c_uint64),
         ("unit_clocks_idle", ctypes.c_uint64),
         ("b_rows_idle", ctypes.c_uint64),
+        ("merge_ms", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:

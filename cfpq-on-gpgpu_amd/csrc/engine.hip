@@ -152,6 +152,7 @@ constexpr size_t NOALLOC_TILES = (size_t)NOALLOC_T * NOALLOC_T;
 constexpr size_t NA_MATS_CAP = 4, NA_HEADS_CAP = 1, NA_TERMS_CAP = 3, NA_TPTR_CAP = 6, NA_IDS_CAP = 8;
 constexpr size_t NA_UNITS_CAP = 3 * QUARTERS * NOALLOC_TILES, NA_LIST_CAP = (1 + SHARDS) * NOALLOC_TILES;
 constexpr size_t NA_DIRTY_CAP = (NOALLOC_TILES + 31) / 32;
+constexpr int EV_SLOT = 5;  // timing events per ring slot (sparse, merge, dense, MFMA bounds)
 constexpr size_t NA_RING = align256(mat_bytes(NOALLOC_T));
 constexpr size_t NA_MATS = NA_RING + align256(sizeof(StepCounters) * RING);
 constexpr size_t NA_HEADS = NA_MATS + align256(sizeof(MatDev) * NA_MATS_CAP);
@@ -796,8 +797,11 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                            tile_units, ctx->splits);
     }
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
-    // product kernels, timed: ev[4i] sparse ev[4i+1] dense ev[4i+2] mfma ev[4i+3]
-    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx], ctx->stream));
+    // product kernels, timed by consecutive events of slot i: sparse
+    // ev[5i]..ev[5i+1], merge of split quarters ..ev[5i+2], dense ..ev[5i+3],
+    // MFMA ..ev[5i+4]
+    hipEvent_t *ev = ev_idx >= 0 ? &ctx->ev[EV_SLOT * (size_t)ev_idx] : nullptr;
+    if (ev) HIPCHK(hipEventRecord(ev[0], ctx->stream));
     const dim3 sgrid((uint32_t)ctx->num_cu * (uint32_t)ctx->opts.sparse_waves_per_cu);
     const uint32_t chunk = (uint32_t)ctx->opts.sparse_chunk;
     const MatDev *dm = ctx->d_mats;
@@ -820,6 +824,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                                    heavy_base, TouchedLists{touched, seg_cap}, chunk, fz, ctx->partials, ctx->pmask,
                                    phase, (const uint64_t *)prio_list);
         }
+        if (ev) HIPCHK(hipEventRecord(ev[1], ctx->stream));
         // the split quarters, from their sub-units' partial rows
         if (split)
             // (a grid of 4 waves per CU measured best: larger grids cost more
@@ -837,18 +842,19 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
         hipLaunchKernelGGL(k_product_sparse<4>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
                            fz);
     }
-    if (ev_idx >= 0) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 1], ctx->stream));
+    if (ev && !lane_parallel) HIPCHK(hipEventRecord(ev[1], ctx->stream));
+    if (ev) HIPCHK(hipEventRecord(ev[2], ctx->stream));
     // (the dense and MFMA launches, and their events, only with a heavy-tile path on)
     if (heavy)
         hipLaunchKernelGGL(k_product_dense, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
                            dense_threshold(ctx->opts), m4r_threshold(ctx->opts), fz, ctx->opts.m4r_keys == 8 ? 1 : 0);
-    if (ev_idx >= 0 && heavy) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 2], ctx->stream));
+    if (ev && heavy) HIPCHK(hipEventRecord(ev[3], ctx->stream));
     if (heavy)
         hipLaunchKernelGGL(k_product_mfma, dim3((uint32_t)ctx->num_cu), dim3(MFMA_BLOCK), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, terms, q, c,
                            (const uint64_t *)ctx->mitems, touched, fz);
-    if (ev_idx >= 0 && heavy) HIPCHK(hipEventRecord(ctx->ev[4 * ev_idx + 3], ctx->stream));
+    if (ev && heavy) HIPCHK(hipEventRecord(ev[4], ctx->stream));
     if (use_sched && ctx->tr_cap)
         hipLaunchKernelGGL(k_trace_label, dim3((uint32_t)ctx->num_cu), dim3(256), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, (const StepCounters *)c, sa);
@@ -1089,9 +1095,9 @@ static int run_sweeps(cfpq_ctx *ctx, const Plan &plan, bool heavy_paths, bool ce
             for (size_t s = 0; s < slots.size(); ++s) {
                 StepCounters &c = ctx->ring_host[slots[s]];
                 fold_shards(c);
-                float step_ms[3] = {0, 0, 0};
-                for (int k2 = 0; k2 < (heavy_paths ? 3 : 1); ++k2) {
-                    HIPCHK(hipEventElapsedTime(&step_ms[k2], ctx->ev[4 * s + k2], ctx->ev[4 * s + k2 + 1]));
+                float step_ms[4] = {0, 0, 0, 0};
+                for (int k2 = 0; k2 < (heavy_paths ? 4 : 2); ++k2) {
+                    HIPCHK(hipEventElapsedTime(&step_ms[k2], ctx->ev[EV_SLOT * s + k2], ctx->ev[EV_SLOT * s + k2 + 1]));
                     part_ms[k2] += step_ms[k2];
                     product_ms += step_ms[k2];
                 }
@@ -1146,11 +1152,11 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             ctx->graph_sig = sig;
         }
     }
-    CHK(ensure_events(ctx, 4 * (size_t)RING + 2));
-    hipEvent_t ev_begin = ctx->ev[4 * RING], ev_end = ctx->ev[4 * RING + 1];
+    CHK(ensure_events(ctx, EV_SLOT * (size_t)RING + 2));
+    hipEvent_t ev_begin = ctx->ev[EV_SLOT * RING], ev_end = ctx->ev[EV_SLOT * RING + 1];
     HIPCHK(hipEventRecord(ev_begin, ctx->stream));
     StepStats tot;
-    double product_ms = 0, part_ms[3] = {0, 0, 0};
+    double product_ms = 0, part_ms[4] = {0, 0, 0, 0};  // sparse, merge, dense, MFMA
     uint32_t steps = 0, syncs = 0, launches = 0, full_passes = 0, flush_steps = 0;
     uint64_t sched_scored = 0, sched_deferred = 0, sched_full = 0;
     int64_t dirty_total = 0;  // tiles deferred and not yet evaluated in full
@@ -1196,9 +1202,9 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             // after a quiet one still retires dirty tiles whose head tile is full)
             dirty_total += (int64_t)c.dirty_new - (int64_t)c.dirty_cleared;
             if (quiet) continue;
-            float step_ms[3] = {0, 0, 0};
-            for (int k2 = 0; k2 < (heavy_paths ? 3 : 1) && !use_graph; ++k2) {
-                HIPCHK(hipEventElapsedTime(&step_ms[k2], ctx->ev[4 * s + k2], ctx->ev[4 * s + k2 + 1]));
+            float step_ms[4] = {0, 0, 0, 0};
+            for (int k2 = 0; k2 < (heavy_paths ? 4 : 2) && !use_graph; ++k2) {
+                HIPCHK(hipEventElapsedTime(&step_ms[k2], ctx->ev[EV_SLOT * s + k2], ctx->ev[EV_SLOT * s + k2 + 1]));
                 part_ms[k2] += step_ms[k2];
                 product_ms += step_ms[k2];
             }
@@ -1208,7 +1214,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
                              "max_unit %u split %u a_rows %llu b_rows %llu busy %.3f "
                              "wave_max %llu unit_max %llu clk (nsub %u cands %u pairs %u gathers %u; clk scan %u filter %u "
                              "pairs %u epilogue %u) | sparse %.3f "
-                             "dense %.3f mfma %.3f ms\n",
+                             "merge %.3f dense %.3f mfma %.3f ms\n",
                              steps - k + s, c.n_units + c.n_heavy, c.n_dense, c.n_mfma, c.n_touched,
                              (unsigned long long)c.new_bits, (unsigned long long)c.tile_products,
                              (unsigned long long)c.cand_hits, c.max_unit_cands, c.n_split, (unsigned long long)c.a_rows,
@@ -1222,7 +1228,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
                              (unsigned)((c.sparse_unit_max_d >> 12) & 4095u) * 256u,
                              (unsigned)((c.sparse_unit_max_c >> 12) & 4095u) * 256u,
                              (unsigned)(c.sparse_unit_max_c & 4095u) * 256u, (unsigned)(c.sparse_unit_max_d & 4095u) * 256u,
-                             step_ms[0], step_ms[1], step_ms[2]);
+                             step_ms[0], step_ms[1], step_ms[2], step_ms[3]);
             accumulate(tot, c);
             sched_scored += c.n_scored;
             sched_deferred += c.n_deferred;
@@ -1291,8 +1297,9 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         stats->host_syncs = syncs;
         stats->mfma_instr = tot.mfma_instr;
         stats->sparse_ms = part_ms[0];
-        stats->dense_ms = part_ms[1];
-        stats->mfma_ms = part_ms[2];
+        stats->merge_ms = part_ms[1];
+        stats->dense_ms = part_ms[2];
+        stats->mfma_ms = part_ms[3];
         stats->sched_scored = sched_scored;
         stats->sched_deferred = sched_deferred;
         stats->sched_full = sched_full;
@@ -1866,8 +1873,8 @@ extern "C" int cfpq_nt_closure(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nru
             HIPCHK(hipMemcpyAsync(d_rules, packed.data(), nrules * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipMemcpyAsync(T, elems, total * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        CHK(ensure_events(ctx, 4 * (size_t)RING + 2));
-        HIPCHK(hipEventRecord(ctx->ev[4 * RING], ctx->stream));
+        CHK(ensure_events(ctx, EV_SLOT * (size_t)RING + 2));
+        HIPCHK(hipEventRecord(ctx->ev[EV_SLOT * RING], ctx->stream));
         const uint32_t nb = (n + NT_BLOCK - 1) / NT_BLOCK;
         const uint32_t ga = (uint32_t)std::min<size_t>((total + 255) / 256, (size_t)ctx->num_cu * 16);
         for (;;) {
@@ -1888,10 +1895,10 @@ extern "C" int cfpq_nt_closure(cfpq_ctx *ctx, const cfpq_rule *rules, size_t nru
             if (ctx->opts.max_iterations && steps >= ctx->opts.max_iterations)
                 return fail(ctx, CFPQ_E_LIMIT, "cfpq_nt_closure: iteration cap %u reached", ctx->opts.max_iterations);
         }
-        HIPCHK(hipEventRecord(ctx->ev[4 * RING + 1], ctx->stream));
+        HIPCHK(hipEventRecord(ctx->ev[EV_SLOT * RING + 1], ctx->stream));
         HIPCHK(hipMemcpyAsync(elems, T, total * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        HIPCHK(hipEventElapsedTime(&ms, ctx->ev[4 * RING], ctx->ev[4 * RING + 1]));
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev[EV_SLOT * RING], ctx->ev[EV_SLOT * RING + 1]));
         return CFPQ_OK;
     };
     rc = run();
@@ -2069,8 +2076,8 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
     };
 
     auto run = [&]() -> int {
-        CHK(ensure_events(ctx, 4 * (size_t)RING + 2));
-        HIPCHK(hipEventRecord(ctx->ev[4 * RING], st));
+        CHK(ensure_events(ctx, EV_SLOT * (size_t)RING + 2));
+        HIPCHK(hipEventRecord(ctx->ev[EV_SLOT * RING], st));
         // load: pack, sort, drop duplicates (one wait for all sets), build the column order
         CHK(pairs_grow(ctx, words, std::max<size_t>(nsets, group + 2)));
         for (uint32_t x = 0; x < nsets; ++x) {
@@ -2249,9 +2256,9 @@ extern "C" int cfpq_pairs_fixpoint(cfpq_ctx *ctx, const cfpq_rule *rules, size_t
             if (ctx->opts.max_iterations && steps >= ctx->opts.max_iterations)
                 return fail(ctx, CFPQ_E_LIMIT, "cfpq_pairs_fixpoint: iteration cap %u reached", ctx->opts.max_iterations);
         }
-        HIPCHK(hipEventRecord(ctx->ev[4 * RING + 1], st));
+        HIPCHK(hipEventRecord(ctx->ev[EV_SLOT * RING + 1], st));
         CHK(host_wait());
-        HIPCHK(hipEventElapsedTime(&ms, ctx->ev[4 * RING], ctx->ev[4 * RING + 1]));
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev[EV_SLOT * RING], ctx->ev[EV_SLOT * RING + 1]));
         bool fits = true;
         for (uint32_t x = 0; x < nsets; ++x) fits = fits && rel[x].count <= sets[x].cap;
         if (!fits) {

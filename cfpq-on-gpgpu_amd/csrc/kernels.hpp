@@ -842,21 +842,26 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     uint64_t unit_clk = 0, idle_clk = 0;  // lane 0: clocks of all units / of units that added no bit
     uint32_t idle_units = 0, idle_brows = 0, units_done = 0;
     uint32_t sh = home, used_up = 0;
+    // The claim of the next chunk is issued when a chunk starts and read when
+    // it ends, so the atomic's round trip overlaps the chunk's own loads.  A
+    // claim is always taken by the wave that issued it (cursors only grow, so
+    // a claim past the end of a shard only says the shard is used up).
+    auto claim = [&](uint32_t s) -> uint32_t {
+        uint32_t k = 0xffffffffu;
+        if (lane == 0) k = atomicAdd(phase == 1 ? &shards[s].prio_next : &shards[s].unit_next, chunk);
+        return k;
+    };
+    uint32_t pending = claim(sh);
     while (used_up < SHARDS) {
-    uint32_t kc = 0xffffffffu;
-    if (lane == 0) {
-        // a stale (smaller) reading of the cursor only costs a wasted atomic
-        uint32_t *cur = phase == 1 ? &shards[sh].prio_next : &shards[sh].unit_next;
-        const uint32_t seen = __hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((seen / 64u) * SHARDS + sh < nblk) kc = atomicAdd(cur, chunk);
-    }
-    kc = uniform(kc);
-    const uint32_t blk = kc == 0xffffffffu ? nblk : (kc / 64u) * SHARDS + sh;
-    if (blk >= nblk) {  // this shard's blocks are all claimed (cursors only grow)
+    const uint32_t kc = uniform(pending);
+    const uint32_t blk = (kc / 64u) * SHARDS + sh;
+    if (blk >= nblk) {  // this shard's blocks are all claimed
         sh = (sh + 1) % SHARDS;
         ++used_up;
+        if (used_up < SHARDS) pending = claim(sh);
         continue;
     }
+    pending = claim(sh);
     const uint32_t u0 = blk * 64u + kc % 64u;
     const uint32_t u1 = (u0 + chunk < n_claim) ? u0 + chunk : n_claim;
     for (uint32_t x = u0; x < u1; ++x) {

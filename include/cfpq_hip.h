@@ -149,7 +149,8 @@ typedef struct cfpq_stats {
     uint32_t product_launches;  /* fixpoint steps whose product kernels were timed       */
     uint32_t host_syncs;        /* host waits on the device flag                         */
     uint64_t mfma_instr;        /* v_mfma_i32_32x32x32_i8 issued (per wave; 2*32^3 ops each) */
-    double sparse_ms;           /* device time in the sparse (wave-per-quarter) product  */
+    double sparse_ms;           /* device time in the sparse product (k_product_sparse*),
+                                   the merge of split quarters excluded (merge_ms)       */
     double dense_ms;            /* device time in the LDS / Four Russians product        */
     double mfma_ms;             /* device time in the MFMA product                       */
     uint64_t sched_scored;      /* (rule, tile) items scored by the learned scheduler    */
@@ -169,6 +170,8 @@ typedef struct cfpq_stats {
     uint64_t unit_clocks_idle;  /* summed clocks of the units that added no bit: what a
                                    scheduler skipping exactly those would save         */
     uint64_t b_rows_idle;       /* B rows gathered by the units that added no bit        */
+    double merge_ms;            /* device time merging split quarters' partial rows
+                                   (k_merge_split; part of product_ms)                   */
 } cfpq_stats;
 
 /* ---- lifecycle ---------------------------------------------------------- */
