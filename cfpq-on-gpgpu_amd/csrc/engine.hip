@@ -280,6 +280,7 @@ static void normalise(cfpq_opts *o) {
     if (o->schedule != CFPQ_SCHEDULE_SWEEP && o->schedule != CFPQ_SCHEDULE_RULES) o->schedule = CFPQ_SCHEDULE_DELTA;
     if (o->m4r_keys != 8) o->m4r_keys = 4;
     if (o->order_threshold == 0) o->order_threshold = 32768;  // P(active) >= 1/2
+    if (o->split_cands == 0) o->split_cands = SPLIT_CANDS;
 }
 
 extern "C" int cfpq_create(int device, const cfpq_opts *opts, cfpq_ctx **out) {
@@ -773,7 +774,8 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     const int lane_parallel = ctx->opts.sparse_mode != 1;  // modes 0, 3, 2: quarter, half-tile, tile units
     // work-unit shape of the lane-parallel kernel: 2 = whole tiles, 3 = halves
     const int tile_units = ctx->opts.sparse_mode == 2 ? 1 : ctx->opts.sparse_mode == 3 ? 2 : 0;
-    const int split = (lane_parallel && !(fuse && ctx->opts.fuse_apply)) ? 1 : 0;
+    // candidate count above which a quarter is split (0: no splitting)
+    const uint32_t split = (lane_parallel && !(fuse && ctx->opts.fuse_apply)) ? ctx->opts.split_cands : 0u;
     const uint32_t split_budget = split_slots_of(ctx, plan);
     const uint32_t heavy_base = (uint32_t)(QUARTERS * plan.total_head_tiles);
     const bool use_sched =
@@ -1008,6 +1010,7 @@ static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &p
             plan.ids_all_off, plan.total_head_tiles, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
             (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk, (uint64_t)o.sched, o.sched_threshold,
             (uint64_t)o.sparse_mode, (uint64_t)o.fuse_apply, (uint64_t)o.m4r_keys, (uint64_t)o.sched_order, (uint64_t)o.order_threshold,
+            (uint64_t)o.split_cands,
             (uint64_t)ctx->sched_have, (uint64_t)(uintptr_t)ctx->sched_w, (uint64_t)(uintptr_t)ctx->dirty,
             (uint64_t)(uintptr_t)ctx->dirty_off, (uint64_t)ctx->tr_cap, (uint64_t)(uintptr_t)ctx->tr_x};
 }

@@ -407,7 +407,7 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
 }
 
 // Append the sparse units of one (head, C tile): quarters in qmask, a heavy
-// quarter (more than SPLIT_CANDS candidates) as up to SPLIT_MAX sub-units
+// quarter (more than `split` candidates; split = 0: none) as up to SPLIT_MAX sub-units
 // while the split budget lasts.  Sub-unit s of a split quarter takes the
 // candidates g = s (mod nsub) of the quarter and leaves its rows in a slot
 // of the partial-row buffer (slot = its index in the sub-unit list); the
@@ -419,7 +419,7 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
 // at units + heavy_base (count n_heavy), which the sparse kernel claims
 // first: the longest units start early instead of ending the launch late.
 __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const uint32_t *qcnt, uint64_t base,
-                                             int split, uint32_t split_budget, StepCounters *__restrict__ c,
+                                             uint32_t split, uint32_t split_budget, StepCounters *__restrict__ c,
                                              uint64_t *__restrict__ units, uint32_t heavy_base, int tile_units,
                                              uint64_t *__restrict__ splits, bool prio = false,
                                              uint64_t *__restrict__ prio_list = nullptr) {
@@ -449,7 +449,7 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
             const uint32_t m = rest & gm;
             bool heavyq = false;
 #pragma unroll
-            for (uint32_t q = 0; q < QUARTERS; ++q) heavyq |= split && ((m >> q) & 1u) && qcnt[q] > SPLIT_CANDS;
+            for (uint32_t q = 0; q < QUARTERS; ++q) heavyq |= split && ((m >> q) & 1u) && qcnt[q] > split;
             const bool tu = m && !heavyq;
             wave_append(tu, base | ITEM_QSET | ((uint64_t)m << ITEM_QSET_SHIFT), &c->n_units, units);
             if (tu) rest &= ~gm;
@@ -462,8 +462,8 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
     for (uint32_t q = 0; q < QUARTERS; ++q) {
         const bool wq = ((rest >> q) & 1u) != 0;
         nsub[q] = wq ? 1u : 0u;
-        if (wq && split && qcnt[q] > SPLIT_CANDS) {
-            uint32_t ns = (qcnt[q] + SPLIT_CANDS - 1) / SPLIT_CANDS;
+        if (wq && split && qcnt[q] > split) {
+            uint32_t ns = (uint32_t)(((uint64_t)qcnt[q] + split - 1) / split);
             ns = ns > SPLIT_MAX ? SPLIT_MAX : (ns < 1 ? 1 : ns);
             // the budget counts sub-units (partial-row slots); a refused
             // reservation only leaves budget unused
@@ -502,7 +502,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
                                                  const TermDev *__restrict__ terms, int parity, int full_pass,
                                                  StepCounters *__restrict__ c, uint64_t *__restrict__ units,
                                                  uint64_t *__restrict__ dense_items,
-                                                 uint64_t *__restrict__ mfma_items, int split,
+                                                 uint64_t *__restrict__ mfma_items, uint32_t split,
                                                  uint32_t split_budget, int heavy, uint32_t heavy_base,
                                                  uint32_t h0, const StepCounters *__restrict__ prevc,
                                                  int tile_units, uint64_t *__restrict__ splits) {
@@ -681,6 +681,35 @@ __device__ __forceinline__ int select_bit(uint64_t m, uint32_t j) {
 // per round in its column-major gathers: their loads are in flight together.
 constexpr int COL_ITEMS = 4;
 
+// A lane ORs its items into the rows their 64-bit row masks name, one LDS
+// atomic per row and word.  Lanes walk their masks starting at bit (lane), not
+// at bit 0: where many lanes hold dense masks (a hub column shared by most
+// rows of a quarter) they then hit 64 different rows at each step of the walk
+// instead of all hitting row 0, then row 1, ..., which serialized the atomics
+// 64-fold.  rot_rows rotates the mask right by `lane`; rot_row maps the
+// lowest set bit of a rotated mask back to its row.
+__device__ __forceinline__ uint64_t rot_rows(uint64_t m, int lane) {
+    const uint32_t s = (uint32_t)lane & 63u;
+    return s ? (m >> s) | (m << (64u - s)) : m;
+}
+__device__ __forceinline__ uint32_t rot_row(uint64_t rr, int lane) {
+    return ((uint32_t)__builtin_ctzll(rr) + (uint32_t)lane) & 63u;
+}
+
+// The row accumulator in LDS keeps word i of row r in slot (i + r / 8) mod 4.
+// Rows are 32 bytes apart, so without the rotation the 64 lanes of one
+// atomic instruction, on 64 consecutive rows, fall on 8 bank groups (8-way
+// conflicts); rotating by r / 8 spreads them over 32.
+__device__ __forceinline__ uint32_t acc_word(uint32_t r, int i) { return ((uint32_t)i + (r >> 3)) & 3u; }
+// A row read back from the accumulator (row r mod 64 == lane) in word order.
+__device__ __forceinline__ Row4 acc_unswizzle(const Row4 &v, uint32_t r) {
+    const uint32_t k = (r >> 3) & 3u;
+    Row4 o = v;
+    if (k & 1u) o = Row4{{o.w[1], o.w[2], o.w[3], o.w[0]}};
+    if (k & 2u) o = Row4{{o.w[2], o.w[3], o.w[0], o.w[1]}};
+    return o;
+}
+
 // Column of the j-th (0-based) set bit of a 256-bit row mask (j < popcount).
 __device__ __forceinline__ uint32_t select_bit256(const Row4 &m, uint32_t j) {
     uint32_t base = 0, k = 0;
@@ -813,9 +842,6 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     __shared__ const uint64_t *s_bb[WAVE];
     __shared__ uint32_t s_end[WAVE], s_kb[WAVE], s_cend[WAVE];
     __shared__ LaneOps s_ops[WAVE];
-    __shared__ Row4 s_gx[WAVE];
-    __shared__ const uint64_t *s_gb[WAVE];
-    __shared__ uint32_t s_gr[WAVE], s_gend[WAVE];
     const int lane = threadIdx.x & 63;
     const uint64_t clk0 = __builtin_amdgcn_s_memtime();
     // phase 1: the units the scheduler ranked first (prio_list); otherwise
@@ -1029,10 +1055,10 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                 r0[j] = QSET ? s_hq[lo] * WAVE : 0u;
                                 const uint64_t *Bt = s_bb[lo];
                                 if ((uintptr_t)Bt & 1u) {  // full B tile: every row of the quarter becomes all ones
-                                    for (uint64_t rr = s_arm[lo]; rr; rr &= rr - 1) {
-                                        const uint32_t r = r0[j] + (uint32_t)__builtin_ctzll(rr);
+                                    for (uint64_t rr = rot_rows(s_arm[lo], lane); rr; rr &= rr - 1) {
+                                        const uint32_t r = r0[j] + rot_row(rr, lane);
 #pragma unroll
-                                        for (int i = 0; i < 4; ++i) atomicOr((unsigned long long *)&acc_s[r].w[i], ~0ull);
+                                        for (int i = 0; i < 4; ++i) atomicOr((unsigned long long *)&acc_s[r].w[i], ~0ull);  // (all words: no swizzle)
                                     }
                                 } else {
                                     const Row4 rel = s_brm[lo];
@@ -1045,11 +1071,11 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                         }
 #pragma unroll
                         for (int j = 0; j < COL_ITEMS; ++j) {
-                            for (uint64_t rr = rows[j]; rr; rr &= rr - 1) {
-                                const uint32_t r = r0[j] + (uint32_t)__builtin_ctzll(rr);
+                            for (uint64_t rr = rot_rows(rows[j], lane); rr; rr &= rr - 1) {
+                                const uint32_t r = r0[j] + rot_row(rr, lane);
 #pragma unroll
                                 for (int i = 0; i < 4; ++i)
-                                    if (b[j].w[i]) atomicOr((unsigned long long *)&acc_s[r].w[i], (unsigned long long)b[j].w[i]);
+                                    if (b[j].w[i]) atomicOr((unsigned long long *)&acc_s[r].w[acc_word(r, i)], (unsigned long long)b[j].w[i]);
                             }
                         }
                     }
@@ -1068,7 +1094,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             // slot (its index u in the sub-unit list), non-empty ones only,
             // with their mask; k_merge_split finishes the quarter
             const uint32_t qq = (uint32_t)__builtin_ctz(qset);
-            const Row4 acc = acc_s[(QSET ? qq * WAVE : 0u) + lane];
+            const Row4 acc = acc_unswizzle(acc_s[(QSET ? qq * WAVE : 0u) + lane], (uint32_t)lane);
             const bool nz = row_nonzero(acc);
             if (nz) store_row(partials + ((size_t)u * WAVE + lane) * TROW_WORDS, acc);
             const uint64_t bal = __ballot(nz);
@@ -1077,7 +1103,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
 #pragma unroll
             for (uint32_t qq = 0; qq < QUARTERS; ++qq) {
                 if (!((qset >> qq) & 1u)) continue;
-                const Row4 acc = acc_s[(QSET ? qq * WAVE : 0u) + lane];
+                const Row4 acc = acc_unswizzle(acc_s[(QSET ? qq * WAVE : 0u) + lane], (uint32_t)lane);
                 pc += quarter_epilogue(O, M, p, qq, I, J, acc, fuse, out_rows).pc;
             }
         }

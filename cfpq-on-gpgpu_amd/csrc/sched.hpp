@@ -255,7 +255,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                                                        const TermDev *__restrict__ terms, int parity,
                                                        StepCounters *__restrict__ c, uint64_t *__restrict__ units,
                                                        uint64_t *__restrict__ dense_items,
-                                                       uint64_t *__restrict__ mfma_items, SchedArgs sa, int split,
+                                                       uint64_t *__restrict__ mfma_items, SchedArgs sa, uint32_t split,
                                                        uint32_t split_budget, int heavy, uint32_t heavy_base,
                                                        int tile_units, uint64_t *__restrict__ splits,
                                                        uint64_t *__restrict__ prio_list) {

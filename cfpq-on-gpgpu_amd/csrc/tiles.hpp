@@ -240,8 +240,9 @@ constexpr int ITEM_QSET_SHIFT = 34;
 constexpr uint64_t ITEM_FULL = 1ull << 63;
 constexpr uint32_t MAX_HEADS = 1u << 15;   // heads per plan (15-bit field)
 constexpr uint32_t SPLIT_MAX = 16;         // sub-units per heavy sparse unit
-constexpr uint32_t SPLIT_CANDS = 64;       // a unit with more candidates than this is split,
-                                           // one sub-unit per SPLIT_CANDS candidates
+constexpr uint32_t SPLIT_CANDS = 64;       // default of cfpq_opts.split_cands: a unit with more
+                                           // candidates than this is split, one sub-unit per
+                                           // split_cands candidates
 __host__ __device__ inline uint32_t item_head(uint64_t it) { return (uint32_t)((it >> 40) & 0x7fffu); }
 __host__ __device__ inline uint32_t unit_sub(uint64_t it) { return (uint32_t)((it >> 55) & 15u); }
 __host__ __device__ inline uint32_t unit_tile(uint64_t it) { return (uint32_t)((it >> 2) & 0xffffffffu); }

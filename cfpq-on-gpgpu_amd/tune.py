@@ -16,6 +16,7 @@ hipEvents, median of a few runs) while walking the parameters one at a time
   mfma_bits            A-tile popcount from which a tile goes to the int8 MFMA kernel
   iters_per_sync       fixpoint steps enqueued per host read of the counters
   fuse_apply           merge new rows into the head matrices inside the products
+  split_cands          candidate k-tiles above which a quarter is split into sub-units
 
 Every candidate is checked against the first run's fingerprints, so a tuning
 run is also a parity run.  Results are cached in ``tuned.json`` next to this
@@ -44,12 +45,13 @@ SPACE = [
     ("mfma_bits", [8192, 16384, 32768, 0xFFFFFFF0]),
     ("iters_per_sync", [2, 4, 8]),
     ("fuse_apply", [0, 1]),
+    ("split_cands", [32, 64, 128, 0xFFFFFFFF]),
 ]
 # Graph replay (use_graph) is a dimension for batches (tune_batch), where the
 # steps are short and launch-bound; for single queries the caller decides, as
 # per-kernel event timing needs the eager launch path.
 DEFAULTS = {"schedule": 0, "sparse_mode": 0, "sparse_batch": 2, "sparse_waves_per_cu": 8, "sparse_chunk": 4, "dense_bits": 2048, "m4r_bits": 16384,
-            "mfma_bits": 32768, "iters_per_sync": 4, "fuse_apply": 0}
+            "mfma_bits": 32768, "iters_per_sync": 4, "fuse_apply": 0, "split_cands": 64}
 
 
 def workload_key(n: int, grammar_text: str) -> str:
@@ -106,9 +108,19 @@ def _store(key, name, best, best_ms, trace, log):
     log(f"[tune] best {best} -> {best_ms:.3f} ms (cached as {key})")
 
 
-def tune(pkg, cf, query, repeats: int = 3, log=print, passes: int = 1) -> dict:
+def _space_and_start(dims, cached: dict, base: dict):
+    """With `dims` (parameter names), walk only those, starting from the
+    cached choice completed by `base`; otherwise the whole space from `base`."""
+    if not dims:
+        return None, None
+    space = [(n, v) for n, v in SPACE + [("use_graph", [0, 1])] if n in dims]
+    return space, dict(base, **cached)
+
+
+def tune(pkg, cf, query, repeats: int = 3, log=print, passes: int = 1, dims=None) -> dict:
     """Tune on `query` (a gen.Query); returns the best option dict and stores it.
-    `passes` > 1 walks the parameters again from the best point found."""
+    `passes` > 1 walks the parameters again from the best point found; `dims`
+    restricts the walk to those parameters, from the cached choice."""
     eng = pkg.Engine(0)
     gr = cf.Grammar(query.grammar, text=True)
     mats = gr.load(eng, cf.Graph.from_query(query))
@@ -136,7 +148,8 @@ def tune(pkg, cf, query, repeats: int = 3, log=print, passes: int = 1) -> dict:
             raise AssertionError(f"tuning candidate {opts} changed the result")
         return statistics.median(times)
 
-    best, best_ms, trace = _descend(run, log, passes)
+    space, start = _space_and_start(dims, tuned_opts(query.n, query.grammar), DEFAULTS)
+    best, best_ms, trace = _descend(run, log, passes, space=space, start=start)
     for c in init:
         c.free()
     gr.free()
@@ -145,7 +158,7 @@ def tune(pkg, cf, query, repeats: int = 3, log=print, passes: int = 1) -> dict:
     return best
 
 
-def tune_batch(pkg, cf, queries, repeats: int = 1, log=print, passes: int = 1) -> dict:
+def tune_batch(pkg, cf, queries, repeats: int = 1, log=print, passes: int = 1, dims=None) -> dict:
     """Tune the batched fixpoint of `queries` (gen.Query list); cached under batch_key."""
     eng = pkg.Engine(0)
     grs, inits = [], []
@@ -178,8 +191,9 @@ def tune_batch(pkg, cf, queries, repeats: int = 1, log=print, passes: int = 1) -
             raise AssertionError(f"tuning candidate {opts} changed a result")
         return statistics.median(times)
 
-    best, best_ms, trace = _descend(run, log, passes, space=SPACE + [("use_graph", [0, 1])],
-                                    start=dict(DEFAULTS, use_graph=0))
+    space, start = _space_and_start(dims, tuned_batch_opts(queries), dict(DEFAULTS, use_graph=0))
+    best, best_ms, trace = _descend(run, log, passes, space=space or SPACE + [("use_graph", [0, 1])],
+                                    start=start or dict(DEFAULTS, use_graph=0))
     for gr, cp in zip(grs, inits):
         for c in cp:
             c.free()

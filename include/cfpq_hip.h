@@ -110,6 +110,10 @@ typedef struct cfpq_opts {
     uint32_t order_threshold;   /* sched_order: an item goes to the first phase when its
                                    predicted activity probability is at least
                                    order_threshold / 65536 (0 = library default, 1/2)    */
+    uint32_t split_cands;       /* lane-parallel sparse product: a 64-row quarter with more
+                                   candidate k-tiles than this is split into sub-units of
+                                   this many candidates each, merged afterwards
+                                   (0 = library default, 64; 0xffffffff = never)         */
 } cfpq_opts;
 
 /* cfpq_opts.schedule */
