@@ -118,7 +118,15 @@ def product_roofline(stats, wname):
                 "alg_ops_per_launch": round(ops / launches), "executed_ops_per_launch": round(executed / launches)}
     if t_sparse >= t_dense:
         kern, t_k = "k_product_sparse", t_sparse
-        alg_bytes = sum(32 * (s["a_rows"] + s["b_rows"] + s["out_rows"]) + 8 * s["a_cols"] for s in stats)
+        # what the sparse kernel's algorithm reads and writes: per candidate
+        # tile-product the filter's metadata (B tile row mask 32 B, A quarter
+        # column union 32 B and row mask 8 B, B tile count 4 B); per gathered
+        # (hit, column) item the quarter's column word (8 B) and the B row
+        # (32 B), per A row of the row-major gathers 32 B; per output row the
+        # head row read and the new row written (2 x 32 B)
+        tp_sparse = sum(s["tile_products"] - s["tile_products_dense"] - s["tile_products_mfma"] for s in stats)
+        alg_bytes = 76 * tp_sparse + sum(32 * (s["a_rows"] + s["b_rows"]) + 8 * s["a_cols"] + 64 * s["out_rows"]
+                                         for s in stats)
     else:
         kern, t_k = "k_product_dense", t_dense
         alg_bytes = sum(32 * (s["a_rows_dense"] + s["b_rows_dense"] + s["out_rows_dense"]) +

@@ -281,6 +281,7 @@ static void normalise(cfpq_opts *o) {
     if (o->m4r_keys != 8) o->m4r_keys = 4;
     if (o->order_threshold == 0) o->order_threshold = 32768;  // P(active) >= 1/2
     if (o->split_cands == 0) o->split_cands = SPLIT_CANDS;
+    if (o->split_rows != 1) o->split_rows = 0;
 }
 
 extern "C" int cfpq_create(int device, const cfpq_opts *opts, cfpq_ctx **out) {
@@ -790,13 +791,14 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                                (uint32_t)ctx->tr_cap);
         hipLaunchKernelGGL(k_build_sched, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems, sa, split,
-                           split_budget, heavy, heavy_base, tile_units, ctx->splits, prio_list);
+                           split_budget, heavy, heavy_base, tile_units, ctx->splits, prio_list,
+                           ctx->opts.split_rows);
     } else {
         hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, (int)full_pass, c, ctx->units, ctx->items,
                            ctx->mitems, split, split_budget, heavy, heavy_base, h0,
                            (const StepCounters *)(prev_slot >= 0 && !full_pass ? ctx->ring + prev_slot : nullptr),
-                           tile_units, ctx->splits);
+                           tile_units, ctx->splits, ctx->opts.split_rows);
     }
     const uint32_t pgrid = (uint32_t)ctx->num_cu * 4;
     // product kernels, timed by consecutive events of slot i: sparse
@@ -827,8 +829,9 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                                    phase, (const uint64_t *)prio_list);
         }
         if (ev) HIPCHK(hipEventRecord(ev[1], ctx->stream));
-        // the split quarters, from their sub-units' partial rows
-        if (split)
+        // the split quarters, from their sub-units' partial rows (row-split
+        // sub-units write their rows themselves)
+        if (split && !ctx->opts.split_rows)
             // (a grid of 4 waves per CU measured best: larger grids cost more
             // in dispatch than the merge work they spread out)
             hipLaunchKernelGGL(k_merge_split, dim3((uint32_t)ctx->num_cu * 4), dim3(WAVE), 0, ctx->stream, dm, dh, q, c,
@@ -1010,7 +1013,7 @@ static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &p
             plan.ids_all_off, plan.total_head_tiles, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
             (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk, (uint64_t)o.sched, o.sched_threshold,
             (uint64_t)o.sparse_mode, (uint64_t)o.fuse_apply, (uint64_t)o.m4r_keys, (uint64_t)o.sched_order, (uint64_t)o.order_threshold,
-            (uint64_t)o.split_cands,
+            (uint64_t)o.split_cands, (uint64_t)o.split_rows,
             (uint64_t)ctx->sched_have, (uint64_t)(uintptr_t)ctx->sched_w, (uint64_t)(uintptr_t)ctx->dirty,
             (uint64_t)(uintptr_t)ctx->dirty_off, (uint64_t)ctx->tr_cap, (uint64_t)(uintptr_t)ctx->tr_x};
 }

@@ -291,7 +291,7 @@ struct QuarterOut {
 };
 __device__ __forceinline__ QuarterOut quarter_epilogue(const MatDev &O, const MatDev &M, uint32_t p, uint32_t q,
                                                        uint32_t I, uint32_t J, const Row4 &acc, int fuse,
-                                                       uint32_t &out_rows) {
+                                                       uint32_t &out_rows, bool shared = false) {
     const int lane = threadIdx.x & 63;
     const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
     Row4 nw = {{0, 0, 0, 0}};
@@ -316,8 +316,17 @@ __device__ __forceinline__ QuarterOut quarter_epilogue(const MatDev &O, const Ma
     if (bal) {
         const Row4 cu = ru.cu;
         if (lane == 0) {
-            gst(&O.rmask[(size_t)p * 4 + q], bal);
-            store_row(colq_of(O, p, q), cu);
+            if (shared) {
+                // other sub-units write other rows of this quarter: OR, not store
+                g_or(&O.rmask[(size_t)p * 4 + q], bal);
+                uint64_t *oc = colq_of(O, p, q);
+#pragma unroll
+                for (int w = 0; w < 4; ++w)
+                    if (cu.w[w]) g_or(oc + w, cu.w[w]);
+            } else {
+                gst(&O.rmask[(size_t)p * 4 + q], bal);
+                store_row(colq_of(O, p, q), cu);
+            }
             set_occq(O, q, I, J);
             if (fuse) {
                 const uint64_t old = gld(&M.rmask[(size_t)p * 4 + q]);
@@ -422,7 +431,7 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
                                              uint32_t split, uint32_t split_budget, StepCounters *__restrict__ c,
                                              uint64_t *__restrict__ units, uint32_t heavy_base, int tile_units,
                                              uint64_t *__restrict__ splits, bool prio = false,
-                                             uint64_t *__restrict__ prio_list = nullptr) {
+                                             uint64_t *__restrict__ prio_list = nullptr, int split_rows = 0) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
     if (prio_list) {
@@ -486,12 +495,14 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
 #pragma unroll
     for (uint32_t q = 0; q < QUARTERS; ++q) {
         if (nsub[q] > 1) {
-            const uint64_t u = base | (uint64_t)q | ((uint64_t)(nsub[q] - 1) << 59);
+            const uint64_t u = base | (uint64_t)q | ((uint64_t)(nsub[q] - 1) << 59) | (split_rows ? ITEM_ROWSPLIT : 0ull);
             const uint32_t at = atomicAdd(&c->n_heavy, nsub[q]);
             for (uint32_t sidx = 0; sidx < nsub[q]; ++sidx) units[heavy_base + at + sidx] = u | ((uint64_t)sidx << 55);
-            const uint32_t sq = atomicAdd(&c->n_split, 1u);
-            splits[2 * (size_t)sq] = u;
-            splits[2 * (size_t)sq + 1] = at;
+            if (!split_rows) {  // candidate split: the quarter is finished by k_merge_split
+                const uint32_t sq = atomicAdd(&c->n_split, 1u);
+                splits[2 * (size_t)sq] = u;
+                splits[2 * (size_t)sq + 1] = at;
+            }
         }
     }
 }
@@ -505,7 +516,8 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
                                                  uint64_t *__restrict__ mfma_items, uint32_t split,
                                                  uint32_t split_budget, int heavy, uint32_t heavy_base,
                                                  uint32_t h0, const StepCounters *__restrict__ prevc,
-                                                 int tile_units, uint64_t *__restrict__ splits) {
+                                                 int tile_units, uint64_t *__restrict__ splits,
+                                                 int split_rows) {
     // a delta step after one that added no bit has an empty delta: no work
     // (the steps the host enqueues past convergence end here)
     if (prevc) {
@@ -533,7 +545,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
     append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units,
-                 heavy_base, tile_units, splits);
+                 heavy_base, tile_units, splits, false, nullptr, split_rows);
 }
 
 // ---------------------------------------------------------------------------
@@ -680,6 +692,13 @@ __device__ __forceinline__ int select_bit(uint64_t m, uint32_t j) {
 // (hit, shared column) items a lane of the lane-parallel sparse kernel takes
 // per round in its column-major gathers: their loads are in flight together.
 constexpr int COL_ITEMS = 4;
+
+// Rows [64 s / n, 64 (s + 1) / n) of a quarter, as a mask (n <= 64).
+__device__ __forceinline__ uint64_t rows_of_part(uint32_t s, uint32_t n) {
+    const uint32_t a = 64u * s / n, b = 64u * (s + 1u) / n;
+    const uint64_t hi = b >= 64u ? ~0ull : (1ull << b) - 1ull;
+    return hi & ~((1ull << a) - 1ull);
+}
 
 // A lane ORs its items into the rows their 64-bit row masks name, one LDS
 // atomic per row and word.  Lanes walk their masks starting at bit (lane), not
@@ -857,7 +876,12 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     // of 64) claims at a time from its own shard's cursor, then from the
     // other shards' in turn once its own is used up.
     const uint32_t n_claim = (n_units + 63u) & ~63u;
-    if ((uint64_t)blockIdx.x * chunk >= n_claim) return;  // more waves than claims
+    // claims of `chunk` units, halved while there are not enough of them for
+    // every wave (the steps near convergence have a few hundred units: one
+    // wave per unit instead of a few waves walking chunks of them)
+    uint32_t ck = chunk;
+    while (ck > 1u && (uint64_t)gridDim.x * ck > n_claim) ck >>= 1;
+    if ((uint64_t)blockIdx.x * ck >= n_claim) return;  // more waves than claims
     const uint32_t nblk = n_claim / 64u;
     const uint32_t home = blockIdx.x % SHARDS;
     CounterShard *shards = c->shard;
@@ -874,8 +898,49 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     // a claim past the end of a shard only says the shard is used up).
     auto claim = [&](uint32_t s) -> uint32_t {
         uint32_t k = 0xffffffffu;
-        if (lane == 0) k = atomicAdd(phase == 1 ? &shards[s].prio_next : &shards[s].unit_next, chunk);
+        if (lane == 0) k = atomicAdd(phase == 1 ? &shards[s].prio_next : &shards[s].unit_next, ck);
         return k;
+    };
+    // Productive units, queued by lane 0: the head tile's count and occupancy
+    // words and its touched-list entry.  A flush adds the counts for up to
+    // PEND units at once (lane i, unit i); the lanes whose tile had no bit yet
+    // set its occupancy and append it to the touched list with one atomic.
+    // Nothing in this launch reads these (the next step's build and the
+    // retire launch do), so deferring them only takes two dependent atomic
+    // round trips per unit off the wave's path.
+    constexpr uint32_t PEND = 32;
+    __shared__ uint32_t *s_pcnt[PEND];
+    __shared__ uint32_t *s_poR[PEND];
+    __shared__ uint32_t *s_poC[PEND];
+    __shared__ uint32_t s_pbit[PEND], s_ppc[PEND];
+    __shared__ uint64_t s_pent[PEND];
+    uint32_t npend = 0;
+    auto flush_pending = [&]() {
+        if (!npend) return;
+        __syncthreads();
+        bool first = false;
+        uint64_t ent = 0;
+        if ((uint32_t)lane < npend) {
+            if (g_add(s_pcnt[lane], s_ppc[lane]) == 0) {
+                const uint32_t b = s_pbit[lane];
+                g_or(s_poR[lane], 1u << (b & 31u));
+                g_or(s_poC[lane], 1u << (b >> 5));
+                first = true;
+                ent = s_pent[lane];
+            }
+        }
+        const uint64_t bal = __ballot(first);
+        if (bal) {
+            const int leader = __ffsll((unsigned long long)bal) - 1;
+            uint32_t at = 0;
+            if (lane == leader) at = atomicAdd(&shards[home].n_touched, (uint32_t)__popcll(bal));
+            at = (uint32_t)__shfl((int)at, leader, 64);
+            if (first)
+                gst(&tl.list[(size_t)(1 + home) * tl.seg_cap + at + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))],
+                    ent);
+        }
+        npend = 0;
+        __syncthreads();
     };
     uint32_t pending = claim(sh);
     while (used_up < SHARDS) {
@@ -889,25 +954,48 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     }
     pending = claim(sh);
     const uint32_t u0 = blk * 64u + kc % 64u;
-    const uint32_t u1 = (u0 + chunk < n_claim) ? u0 + chunk : n_claim;
+    const uint32_t u1 = (u0 + ck < n_claim) ? u0 + ck : n_claim;
+    // The chunk's list entries and their heads' fields, lane j for unit
+    // u0 + j, loaded at once: a unit starts from registers instead of two
+    // dependent round trips of its own.
+    uint64_t c_it = 0;
+    uint32_t c_T = 0, c_t0 = 0, c_t1 = 0, c_out = 0, c_tgt = 0;
+    {
+        const uint32_t x = u0 + (uint32_t)lane;
+        const uint32_t u = (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
+        if (x < u1 && u < n_units) {
+            c_it = gld(phase == 1 ? prio_list + u : units + (u < n_heavy ? heavy_base + u : u - n_heavy));
+            const HeadDev *hd = heads + item_head(c_it);
+            const bool fl = (c_it & ITEM_FULL) != 0;
+            c_T = gld(&hd->T);
+            c_t0 = fl ? gld(&hd->f0) : gld(&hd->t0);
+            c_t1 = fl ? gld(&hd->f1) : gld(&hd->t1);
+            c_out = gld(&hd->out[parity]);
+            c_tgt = gld(&hd->target);
+        }
+    }
     for (uint32_t x = u0; x < u1; ++x) {
         const uint32_t u = (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
         if (u >= n_units) continue;
         const uint64_t tu0 = __builtin_amdgcn_s_memtime();
-        const uint64_t it = phase == 1 ? ld_const(prio_list + u) : ld_const(units + (u < n_heavy ? heavy_base + u : u - n_heavy));
-        const uint32_t h = uniform(item_head(it));
-        const uint32_t p = uniform(unit_tile(it));
+        const uint32_t cl = x - u0;  // the lane holding this unit's entry
+        const uint64_t it = lane_read64(c_it, cl);
+        const uint32_t h = item_head(it);
+        const uint32_t p = unit_tile(it);
         // the unit's quarters: one, or every quarter of the tile with candidates
-        const uint32_t qset = uniform(unit_quarters(it));
+        const uint32_t qset = unit_quarters(it);
         const bool full = (it & ITEM_FULL) != 0;
-        const uint32_t sub = uniform(unit_sub(it)), nsub = uniform(unit_nsub(it));
-        const uint32_t T = ld_const(&heads[h].T);
-        const uint32_t t0 = full ? ld_const(&heads[h].f0) : ld_const(&heads[h].t0);
-        const uint32_t t1 = full ? ld_const(&heads[h].f1) : ld_const(&heads[h].t1);
+        const uint32_t sub = unit_sub(it), nsub = unit_nsub(it);
+        // a row-split sub-unit: its rows of the quarter
+        const bool rsplit = nsub > 1 && (it & ITEM_ROWSPLIT) != 0;
+        const uint64_t rowsel = rsplit ? rows_of_part(sub, nsub) : ~0ull;
+        const uint32_t T = lane_read(c_T, cl);
+        const uint32_t t0 = lane_read(c_t0, cl);
+        const uint32_t t1 = lane_read(c_t1, cl);
         const uint32_t I = p / T, J = p - (p / T) * T;
         // the output descriptors are read now, while the unit's first loads are in flight
-        const MatDev O = mats[uniform(heads[h].out[parity])];
-        const MatDev M = mats[uniform(heads[h].target)];
+        const MatDev O = mats[lane_read(c_out, cl)];
+        const MatDev M = mats[lane_read(c_tgt, cl)];
         const uint32_t tp_unit0 = tp, br_unit0 = brows;
         uint32_t unit_pairs = 0;  // (hit, shared column) items of the unit (its shape, for the step log)
         // a split quarter's sub-unit `sub` of `nsub` takes the candidates
@@ -954,9 +1042,11 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                 // 2. filter up to 64 candidates: lane g takes the g-th candidate
                 //    (of a sub-unit: the candidates of its residue class)
                 const uint32_t g = cb + lane;
-                const bool mine = (uint32_t)lane < nc && (nsub == 1 || g % nsub == sub);
-                tp += (uint32_t)__popcll(__ballot(mine));
-                uint64_t arm = 0;
+                const bool mine = (uint32_t)lane < nc && (nsub == 1 || rsplit || g % nsub == sub);
+                // (the sub-units of a row-split quarter share its candidates: counted once)
+                if (!rsplit || sub == 0) tp += (uint32_t)__popcll(__ballot(mine));
+                uint64_t arm = 0;  // (a hit on a full B tile: the A quarter's non-empty rows)
+                bool hit = false;
                 Row4 brm = {{0, 0, 0, 0}};
                 const uint64_t *abase = nullptr, *bbase = nullptr;
                 uint32_t cq = 0;  // the candidate's quarter
@@ -988,23 +1078,30 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     const uint32_t K = (jj % ow) * 32 + (uint32_t)bit;
                     const LaneOps o = s_ops[lo];
                     const size_t lt = (size_t)I * T + K, rt = (size_t)K * T + J;
+                    // two loads per candidate: the B tile's row mask and the A
+                    // quarter's column union (the filter's L2 requests are
+                    // most of the kernel's)
                     brm = load_row(o.rrmask + rt * 4);
                     const Row4 cu = load_row(o.lcolq + (lt * QUARTERS + q) * TROW_WORDS);
-                    arm = gld(&o.lrmask[lt * 4 + q]);
-                    const bool hit = ((cu.w[0] & brm.w[0]) | (cu.w[1] & brm.w[1]) | (cu.w[2] & brm.w[2]) |
-                                      (cu.w[3] & brm.w[3])) != 0;
-                    if (!hit) arm = 0;
+                    hit = ((cu.w[0] & brm.w[0]) | (cu.w[1] & brm.w[1]) | (cu.w[2] & brm.w[2]) |
+                           (cu.w[3] & brm.w[3])) != 0;
+                    // a B tile can only be full when all of its 256 rows are non-empty
+                    const bool all_rows = (brm.w[0] & brm.w[1] & brm.w[2] & brm.w[3]) == ~0ull;
                     // column-major gathers: the columns the A quarter and the B
                     // tile's non-empty rows share, and the quarter's column words
                     brm = Row4{{cu.w[0] & brm.w[0], cu.w[1] & brm.w[1], cu.w[2] & brm.w[2], cu.w[3] & brm.w[3]}};
                     abase = o.lcolT + (lt * QUARTERS + q) * TILE;
                     bbase = o.rbits + rt * TILE_WORDS;
-                    // a full B tile needs no gathers: bit 0 of the (8-byte aligned) pointer marks it
-                    if (gld(&o.rcnt[rt]) == TILE_BITS) bbase = (const uint64_t *)((uintptr_t)bbase | 1u);
+                    // a full B tile needs no gathers (every row of the quarter
+                    // becomes all ones): bit 0 of the (8-byte aligned) pointer
+                    // marks it, and the quarter's row mask names the rows
+                    if (hit && all_rows && gld(&o.rcnt[rt]) == TILE_BITS) {
+                        bbase = (const uint64_t *)((uintptr_t)bbase | 1u);
+                        arm = gld(&o.lrmask[lt * 4 + q]);
+                    }
                 }
                 // 3. stage the hits in lane order and spread their (candidate,
                 //    A row) pairs over the lanes
-                const bool hit = arm != 0;
                 const uint64_t hmask = __ballot(hit);
                 if (!hmask) continue;
                 const uint32_t nh = __popcll(hmask);
@@ -1023,7 +1120,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     s_ab[slot] = abase;
                     s_bb[slot] = bbase;
                 }
-                hits_n += hit;
+                if (!rsplit || sub == 0) hits_n += hit;
                 acols += ((uintptr_t)bbase & 1u) ? 0u : cnt;
                 __syncthreads();
                 const uint64_t t_pairs = __builtin_amdgcn_s_memtime();
@@ -1055,7 +1152,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                 r0[j] = QSET ? s_hq[lo] * WAVE : 0u;
                                 const uint64_t *Bt = s_bb[lo];
                                 if ((uintptr_t)Bt & 1u) {  // full B tile: every row of the quarter becomes all ones
-                                    for (uint64_t rr = rot_rows(s_arm[lo], lane); rr; rr &= rr - 1) {
+                                    for (uint64_t rr = rot_rows(s_arm[lo] & rowsel, lane); rr; rr &= rr - 1) {
                                         const uint32_t r = r0[j] + rot_row(rr, lane);
 #pragma unroll
                                         for (int i = 0; i < 4; ++i) atomicOr((unsigned long long *)&acc_s[r].w[i], ~0ull);  // (all words: no swizzle)
@@ -1063,7 +1160,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                 } else {
                                     const Row4 rel = s_brm[lo];
                                     const uint32_t k = select_bit256(rel, gp - (s_end[lo] - row_popc(rel)));
-                                    rows[j] = gld(s_ab[lo] + k);
+                                    rows[j] = gld(s_ab[lo] + k) & rowsel;
                                     b[j] = load_row(Bt + k * TROW_WORDS);
                                     ++brows;
                                 }
@@ -1089,7 +1186,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         const uint64_t t_epi = __builtin_amdgcn_s_memtime();
         max_cands = max(max_cands, tp - tp_unit0);
         uint32_t pc = 0;  // new bits of the unit's quarters
-        if (nsub > 1) {
+        if (nsub > 1 && !rsplit) {
             // a sub-unit of a split quarter: its rows go to its partial-row
             // slot (its index u in the sub-unit list), non-empty ones only,
             // with their mask; k_merge_split finishes the quarter
@@ -1104,17 +1201,22 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             for (uint32_t qq = 0; qq < QUARTERS; ++qq) {
                 if (!((qset >> qq) & 1u)) continue;
                 const Row4 acc = acc_unswizzle(acc_s[(QSET ? qq * WAVE : 0u) + lane], (uint32_t)lane);
-                pc += quarter_epilogue(O, M, p, qq, I, J, acc, fuse, out_rows).pc;
+                pc += quarter_epilogue(O, M, p, qq, I, J, acc, fuse, out_rows, rsplit).pc;
             }
         }
-        if (pc && lane == 0) {
-            const uint32_t old = g_add(&O.cnt[p], pc);
-            if (old == 0) {
-                set_occ(O, I, J);
-                const uint32_t at = atomicAdd(&shards[home].n_touched, 1u);
-                gst(&tl.list[(size_t)(1 + home) * tl.seg_cap + at], ((uint64_t)h << 40) | p);
+        if (pc) {
+            // the tile's count, occupancy and touched-list entry are queued
+            // and written for up to PEND units at once (flush_pending)
+            if (lane == 0) {
+                s_pcnt[npend] = &O.cnt[p];
+                s_poR[npend] = &O.occR[I * O.ow + (J >> 5)];
+                s_poC[npend] = &O.occC[J * O.ow + (I >> 5)];
+                s_pbit[npend] = (J & 31u) | ((I & 31u) << 5);
+                s_ppc[npend] = pc;
+                s_pent[npend] = ((uint64_t)h << 40) | p;
+                new_bits += pc;
             }
-            new_bits += pc;
+            if (++npend == PEND) flush_pending();
         }
         const uint64_t t_end = __builtin_amdgcn_s_memtime();
         const uint64_t tu = t_end - tu0;
@@ -1139,6 +1241,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         __syncthreads();  // the accumulator is reset by the next unit
     }
     }
+    flush_pending();
     const uint32_t ar = wave_sum(arows), br = wave_sum(brows), orow = wave_sum(out_rows), hs = wave_sum(hits_n);
     const uint32_t ac = wave_sum(acols);
     if (lane == 0) {

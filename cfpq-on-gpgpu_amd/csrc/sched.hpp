@@ -258,7 +258,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                                                        uint64_t *__restrict__ mfma_items, SchedArgs sa, uint32_t split,
                                                        uint32_t split_budget, int heavy, uint32_t heavy_base,
                                                        int tile_units, uint64_t *__restrict__ splits,
-                                                       uint64_t *__restrict__ prio_list) {
+                                                       uint64_t *__restrict__ prio_list, int split_rows) {
     __shared__ float sw[MLP_PARAMS];
     if (sa.w) mlp_load(sa.w, sw);
     const uint32_t h = blockIdx.y;
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
     append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units,
-                 heavy_base, tile_units, splits, prio, sa.order ? prio_list : nullptr);
+                 heavy_base, tile_units, splits, prio, sa.order ? prio_list : nullptr, split_rows);
     deferred = wave_sum(deferred);
     fulls = wave_sum(fulls);
     scored = wave_sum(scored);

@@ -238,6 +238,11 @@ constexpr uint64_t ITEM_TILE_MASK = 0xffffffffffull;
 constexpr uint64_t ITEM_QSET = 1ull << 38;
 constexpr int ITEM_QSET_SHIFT = 34;
 constexpr uint64_t ITEM_FULL = 1ull << 63;
+// A sub-unit of a quarter split by rows: sub-unit s of n takes every
+// candidate of the quarter but only its rows [64 s / n, 64 (s + 1) / n), and
+// finishes them itself (no merge pass); without the flag a split quarter's
+// sub-units take the candidates g = s (mod n) and leave partial rows.
+constexpr uint64_t ITEM_ROWSPLIT = 1ull << 39;
 constexpr uint32_t MAX_HEADS = 1u << 15;   // heads per plan (15-bit field)
 constexpr uint32_t SPLIT_MAX = 16;         // sub-units per heavy sparse unit
 constexpr uint32_t SPLIT_CANDS = 64;       // default of cfpq_opts.split_cands: a unit with more

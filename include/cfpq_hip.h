@@ -114,6 +114,10 @@ typedef struct cfpq_opts {
                                    candidate k-tiles than this is split into sub-units of
                                    this many candidates each, merged afterwards
                                    (0 = library default, 64; 0xffffffff = never)         */
+    int32_t split_rows;         /* how a split quarter is shared out: 0 = by candidates,
+                                   the sub-units' partial rows OR-ed by a merge pass
+                                   (default); 1 = by rows, each sub-unit evaluating every
+                                   candidate for its own rows and writing them itself    */
 } cfpq_opts;
 
 /* cfpq_opts.schedule */

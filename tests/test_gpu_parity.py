@@ -330,6 +330,9 @@ def test_fixpoint_dense_stress_32768_golden(pkg, gen, variant):
     dict(sparse_waves_per_cu=1, sparse_chunk=16, use_graph=True),
     # tile units: one unit per tile for its quarters with candidates (heavy quarters still split)
     dict(dense_bits=0xFFFFFFF0, mfma_bits=0xFFFFFFF0, sparse_mode=2, sparse_waves_per_cu=8, sparse_chunk=4),
+    # heavy quarters shared out by rows (no merge pass), and by candidates from 16 on
+    dict(dense_bits=0xFFFFFFF0, mfma_bits=0xFFFFFFF0, split_rows=True, split_cands=16),
+    dict(dense_bits=0xFFFFFFF0, mfma_bits=0xFFFFFFF0, split_rows=False, split_cands=16),
 ])
 def test_fixpoint_sg_forest_65536_options(pkg, gen, opts):
     """Full-size ontology workload under other launch shapes: fingerprints
@@ -519,6 +522,23 @@ def test_fixpoint_tile_units(pkg, gen, oracle, which):
                                          ("auto", True, "rules", 3)):
         st = _fixpoint_vs_oracle(pkg, gen, oracle, q, kernel=kernel, fuse_apply=fuse, schedule=schedule,
                                  sparse_mode=mode, dense_bits=0xFFFFFFF0 if kernel == "auto" else 0,
+                                 mfma_bits=0xFFFFFFF0)
+        assert st["iterations"] >= 2
+
+
+@pytest.mark.parametrize("which", ["dense", "dyck", "sg"])
+@pytest.mark.parametrize("split_rows", [False, True])
+def test_fixpoint_split_quarters(pkg, gen, oracle, which, split_rows):
+    """Quarters with more than split_cands candidates are shared out over
+    sub-units, by candidates (partial rows, merge pass) or by rows (each
+    sub-unit writes its own rows and ORs the quarter's metadata): with
+    thresholds of 1 and 2 nearly every quarter is split, and the fixpoint
+    must still equal the oracle's word for word, for quarter and tile units."""
+    q = {"dense": lambda: gen.dense_stress(1100), "dyck": lambda: gen.dyck_er(1200, 4000, seed=9),
+         "sg": lambda: gen.sg_forest(6000, seed=10, types_per_vertex=4)}[which]()
+    for cands, mode, schedule in ((1, 0, "delta"), (2, 2, "delta"), (1, 0, "rules")):
+        st = _fixpoint_vs_oracle(pkg, gen, oracle, q, kernel="auto", split_rows=split_rows, split_cands=cands,
+                                 sparse_mode=mode, schedule=schedule, dense_bits=0xFFFFFFF0,
                                  mfma_bits=0xFFFFFFF0)
         assert st["iterations"] >= 2
 
