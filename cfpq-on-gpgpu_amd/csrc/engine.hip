@@ -776,7 +776,9 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     // work-unit shape of the lane-parallel kernel: 2 = whole tiles, 3 = halves
     const int tile_units = ctx->opts.sparse_mode == 2 ? 1 : ctx->opts.sparse_mode == 3 ? 2 : 0;
     // candidate count above which a quarter is split (0: no splitting)
-    const uint32_t split = (lane_parallel && !(fuse && ctx->opts.fuse_apply)) ? ctx->opts.split_cands : 0u;
+    // (0xffffffff: never split, so no merge launch either)
+    const uint32_t split = (lane_parallel && !(fuse && ctx->opts.fuse_apply) && ctx->opts.split_cands != 0xffffffffu)
+                               ? ctx->opts.split_cands : 0u;
     const uint32_t split_budget = split_slots_of(ctx, plan);
     const uint32_t heavy_base = (uint32_t)(QUARTERS * plan.total_head_tiles);
     const bool use_sched =
@@ -849,13 +851,14 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     }
     if (ev && !lane_parallel) HIPCHK(hipEventRecord(ev[1], ctx->stream));
     if (ev) HIPCHK(hipEventRecord(ev[2], ctx->stream));
-    // (the dense and MFMA launches, and their events, only with a heavy-tile path on)
-    if (heavy)
+    // (the dense and MFMA launches, and their events, only with a heavy-tile
+    // path on; each kernel only when its own crossover can route a tile to it)
+    if (heavy && dense_threshold(ctx->opts) <= TILE_BITS)
         hipLaunchKernelGGL(k_product_dense, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
                            dense_threshold(ctx->opts), m4r_threshold(ctx->opts), fz, ctx->opts.m4r_keys == 8 ? 1 : 0);
     if (ev && heavy) HIPCHK(hipEventRecord(ev[3], ctx->stream));
-    if (heavy)
+    if (heavy && mfma_threshold(ctx->opts) <= TILE_BITS)
         hipLaunchKernelGGL(k_product_mfma, dim3((uint32_t)ctx->num_cu), dim3(MFMA_BLOCK), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, terms, q, c,
                            (const uint64_t *)ctx->mitems, touched, fz);
