@@ -72,7 +72,7 @@ struct cfpq_ctx {
     size_t lists_cap = 0;
     // split quarters of the sparse product: partial rows (64 rows per slot),
     // their masks, and the list of split quarters (2 words each); at most
-    // split_slots sub-units per step
+    // split_slots sub-units per step (2 KiB of partial rows each: 64 MiB)
     uint64_t *partials = nullptr;
     size_t partials_cap = 0;
     uint64_t *pmask = nullptr;
@@ -779,6 +779,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     // (0xffffffff: never split, so no merge launch either)
     const uint32_t split = (lane_parallel && !(fuse && ctx->opts.fuse_apply) && ctx->opts.split_cands != 0xffffffffu)
                                ? ctx->opts.split_cands : 0u;
+    const uint32_t piece = ctx->opts.split_piece ? ctx->opts.split_piece : std::max<uint32_t>(split, 1u);
     const uint32_t split_budget = split_slots_of(ctx, plan);
     const uint32_t heavy_base = (uint32_t)(QUARTERS * plan.total_head_tiles);
     const bool use_sched =
@@ -793,12 +794,12 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
                                (uint32_t)ctx->tr_cap);
         hipLaunchKernelGGL(k_build_sched, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, ctx->units, ctx->items, ctx->mitems, sa, split,
-                           split_budget, heavy, heavy_base, tile_units, ctx->splits, prio_list,
+                           piece, split_budget, heavy, heavy_base, tile_units, ctx->splits, prio_list,
                            ctx->opts.split_rows);
     } else {
         hipLaunchKernelGGL(k_build, dim3(bx, nh), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, (int)full_pass, c, ctx->units, ctx->items,
-                           ctx->mitems, split, split_budget, heavy, heavy_base, h0,
+                           ctx->mitems, split, piece, split_budget, heavy, heavy_base, h0,
                            (const StepCounters *)(prev_slot >= 0 && !full_pass ? ctx->ring + prev_slot : nullptr),
                            tile_units, ctx->splits, ctx->opts.split_rows);
     }
@@ -1016,7 +1017,7 @@ static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &p
             plan.ids_all_off, plan.total_head_tiles, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
             (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk, (uint64_t)o.sched, o.sched_threshold,
             (uint64_t)o.sparse_mode, (uint64_t)o.fuse_apply, (uint64_t)o.m4r_keys, (uint64_t)o.sched_order, (uint64_t)o.order_threshold,
-            (uint64_t)o.split_cands, (uint64_t)o.split_rows,
+            (uint64_t)o.split_cands, (uint64_t)o.split_rows, (uint64_t)o.split_piece,
             (uint64_t)ctx->sched_have, (uint64_t)(uintptr_t)ctx->sched_w, (uint64_t)(uintptr_t)ctx->dirty,
             (uint64_t)(uintptr_t)ctx->dirty_off, (uint64_t)ctx->tr_cap, (uint64_t)(uintptr_t)ctx->tr_x};
 }

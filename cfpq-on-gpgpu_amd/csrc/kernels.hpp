@@ -416,7 +416,8 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
 }
 
 // Append the sparse units of one (head, C tile): quarters in qmask, a heavy
-// quarter (more than `split` candidates; split = 0: none) as up to SPLIT_MAX sub-units
+// quarter (more than `split` candidates; split = 0: none) as up to SPLIT_MAX
+// sub-units of about `piece` candidates each
 // while the split budget lasts.  Sub-unit s of a split quarter takes the
 // candidates g = s (mod nsub) of the quarter and leaves its rows in a slot
 // of the partial-row buffer (slot = its index in the sub-unit list); the
@@ -428,7 +429,8 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
 // at units + heavy_base (count n_heavy), which the sparse kernel claims
 // first: the longest units start early instead of ending the launch late.
 __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const uint32_t *qcnt, uint64_t base,
-                                             uint32_t split, uint32_t split_budget, StepCounters *__restrict__ c,
+                                             uint32_t split, uint32_t piece, uint32_t split_budget,
+                                             StepCounters *__restrict__ c,
                                              uint64_t *__restrict__ units, uint32_t heavy_base, int tile_units,
                                              uint64_t *__restrict__ splits, bool prio = false,
                                              uint64_t *__restrict__ prio_list = nullptr, int split_rows = 0) {
@@ -472,7 +474,7 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
         const bool wq = ((rest >> q) & 1u) != 0;
         nsub[q] = wq ? 1u : 0u;
         if (wq && split && qcnt[q] > split) {
-            uint32_t ns = (uint32_t)(((uint64_t)qcnt[q] + split - 1) / split);
+            uint32_t ns = (uint32_t)(((uint64_t)qcnt[q] + piece - 1) / piece);
             ns = ns > SPLIT_MAX ? SPLIT_MAX : (ns < 1 ? 1 : ns);
             // the budget counts sub-units (partial-row slots); a refused
             // reservation only leaves budget unused
@@ -513,7 +515,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
                                                  const TermDev *__restrict__ terms, int parity, int full_pass,
                                                  StepCounters *__restrict__ c, uint64_t *__restrict__ units,
                                                  uint64_t *__restrict__ dense_items,
-                                                 uint64_t *__restrict__ mfma_items, uint32_t split,
+                                                 uint64_t *__restrict__ mfma_items, uint32_t split, uint32_t piece,
                                                  uint32_t split_budget, int heavy, uint32_t heavy_base,
                                                  uint32_t h0, const StepCounters *__restrict__ prevc,
                                                  int tile_units, uint64_t *__restrict__ splits,
@@ -544,8 +546,8 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
     wave_append(mfma, flag | ((uint64_t)h << 40) | p, &c->n_mfma, mfma_items);
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
-    append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units,
-                 heavy_base, tile_units, splits, false, nullptr, split_rows);
+    append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, piece, split_budget, c,
+                 units, heavy_base, tile_units, splits, false, nullptr, split_rows);
 }
 
 // ---------------------------------------------------------------------------
@@ -700,6 +702,17 @@ __device__ __forceinline__ uint64_t rows_of_part(uint32_t s, uint32_t n) {
     return hi & ~((1ull << a) - 1ull);
 }
 
+// The LDS row accumulator of the lane-parallel sparse kernel: row x at word
+// ACC_STRIDE * x.
+constexpr uint32_t ACC_STRIDE = 5;
+__device__ __forceinline__ void acc_put(uint32_t x, uint64_t *acc, const Row4 &v) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[x * ACC_STRIDE + i] = v.w[i];
+}
+__device__ __forceinline__ Row4 acc_get(uint32_t x, const uint64_t *acc) {
+    return Row4{{acc[x * ACC_STRIDE], acc[x * ACC_STRIDE + 1], acc[x * ACC_STRIDE + 2], acc[x * ACC_STRIDE + 3]}};
+}
+
 // A lane ORs its items into the rows their 64-bit row masks name, one LDS
 // atomic per row and word.  Lanes walk their masks starting at bit (lane), not
 // at bit 0: where many lanes hold dense masks (a hub column shared by most
@@ -715,19 +728,6 @@ __device__ __forceinline__ uint32_t rot_row(uint64_t rr, int lane) {
     return ((uint32_t)__builtin_ctzll(rr) + (uint32_t)lane) & 63u;
 }
 
-// The row accumulator in LDS keeps word i of row r in slot (i + r / 8) mod 4.
-// Rows are 32 bytes apart, so without the rotation the 64 lanes of one
-// atomic instruction, on 64 consecutive rows, fall on 8 bank groups (8-way
-// conflicts); rotating by r / 8 spreads them over 32.
-__device__ __forceinline__ uint32_t acc_word(uint32_t r, int i) { return ((uint32_t)i + (r >> 3)) & 3u; }
-// A row read back from the accumulator (row r mod 64 == lane) in word order.
-__device__ __forceinline__ Row4 acc_unswizzle(const Row4 &v, uint32_t r) {
-    const uint32_t k = (r >> 3) & 3u;
-    Row4 o = v;
-    if (k & 1u) o = Row4{{o.w[1], o.w[2], o.w[3], o.w[0]}};
-    if (k & 2u) o = Row4{{o.w[2], o.w[3], o.w[0], o.w[1]}};
-    return o;
-}
 
 // Column of the j-th (0-based) set bit of a 256-bit row mask (j < popcount).
 __device__ __forceinline__ uint32_t select_bit256(const Row4 &m, uint32_t j) {
@@ -850,7 +850,10 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                                             uint64_t *__restrict__ pmask, int phase,
                                                             const uint64_t *__restrict__ prio_list) {
     constexpr uint32_t ACC_ROWS = QSET ? TILE : WAVE;
-    __shared__ Row4 acc_s[ACC_ROWS];  // rows of the unit's quarters (row 64 q + r with QSET, else r)
+    // rows of the unit's quarters (row 64 q + r with QSET, else r), ACC_STRIDE
+    // words apart: 40-byte rows put the 64 lanes of one atomic, on 64
+    // different rows, on 32 bank pairs (32-byte rows: on 8)
+    __shared__ uint64_t acc_s[ACC_ROWS * ACC_STRIDE];
     __shared__ Row4 s_brm[WAVE];
     __shared__ uint64_t s_arm[WAVE];
     // with QSET: the quarter of each staged hit, and the candidate masks per
@@ -1003,7 +1006,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         // several waves; its rows go to a partial-row slot (k_merge_split)
 #pragma unroll
         for (uint32_t qq = 0; qq < QUARTERS; ++qq)
-            if ((qset >> qq) & 1u) acc_s[(QSET ? qq * WAVE : 0u) + lane] = Row4{{0, 0, 0, 0}};
+            if ((qset >> qq) & 1u) acc_put((QSET ? qq * WAVE : 0u) + lane, acc_s, Row4{{0, 0, 0, 0}});
         __syncthreads();
         const uint32_t ow = (T + 31) / 32;
         const uint32_t ntw = (t1 - t0) * ow;  // (term, occupancy word) pairs of the unit
@@ -1155,7 +1158,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                     for (uint64_t rr = rot_rows(s_arm[lo] & rowsel, lane); rr; rr &= rr - 1) {
                                         const uint32_t r = r0[j] + rot_row(rr, lane);
 #pragma unroll
-                                        for (int i = 0; i < 4; ++i) atomicOr((unsigned long long *)&acc_s[r].w[i], ~0ull);  // (all words: no swizzle)
+                                        for (int i = 0; i < 4; ++i) atomicOr((unsigned long long *)&acc_s[r * ACC_STRIDE + i], ~0ull);
                                     }
                                 } else {
                                     const Row4 rel = s_brm[lo];
@@ -1168,11 +1171,14 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                         }
 #pragma unroll
                         for (int j = 0; j < COL_ITEMS; ++j) {
+                            // one atomic per row and word, zero words included: a
+                            // per-lane test of each word costs more (exec-mask
+                            // branches around every atomic) than the OR of a zero
                             for (uint64_t rr = rot_rows(rows[j], lane); rr; rr &= rr - 1) {
                                 const uint32_t r = r0[j] + rot_row(rr, lane);
 #pragma unroll
                                 for (int i = 0; i < 4; ++i)
-                                    if (b[j].w[i]) atomicOr((unsigned long long *)&acc_s[r].w[acc_word(r, i)], (unsigned long long)b[j].w[i]);
+                                    atomicOr((unsigned long long *)&acc_s[r * ACC_STRIDE + i], (unsigned long long)b[j].w[i]);
                             }
                         }
                     }
@@ -1191,7 +1197,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             // slot (its index u in the sub-unit list), non-empty ones only,
             // with their mask; k_merge_split finishes the quarter
             const uint32_t qq = (uint32_t)__builtin_ctz(qset);
-            const Row4 acc = acc_unswizzle(acc_s[(QSET ? qq * WAVE : 0u) + lane], (uint32_t)lane);
+            const Row4 acc = acc_get((QSET ? qq * WAVE : 0u) + lane, acc_s);
             const bool nz = row_nonzero(acc);
             if (nz) store_row(partials + ((size_t)u * WAVE + lane) * TROW_WORDS, acc);
             const uint64_t bal = __ballot(nz);
@@ -1200,7 +1206,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
 #pragma unroll
             for (uint32_t qq = 0; qq < QUARTERS; ++qq) {
                 if (!((qset >> qq) & 1u)) continue;
-                const Row4 acc = acc_unswizzle(acc_s[(QSET ? qq * WAVE : 0u) + lane], (uint32_t)lane);
+                const Row4 acc = acc_get((QSET ? qq * WAVE : 0u) + lane, acc_s);
                 pc += quarter_epilogue(O, M, p, qq, I, J, acc, fuse, out_rows, rsplit).pc;
             }
         }

@@ -255,7 +255,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
                                                        const TermDev *__restrict__ terms, int parity,
                                                        StepCounters *__restrict__ c, uint64_t *__restrict__ units,
                                                        uint64_t *__restrict__ dense_items,
-                                                       uint64_t *__restrict__ mfma_items, SchedArgs sa, uint32_t split,
+                                                       uint64_t *__restrict__ mfma_items, SchedArgs sa, uint32_t split, uint32_t piece,
                                                        uint32_t split_budget, int heavy, uint32_t heavy_base,
                                                        int tile_units, uint64_t *__restrict__ splits,
                                                        uint64_t *__restrict__ prio_list, int split_rows) {
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_sched(const MatDev *__restrict_
     wave_append(mfma, flag | ((uint64_t)h << 40) | p, &c->n_mfma, mfma_items);
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
-    append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, split_budget, c, units,
+    append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, piece, split_budget, c, units,
                  heavy_base, tile_units, splits, prio, sa.order ? prio_list : nullptr, split_rows);
     deferred = wave_sum(deferred);
     fulls = wave_sum(fulls);

@@ -26,8 +26,9 @@
 namespace {
 
 constexpr uint64_t kHashMod = 1000000007ull;  // cfpq-cpp/Grammar.cpp:13
-// Device bytes the bit-matrix engine may plan for: a matrix and its delta
-// buffers per nonterminal take about 3 n^2 / 8 bytes (of 288 GB of HBM).
+// Device bytes the bit-matrix engine may plan for: a matrix and its two delta
+// buffers per nonterminal take about 6 n^2 / 8 bytes, the bits and their
+// column-major copy (of 288 GB of HBM).
 constexpr double kBitsBudget = 200.0 * (1ull << 30);
 
 uint64_t pow_mod(uint64_t b, uint64_t e) {
@@ -115,7 +116,7 @@ int main(int argc, char *argv[]) {
         Graph graph(argv[2]);
         const double n = graph.vertices_count;
         const bool pairs = std::strcmp(mode, "--pairs") == 0 ||
-                           (!*mode && 3.0 * grammar.get_nonterminals_count() * n * n / 8 > kBitsBudget);
+                           (!*mode && 6.0 * grammar.get_nonterminals_count() * n * n / 8 > kBitsBudget);
         if (pairs) {
             auto times = run_pairs(grammar, graph, argv[3]);
             std::cout << times.first << ' ' << times.second << std::endl;

@@ -18,6 +18,7 @@ hipEvents, median of a few runs) while walking the parameters one at a time
   fuse_apply           merge new rows into the head matrices inside the products
   split_cands          candidate k-tiles above which a quarter is split into sub-units
   split_rows           split quarters shared out by candidates (merge pass) or by rows
+  split_piece          candidates per sub-unit of a split quarter (0: split_cands)
 
 Every candidate is checked against the first run's fingerprints, so a tuning
 run is also a parity run.  Results are cached in ``tuned.json`` next to this
@@ -48,12 +49,13 @@ SPACE = [
     ("fuse_apply", [0, 1]),
     ("split_cands", [32, 64, 128, 0xFFFFFFFF]),
     ("split_rows", [0, 1]),
+    ("split_piece", [0, 16, 32, 64]),
 ]
 # Graph replay (use_graph) is a dimension for batches (tune_batch), where the
 # steps are short and launch-bound; for single queries the caller decides, as
 # per-kernel event timing needs the eager launch path.
 DEFAULTS = {"schedule": 0, "sparse_mode": 0, "sparse_batch": 2, "sparse_waves_per_cu": 8, "sparse_chunk": 4, "dense_bits": 2048, "m4r_bits": 16384,
-            "mfma_bits": 32768, "iters_per_sync": 4, "fuse_apply": 0, "split_cands": 64, "split_rows": 0}
+            "mfma_bits": 32768, "iters_per_sync": 4, "fuse_apply": 0, "split_cands": 64, "split_rows": 0, "split_piece": 0}
 
 
 def workload_key(n: int, grammar_text: str) -> str:

@@ -118,6 +118,8 @@ typedef struct cfpq_opts {
                                    the sub-units' partial rows OR-ed by a merge pass
                                    (default); 1 = by rows, each sub-unit evaluating every
                                    candidate for its own rows and writing them itself    */
+    uint32_t split_piece;       /* candidates per sub-unit of a split quarter, at most 16
+                                   sub-units (0 = split_cands)                           */
 } cfpq_opts;
 
 /* cfpq_opts.schedule */
