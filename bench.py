@@ -397,6 +397,10 @@ def main():
         return wall * 1e3 / args.steps, stats, ok
 
     runs = {"plain": measure("plain", opts)}
+    # the same eager engine without the per-kernel timing events (the plain
+    # leg's events give the roofline's kernel times; each costs a few
+    # microseconds of gap between launches)
+    runs["untimed"] = measure("untimed", dict(opts, no_kernel_timing=True))
     if not args.no_graph_leg and not opts.get("use_graph"):
         # the same engine with every step replayed from captured hipGraphs
         # (fewer launch gaps; per-kernel timing needs the eager plain leg)
@@ -424,6 +428,7 @@ def main():
             pairs_run = str(e)
     total_window = None if args.no_total else measure_total(args, eng, gr, mats, init, rules, opts, want)
     mfma_leg = measure_mfma(eng, gr, mats, init, rules, opts, want, step) if (args.config == 2 and not args.no_mfma) else None
+    sched_name = {1: "head-sweep fixpoint", 2: "rule-sweep fixpoint"}.get(opts.get("schedule"), "delta fixpoint")
     ok_runs = {k: v for k, v in runs.items() if v[2] and k != "worklist"}
     chosen = min(ok_runs, key=lambda k: ok_runs[k][0]) if ok_runs else "plain"
     ms_per_step, stats, bit_exact = runs[chosen]
@@ -490,11 +495,9 @@ def main():
         "data": "synthetic (seeded generator, same inputs as the CPU oracle)",
         "config": {"workload": wname, "n": n, "edges": q.n_edges, "nonterminals": gr.nonterminals_count,
                    "rules": len(rules), "A_ref": a_ref, "A_ref_kind": work_key,
-                   "engine": {"plain": {1: "head-sweep fixpoint", 2: "rule-sweep fixpoint"}.get(opts.get("schedule"),
-                                                                                              "delta fixpoint"),
-                              "graph": {1: "head-sweep fixpoint", 2: "rule-sweep fixpoint"}.get(opts.get("schedule"),
-                                                                                              "delta fixpoint") +
-                                       ", steps replayed from captured hipGraphs",
+                   "engine": {"plain": sched_name,
+                              "untimed": sched_name + " without per-kernel timing events",
+                              "graph": sched_name + ", steps replayed from captured hipGraphs",
                               "learned": "delta fixpoint + learned tile scheduler"}[chosen],
                    "kernel": args.kernel, "parallelism": "single GPU"},
         "fixpoint_s": round(statistics.median(fix_ms) / 1e3, 6),

@@ -57,6 +57,7 @@ class Opts(ctypes.Structure):
         ("split_cands", ctypes.c_uint32),
         ("split_rows", ctypes.c_int32),
         ("split_piece", ctypes.c_uint32),
+        ("no_kernel_timing", ctypes.c_int32),
     ]
 
 
@@ -200,7 +201,7 @@ def make_opts(kernel="auto", dense_bits=0, mfma_bits=0, m4r_bits=0, max_iteratio
               iters_per_sync=0, use_graph=False, sparse_batch=0, sparse_waves_per_cu=0,
               sparse_chunk=0, sched=False, sched_threshold=0, sparse_mode=0, fuse_apply=False,
               schedule=0, m4r_keys=0, sched_order=False, order_threshold=0, split_cands=0,
-              split_rows=False, split_piece=0) -> Opts:
+              split_rows=False, split_piece=0, no_kernel_timing=False) -> Opts:
     o = Opts()
     lib().cfpq_default_opts(ctypes.byref(o))
     o.kernel = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
@@ -223,6 +224,7 @@ def make_opts(kernel="auto", dense_bits=0, mfma_bits=0, m4r_bits=0, max_iteratio
     o.split_cands = split_cands
     o.split_rows = int(bool(split_rows))
     o.split_piece = split_piece
+    o.no_kernel_timing = int(bool(no_kernel_timing))
     o.sched = int(bool(sched))
     # a float in (0, 1) is a probability; an int is already in units of 1/65536
     o.sched_threshold = int(round(sched_threshold * 65536)) if isinstance(sched_threshold, float) else sched_threshold

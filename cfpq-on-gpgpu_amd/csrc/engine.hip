@@ -282,6 +282,7 @@ static void normalise(cfpq_opts *o) {
     if (o->order_threshold == 0) o->order_threshold = 32768;  // P(active) >= 1/2
     if (o->split_cands == 0) o->split_cands = SPLIT_CANDS;
     if (o->split_rows != 1) o->split_rows = 0;
+    if (o->no_kernel_timing != 1) o->no_kernel_timing = 0;
 }
 
 extern "C" int cfpq_create(int device, const cfpq_opts *opts, cfpq_ctx **out) {
@@ -1017,7 +1018,7 @@ static std::vector<uint64_t> launch_signature(const cfpq_ctx *ctx, const Plan &p
             plan.ids_all_off, plan.total_head_tiles, (uint64_t)o.kernel, o.dense_bits, o.mfma_bits, o.m4r_bits, (uint64_t)o.sparse_batch,
             (uint64_t)o.sparse_waves_per_cu, (uint64_t)o.sparse_chunk, (uint64_t)o.sched, o.sched_threshold,
             (uint64_t)o.sparse_mode, (uint64_t)o.fuse_apply, (uint64_t)o.m4r_keys, (uint64_t)o.sched_order, (uint64_t)o.order_threshold,
-            (uint64_t)o.split_cands, (uint64_t)o.split_rows, (uint64_t)o.split_piece,
+            (uint64_t)o.split_cands, (uint64_t)o.split_rows, (uint64_t)o.split_piece, (uint64_t)o.no_kernel_timing,
             (uint64_t)ctx->sched_have, (uint64_t)(uintptr_t)ctx->sched_w, (uint64_t)(uintptr_t)ctx->dirty,
             (uint64_t)(uintptr_t)ctx->dirty_off, (uint64_t)ctx->tr_cap, (uint64_t)(uintptr_t)ctx->tr_x};
 }
@@ -1088,7 +1089,8 @@ static int run_sweeps(cfpq_ctx *ctx, const Plan &plan, bool heavy_paths, bool ce
         for (size_t h = 0; h <= nh; ++h) {
             if (h < nh && visit[h]) {
                 const uint32_t slot = steps % RING;
-                CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, true, (int)slots.size(), false, true,
+                CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, true,
+                                 ctx->opts.no_kernel_timing ? -1 : (int)slots.size(), false, true,
                                  prev_slot < 0, (int)h));
                 ++launches;
                 slots.push_back(slot);
@@ -1106,7 +1108,7 @@ static int run_sweeps(cfpq_ctx *ctx, const Plan &plan, bool heavy_paths, bool ce
                 StepCounters &c = ctx->ring_host[slots[s]];
                 fold_shards(c);
                 float step_ms[4] = {0, 0, 0, 0};
-                for (int k2 = 0; k2 < (heavy_paths ? 4 : 2); ++k2) {
+                for (int k2 = 0; k2 < (heavy_paths ? 4 : 2) && !ctx->opts.no_kernel_timing; ++k2) {
                     HIPCHK(hipEventElapsedTime(&step_ms[k2], ctx->ev[EV_SLOT * s + k2], ctx->ev[EV_SLOT * s + k2 + 1]));
                     part_ms[k2] += step_ms[k2];
                     product_ms += step_ms[k2];
@@ -1191,7 +1193,8 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             // (with d = M the two semi-naive terms would repeat the same product)
             const bool full = kind == 1 || steps == 0;
             if (use_graph) CHK(launch_step_graph(ctx, plan, slot, prev_slot, full, kind == 2));
-            else CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, full, s, kind == 2, true,
+            else CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, full,
+                                  ctx->opts.no_kernel_timing ? -1 : s, kind == 2, true,
                                   prev_slot < 0 || (uint32_t)(prev_slot + 1) % RING != slot));
             ++launches;
             slots.push_back(slot);
@@ -1213,7 +1216,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             dirty_total += (int64_t)c.dirty_new - (int64_t)c.dirty_cleared;
             if (quiet) continue;
             float step_ms[4] = {0, 0, 0, 0};
-            for (int k2 = 0; k2 < (heavy_paths ? 4 : 2) && !use_graph; ++k2) {
+            for (int k2 = 0; k2 < (heavy_paths ? 4 : 2) && !use_graph && !ctx->opts.no_kernel_timing; ++k2) {
                 HIPCHK(hipEventElapsedTime(&step_ms[k2], ctx->ev[EV_SLOT * s + k2], ctx->ev[EV_SLOT * s + k2 + 1]));
                 part_ms[k2] += step_ms[k2];
                 product_ms += step_ms[k2];

@@ -138,7 +138,8 @@ def tune(pkg, cf, query, repeats: int = 3, log=print, passes: int = 1, dims=None
 
     def run(opts) -> float:
         nonlocal ref
-        eng.set_opts(**opts)
+        # timed as the engine runs without per-kernel timing events
+        eng.set_opts(**opts, no_kernel_timing=True)
         times = []
         for _ in range(repeats):
             for m, c in zip(mats, init):
@@ -181,7 +182,7 @@ def tune_batch(pkg, cf, queries, repeats: int = 1, log=print, passes: int = 1, d
 
     def run(opts) -> float:
         nonlocal ref
-        eng.set_opts(**opts)
+        eng.set_opts(**opts, no_kernel_timing=True)
         times = []
         for _ in range(repeats):
             for gr, cp in zip(grs, inits):

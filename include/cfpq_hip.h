@@ -120,6 +120,11 @@ typedef struct cfpq_opts {
                                    candidate for its own rows and writing them itself    */
     uint32_t split_piece;       /* candidates per sub-unit of a split quarter, at most 16
                                    sub-units (0 = split_cands)                           */
+    int32_t no_kernel_timing;   /* 1: no timing events around the product kernels of each
+                                   step (cfpq_stats sparse_ms, merge_ms, dense_ms, mfma_ms
+                                   and product_ms stay 0; fixpoint_ms is still measured):
+                                   each event costs a few microseconds of gap, which
+                                   matters for small graphs of many short steps          */
 } cfpq_opts;
 
 /* cfpq_opts.schedule */

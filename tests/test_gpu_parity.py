@@ -126,6 +126,15 @@ def test_fixpoint_dyck_er_1024(pkg, gen, oracle, kernel):
     assert stats["iterations"] >= 2
 
 
+def test_fixpoint_without_kernel_timing(pkg, gen, oracle):
+    """no_kernel_timing: the same fixpoint, with no per-kernel times but the
+    whole call still timed."""
+    for schedule in ("delta", "rules"):
+        stats = _fixpoint_vs_oracle(pkg, gen, oracle, gen.config(0), no_kernel_timing=True, schedule=schedule)
+        assert stats["sparse_ms"] == 0 and stats["dense_ms"] == 0 and stats["product_ms"] == 0
+        assert stats["fixpoint_ms"] > 0
+
+
 def test_fixpoint_sg_forest_16384(pkg, gen, oracle):
     _fixpoint_vs_oracle(pkg, gen, oracle, gen.config(1))
 
