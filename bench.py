@@ -506,6 +506,7 @@ def main():
         "iterations": s0["iterations"],
         "tile_products": s0["tile_products"],
         "work": {k: s0[k] for k in ("items", "tile_products", "tile_products_dense", "tile_products_mfma",
+                                    "tile_products_unchanged",
                                     "a_rows", "a_cols", "b_rows", "b_rows_lds", "m4r_lookups", "mfma_instr", "out_rows",
                                     "new_bits", "product_launches", "host_syncs", "cand_hits",
                                     "max_unit_cands", "a_rows_dense", "b_rows_dense", "out_rows_dense")},

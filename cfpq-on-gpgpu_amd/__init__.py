@@ -103,6 +103,7 @@ class Stats(ctypes.Structure):
         ("unit_clocks_idle", ctypes.c_uint64),
         ("b_rows_idle", ctypes.c_uint64),
         ("merge_ms", ctypes.c_double),
+        ("tile_products_unchanged", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

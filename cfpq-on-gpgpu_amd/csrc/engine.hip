@@ -111,6 +111,10 @@ struct cfpq_ctx {
     size_t dirty_cap = 0;
     uint32_t *dirty_off = nullptr;    // word offset of each head's bitmap
     size_t dirty_off_cap = 0;
+    // head and rule sweeps: per plan entry, the step (from 1) that last
+    // evaluated it, 0 before its first evaluation in the call
+    uint32_t *last_eval = nullptr;
+    size_t last_eval_cap = 0;
     // trace rows
     float *tr_x = nullptr, *tr_y = nullptr;
     uint32_t *tr_rule = nullptr, *tr_item = nullptr, *tr_count = nullptr;
@@ -131,7 +135,7 @@ static constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Bytes of a matrix with T tiles per side (the layout of mat_layout).
 static constexpr size_t mat_bytes(size_t T) {
-    return 2 * align256(T * T * TILE_WORDS * 8) + align256(T * T * 4) + align256(T * T * 4 * 8) +
+    return 2 * align256(T * T * TILE_WORDS * 8) + 2 * align256(T * T * 4) + align256(T * T * 4 * 8) +
            align256(T * T * QUARTERS * TROW_WORDS * 8) + align256(OCC_ARRAYS * T * ((T + 31) / 32) * 4);
 }
 
@@ -347,6 +351,7 @@ extern "C" int cfpq_destroy(cfpq_ctx *ctx) {
     void *dev[] = {ctx->stage, ctx->ring, ctx->scratch64, ctx->items, ctx->units, ctx->mitems, ctx->lists[0], ctx->lists[1],
                    ctx->partials, ctx->pmask, ctx->splits,
                    ctx->d_mats, ctx->d_heads, ctx->d_terms, ctx->d_tptr, ctx->d_ids, ctx->sched_w, ctx->dirty, ctx->dirty_off,
+                   ctx->last_eval,
                    ctx->tr_x, ctx->tr_y, ctx->tr_rule, ctx->tr_item, ctx->tr_count, ctx->t_master, ctx->t_m1,
                    ctx->t_m2, ctx->t_grad, ctx->t_loss, ctx->t_x, ctx->t_y, ctx->t_rule};
     for (void *p : dev)
@@ -389,6 +394,7 @@ static size_t mat_layout(uint32_t Tn, char *base, MatDev *d) {
     const size_t b_cq = align256(T * T * QUARTERS * TROW_WORDS * sizeof(uint64_t));
     const size_t b_occ = align256(OCC_ARRAYS * T * ow * sizeof(uint32_t));  // occR, occC, occQ, occD, occX
     const size_t b_ct = b_bits;  // the quarters in column-major form: as many words as the tiles
+    const size_t b_st = b_cnt;   // sweep stamps, one word per tile
     if (base && d) {
         d->bits = (uint64_t *)base;
         d->cnt = (uint32_t *)(base + b_bits);
@@ -400,10 +406,11 @@ static size_t mat_layout(uint32_t Tn, char *base, MatDev *d) {
         d->occQ = d->occC + T * ow;
         d->occD = d->occQ + QUARTERS * T * ow;
         d->occX = d->occD + T * ow;
+        d->stamp = (uint32_t *)(base + b_bits + b_cnt + b_rm + b_cq + b_occ + b_ct);
         d->T = Tn;
         d->ow = (uint32_t)ow;
     }
-    return b_bits + b_cnt + b_rm + b_cq + b_occ + b_ct;
+    return b_bits + b_cnt + b_rm + b_cq + b_occ + b_ct + b_st;
 }
 
 static int mat_create(cfpq_ctx *ctx, uint32_t n, cfpq_mat **out) {
@@ -674,6 +681,7 @@ struct StepStats {
     uint64_t new_bits = 0, items = 0, tp = 0, tpd = 0, tpm = 0, arows = 0, brows = 0, orows = 0, brows_lds = 0,
              lookups = 0, mfma_instr = 0, cand_hits = 0, arows_d = 0, brows_d = 0, orows_d = 0, acols = 0;
     uint64_t units_done = 0, units_idle = 0, unit_clk = 0, idle_clk = 0, idle_brows = 0;
+    uint64_t tpu = 0;
     uint32_t max_unit_cands = 0;
 };
 
@@ -746,8 +754,13 @@ static SchedArgs sched_args(const cfpq_ctx *ctx, bool flush) {
 // zero_counters: the step zeroes its own counter block (a run's first step,
 // or a slot the previous step's retire launch did not zero for it).
 // head >= 0: only that head's items are built (a head-sweep step).
+// Sweeps (head >= 0) pass stamp_step (the step number from 1): the retire
+// stamps the head tiles that gained bits with it and records it as the
+// entry's last evaluation; with `unchanged`, the dense kernel skips
+// tile-products whose operand tiles both carry older stamps than that.
 static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, int prev_slot, bool full_pass,
-                        int ev_idx, bool flush, bool fuse, bool zero_counters, int head = -1) {
+                        int ev_idx, bool flush, bool fuse, bool zero_counters, int head = -1,
+                        uint32_t stamp_step = 0, bool unchanged = false) {
     StepCounters *c = ctx->ring + slot;
     const uint32_t nh = head >= 0 ? 1u : (uint32_t)plan.heads.size();
     const uint32_t h0 = head >= 0 ? (uint32_t)head : 0u;
@@ -858,7 +871,8 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     if (heavy && dense_threshold(ctx->opts) <= TILE_BITS)
         hipLaunchKernelGGL(k_product_dense, dim3(pgrid), dim3(BLOCK), 0, ctx->stream, (const MatDev *)ctx->d_mats,
                            (const HeadDev *)ctx->d_heads, terms, q, c, (const uint64_t *)ctx->items, touched,
-                           dense_threshold(ctx->opts), m4r_threshold(ctx->opts), fz, ctx->opts.m4r_keys == 8 ? 1 : 0);
+                           dense_threshold(ctx->opts), m4r_threshold(ctx->opts), fz, ctx->opts.m4r_keys == 8 ? 1 : 0,
+                           (const uint32_t *)(unchanged && head >= 0 && stamp_step ? ctx->last_eval : nullptr));
     if (ev && heavy) HIPCHK(hipEventRecord(ev[3], ctx->stream));
     if (heavy && mfma_threshold(ctx->opts) <= TILE_BITS)
         hipLaunchKernelGGL(k_product_mfma, dim3((uint32_t)ctx->num_cu), dim3(MFMA_BLOCK), 0, ctx->stream,
@@ -885,7 +899,8 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     hipLaunchKernelGGL(k_retire, dim3(na + nc + nido * per + 1), dim3(WAVE), 0, ctx->stream,
                        (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, (const StepCounters *)c,
                        TouchedLists{touched, seg_cap}, prev, ptl, ids_q, nido, per, na, nc,
-                       ctx->ring + (slot + 1) % RING, fuse ? 0 : 1);
+                       ctx->ring + (slot + 1) % RING, fuse ? 0 : 1, head >= 0 ? stamp_step : 0u, head,
+                       head >= 0 && stamp_step ? ctx->last_eval : nullptr);
     if (prev_slot < 0 && !full_pass) {
         // a first step over delta terms: its delta buffers are cleared in full
         hipLaunchKernelGGL(k_clear_full, dim3(std::max<uint32_t>(1, ctx->num_cu * 4 / std::max<uint32_t>(1, nids_q)),
@@ -963,6 +978,8 @@ static int upload_plan(cfpq_ctx *ctx, Plan &plan) {
     HIPCHK(hipMemcpyAsync(ctx->dirty_off, plan.dirty_off.data(), plan.dirty_off.size() * 4, hipMemcpyHostToDevice,
                           ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->dirty, 0, plan.dirty_words * 4, ctx->stream));
+    CHK(grow(ctx, &ctx->last_eval, &ctx->last_eval_cap, plan.heads.size()));
+    HIPCHK(hipMemsetAsync(ctx->last_eval, 0, plan.heads.size() * 4, ctx->stream));
     // the tables above were copied from pageable host vectors that die here
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return CFPQ_OK;
@@ -980,6 +997,7 @@ static void accumulate(StepStats &s, const StepCounters &c) {
     s.tp += c.tile_products;
     s.tpd += c.tp_dense;
     s.tpm += c.tp_mfma;
+    s.tpu += c.tp_unchanged;
     s.arows += c.a_rows;
     s.acols += c.a_cols;
     s.units_done += c.units_done;
@@ -1083,15 +1101,23 @@ static int run_sweeps(cfpq_ctx *ctx, const Plan &plan, bool heavy_paths, bool ce
     std::vector<char> visit(nh, 1), changed(nh, 0);
     std::vector<uint32_t> slots, hs;
     bool certifying = false;
+    // stamps from earlier calls would only make tiles look changed
+    const uint32_t nall = (uint32_t)plan.mats.size();
+    hipLaunchKernelGGL(k_zero_stamps, dim3(std::max<uint32_t>(1, (uint32_t)((plan.max_tiles_per_head + 255) / 256)),
+                                           std::min<uint32_t>(std::max<uint32_t>(nall, 1), 65535)),
+                       dim3(256), 0, ctx->stream, (const MatDev *)ctx->d_mats, ctx->d_ids + plan.ids_all_off, nall);
+    CHK(launch_check(ctx, "k_zero_stamps"));
     for (;;) {
         std::fill(changed.begin(), changed.end(), 0);
         bool any = false;
         for (size_t h = 0; h <= nh; ++h) {
             if (h < nh && visit[h]) {
                 const uint32_t slot = steps % RING;
+                // steps are numbered from 1 for the stamps; the certifying
+                // sweep evaluates every tile-product
                 CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, true,
                                  ctx->opts.no_kernel_timing ? -1 : (int)slots.size(), false, true,
-                                 prev_slot < 0, (int)h));
+                                 prev_slot < 0, (int)h, steps + 1, !certifying));
                 ++launches;
                 slots.push_back(slot);
                 hs.push_back((uint32_t)h);
@@ -1284,7 +1310,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         hipLaunchKernelGGL(k_retire, dim3(nc + nids_q * per + 1), dim3(WAVE), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q,
                            (const StepCounters *)(ctx->ring + slot), ptl, (const StepCounters *)(ctx->ring + prev_slot),
-                           ptl, ids_q, nids_q, per, 0u, nc, ctx->ring + slot, 1);
+                           ptl, ids_q, nids_q, per, 0u, nc, ctx->ring + slot, 1, 0u, -1, (uint32_t *)nullptr);
         CHK(launch_check(ctx, "final clear"));
     }
     HIPCHK(hipEventRecord(ev_end, ctx->stream));
@@ -1299,6 +1325,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         stats->tile_products = tot.tp;
         stats->tile_products_dense = tot.tpd;
         stats->tile_products_mfma = tot.tpm;
+        stats->tile_products_unchanged = tot.tpu;
         stats->a_rows = tot.arows;
         stats->b_rows = tot.brows;
         stats->out_rows = tot.orows;
@@ -1553,19 +1580,19 @@ struct NoallocBuffers {
     MatDev *d_mats; HeadDev *d_heads; TermDev *d_terms; TermPtr *d_tptr; uint32_t *d_ids;
     uint64_t *items, *units, *mitems, *lists[2];
     StepCounters *ring;
-    uint32_t *dirty, *dirty_off;
+    uint32_t *dirty, *dirty_off, *last_eval;
     uint64_t *partials, *pmask, *splits;
     uint32_t split_slots;
-    size_t caps[14];
+    size_t caps[15];
     explicit NoallocBuffers(cfpq_ctx *c) : ctx(c) {
         d_mats = c->d_mats; d_heads = c->d_heads; d_terms = c->d_terms; d_tptr = c->d_tptr; d_ids = c->d_ids;
         items = c->items; units = c->units; mitems = c->mitems; lists[0] = c->lists[0]; lists[1] = c->lists[1];
-        ring = c->ring; dirty = c->dirty; dirty_off = c->dirty_off;
+        ring = c->ring; dirty = c->dirty; dirty_off = c->dirty_off; last_eval = c->last_eval;
         partials = c->partials; pmask = c->pmask; splits = c->splits; split_slots = c->split_slots;
-        const size_t cv[14] = {c->d_mats_cap, c->d_heads_cap, c->d_terms_cap, c->d_tptr_cap, c->d_ids_cap, c->items_cap,
+        const size_t cv[15] = {c->d_mats_cap, c->d_heads_cap, c->d_terms_cap, c->d_tptr_cap, c->d_ids_cap, c->items_cap,
                                c->units_cap, c->mitems_cap, c->lists_cap, c->dirty_cap, c->dirty_off_cap,
-                               c->partials_cap, c->pmask_cap, c->splits_cap};
-        std::copy(cv, cv + 14, caps);
+                               c->partials_cap, c->pmask_cap, c->splits_cap, c->last_eval_cap};
+        std::copy(cv, cv + 15, caps);
         char *base = c->noalloc_base;
         c->d_mats = (MatDev *)(base + NA_MATS);       c->d_mats_cap = NA_MATS_CAP;
         c->d_heads = (HeadDev *)(base + NA_HEADS);    c->d_heads_cap = NA_HEADS_CAP;
@@ -1580,6 +1607,7 @@ struct NoallocBuffers {
         c->ring = (StepCounters *)(base + NA_RING);
         c->dirty = (uint32_t *)(base + NA_DIRTY);     c->dirty_cap = NA_DIRTY_CAP;
         c->dirty_off = (uint32_t *)(base + NA_DIRTY_OFF); c->dirty_off_cap = 1;
+        c->last_eval = (uint32_t *)(base + NA_DIRTY_OFF + 128); c->last_eval_cap = 1;  // (in the same 256 B)
         c->partials = (uint64_t *)(base + NA_PARTIALS); c->partials_cap = (size_t)NA_SPLIT_SLOTS * WAVE * TROW_WORDS;
         c->pmask = (uint64_t *)(base + NA_PMASK);     c->pmask_cap = NA_SPLIT_SLOTS;
         c->splits = (uint64_t *)(base + NA_SPLITS);   c->splits_cap = 2 * (NA_SPLIT_SLOTS / 2 + 1);
@@ -1590,9 +1618,9 @@ struct NoallocBuffers {
         cfpq_ctx *c = ctx;
         c->d_mats = d_mats; c->d_heads = d_heads; c->d_terms = d_terms; c->d_tptr = d_tptr; c->d_ids = d_ids;
         c->items = items; c->units = units; c->mitems = mitems; c->lists[0] = lists[0]; c->lists[1] = lists[1];
-        c->ring = ring; c->dirty = dirty; c->dirty_off = dirty_off;
+        c->ring = ring; c->dirty = dirty; c->dirty_off = dirty_off; c->last_eval = last_eval;
         c->partials = partials; c->pmask = pmask; c->splits = splits; c->split_slots = split_slots;
-        c->partials_cap = caps[11]; c->pmask_cap = caps[12]; c->splits_cap = caps[13];
+        c->partials_cap = caps[11]; c->pmask_cap = caps[12]; c->splits_cap = caps[13]; c->last_eval_cap = caps[14];
         c->d_mats_cap = caps[0]; c->d_heads_cap = caps[1]; c->d_terms_cap = caps[2]; c->d_tptr_cap = caps[3];
         c->d_ids_cap = caps[4]; c->items_cap = caps[5]; c->units_cap = caps[6]; c->mitems_cap = caps[7];
         c->lists_cap = caps[8]; c->dirty_cap = caps[9]; c->dirty_off_cap = caps[10];

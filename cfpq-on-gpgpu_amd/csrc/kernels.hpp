@@ -1479,7 +1479,8 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
                                                          StepCounters *__restrict__ c,
                                                          const uint64_t *__restrict__ items,
                                                          uint64_t *__restrict__ touched, uint32_t dense_bits,
-                                                         uint32_t m4r_bits, int fuse, int m4r8) {
+                                                         uint32_t m4r_bits, int fuse, int m4r8,
+                                                         const uint32_t *__restrict__ last_eval) {
     __shared__ Row4 Bs[TILE];
     __shared__ Row4 Tab[M4R_GROUPS * 16];
     __shared__ uint32_t s_idx;
@@ -1489,7 +1490,7 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
     const int r = threadIdx.x, wv = r >> 6, lane = r & 63;
     const uint32_t n_items = c->n_dense;
     if (blockIdx.x >= n_items) return;  // surplus work-groups would only queue on the cursor
-    uint32_t tp = 0, tpd = 0, arows = 0, brows = 0, brows_lds = 0, out_rows = 0, lookups = 0;
+    uint32_t tp = 0, tpd = 0, tpu = 0, arows = 0, brows = 0, brows_lds = 0, out_rows = 0, lookups = 0;
     uint64_t new_bits = 0;  // thread 0: new bits of this work-group's items
     for (uint32_t idx = wg_next(&c->dense_next, &s_idx); idx < n_items; idx = wg_next(&c->dense_next, &s_idx)) {
         const uint64_t it = items[idx];
@@ -1499,6 +1500,10 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
         const uint32_t T = uniform(heads[h].T);
         const uint32_t t0 = uniform(full ? heads[h].f0 : heads[h].t0), t1 = uniform(full ? heads[h].f1 : heads[h].t1);
         const uint32_t I = p / T, J = p - (p / T) * T;
+        // sweeps: the step that last evaluated this entry (0: none in this
+        // call); a tile-product whose two operand tiles have not changed since
+        // then was evaluated then, with the same operands, and is skipped
+        const uint32_t last = last_eval ? uniform(ld_const(&last_eval[h])) : 0u;
         Row4 acc = {{0, 0, 0, 0}};
         uint64_t acc8[QUARTERS] = {0, 0, 0, 0};  // 8-bit Four Russians: slice wv of rows lane + 64 j
         bool used8 = false;
@@ -1514,10 +1519,17 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
                     const uint64_t *At = L.bits + (size_t)lt * TILE_WORDS;
                     const uint64_t *Bt = R.bits + (size_t)rt * TILE_WORDS;
                     const uint64_t rm = L.rmask[(size_t)lt * 4 + wv];
+                    // A-tile count, bit 31 = "B tile full", bit 30 = "both
+                    // operand tiles unchanged since the last evaluation" (counts are <= 2^16)
+                    const uint32_t ab = wg_bcast(L.cnt[lt] | (R.cnt[rt] == TILE_BITS ? 0x80000000u : 0u) |
+                                                     (last && L.stamp[lt] < last && R.stamp[rt] < last ? 0x40000000u : 0u),
+                                                 s_bc, nbc);
+                    if (ab & 0x40000000u) {
+                        ++tpu;
+                        continue;
+                    }
                     ++tp;
-                    // A-tile count, and bit 31 = "B tile full" (counts are <= 2^16)
-                    const uint32_t ab = wg_bcast(L.cnt[lt] | (R.cnt[rt] == TILE_BITS ? 0x80000000u : 0u), s_bc, nbc);
-                    const uint32_t acnt = ab & 0x7fffffffu;
+                    const uint32_t acnt = ab & 0x3fffffffu;
                     if (ab >> 31) {
                         // B all ones: every non-empty A row gives an all-ones row
                         if ((rm >> (r & 63)) & 1ull) acc = Row4{{~0ull, ~0ull, ~0ull, ~0ull}};
@@ -1583,6 +1595,7 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
         atomicAdd((unsigned long long *)&c->tile_products, (unsigned long long)tp);
         atomicAdd((unsigned long long *)&c->tp_dense, (unsigned long long)tpd);
     }
+    if (r == 0 && tpu) atomicAdd((unsigned long long *)&c->tp_unchanged, (unsigned long long)tpu);
     if (r == 0 && new_bits) atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)new_bits);
 }
 
@@ -1810,7 +1823,7 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
 // product kernels already merged rows and metadata into the head).
 __device__ __forceinline__ void apply_waves(const MatDev *__restrict__ mats, const HeadDev *__restrict__ heads,
                                             int parity, const StepCounters *__restrict__ c, TouchedLists tl,
-                                            uint32_t blk, uint32_t nblk, int merge) {
+                                            uint32_t blk, uint32_t nblk, int merge, uint32_t stamp_step) {
     const int lane = threadIdx.x & 63;
     const uint32_t n = touched_total(c) * QUARTERS;
     for (uint32_t idx = blk; idx < n; idx += nblk) {  // static: the units are small and alike
@@ -1826,6 +1839,8 @@ __device__ __forceinline__ void apply_waves(const MatDev *__restrict__ mats, con
             gst(&M.cnt[p], old + gld(&O.cnt[p]));
             if (!old) set_occ(M, I, J);
         }
+        // sweeps: the head tile gained bits in this step
+        if (stamp_step && lane == 0 && q == 0) gst(&M.stamp[p], stamp_step);
         const uint64_t rm = uniform64(gld(&O.rmask[(size_t)p * 4 + q]));
         if (!rm) continue;
         const bool keepT = uniform(heads[h].colT) != 0;
@@ -1890,7 +1905,7 @@ __device__ __forceinline__ void clear_waves(const MatDev *__restrict__ mats, con
 __global__ __launch_bounds__(WAVE) void k_apply(const MatDev *__restrict__ mats,
                                                 const HeadDev *__restrict__ heads, int parity,
                                                 StepCounters *__restrict__ c, TouchedLists tl) {
-    apply_waves(mats, heads, parity, c, tl, blockIdx.x, gridDim.x, 1);
+    apply_waves(mats, heads, parity, c, tl, blockIdx.x, gridDim.x, 1, 0u);
 }
 
 __global__ __launch_bounds__(WAVE) void k_clear(const MatDev *__restrict__ mats,
@@ -1912,6 +1927,15 @@ __device__ __forceinline__ void zero_occ(const MatDev &O, uint32_t x0, uint32_t 
     }
 }
 
+// Sweep stamps of matrices ids[0..nid) to 0 ("changed before this call").
+__global__ void k_zero_stamps(const MatDev *__restrict__ mats, const uint32_t *__restrict__ ids, uint32_t nid) {
+    for (uint32_t m = blockIdx.y; m < nid; m += gridDim.y) {
+        const MatDev M = mats[ids[m]];
+        const uint32_t tiles = M.T * M.T;
+        for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < tiles; p += gridDim.x * blockDim.x) M.stamp[p] = 0;
+    }
+}
+
 __global__ void k_clear_occ(const MatDev *__restrict__ mats, const uint32_t *__restrict__ ids, uint32_t nid) {
     for (uint32_t q = blockIdx.y; q < nid; q += gridDim.y)
         zero_occ(mats[ids[q]], blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
@@ -1922,6 +1946,8 @@ __global__ void k_clear_occ(const MatDev *__restrict__ mats, const uint32_t *__r
 //                              `parity`, list `tl`, counts in c) unless the
 //                              products merged them already (merge = 0), and
 //                              the column-major form of every new quarter;
+//                              sweeps stamp the head tiles with stamp_step and
+//                              record it as entry `entry`'s last evaluation;
 //   blocks [na, na + nc)       zero the rows and metadata of the delta the step
 //                              consumed (parity 1 - parity, the previous step's
 //                              list `ptl`, counts in prev);
@@ -1935,10 +1961,13 @@ __global__ __launch_bounds__(WAVE) void k_retire(const MatDev *__restrict__ mats
                                                  const StepCounters *__restrict__ prev, TouchedLists ptl,
                                                  const uint32_t *__restrict__ ids, uint32_t nid, uint32_t per,
                                                  uint32_t na, uint32_t nc, StepCounters *__restrict__ next,
-                                                 int merge) {
+                                                 int merge, uint32_t stamp_step, int entry,
+                                                 uint32_t *__restrict__ last_eval) {
     const uint32_t b = blockIdx.x;
+    // sweeps: this step evaluated plan entry `entry` (its products have run)
+    if (b == 0 && threadIdx.x == 0 && last_eval && entry >= 0) gst(&last_eval[entry], stamp_step);
     if (b < na) {
-        apply_waves(mats, heads, parity, c, tl, b, na, merge);
+        apply_waves(mats, heads, parity, c, tl, b, na, merge, stamp_step);
     } else if (b < na + nc) {
         clear_waves(mats, heads, 1 - parity, prev, ptl, b - na, nc);
     } else if (b < na + nc + nid * per) {

@@ -47,6 +47,11 @@ struct MatDev {
     uint32_t *occQ;
     uint32_t *occD;
     uint32_t *occX;
+    // per tile: the sweep step (counted from 1 within a fixpoint call) in
+    // which the tile last gained bits as a rule head; written and read by
+    // head and rule sweeps only (any value is safe for a tile that did not
+    // change during the call)
+    uint32_t *stamp;
     uint32_t T;
     uint32_t ow;
 };
@@ -145,6 +150,8 @@ struct StepCounters {
     uint64_t tile_products;   // (term, C tile, k) products evaluated (sparse: per quarter)
     uint64_t tp_dense;        // of which on the LDS / Four Russians paths
     uint64_t tp_mfma;         // of which on the MFMA path
+    uint64_t tp_unchanged;    // sweeps: tile-products skipped, both operand tiles unchanged since the
+                              // entry's last evaluation (dense kernel)
     uint64_t a_rows;          // non-empty A rows visited
     uint64_t a_cols;          // A-quarter column words read (column-major gathers, 8 B each)
     uint64_t unit_clk, idle_clk, idle_units, idle_brows, units_done;  // (CounterShard)

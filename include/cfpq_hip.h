@@ -187,6 +187,9 @@ typedef struct cfpq_stats {
     uint64_t b_rows_idle;       /* B rows gathered by the units that added no bit        */
     double merge_ms;            /* device time merging split quarters' partial rows
                                    (k_merge_split; part of product_ms)                   */
+    uint64_t tile_products_unchanged; /* head and rule sweeps: dense tile-products skipped
+                                   because neither operand tile changed since the rule
+                                   was last evaluated (not in tile_products)           */
 } cfpq_stats;
 
 /* ---- lifecycle ---------------------------------------------------------- */
