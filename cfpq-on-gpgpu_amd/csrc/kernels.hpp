@@ -1335,13 +1335,23 @@ __global__ __launch_bounds__(WAVE) void k_merge_split(const MatDev *__restrict__
 // ---------------------------------------------------------------------------
 
 // A tile with few bits: gather the B rows named by A's set bits from HBM/L2.
+// A row whose accumulated bits together with the head row's known bits
+// (`known`) are all ones can gain nothing more: its remaining lookups and
+// gathers are skipped (the new bits are acc & ~M).
+__device__ __forceinline__ bool row_saturated(const Row4 &acc, const Row4 &known) {
+    return ((acc.w[0] | known.w[0]) & (acc.w[1] | known.w[1]) & (acc.w[2] | known.w[2]) & (acc.w[3] | known.w[3])) ==
+           ~0ull;
+}
+
 __device__ __forceinline__ void tp_gather(const uint64_t *__restrict__ At, const uint64_t *__restrict__ Bt,
-                                          uint64_t rm, int r, Row4 &acc, uint32_t &arows, uint32_t &brows) {
-    if (!((rm >> (r & 63)) & 1ull)) return;
+                                          uint64_t rm, int r, Row4 &acc, const Row4 &known, uint32_t &arows,
+                                          uint32_t &brows) {
+    if (!((rm >> (r & 63)) & 1ull) || row_saturated(acc, known)) return;
     ++arows;
     const Row4 a = load_row(At + r * TROW_WORDS);
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
+        if (row_saturated(acc, known)) break;
         uint64_t x = a.w[w];
         while (x) {
             const int k = w * 64 + __builtin_ctzll(x);
@@ -1355,16 +1365,17 @@ __device__ __forceinline__ void tp_gather(const uint64_t *__restrict__ At, const
 // Heavier A tile: stage the B tile in LDS (8 KiB, one coalesced row per
 // thread), then OR rows from LDS.  Must be reached by the whole work-group.
 __device__ __forceinline__ void tp_lds(const uint64_t *__restrict__ At, const uint64_t *__restrict__ Bt,
-                                       uint64_t rm, int r, Row4 *Bs, Row4 &acc, uint32_t &arows,
+                                       uint64_t rm, int r, Row4 *Bs, Row4 &acc, const Row4 &known, uint32_t &arows,
                                        uint32_t &brows_lds) {
     __syncthreads();
     Bs[r] = load_row(Bt + r * TROW_WORDS);
     __syncthreads();
-    if (!((rm >> (r & 63)) & 1ull)) return;
+    if (!((rm >> (r & 63)) & 1ull) || row_saturated(acc, known)) return;
     ++arows;
     const Row4 a = load_row(At + r * TROW_WORDS);
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
+        if (row_saturated(acc, known)) break;
         uint64_t x = a.w[w];
         while (x) {
             const int k = w * 64 + __builtin_ctzll(x);
@@ -1382,8 +1393,8 @@ __device__ __forceinline__ void tp_lds(const uint64_t *__restrict__ At, const ui
 // Must be reached by the whole work-group.
 constexpr int M4R_GROUPS = TILE / 4;  // 64 tables of 16 entries
 __device__ __forceinline__ void tp_m4r(const uint64_t *__restrict__ At, const uint64_t *__restrict__ Bt,
-                                       uint64_t rm, int r, Row4 *Bs, Row4 *Tab, Row4 &acc, uint32_t &arows,
-                                       uint32_t &lookups) {
+                                       uint64_t rm, int r, Row4 *Bs, Row4 *Tab, Row4 &acc, const Row4 &known,
+                                       uint32_t &arows, uint32_t &lookups) {
     __syncthreads();
     Bs[r] = load_row(Bt + r * TROW_WORDS);
     __syncthreads();
@@ -1398,11 +1409,12 @@ __device__ __forceinline__ void tp_m4r(const uint64_t *__restrict__ At, const ui
         Tab[e] = v;
     }
     __syncthreads();
-    if (!((rm >> (r & 63)) & 1ull)) return;
+    if (!((rm >> (r & 63)) & 1ull) || row_saturated(acc, known)) return;
     ++arows;
     const Row4 a = load_row(At + r * TROW_WORDS);
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
+        if (row_saturated(acc, known)) break;
         uint64_t x = a.w[w];
         while (x) {
             const int nib = __builtin_ctzll(x) >> 2;   // lowest non-zero nibble
@@ -1504,6 +1516,10 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
         // call); a tile-product whose two operand tiles have not changed since
         // then was evaluated then, with the same operands, and is skipped
         const uint32_t last = last_eval ? uniform(ld_const(&last_eval[h])) : 0u;
+        const MatDev M = mats[uniform(heads[h].target)];
+        // the head's row r as it is now (a subset of it later, with the head
+        // update fused): the bits this item cannot add
+        const Row4 known = load_row(M.bits + (size_t)p * TILE_WORDS + r * TROW_WORDS);
         Row4 acc = {{0, 0, 0, 0}};
         uint64_t acc8[QUARTERS] = {0, 0, 0, 0};  // 8-bit Four Russians: slice wv of rows lane + 64 j
         bool used8 = false;
@@ -1549,12 +1565,12 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
                         tp_m4r8(At, Bt, L.rmask + (size_t)lt * 4, r, Bs, (uint64_t *)Tab, acc8, arows, lookups);
                     } else if (acnt >= m4r_bits) {
                         ++tpd;
-                        tp_m4r(At, Bt, rm, r, Bs, Tab, acc, arows, lookups);
+                        tp_m4r(At, Bt, rm, r, Bs, Tab, acc, known, arows, lookups);
                     } else if (acnt >= dense_bits) {
                         ++tpd;
-                        tp_lds(At, Bt, rm, r, Bs, acc, arows, brows_lds);
+                        tp_lds(At, Bt, rm, r, Bs, acc, known, arows, brows_lds);
                     } else {
-                        tp_gather(At, Bt, rm, r, acc, arows, brows);
+                        tp_gather(At, Bt, rm, r, acc, known, arows, brows);
                     }
                 }
             }
@@ -1568,7 +1584,6 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
             or_row(acc, Bs[r]);
         }
         const MatDev O = mats[uniform(heads[h].out[parity])];
-        const MatDev M = mats[uniform(heads[h].target)];
         const QuarterOut qo = quarter_epilogue(O, M, p, wv, I, J, acc, fuse, out_rows);
         if (lane == 0) s_sum[wv] = qo.pc;
         __syncthreads();
