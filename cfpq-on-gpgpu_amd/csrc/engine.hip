@@ -72,14 +72,14 @@ struct cfpq_ctx {
     size_t lists_cap = 0;
     // split quarters of the sparse product: partial rows (64 rows per slot),
     // their masks, and the list of split quarters (2 words each); at most
-    // split_slots sub-units per step (2 KiB of partial rows each: 64 MiB)
+    // split_slots sub-units per step (2 KiB of partial rows each: 256 MiB)
     uint64_t *partials = nullptr;
     size_t partials_cap = 0;
     uint64_t *pmask = nullptr;
     size_t pmask_cap = 0;
     uint64_t *splits = nullptr;
     size_t splits_cap = 0;
-    uint32_t split_slots = 32768;
+    uint32_t split_slots = 131072;
 
     // descriptor tables
     MatDev *d_mats = nullptr;
