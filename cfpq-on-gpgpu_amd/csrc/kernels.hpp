@@ -1412,16 +1412,21 @@ __device__ __forceinline__ void tp_m4r(const uint64_t *__restrict__ At, const ui
     if (!((rm >> (r & 63)) & 1ull) || row_saturated(acc, known)) return;
     ++arows;
     const Row4 a = load_row(At + r * TROW_WORDS);
+    // Nibble positions in the same order in every lane: at each step the
+    // lanes read from one 16-entry table (512 contiguous bytes, every bank
+    // once, identical keys broadcast) instead of from tables scattered over
+    // 32 KiB.  Zero nibbles are skipped per lane.
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
         if (row_saturated(acc, known)) break;
-        uint64_t x = a.w[w];
-        while (x) {
-            const int nib = __builtin_ctzll(x) >> 2;   // lowest non-zero nibble
+        const uint64_t x = a.w[w];
+#pragma unroll 4
+        for (int nib = 0; nib < 16; ++nib) {
             const int key = (int)((x >> (4 * nib)) & 15u);
-            x &= ~(15ull << (4 * nib));
-            or_row(acc, Tab[(w * 16 + nib) * 16 + key]);
-            ++lookups;
+            if (key) {
+                or_row(acc, Tab[(w * 16 + nib) * 16 + key]);
+                ++lookups;
+            }
         }
     }
 }
