@@ -1420,7 +1420,7 @@ __device__ __forceinline__ void tp_m4r(const uint64_t *__restrict__ At, const ui
     for (int w = 0; w < 4; ++w) {
         if (row_saturated(acc, known)) break;
         const uint64_t x = a.w[w];
-#pragma unroll 4
+#pragma unroll
         for (int nib = 0; nib < 16; ++nib) {
             const int key = (int)((x >> (4 * nib)) & 15u);
             if (key) {
