@@ -836,14 +836,12 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
         // phase 0 the rest; the launch boundary makes phase 1's head updates
         // (fused apply) visible to every unit of phase 0
         for (int phase = ordered ? 1 : 0; phase >= 0; --phase) {
-            if (tile_units)
-                hipLaunchKernelGGL(k_product_sparse_lp<true>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
-                                   heavy_base, TouchedLists{touched, seg_cap}, chunk, fz, ctx->partials, ctx->pmask,
-                                   phase, (const uint64_t *)prio_list);
-            else
-                hipLaunchKernelGGL(k_product_sparse_lp<false>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un,
-                                   heavy_base, TouchedLists{touched, seg_cap}, chunk, fz, ctx->partials, ctx->pmask,
-                                   phase, (const uint64_t *)prio_list);
+            const bool rs = ctx->opts.split_rows != 0;
+            auto *kern = tile_units ? (rs ? k_product_sparse_lp<true, true> : k_product_sparse_lp<true, false>)
+                                    : (rs ? k_product_sparse_lp<false, true> : k_product_sparse_lp<false, false>);
+            hipLaunchKernelGGL(kern, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, tp, q, c, un, heavy_base,
+                               TouchedLists{touched, seg_cap}, chunk, fz, ctx->partials, ctx->pmask, phase,
+                               (const uint64_t *)prio_list);
         }
         if (ev) HIPCHK(hipEventRecord(ev[1], ctx->stream));
         // the split quarters, from their sub-units' partial rows (row-split

@@ -702,33 +702,6 @@ __device__ __forceinline__ uint64_t rows_of_part(uint32_t s, uint32_t n) {
     return hi & ~((1ull << a) - 1ull);
 }
 
-// The LDS row accumulator of the lane-parallel sparse kernel: row x at word
-// ACC_STRIDE * x.
-constexpr uint32_t ACC_STRIDE = 5;
-__device__ __forceinline__ void acc_put(uint32_t x, uint64_t *acc, const Row4 &v) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[x * ACC_STRIDE + i] = v.w[i];
-}
-__device__ __forceinline__ Row4 acc_get(uint32_t x, const uint64_t *acc) {
-    return Row4{{acc[x * ACC_STRIDE], acc[x * ACC_STRIDE + 1], acc[x * ACC_STRIDE + 2], acc[x * ACC_STRIDE + 3]}};
-}
-
-// A lane ORs its items into the rows their 64-bit row masks name, one LDS
-// atomic per row and word.  Lanes walk their masks starting at bit (lane), not
-// at bit 0: where many lanes hold dense masks (a hub column shared by most
-// rows of a quarter) they then hit 64 different rows at each step of the walk
-// instead of all hitting row 0, then row 1, ..., which serialized the atomics
-// 64-fold.  rot_rows rotates the mask right by `lane`; rot_row maps the
-// lowest set bit of a rotated mask back to its row.
-__device__ __forceinline__ uint64_t rot_rows(uint64_t m, int lane) {
-    const uint32_t s = (uint32_t)lane & 63u;
-    return s ? (m >> s) | (m << (64u - s)) : m;
-}
-__device__ __forceinline__ uint32_t rot_row(uint64_t rr, int lane) {
-    return ((uint32_t)__builtin_ctzll(rr) + (uint32_t)lane) & 63u;
-}
-
-
 // Column of the j-th (0-based) set bit of a 256-bit row mask (j < popcount).
 __device__ __forceinline__ uint32_t select_bit256(const Row4 &m, uint32_t j) {
     uint32_t base = 0, k = 0;
@@ -827,19 +800,22 @@ struct LaneOps {
 //           selection within it) and tests it with vector loads of the B
 //           tile's row mask and the A quarter's column union and row mask; a
 //           ballot gives the hits;
-//   gather  the hits' (candidate, non-empty A row) pairs are numbered by a
-//           prefix sum over the lanes and dealt out 64 at a time: a lane
-//           finds its pair's candidate by binary search and the row by bit
-//           selection, loads the A row, ORs the B rows named by its bits
-//           (up to four B rows in flight) and merges the result into the
-//           unit's row accumulator in LDS (ds_or_b64);
+//   gather  column-major: the hits' (candidate, shared column k) items are
+//           numbered by a prefix sum over the lanes and dealt out 64 at a
+//           time; a lane finds its item's candidate by binary search and the
+//           column by bit selection, loads the A quarter's column word k (the
+//           rows holding k) and B row k together, and ORs B row k into each
+//           of those rows of the unit's LDS row accumulator (ds_or_b64);
 //   epilogue  lane r owns row r of the quarter, as in the batched kernel.
-// 64 candidates are filtered per load round and 64 A rows are in flight per
-// wave, however the rows are spread over the candidates.
+// 64 candidates are filtered per load round and COL_ITEMS x 64 items are in
+// flight per wave, however they are spread over the candidates.
 // QSET: the units may cover several quarters of a tile (tile units, a
 // 256-row accumulator); otherwise every unit is one quarter and the
 // accumulator has 64 rows (a smaller LDS footprint, more waves per CU).
-template <bool QSET>
+// RSPLIT: the step may hold row-split sub-units (ITEM_ROWSPLIT); the kernel
+// without that support is launched otherwise (its per-item row selection
+// measured a few per cent of the kernel's time).
+template <bool QSET, bool RSPLIT>
 __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__restrict__ mats,
                                                             const HeadDev *__restrict__ heads,
                                                             const TermPtr *__restrict__ tptr, int parity,
@@ -850,10 +826,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                                             uint64_t *__restrict__ pmask, int phase,
                                                             const uint64_t *__restrict__ prio_list) {
     constexpr uint32_t ACC_ROWS = QSET ? TILE : WAVE;
-    // rows of the unit's quarters (row 64 q + r with QSET, else r), ACC_STRIDE
-    // words apart: 40-byte rows put the 64 lanes of one atomic, on 64
-    // different rows, on 32 bank pairs (32-byte rows: on 8)
-    __shared__ uint64_t acc_s[ACC_ROWS * ACC_STRIDE];
+    __shared__ Row4 acc_s[ACC_ROWS];  // rows of the unit's quarters (row 64 q + r with QSET, else r)
     __shared__ Row4 s_brm[WAVE];
     __shared__ uint64_t s_arm[WAVE];
     // with QSET: the quarter of each staged hit, and the candidate masks per
@@ -879,12 +852,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     // of 64) claims at a time from its own shard's cursor, then from the
     // other shards' in turn once its own is used up.
     const uint32_t n_claim = (n_units + 63u) & ~63u;
-    // claims of `chunk` units, halved while there are not enough of them for
-    // every wave (the steps near convergence have a few hundred units: one
-    // wave per unit instead of a few waves walking chunks of them)
-    uint32_t ck = chunk;
-    while (ck > 1u && (uint64_t)gridDim.x * ck > n_claim) ck >>= 1;
-    if ((uint64_t)blockIdx.x * ck >= n_claim) return;  // more waves than claims
+    if ((uint64_t)blockIdx.x * chunk >= n_claim) return;  // more waves than claims
     const uint32_t nblk = n_claim / 64u;
     const uint32_t home = blockIdx.x % SHARDS;
     CounterShard *shards = c->shard;
@@ -901,49 +869,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     // a claim past the end of a shard only says the shard is used up).
     auto claim = [&](uint32_t s) -> uint32_t {
         uint32_t k = 0xffffffffu;
-        if (lane == 0) k = atomicAdd(phase == 1 ? &shards[s].prio_next : &shards[s].unit_next, ck);
+        if (lane == 0) k = atomicAdd(phase == 1 ? &shards[s].prio_next : &shards[s].unit_next, chunk);
         return k;
-    };
-    // Productive units, queued by lane 0: the head tile's count and occupancy
-    // words and its touched-list entry.  A flush adds the counts for up to
-    // PEND units at once (lane i, unit i); the lanes whose tile had no bit yet
-    // set its occupancy and append it to the touched list with one atomic.
-    // Nothing in this launch reads these (the next step's build and the
-    // retire launch do), so deferring them only takes two dependent atomic
-    // round trips per unit off the wave's path.
-    constexpr uint32_t PEND = 32;
-    __shared__ uint32_t *s_pcnt[PEND];
-    __shared__ uint32_t *s_poR[PEND];
-    __shared__ uint32_t *s_poC[PEND];
-    __shared__ uint32_t s_pbit[PEND], s_ppc[PEND];
-    __shared__ uint64_t s_pent[PEND];
-    uint32_t npend = 0;
-    auto flush_pending = [&]() {
-        if (!npend) return;
-        __syncthreads();
-        bool first = false;
-        uint64_t ent = 0;
-        if ((uint32_t)lane < npend) {
-            if (g_add(s_pcnt[lane], s_ppc[lane]) == 0) {
-                const uint32_t b = s_pbit[lane];
-                g_or(s_poR[lane], 1u << (b & 31u));
-                g_or(s_poC[lane], 1u << (b >> 5));
-                first = true;
-                ent = s_pent[lane];
-            }
-        }
-        const uint64_t bal = __ballot(first);
-        if (bal) {
-            const int leader = __ffsll((unsigned long long)bal) - 1;
-            uint32_t at = 0;
-            if (lane == leader) at = atomicAdd(&shards[home].n_touched, (uint32_t)__popcll(bal));
-            at = (uint32_t)__shfl((int)at, leader, 64);
-            if (first)
-                gst(&tl.list[(size_t)(1 + home) * tl.seg_cap + at + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))],
-                    ent);
-        }
-        npend = 0;
-        __syncthreads();
     };
     uint32_t pending = claim(sh);
     while (used_up < SHARDS) {
@@ -957,48 +884,32 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     }
     pending = claim(sh);
     const uint32_t u0 = blk * 64u + kc % 64u;
-    const uint32_t u1 = (u0 + ck < n_claim) ? u0 + ck : n_claim;
-    // The chunk's list entries and their heads' fields, lane j for unit
-    // u0 + j, loaded at once: a unit starts from registers instead of two
-    // dependent round trips of its own.
-    uint64_t c_it = 0;
-    uint32_t c_T = 0, c_t0 = 0, c_t1 = 0, c_out = 0, c_tgt = 0;
-    {
-        const uint32_t x = u0 + (uint32_t)lane;
-        const uint32_t u = (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
-        if (x < u1 && u < n_units) {
-            c_it = gld(phase == 1 ? prio_list + u : units + (u < n_heavy ? heavy_base + u : u - n_heavy));
-            const HeadDev *hd = heads + item_head(c_it);
-            const bool fl = (c_it & ITEM_FULL) != 0;
-            c_T = gld(&hd->T);
-            c_t0 = fl ? gld(&hd->f0) : gld(&hd->t0);
-            c_t1 = fl ? gld(&hd->f1) : gld(&hd->t1);
-            c_out = gld(&hd->out[parity]);
-            c_tgt = gld(&hd->target);
-        }
-    }
+    const uint32_t u1 = (u0 + chunk < n_claim) ? u0 + chunk : n_claim;
     for (uint32_t x = u0; x < u1; ++x) {
         const uint32_t u = (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
         if (u >= n_units) continue;
         const uint64_t tu0 = __builtin_amdgcn_s_memtime();
-        const uint32_t cl = x - u0;  // the lane holding this unit's entry
-        const uint64_t it = lane_read64(c_it, cl);
-        const uint32_t h = item_head(it);
-        const uint32_t p = unit_tile(it);
+        // (scalar loads: the list entry and the head's fields come through the
+        // scalar cache, which neighbouring units of a chunk share; loading a
+        // chunk's entries one per lane through the vector path measured
+        // slower on batches of small units)
+        const uint64_t it = phase == 1 ? ld_const(prio_list + u) : ld_const(units + (u < n_heavy ? heavy_base + u : u - n_heavy));
+        const uint32_t h = uniform(item_head(it));
+        const uint32_t p = uniform(unit_tile(it));
         // the unit's quarters: one, or every quarter of the tile with candidates
-        const uint32_t qset = unit_quarters(it);
+        const uint32_t qset = uniform(unit_quarters(it));
         const bool full = (it & ITEM_FULL) != 0;
-        const uint32_t sub = unit_sub(it), nsub = unit_nsub(it);
+        const uint32_t sub = uniform(unit_sub(it)), nsub = uniform(unit_nsub(it));
         // a row-split sub-unit: its rows of the quarter
-        const bool rsplit = nsub > 1 && (it & ITEM_ROWSPLIT) != 0;
+        const bool rsplit = RSPLIT && nsub > 1 && (it & ITEM_ROWSPLIT) != 0;
         const uint64_t rowsel = rsplit ? rows_of_part(sub, nsub) : ~0ull;
-        const uint32_t T = lane_read(c_T, cl);
-        const uint32_t t0 = lane_read(c_t0, cl);
-        const uint32_t t1 = lane_read(c_t1, cl);
+        const uint32_t T = ld_const(&heads[h].T);
+        const uint32_t t0 = full ? ld_const(&heads[h].f0) : ld_const(&heads[h].t0);
+        const uint32_t t1 = full ? ld_const(&heads[h].f1) : ld_const(&heads[h].t1);
         const uint32_t I = p / T, J = p - (p / T) * T;
         // the output descriptors are read now, while the unit's first loads are in flight
-        const MatDev O = mats[lane_read(c_out, cl)];
-        const MatDev M = mats[lane_read(c_tgt, cl)];
+        const MatDev O = mats[uniform(heads[h].out[parity])];
+        const MatDev M = mats[uniform(heads[h].target)];
         const uint32_t tp_unit0 = tp, br_unit0 = brows;
         uint32_t unit_pairs = 0;  // (hit, shared column) items of the unit (its shape, for the step log)
         // a split quarter's sub-unit `sub` of `nsub` takes the candidates
@@ -1006,7 +917,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         // several waves; its rows go to a partial-row slot (k_merge_split)
 #pragma unroll
         for (uint32_t qq = 0; qq < QUARTERS; ++qq)
-            if ((qset >> qq) & 1u) acc_put((QSET ? qq * WAVE : 0u) + lane, acc_s, Row4{{0, 0, 0, 0}});
+            if ((qset >> qq) & 1u) acc_s[(QSET ? qq * WAVE : 0u) + lane] = Row4{{0, 0, 0, 0}};
         __syncthreads();
         const uint32_t ow = (T + 31) / 32;
         const uint32_t ntw = (t1 - t0) * ow;  // (term, occupancy word) pairs of the unit
@@ -1081,15 +992,17 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     const uint32_t K = (jj % ow) * 32 + (uint32_t)bit;
                     const LaneOps o = s_ops[lo];
                     const size_t lt = (size_t)I * T + K, rt = (size_t)K * T + J;
-                    // two loads per candidate: the B tile's row mask and the A
-                    // quarter's column union (the filter's L2 requests are
-                    // most of the kernel's)
+                    // the B tile's row mask, the A quarter's column union and
+                    // row mask and the B tile's count, loaded together (reading
+                    // the last two only for hits on B tiles whose rows are all
+                    // non-empty costs a second round trip where such tiles are
+                    // common, as in saturating queries)
                     brm = load_row(o.rrmask + rt * 4);
                     const Row4 cu = load_row(o.lcolq + (lt * QUARTERS + q) * TROW_WORDS);
+                    const uint64_t qrm = gld(&o.lrmask[lt * 4 + q]);
+                    const uint32_t bcnt = gld(&o.rcnt[rt]);
                     hit = ((cu.w[0] & brm.w[0]) | (cu.w[1] & brm.w[1]) | (cu.w[2] & brm.w[2]) |
                            (cu.w[3] & brm.w[3])) != 0;
-                    // a B tile can only be full when all of its 256 rows are non-empty
-                    const bool all_rows = (brm.w[0] & brm.w[1] & brm.w[2] & brm.w[3]) == ~0ull;
                     // column-major gathers: the columns the A quarter and the B
                     // tile's non-empty rows share, and the quarter's column words
                     brm = Row4{{cu.w[0] & brm.w[0], cu.w[1] & brm.w[1], cu.w[2] & brm.w[2], cu.w[3] & brm.w[3]}};
@@ -1098,9 +1011,9 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                     // a full B tile needs no gathers (every row of the quarter
                     // becomes all ones): bit 0 of the (8-byte aligned) pointer
                     // marks it, and the quarter's row mask names the rows
-                    if (hit && all_rows && gld(&o.rcnt[rt]) == TILE_BITS) {
+                    if (hit && bcnt == TILE_BITS) {
                         bbase = (const uint64_t *)((uintptr_t)bbase | 1u);
-                        arm = gld(&o.lrmask[lt * 4 + q]);
+                        arm = qrm;
                     }
                 }
                 // 3. stage the hits in lane order and spread their (candidate,
@@ -1155,10 +1068,10 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                                 r0[j] = QSET ? s_hq[lo] * WAVE : 0u;
                                 const uint64_t *Bt = s_bb[lo];
                                 if ((uintptr_t)Bt & 1u) {  // full B tile: every row of the quarter becomes all ones
-                                    for (uint64_t rr = rot_rows(s_arm[lo] & rowsel, lane); rr; rr &= rr - 1) {
-                                        const uint32_t r = r0[j] + rot_row(rr, lane);
+                                    for (uint64_t rr = s_arm[lo] & rowsel; rr; rr &= rr - 1) {
+                                        const uint32_t r = r0[j] + (uint32_t)__builtin_ctzll(rr);
 #pragma unroll
-                                        for (int i = 0; i < 4; ++i) atomicOr((unsigned long long *)&acc_s[r * ACC_STRIDE + i], ~0ull);
+                                        for (int i = 0; i < 4; ++i) atomicOr((unsigned long long *)&acc_s[r].w[i], ~0ull);
                                     }
                                 } else {
                                     const Row4 rel = s_brm[lo];
@@ -1171,14 +1084,17 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
                         }
 #pragma unroll
                         for (int j = 0; j < COL_ITEMS; ++j) {
-                            // one atomic per row and word, zero words included: a
-                            // per-lane test of each word costs more (exec-mask
-                            // branches around every atomic) than the OR of a zero
-                            for (uint64_t rr = rot_rows(rows[j], lane); rr; rr &= rr - 1) {
-                                const uint32_t r = r0[j] + rot_row(rr, lane);
+                            // one LDS atomic per row and word.  Quarter-set units
+                            // (small graphs: B rows of a few bits) skip the zero
+                            // words; single-quarter units OR them too, since the
+                            // exec-mask branch around each atomic costs more there
+                            // than the OR of a zero (both measured)
+                            for (uint64_t rr = rows[j]; rr; rr &= rr - 1) {
+                                const uint32_t r = r0[j] + (uint32_t)__builtin_ctzll(rr);
 #pragma unroll
                                 for (int i = 0; i < 4; ++i)
-                                    atomicOr((unsigned long long *)&acc_s[r * ACC_STRIDE + i], (unsigned long long)b[j].w[i]);
+                                    if (!QSET || b[j].w[i])
+                                        atomicOr((unsigned long long *)&acc_s[r].w[i], (unsigned long long)b[j].w[i]);
                             }
                         }
                     }
@@ -1197,7 +1113,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
             // slot (its index u in the sub-unit list), non-empty ones only,
             // with their mask; k_merge_split finishes the quarter
             const uint32_t qq = (uint32_t)__builtin_ctz(qset);
-            const Row4 acc = acc_get((QSET ? qq * WAVE : 0u) + lane, acc_s);
+            const Row4 acc = acc_s[(QSET ? qq * WAVE : 0u) + lane];
             const bool nz = row_nonzero(acc);
             if (nz) store_row(partials + ((size_t)u * WAVE + lane) * TROW_WORDS, acc);
             const uint64_t bal = __ballot(nz);
@@ -1206,23 +1122,17 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
 #pragma unroll
             for (uint32_t qq = 0; qq < QUARTERS; ++qq) {
                 if (!((qset >> qq) & 1u)) continue;
-                const Row4 acc = acc_get((QSET ? qq * WAVE : 0u) + lane, acc_s);
+                const Row4 acc = acc_s[(QSET ? qq * WAVE : 0u) + lane];
                 pc += quarter_epilogue(O, M, p, qq, I, J, acc, fuse, out_rows, rsplit).pc;
             }
         }
-        if (pc) {
-            // the tile's count, occupancy and touched-list entry are queued
-            // and written for up to PEND units at once (flush_pending)
-            if (lane == 0) {
-                s_pcnt[npend] = &O.cnt[p];
-                s_poR[npend] = &O.occR[I * O.ow + (J >> 5)];
-                s_poC[npend] = &O.occC[J * O.ow + (I >> 5)];
-                s_pbit[npend] = (J & 31u) | ((I & 31u) << 5);
-                s_ppc[npend] = pc;
-                s_pent[npend] = ((uint64_t)h << 40) | p;
-                new_bits += pc;
+        if (pc && lane == 0) {
+            if (g_add(&O.cnt[p], pc) == 0) {
+                set_occ(O, I, J);
+                const uint32_t at = atomicAdd(&shards[home].n_touched, 1u);
+                gst(&tl.list[(size_t)(1 + home) * tl.seg_cap + at], ((uint64_t)h << 40) | p);
             }
-            if (++npend == PEND) flush_pending();
+            new_bits += pc;
         }
         const uint64_t t_end = __builtin_amdgcn_s_memtime();
         const uint64_t tu = t_end - tu0;
@@ -1247,7 +1157,6 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         __syncthreads();  // the accumulator is reset by the next unit
     }
     }
-    flush_pending();
     const uint32_t ar = wave_sum(arows), br = wave_sum(brows), orow = wave_sum(out_rows), hs = wave_sum(hits_n);
     const uint32_t ac = wave_sum(acols);
     if (lane == 0) {
@@ -1367,7 +1276,7 @@ __device__ __forceinline__ void tp_gather(const uint64_t *__restrict__ At, const
 __device__ __forceinline__ void tp_lds(const uint64_t *__restrict__ At, const uint64_t *__restrict__ Bt,
                                        uint64_t rm, int r, Row4 *Bs, Row4 &acc, const Row4 &known, uint32_t &arows,
                                        uint32_t &brows_lds) {
-    __syncthreads();
+    // (the caller's broadcast before this call is the barrier that protects Bs)
     Bs[r] = load_row(Bt + r * TROW_WORDS);
     __syncthreads();
     if (!((rm >> (r & 63)) & 1ull) || row_saturated(acc, known)) return;
@@ -1395,7 +1304,9 @@ constexpr int M4R_GROUPS = TILE / 4;  // 64 tables of 16 entries
 __device__ __forceinline__ void tp_m4r(const uint64_t *__restrict__ At, const uint64_t *__restrict__ Bt,
                                        uint64_t rm, int r, Row4 *Bs, Row4 *Tab, Row4 &acc, const Row4 &known,
                                        uint32_t &arows, uint32_t &lookups) {
-    __syncthreads();
+    // (no barrier before the B tile is staged: the caller's broadcast of this
+    // tile-product's counts is one, after every lane's reads of the previous
+    // tile-product's Bs and tables)
     Bs[r] = load_row(Bt + r * TROW_WORDS);
     __syncthreads();
 #pragma unroll
