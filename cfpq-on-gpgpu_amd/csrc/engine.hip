@@ -889,14 +889,13 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     // the apply waves run in both modes: without fused apply they merge the
     // new rows into the heads, and either way they build the new quarters'
     // column-major form
-    const uint32_t na = (uint32_t)ctx->num_cu * 32;
-    const uint32_t nc = retire_delta ? (uint32_t)ctx->num_cu * 8 : 0u;
+    const uint32_t nw = (uint32_t)ctx->num_cu * 32;
     const uint32_t nido = retire_delta ? nids_q : 0u, per = occ_blocks_per_matrix(plan);
     const StepCounters *prev = retire_delta ? ctx->ring + prev_slot : c;
     const TouchedLists ptl{ctx->lists[retire_delta ? (prev_slot & 1) : (slot & 1)], seg_cap};
-    hipLaunchKernelGGL(k_retire, dim3(na + nc + nido * per + 1), dim3(WAVE), 0, ctx->stream,
+    hipLaunchKernelGGL(k_retire, dim3(nw + nido * per + 1), dim3(WAVE), 0, ctx->stream,
                        (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q, (const StepCounters *)c,
-                       TouchedLists{touched, seg_cap}, prev, ptl, ids_q, nido, per, na, nc,
+                       TouchedLists{touched, seg_cap}, prev, ptl, ids_q, nido, per, nw, 1, retire_delta ? 1 : 0,
                        ctx->ring + (slot + 1) % RING, fuse ? 0 : 1, head >= 0 ? stamp_step : 0u, head,
                        head >= 0 && stamp_step ? ctx->last_eval : nullptr);
     if (prev_slot < 0 && !full_pass) {
@@ -1188,7 +1187,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             ctx->graph_sig = sig;
         }
     }
-    CHK(ensure_events(ctx, EV_SLOT * (size_t)RING + 2));
+    CHK(ensure_events(ctx, EV_SLOT * (size_t)RING + 4));
     hipEvent_t ev_begin = ctx->ev[EV_SLOT * RING], ev_end = ctx->ev[EV_SLOT * RING + 1];
     HIPCHK(hipEventRecord(ev_begin, ctx->stream));
     StepStats tot;
@@ -1207,10 +1206,31 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
                        full_passes));
         done = true;
     }
-    while (!done) {
-        const int kind = pending;
-        const int k = kind ? 1 : per_sync;
+    // A window: `per_sync` delta steps (or one full-pass or flush step), then
+    // a copy of their counters to the host and an event.  Delta windows are
+    // enqueued one ahead: while the host waits for a window's counters the
+    // next one already runs, so the device does not idle for the host's wake-up
+    // and enqueue at every window.  If the waited-for window converged, the
+    // steps ahead found no delta (as the steps after the quiet one within a
+    // window) and are not counted.  Not with deferred tiles, whose balance
+    // every executed step changes, and not with graph replay (measured slower
+    // with a window of graph launches ahead).
+    struct Window {
+        int kind = 0;
+        uint32_t first = 0;  // step number of the first step
         std::vector<uint32_t> slots;
+        hipEvent_t ev = nullptr;
+    };
+    Window win[2];
+    win[0].ev = ctx->ev[EV_SLOT * RING + 2];
+    win[1].ev = ctx->ev[EV_SLOT * RING + 3];
+    const bool ahead_ok = !single_step_full && !use_graph && 2 * per_sync <= RING &&
+                          !(plan.dirty_words && (sched_on(ctx) || order_on(ctx) || ctx->tr_cap));
+    auto enqueue_window = [&](int kind, Window &w) -> int {
+        const int k = kind ? 1 : per_sync;
+        w.kind = kind;
+        w.first = steps;
+        w.slots.clear();
         for (int s = 0; s < k; ++s) {
             const uint32_t slot = steps % RING;
             // the first step evaluates every rule once over the full matrices
@@ -1218,17 +1238,35 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             const bool full = kind == 1 || steps == 0;
             if (use_graph) CHK(launch_step_graph(ctx, plan, slot, prev_slot, full, kind == 2));
             else CHK(enqueue_step(ctx, plan, (int)(steps & 1), slot, prev_slot, full,
-                                  ctx->opts.no_kernel_timing ? -1 : s, kind == 2, true,
+                                  ctx->opts.no_kernel_timing ? -1 : (int)slot, kind == 2, true,
                                   prev_slot < 0 || (uint32_t)(prev_slot + 1) % RING != slot));
             ++launches;
-            slots.push_back(slot);
+            w.slots.push_back(slot);
             prev_slot = (int)slot;
             ++steps;
         }
-        // read the counters of the steps just run
-        HIPCHK(hipMemcpyAsync(ctx->ring_host, ctx->ring, sizeof(StepCounters) * RING, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        // the counters of the window's steps (consecutive ring slots)
+        const uint32_t first = w.slots[0], n1 = std::min<uint32_t>((uint32_t)k, RING - first);
+        HIPCHK(hipMemcpyAsync(ctx->ring_host + first, ctx->ring + first, sizeof(StepCounters) * n1,
+                              hipMemcpyDeviceToHost, ctx->stream));
+        if (n1 < (uint32_t)k)
+            HIPCHK(hipMemcpyAsync(ctx->ring_host, ctx->ring, sizeof(StepCounters) * (k - n1), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        HIPCHK(hipEventRecord(w.ev, ctx->stream));
+        return CFPQ_OK;
+    };
+    int cw = 0;  // the window waited for next
+    if (!done) CHK(enqueue_window(pending, win[cw]));
+    while (!done) {
+        Window &w = win[cw];
+        Window &ahead = win[cw ^ 1];
+        const bool have_ahead = ahead_ok && w.kind == 0;
+        if (have_ahead) CHK(enqueue_window(0, ahead));
+        HIPCHK(hipEventSynchronize(w.ev));
         ++syncs;
+        const int kind = w.kind;
+        const int k = (int)w.slots.size();
+        const std::vector<uint32_t> &slots = w.slots;
         pending = 0;
         uint32_t used = 0;
         bool quiet = false;  // a step added no bit
@@ -1240,8 +1278,9 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
             dirty_total += (int64_t)c.dirty_new - (int64_t)c.dirty_cleared;
             if (quiet) continue;
             float step_ms[4] = {0, 0, 0, 0};
+            const uint32_t e0 = EV_SLOT * slots[s];
             for (int k2 = 0; k2 < (heavy_paths ? 4 : 2) && !use_graph && !ctx->opts.no_kernel_timing; ++k2) {
-                HIPCHK(hipEventElapsedTime(&step_ms[k2], ctx->ev[EV_SLOT * s + k2], ctx->ev[EV_SLOT * s + k2 + 1]));
+                HIPCHK(hipEventElapsedTime(&step_ms[k2], ctx->ev[e0 + k2], ctx->ev[e0 + k2 + 1]));
                 part_ms[k2] += step_ms[k2];
                 product_ms += step_ms[k2];
             }
@@ -1252,7 +1291,7 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
                              "wave_max %llu unit_max %llu clk (nsub %u cands %u pairs %u gathers %u; clk scan %u filter %u "
                              "pairs %u epilogue %u) | sparse %.3f "
                              "merge %.3f dense %.3f mfma %.3f ms\n",
-                             steps - k + s, c.n_units + c.n_heavy, c.n_dense, c.n_mfma, c.n_touched,
+                             w.first + s, c.n_units + c.n_heavy, c.n_dense, c.n_mfma, c.n_touched,
                              (unsigned long long)c.new_bits, (unsigned long long)c.tile_products,
                              (unsigned long long)c.cand_hits, c.max_unit_cands, c.n_split, (unsigned long long)c.a_rows,
                              (unsigned long long)c.b_rows,
@@ -1276,7 +1315,9 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         if (kind == 1) ++full_passes;
         if (kind == 2) ++flush_steps;
         // steps enqueued after a quiet one found no delta; count only up to it
-        steps -= (uint32_t)(k - used);
+        // (with the window ahead of this one)
+        const uint32_t ahead_steps = have_ahead ? (uint32_t)ahead.slots.size() : 0u;
+        steps -= (uint32_t)(k - used) + (quiet ? ahead_steps : 0u);
         if (quiet) {
             if (dirty_total > 0) {
                 // deferred tiles remain: evaluate them all in full before
@@ -1292,8 +1333,14 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         } else if (single_step_full && kind == 1) {
             done = true;
         }
-        if (!done && ctx->opts.max_iterations && steps >= ctx->opts.max_iterations)
+        if (!done && ctx->opts.max_iterations && steps - (quiet ? 0u : ahead_steps) >= ctx->opts.max_iterations)
             return fail(ctx, CFPQ_E_LIMIT, "fixpoint: iteration cap %u reached", ctx->opts.max_iterations);
+        if (done) break;
+        if (have_ahead && !quiet) {
+            cw ^= 1;  // the window ahead is waited for next
+        } else {
+            CHK(enqueue_window(pending, w));
+        }
     }
     // the last step's delta (D of the step after it) is empty; the delta it
     // consumed was cleared by the step itself.  Retire the newest delta list
@@ -1303,12 +1350,12 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         const uint32_t slot = steps % RING;
         const uint32_t *ids_q = ctx->d_ids + (q ? nd : 0);
         const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
-        const uint32_t nc = (uint32_t)ctx->num_cu * 8, per = occ_blocks_per_matrix(plan);
+        const uint32_t nw = (uint32_t)ctx->num_cu * 16, per = occ_blocks_per_matrix(plan);
         const TouchedLists ptl{ctx->lists[prev_slot & 1], (uint32_t)plan.total_head_tiles};
-        hipLaunchKernelGGL(k_retire, dim3(nc + nids_q * per + 1), dim3(WAVE), 0, ctx->stream,
+        hipLaunchKernelGGL(k_retire, dim3(nw + nids_q * per + 1), dim3(WAVE), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q,
                            (const StepCounters *)(ctx->ring + slot), ptl, (const StepCounters *)(ctx->ring + prev_slot),
-                           ptl, ids_q, nids_q, per, 0u, nc, ctx->ring + slot, 1, 0u, -1, (uint32_t *)nullptr);
+                           ptl, ids_q, nids_q, per, nw, 0, 1, ctx->ring + slot, 1, 0u, -1, (uint32_t *)nullptr);
         CHK(launch_check(ctx, "final clear"));
     }
     HIPCHK(hipEventRecord(ev_end, ctx->stream));

@@ -1745,104 +1745,97 @@ __global__ __launch_bounds__(MFMA_BLOCK) void k_product_mfma(const MatDev *__res
 // apply / clear
 // ---------------------------------------------------------------------------
 
-// M[head] |= next-delta tile, for every tile a product kernel wrote: one
-// wave per (tile, quarter); the quarter-0 wave also merges the tile count.
-// Merge the new rows of one step's touched tiles into the heads: one wave per
-// (touched tile, quarter), waves blk, blk + nblk, ... of the work.
-// The quarter's column-major form is computed here for the delta (from its
-// new rows) and OR-ed into the head's, with or without `merge` (merge = 0: the
-// product kernels already merged rows and metadata into the head).
-__device__ __forceinline__ void apply_waves(const MatDev *__restrict__ mats, const HeadDev *__restrict__ heads,
-                                            int parity, const StepCounters *__restrict__ c, TouchedLists tl,
-                                            uint32_t blk, uint32_t nblk, int merge, uint32_t stamp_step) {
+// Merge the new rows of one touched tile (head h, tile p: entry `it`) into the
+// head, whole wave, all four quarters at once: the tile's row masks and column
+// unions (delta and head) come in one round of scalar loads, then every row
+// either quarter needs in one round of vector loads (lane r: row 64 q + r of
+// quarter q), so a tile costs a few dependent round trips instead of a few per
+// quarter.  The column-major form of every quarter with new rows is computed
+// here for the delta (from its new rows) and OR-ed into the head's, with or
+// without `merge` (merge = 0: the product kernels already merged rows and
+// metadata into the head).  Each (head, tile) is touched once per step, so
+// nothing else in the launch writes the head tile.
+__device__ __forceinline__ void apply_tile(const MatDev *__restrict__ mats, const HeadDev *__restrict__ heads,
+                                           int parity, uint64_t it, int merge, uint32_t stamp_step) {
     const int lane = threadIdx.x & 63;
-    const uint32_t n = touched_total(c) * QUARTERS;
-    for (uint32_t idx = blk; idx < n; idx += nblk) {  // static: the units are small and alike
-        const uint64_t it = touched_entry(c, tl, idx >> 2);
-        const uint32_t q = idx & 3u;
-        const uint32_t h = uniform(item_head(it));
-        const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
-        const MatDev M = mats[uniform(heads[h].target)];
-        const MatDev O = mats[uniform(heads[h].out[parity])];
-        const uint32_t T = uniform(heads[h].T), I = p / T, J = p - (p / T) * T;
-        if (merge && lane == 0 && q == 0) {
-            const uint32_t old = gld(&M.cnt[p]);
-            gst(&M.cnt[p], old + gld(&O.cnt[p]));
-            if (!old) set_occ(M, I, J);
-        }
-        // sweeps: the head tile gained bits in this step
-        if (stamp_step && lane == 0 && q == 0) gst(&M.stamp[p], stamp_step);
-        const uint64_t rm = uniform64(gld(&O.rmask[(size_t)p * 4 + q]));
-        if (!rm) continue;
-        const bool keepT = uniform(heads[h].colT) != 0;
-        // the head's rows: with merge, its non-empty rows before the merge and
-        // the new ones; without, the product kernels merged them already
-        const uint64_t mrm = uniform64(gld(&M.rmask[(size_t)p * 4 + q]));
+    const uint32_t h = uniform(item_head(it));
+    const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
+    const MatDev M = mats[uniform(heads[h].target)];
+    const MatDev O = mats[uniform(heads[h].out[parity])];
+    const uint32_t T = uniform(heads[h].T), I = p / T, J = p - (p / T) * T;
+    const bool keepT = uniform(heads[h].colT) != 0;
+    const Row4 rmO = ld_const_row(O.rmask + (size_t)p * 4);
+    const Row4 rmM = ld_const_row(M.rmask + (size_t)p * 4);
+    Row4 ocu[QUARTERS], mcu[QUARTERS];
+#pragma unroll
+    for (int q = 0; q < QUARTERS; ++q) {
+        ocu[q] = ld_const_row(colq_of(O, p, q));
+        mcu[q] = ld_const_row(colq_of(M, p, q));
+    }
+    if (merge && lane == 0) {
+        const uint32_t old = gld(&M.cnt[p]);
+        gst(&M.cnt[p], old + gld(&O.cnt[p]));
+        if (!old) set_occ(M, I, J);
+    }
+    // sweeps: the head tile gained bits in this step
+    if (stamp_step && lane == 0) gst(&M.stamp[p], stamp_step);
+    // the head's rows: with merge, its non-empty rows before the merge and the
+    // new ones; without, the product kernels merged them already
+    Row4 nw[QUARTERS], m[QUARTERS];
+#pragma unroll
+    for (int q = 0; q < QUARTERS; ++q) {
+        const uint64_t rm = rmO.w[q];
+        const uint64_t need = rm ? (keepT ? (rmM.w[q] | rm) : 0ull) | (merge ? rm : 0ull) : 0ull;
         const size_t off = (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS;
-        Row4 nw = {{0, 0, 0, 0}}, m = {{0, 0, 0, 0}};
-        if ((rm >> lane) & 1ull) nw = load_row(O.bits + off);
-        const uint64_t need = (keepT ? (mrm | rm) : 0ull) | (merge ? rm : 0ull);
-        if ((need >> lane) & 1ull) m = load_row(M.bits + off);
+        nw[q] = ((rm >> lane) & 1ull) ? load_row(O.bits + off) : Row4{{0, 0, 0, 0}};
+        m[q] = ((need >> lane) & 1ull) ? load_row(M.bits + off) : Row4{{0, 0, 0, 0}};
+    }
+#pragma unroll
+    for (int q = 0; q < QUARTERS; ++q) {
+        const uint64_t rm = rmO.w[q];
+        if (!rm) continue;
         if (merge && ((rm >> lane) & 1ull)) {
-            or_row(m, nw);
-            store_row(M.bits + off, m);
+            or_row(m[q], nw[q]);
+            store_row(M.bits + (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS, m[q]);
         }
-        const Row4 ocu = load_row(colq_of(O, p, q));
         if (keepT) {
             // column words of the columns that gained bits: the delta's from its
             // rows, the head's recomputed from all of its rows (a head that was
             // cleared and filled again may hold stale words for columns that
             // left its colq, so they are rewritten, never OR-ed into)
-            quarter_transpose(nw, ocu, colT_of(O, p, q));
-            quarter_transpose(m, ocu, colT_of(M, p, q));
+            quarter_transpose(nw[q], ocu[q], colT_of(O, p, q));
+            quarter_transpose(m[q], ocu[q], colT_of(M, p, q));
         }
         if (merge && lane == 0) {
-            const uint64_t old = gld(&M.rmask[(size_t)p * 4 + q]);
-            gst(&M.rmask[(size_t)p * 4 + q], old | rm);
-            Row4 cu = load_row(colq_of(M, p, q));
-            or_row(cu, ocu);
+            gst(&M.rmask[(size_t)p * 4 + q], rmM.w[q] | rm);
+            Row4 cu = mcu[q];
+            or_row(cu, ocu[q]);
             store_row(colq_of(M, p, q), cu);
-            if (!old) set_occq(M, q, I, J);
+            if (!rmM.w[q]) set_occq(M, q, I, J);
         }
     }
 }
 
-// Zero the delta tiles listed in `tl` (written by the step of parity
-// `parity`, counted in `list_ctr`): one wave per (tile, quarter) -- rows named
-// by the quarter's row mask, then the quarter's metadata; the quarter-0 wave
-// also zeroes the count.
-__device__ __forceinline__ void clear_waves(const MatDev *__restrict__ mats, const HeadDev *__restrict__ heads,
-                                            int parity, const StepCounters *__restrict__ list_ctr, TouchedLists tl,
-                                            uint32_t blk, uint32_t nblk) {
+// Zero one delta tile (entry `it` of the list written by the step of parity
+// `parity`), whole wave: its rows named by the four row masks, then its
+// metadata and count.
+__device__ __forceinline__ void clear_tile(const MatDev *__restrict__ mats, const HeadDev *__restrict__ heads,
+                                           int parity, uint64_t it) {
     const int lane = threadIdx.x & 63;
-    const uint32_t n = touched_total(list_ctr) * QUARTERS;
-    for (uint32_t idx = blk; idx < n; idx += nblk) {  // static: the units are small and alike
-        const uint64_t it = touched_entry(list_ctr, tl, idx >> 2);
-        const uint32_t q = idx & 3u;
-        const uint32_t h = uniform(item_head(it));
-        const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
-        const MatDev O = mats[uniform(heads[h].out[parity])];
-        const uint64_t rm = gld(&O.rmask[(size_t)p * 4 + q]);
-        if ((rm >> lane) & 1ull)
+    const uint32_t h = uniform(item_head(it));
+    const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
+    const MatDev O = mats[uniform(heads[h].out[parity])];
+    const Row4 rm = ld_const_row(O.rmask + (size_t)p * 4);
+#pragma unroll
+    for (int q = 0; q < QUARTERS; ++q) {
+        if ((rm.w[q] >> lane) & 1ull)
             store_row(O.bits + (size_t)p * TILE_WORDS + (q * WAVE + lane) * TROW_WORDS, Row4{{0, 0, 0, 0}});
-        if (lane == 0 && rm) {
+        if (lane == 0 && rm.w[q]) {
             gst(&O.rmask[(size_t)p * 4 + q], (uint64_t)0);
             store_row(colq_of(O, p, q), Row4{{0, 0, 0, 0}});
         }
-        if (lane == 0 && q == 0) gst(&O.cnt[p], 0u);
     }
-}
-
-__global__ __launch_bounds__(WAVE) void k_apply(const MatDev *__restrict__ mats,
-                                                const HeadDev *__restrict__ heads, int parity,
-                                                StepCounters *__restrict__ c, TouchedLists tl) {
-    apply_waves(mats, heads, parity, c, tl, blockIdx.x, gridDim.x, 1, 0u);
-}
-
-__global__ __launch_bounds__(WAVE) void k_clear(const MatDev *__restrict__ mats,
-                                                const HeadDev *__restrict__ heads, int parity,
-                                                const StepCounters *__restrict__ list_ctr, TouchedLists tl) {
-    clear_waves(mats, heads, parity, list_ctr, tl, blockIdx.x, gridDim.x);
+    if (lane == 0) gst(&O.cnt[p], 0u);
 }
 
 // Zero the occupancy bitsets (all four kinds) of one matrix, grid-strided.
@@ -1873,36 +1866,41 @@ __global__ void k_clear_occ(const MatDev *__restrict__ mats, const uint32_t *__r
 }
 
 // The end of a step in one launch, its parts on disjoint arrays:
-//   blocks [0, na)             apply this step's new rows to the heads (parity
-//                              `parity`, list `tl`, counts in c) unless the
-//                              products merged them already (merge = 0), and
-//                              the column-major form of every new quarter;
-//                              sweeps stamp the head tiles with stamp_step and
-//                              record it as entry `entry`'s last evaluation;
-//   blocks [na, na + nc)       zero the rows and metadata of the delta the step
-//                              consumed (parity 1 - parity, the previous step's
-//                              list `ptl`, counts in prev);
-//   blocks [.., .. + nid * per) zero that delta's occupancy bitsets (ids[0..nid)),
+//   blocks [0, nw)             one wave per tile, strided over this step's
+//                              touched tiles (parity `parity`, list `tl`,
+//                              counts in c; with `apply`: their new rows merged
+//                              into the heads unless the products did so
+//                              already (merge = 0), and the column-major form
+//                              of every new quarter; sweeps stamp the head
+//                              tiles with stamp_step) and then the tiles of the
+//                              delta the step consumed (with `clear`: parity
+//                              1 - parity, the previous step's list `ptl`,
+//                              counts in prev; rows and metadata zeroed);
+//   blocks [nw, nw + nid * per) zero that delta's occupancy bitsets (ids[0..nid)),
 //                              `per` blocks per matrix;
 //   the last block             zero the counters of the next ring slot, so that
 //                              the next step starts without a launch of its own.
+// Block 0 also records stamp_step as plan entry `entry`'s last evaluation (sweeps).
 __global__ __launch_bounds__(WAVE) void k_retire(const MatDev *__restrict__ mats,
                                                  const HeadDev *__restrict__ heads, int parity,
                                                  const StepCounters *__restrict__ c, TouchedLists tl,
                                                  const StepCounters *__restrict__ prev, TouchedLists ptl,
                                                  const uint32_t *__restrict__ ids, uint32_t nid, uint32_t per,
-                                                 uint32_t na, uint32_t nc, StepCounters *__restrict__ next,
+                                                 uint32_t nw, int apply, int clear, StepCounters *__restrict__ next,
                                                  int merge, uint32_t stamp_step, int entry,
                                                  uint32_t *__restrict__ last_eval) {
     const uint32_t b = blockIdx.x;
     // sweeps: this step evaluated plan entry `entry` (its products have run)
     if (b == 0 && threadIdx.x == 0 && last_eval && entry >= 0) gst(&last_eval[entry], stamp_step);
-    if (b < na) {
-        apply_waves(mats, heads, parity, c, tl, b, na, merge, stamp_step);
-    } else if (b < na + nc) {
-        clear_waves(mats, heads, 1 - parity, prev, ptl, b - na, nc);
-    } else if (b < na + nc + nid * per) {
-        const uint32_t o = b - na - nc, m = o / per;
+    if (b < nw) {
+        const uint32_t n_a = apply ? touched_total(c) : 0u;
+        const uint32_t n_c = clear ? touched_total(prev) : 0u;
+        for (uint32_t i = b; i < n_a + n_c; i += nw) {  // static: the tiles are small and alike
+            if (i < n_a) apply_tile(mats, heads, parity, touched_entry(c, tl, i), merge, stamp_step);
+            else clear_tile(mats, heads, 1 - parity, touched_entry(prev, ptl, i - n_a));
+        }
+    } else if (b < nw + nid * per) {
+        const uint32_t o = b - nw, m = o / per;
         zero_occ(mats[ids[m]], (o - m * per) * WAVE + threadIdx.x, per * WAVE);
     } else if (next) {
         uint64_t *w = reinterpret_cast<uint64_t *>(next);
