@@ -228,7 +228,12 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
 
 // 64 x 64 bit transpose across the wave: on entry lane r holds row r (bit c =
 // element (r, c)), on exit lane c holds column c (bit r = element (r, c)).
-// Stage j swaps the off-diagonal j x j sub-blocks of every 2j x 2j block.
+// Stage j swaps the off-diagonal j x j sub-blocks of every 2j x 2j block:
+// of a lane pair (a, a + j), a keeps its low j-bit groups (mask keep) and
+// takes b's low groups shifted up, b keeps its high groups and takes a's high
+// groups shifted down.  Each lane sends its value already shifted the way its
+// partner needs it, so every lane runs the same bitwise selects (no branch on
+// the lane's side of the pair).
 __device__ __forceinline__ uint64_t wave_transpose64(uint64_t v) {
     const int lane = threadIdx.x & 63;
     const uint64_t keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
@@ -236,9 +241,11 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t v) {
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
         const int j = 32 >> s;
-        const uint64_t m0 = keep[s];
-        const uint64_t p = shfl_xor64(v, j);
-        v = (lane & j) ? ((v & ~m0) | ((p & ~m0) >> j)) : ((v & m0) | ((p & m0) << j));
+        const uint64_t hm = (lane & j) ? ~0ull : 0ull;  // all ones on the pair's upper lane
+        const uint64_t send = ((v << j) & hm) | ((v >> j) & ~hm);
+        const uint64_t recv = shfl_xor64(send, j);
+        const uint64_t k = keep[s] ^ hm;  // the bits this lane keeps
+        v = (v & k) | (recv & ~k);
     }
     return v;
 }
