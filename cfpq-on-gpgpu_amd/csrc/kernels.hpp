@@ -869,7 +869,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     uint64_t unit_max_a = 0, unit_max_b = 0, unit_max_c = 0, unit_max_d = 0;  // its shape (CounterShard)
     uint64_t unit_clk = 0, idle_clk = 0;  // lane 0: clocks of all units / of units that added no bit
     uint32_t idle_units = 0, idle_brows = 0, units_done = 0;
-    uint32_t sh = home, used_up = 0;
+    uint32_t sh = home;
     // The claim of the next chunk is issued when a chunk starts and read when
     // it ends, so the atomic's round trip overlaps the chunk's own loads.  A
     // claim is always taken by the wave that issued it (cursors only grow, so
@@ -880,13 +880,22 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         return k;
     };
     uint32_t pending = claim(sh);
-    while (used_up < SHARDS) {
+    for (;;) {
     const uint32_t kc = uniform(pending);
     const uint32_t blk = (kc / 64u) * SHARDS + sh;
-    if (blk >= nblk) {  // this shard's blocks are all claimed
-        sh = (sh + 1) % SHARDS;
-        ++used_up;
-        if (used_up < SHARDS) pending = claim(sh);
+    if (blk >= nblk) {
+        // this shard's blocks are all claimed: mark it in the step's mask of
+        // used-up shards, which the same atomic returns, and go on with the
+        // next shard not marked, or end (one round trip, instead of a claim on
+        // every other shard in turn at the end of every wave)
+        uint32_t m = 0;
+        if (lane == 0) m = atomicOr(phase == 1 ? &c->prio_shards_done : &c->shards_done, 1u << sh);
+        m = uniform(m) | (1u << sh);
+        const uint32_t left = ~m & ((1u << SHARDS) - 1u);
+        if (!left) break;
+        const uint32_t after = left & ~((2u << sh) - 1u);  // shards past sh
+        sh = (uint32_t)__ffs(after ? after : left) - 1u;
+        pending = claim(sh);
         continue;
     }
     pending = claim(sh);

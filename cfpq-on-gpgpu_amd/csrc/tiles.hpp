@@ -146,6 +146,8 @@ struct StepCounters {
     uint32_t n_split;        // heavy units split
     uint32_t n_heavy;        // sub-units of split units, listed apart so that they are claimed first
     uint32_t n_prio;         // units the scheduler ranked first (evaluated by the step's first phase)
+    uint32_t shards_done;    // lane-parallel sparse kernel: bit s = shard s's units all claimed
+    uint32_t prio_shards_done;  // the same for the scheduler's first phase
     uint64_t new_bits;
     uint64_t tile_products;   // (term, C tile, k) products evaluated (sparse: per quarter)
     uint64_t tp_dense;        // of which on the LDS / Four Russians paths
