@@ -398,9 +398,44 @@ __device__ __forceinline__ void wave_append(bool want, uint64_t val, uint32_t *c
 __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, const TermDev *__restrict__ terms,
                                            uint32_t t0, uint32_t t1, int parity, uint32_t T, uint32_t I, uint32_t J,
                                            uint32_t &qmask, bool &dense, bool &mfma, uint32_t *qcnt, int heavy) {
+    // When a tile row spans whole waves (T a multiple of 64), the wave's
+    // tiles share the tile row I: the A operand's words of row I (occupancy,
+    // per-quarter, heavy-tile bitsets) are the same for every lane and come in
+    // scalar loads, eight words at a time and together with the B column's
+    // words, instead of a chain of dependent loads per word.
+    const bool row_uniform = (T % WAVE) == 0;
     for (uint32_t t = t0; t < t1; ++t) {
         const MatDev L = mats[terms[t].left[parity]];
         const MatDev R = mats[terms[t].right[parity]];
+        if (row_uniform) {
+            const uint32_t Iu = uniform(I);
+            for (uint32_t w0 = 0; w0 < L.ow; w0 += 8) {
+                uint32_t col[8], qw[QUARTERS][8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const uint32_t w = w0 + (uint32_t)i;
+                    col[i] = w < L.ow ? gld(&R.occC[J * R.ow + w]) & ld_const(&L.occR[Iu * L.ow + w]) : 0u;
+#pragma unroll
+                    for (uint32_t q = 0; q < QUARTERS; ++q)
+                        qw[q][i] = w < L.ow ? ld_const(&L.occQ[(q * T + Iu) * L.ow + w]) : 0u;
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const uint32_t w = w0 + (uint32_t)i;
+                    if (heavy && col[i]) {
+                        if (ld_const(&L.occD[Iu * L.ow + w]) & col[i]) dense = true;
+                        if (ld_const(&L.occX[Iu * L.ow + w]) & col[i]) mfma = true;
+                    }
+#pragma unroll
+                    for (uint32_t q = 0; q < QUARTERS; ++q) {
+                        const uint32_t k = qw[q][i] & col[i];
+                        if (k) qmask |= 1u << q;
+                        qcnt[q] += __popc(k);
+                    }
+                }
+            }
+            continue;
+        }
         for (uint32_t w = 0; w < L.ow; ++w) {
             uint32_t col = gld(&R.occC[J * R.ow + w]);
             if (!col) continue;
@@ -440,7 +475,8 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
                                              StepCounters *__restrict__ c,
                                              uint64_t *__restrict__ units, uint32_t heavy_base, int tile_units,
                                              uint64_t *__restrict__ splits, bool prio = false,
-                                             uint64_t *__restrict__ prio_list = nullptr, int split_rows = 0) {
+                                             uint64_t *__restrict__ prio_list = nullptr, int split_rows = 0,
+                                             bool block_agg = false) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
     if (prio_list) {
@@ -480,37 +516,80 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
     for (uint32_t q = 0; q < QUARTERS; ++q) {
         const bool wq = ((rest >> q) & 1u) != 0;
         nsub[q] = wq ? 1u : 0u;
+        uint32_t want = 0;
         if (wq && split && qcnt[q] > split) {
             uint32_t ns = (uint32_t)(((uint64_t)qcnt[q] + piece - 1) / piece);
             ns = ns > SPLIT_MAX ? SPLIT_MAX : (ns < 1 ? 1 : ns);
-            // the budget counts sub-units (partial-row slots); a refused
-            // reservation only leaves budget unused
-            if (ns > 1 && atomicAdd(&c->split_extra, ns) + ns > split_budget) ns = 1;
-            nsub[q] = ns;
+            want = ns > 1 ? ns : 0u;
+        }
+        // the budget counts sub-units (partial-row slots), reserved for the
+        // wave's heavy quarters with one atomic (lane i's slots end at the
+        // prefix sum of the requests); a refused reservation only leaves
+        // budget unused
+        const uint64_t wb = __ballot(want != 0);
+        if (wb) {
+            uint32_t tot = 0;
+            const uint32_t incl = scan_ballot(want, wb, tot);
+            uint32_t old = 0;
+            if (lane == 0) old = atomicAdd(&c->split_extra, tot);
+            old = uniform(old);
+            if (want) nsub[q] = old + incl > split_budget ? 1u : want;
         }
         single[q] = __ballot(nsub[q] == 1);
         total += (uint32_t)__popcll(single[q]);
     }
-    if (total) {
-        uint32_t at = 0;
+    // The single units' list positions: one atomic per work-group (block_agg:
+    // every thread of the group calls this once) or per wave.  Every wave of
+    // the build hits the one count, so fewer atomics on it shorten the build.
+    uint32_t at = 0;
+    if (block_agg) {
+        __shared__ uint32_t s_wcnt[BLOCK / WAVE];
+        __shared__ uint32_t s_base;
+        const uint32_t wv = threadIdx.x / WAVE;
+        if (lane == 0) s_wcnt[wv] = total;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t sum = 0;
+            for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) sum += s_wcnt[w];
+            s_base = sum ? atomicAdd(&c->n_units, sum) : 0u;
+        }
+        __syncthreads();
+        at = s_base;
+        for (uint32_t w = 0; w < wv; ++w) at += s_wcnt[w];
+    } else if (total) {
         if (lane == 0) at = atomicAdd(&c->n_units, total);
         at = uniform(at);
+    }
+    if (total) {
 #pragma unroll
         for (uint32_t q = 0; q < QUARTERS; ++q) {
             if (nsub[q] == 1) units[at + __popcll(single[q] & below)] = base | (uint64_t)q;
             at += (uint32_t)__popcll(single[q]);
         }
     }
+    // split quarters: their sub-units and split entries, one atomic per wave each
 #pragma unroll
     for (uint32_t q = 0; q < QUARTERS; ++q) {
-        if (nsub[q] > 1) {
-            const uint64_t u = base | (uint64_t)q | ((uint64_t)(nsub[q] - 1) << 59) | (split_rows ? ITEM_ROWSPLIT : 0ull);
-            const uint32_t at = atomicAdd(&c->n_heavy, nsub[q]);
-            for (uint32_t sidx = 0; sidx < nsub[q]; ++sidx) units[heavy_base + at + sidx] = u | ((uint64_t)sidx << 55);
+        const uint32_t hv = nsub[q] > 1 ? nsub[q] : 0u;
+        const uint64_t hb = __ballot(hv != 0);
+        if (!hb) continue;
+        uint32_t tot = 0;
+        const uint32_t incl = scan_ballot(hv, hb, tot);
+        uint32_t hbase = 0, sbase = 0;
+        if (lane == 0) {
+            hbase = atomicAdd(&c->n_heavy, tot);
+            if (!split_rows) sbase = atomicAdd(&c->n_split, (uint32_t)__popcll(hb));
+        }
+        hbase = uniform(hbase);
+        sbase = uniform(sbase);
+        if (hv) {
+            const uint32_t hat = hbase + incl - hv;
+            const uint64_t u = base | (uint64_t)q | ((uint64_t)(hv - 1) << 59) | (split_rows ? ITEM_ROWSPLIT : 0ull);
+            for (uint32_t sidx = 0; sidx < hv; ++sidx) units[heavy_base + hat + sidx] = u | ((uint64_t)sidx << 55);
             if (!split_rows) {  // candidate split: the quarter is finished by k_merge_split
-                const uint32_t sq = atomicAdd(&c->n_split, 1u);
+                const uint32_t sq = sbase + (uint32_t)__popcll(hb & below);
                 splits[2 * (size_t)sq] = u;
-                splits[2 * (size_t)sq + 1] = at;
+                splits[2 * (size_t)sq + 1] = hat;
             }
         }
     }
@@ -554,7 +633,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
     dense = dense || mfma;
     append_units(!dense, qmask, qcnt, flag | ((uint64_t)h << 40) | ((uint64_t)p << 2), split, piece, split_budget, c,
-                 units, heavy_base, tile_units, splits, false, nullptr, split_rows);
+                 units, heavy_base, tile_units, splits, false, nullptr, split_rows, true);
 }
 
 // ---------------------------------------------------------------------------

@@ -181,6 +181,43 @@ def test_fixpoint_sync_cadence(pkg, gen, oracle, sync):
     _fixpoint_vs_oracle(pkg, gen, oracle, gen.dyck_er(400, 1200, seed=8), iters_per_sync=sync)
 
 
+def test_fixpoint_windows_ahead(pkg, gen, oracle):
+    """Windows of steps enqueued one ahead of the host's wait: every window
+    length gives the same result and the same step count as one step per
+    window, with and without per-kernel events and under graph replay (no
+    window ahead), the steps ahead of the converging window are not counted,
+    and the delta buffers are left empty (a second call on the converged
+    matrices is one quiet step)."""
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    q = gen.dyck_er(600, 1800, seed=31)
+    g_ref, mats_ref, _ = oracle.solve(q, "seminaive")
+    want = oracle.fingerprints(g_ref, mats_ref, q.n)
+    steps = set()
+    for opts in (dict(iters_per_sync=1), dict(iters_per_sync=2), dict(iters_per_sync=5),
+                 dict(iters_per_sync=2, no_kernel_timing=True), dict(iters_per_sync=3, use_graph=True)):
+        with pkg.Engine(0, **opts) as eng:
+            gr = cf.Grammar(q.grammar, text=True)
+            mats = gr.load(eng, cf.Graph.from_query(q))
+            st = eng.fixpoint(gr.rule_triples(), mats)
+            assert gr.results() == want, opts
+            steps.add(st["iterations"])
+            again = eng.fixpoint(gr.rule_triples(), mats)
+            assert again["iterations"] == 1 and again["new_bits"] == 0, opts
+            assert gr.results() == want, opts
+            gr.free()
+    assert len(steps) == 1, steps
+
+
+def test_fixpoint_iteration_cap_with_windows_ahead(pkg, gen):
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    with pkg.Engine(0, max_iterations=3, iters_per_sync=2) as eng:
+        gr = cf.Grammar(gen.DYCK_GRAMMAR, text=True)
+        with pytest.raises(pkg.CfpqError) as ei:
+            gr.intersection_with_graph(cf.Graph.from_query(gen.config(0)), eng)
+        assert ei.value.code == pkg.E_LIMIT
+        _ = eng.matrix(10).popcount()
+
+
 def test_fixpoint_epsilon_and_self_rules(pkg, gen, oracle):
     q = gen.dyck_er(300, 800, seed=21)
     q.grammar = "S\nS A S1\nS1 S B\nS S S\nA a\nB b\n"
