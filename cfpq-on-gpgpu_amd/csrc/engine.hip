@@ -823,7 +823,11 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     // MFMA ..ev[5i+4]
     hipEvent_t *ev = ev_idx >= 0 ? &ctx->ev[EV_SLOT * (size_t)ev_idx] : nullptr;
     if (ev) HIPCHK(hipEventRecord(ev[0], ctx->stream));
-    const dim3 sgrid((uint32_t)ctx->num_cu * (uint32_t)ctx->opts.sparse_waves_per_cu);
+    // (no more waves than the step can have units: small problems launch
+    // small grids, which dispatch faster)
+    const uint64_t max_units = (uint64_t)QUARTERS * plan.total_head_tiles + split_budget;
+    const dim3 sgrid((uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>((uint64_t)ctx->num_cu * (uint64_t)ctx->opts.sparse_waves_per_cu, max_units)));
     const uint32_t chunk = (uint32_t)ctx->opts.sparse_chunk;
     const MatDev *dm = ctx->d_mats;
     const HeadDev *dh = ctx->d_heads;
@@ -889,7 +893,8 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
     // the apply waves run in both modes: without fused apply they merge the
     // new rows into the heads, and either way they build the new quarters'
     // column-major form
-    const uint32_t nw = (uint32_t)ctx->num_cu * 32;
+    // (at most one wave per tile of the two lists)
+    const uint32_t nw = (uint32_t)std::min<size_t>((size_t)ctx->num_cu * 32, std::max<size_t>(1, 2 * plan.total_head_tiles));
     const uint32_t nido = retire_delta ? nids_q : 0u, per = occ_blocks_per_matrix(plan);
     const StepCounters *prev = retire_delta ? ctx->ring + prev_slot : c;
     const TouchedLists ptl{ctx->lists[retire_delta ? (prev_slot & 1) : (slot & 1)], seg_cap};
@@ -1350,7 +1355,8 @@ static int run_fixpoint(cfpq_ctx *ctx, Plan &plan, cfpq_stats *stats, bool singl
         const uint32_t slot = steps % RING;
         const uint32_t *ids_q = ctx->d_ids + (q ? nd : 0);
         const uint32_t nids_q = (uint32_t)plan.all_delta[q].size();
-        const uint32_t nw = (uint32_t)ctx->num_cu * 16, per = occ_blocks_per_matrix(plan);
+        const uint32_t nw = (uint32_t)std::min<size_t>((size_t)ctx->num_cu * 16, std::max<size_t>(1, plan.total_head_tiles)),
+                       per = occ_blocks_per_matrix(plan);
         const TouchedLists ptl{ctx->lists[prev_slot & 1], (uint32_t)plan.total_head_tiles};
         hipLaunchKernelGGL(k_retire, dim3(nw + nids_q * per + 1), dim3(WAVE), 0, ctx->stream,
                            (const MatDev *)ctx->d_mats, (const HeadDev *)ctx->d_heads, q,
