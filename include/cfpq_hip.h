@@ -81,7 +81,8 @@ typedef struct cfpq_opts {
     uint32_t max_iterations;    /* 0 = unlimited                                         */
     int32_t final_full_pass;    /* 1: after the delta fixpoint, verify with one exact
                                    full pass (every rule over full matrices)             */
-    int32_t iters_per_sync;     /* fixpoint steps enqueued between host flag reads (0 = default) */
+    int32_t iters_per_sync;     /* fixpoint steps per window: the host reads a window's step
+                                   counters while the next window runs (0 = default) */
     int32_t use_graph;          /* 1: replay one captured hipGraph per fixpoint step     */
     int32_t sparse_batch;       /* k-tiles per batch in the sparse product: 1, 2 or 4 (0 = default) */
     int32_t sparse_waves_per_cu; /* persistent sparse-product waves per CU (0 = default)   */
