@@ -855,7 +855,7 @@ static int enqueue_step(cfpq_ctx *ctx, const Plan &plan, int q, uint32_t slot, i
             // in dispatch than the merge work they spread out)
             hipLaunchKernelGGL(k_merge_split, dim3((uint32_t)ctx->num_cu * 4), dim3(WAVE), 0, ctx->stream, dm, dh, q, c,
                                (const uint64_t *)ctx->splits, (const uint64_t *)ctx->partials,
-                               (const uint64_t *)ctx->pmask, touched);
+                               (const uint64_t *)ctx->pmask, TouchedLists{touched, seg_cap});
     } else if (ctx->opts.sparse_batch == 1) {
         hipLaunchKernelGGL(k_product_sparse<1>, sgrid, dim3(WAVE), 0, ctx->stream, dm, dh, terms, q, c, un, touched, chunk,
                            fz);

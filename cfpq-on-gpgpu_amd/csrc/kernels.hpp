@@ -1289,9 +1289,13 @@ __global__ __launch_bounds__(WAVE) void k_merge_split(const MatDev *__restrict__
                                                       const HeadDev *__restrict__ heads, int parity,
                                                       StepCounters *__restrict__ c, const uint64_t *__restrict__ splits,
                                                       const uint64_t *__restrict__ partials,
-                                                      const uint64_t *__restrict__ pmask, uint64_t *__restrict__ touched) {
+                                                      const uint64_t *__restrict__ pmask, TouchedLists tl) {
     const int lane = threadIdx.x & 63;
     const uint32_t n_split = c->n_split;
+    // counts and touched-list entries go to this XCD's shard (6,000 split
+    // quarters in a step would otherwise queue on one address)
+    const uint32_t home = blockIdx.x % SHARDS;
+    CounterShard *shard = &c->shard[home];
     uint32_t out_rows = 0;
     uint64_t new_bits = 0;
     for (uint32_t sq = blockIdx.x; sq < n_split; sq += gridDim.x) {
@@ -1321,15 +1325,16 @@ __global__ __launch_bounds__(WAVE) void k_merge_split(const MatDev *__restrict__
             const uint32_t old = g_add(&O.cnt[p], qo.pc);
             if (old == 0) {
                 set_occ(O, I, J);
-                touched[atomicAdd(&c->n_touched, 1u)] = ((uint64_t)h << 40) | p;
+                const uint32_t at = atomicAdd(&shard->n_touched, 1u);
+                gst(&tl.list[(size_t)(1 + home) * tl.seg_cap + at], ((uint64_t)h << 40) | p);
             }
             new_bits += qo.pc;
         }
     }
     const uint32_t orow = wave_sum(out_rows);
     if (lane == 0) {
-        if (orow) atomicAdd((unsigned long long *)&c->out_rows, (unsigned long long)orow);
-        if (new_bits) atomicAdd((unsigned long long *)&c->new_bits, (unsigned long long)new_bits);
+        if (orow) atomicAdd((unsigned long long *)&shard->out_rows, (unsigned long long)orow);
+        if (new_bits) atomicAdd((unsigned long long *)&shard->new_bits, (unsigned long long)new_bits);
     }
 }
 
