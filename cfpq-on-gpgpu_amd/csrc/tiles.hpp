@@ -131,8 +131,6 @@ struct StepCounters {
     uint32_t n_dense;      // dense work items (head, tile) built
     uint32_t dense_next;   // dense product cursor
     uint32_t n_touched;    // output tiles that received new bits
-    uint32_t apply_next;   // apply-kernel cursor
-    uint32_t clear_next;   // clear-kernel cursor
     uint32_t n_mfma;       // MFMA work items (head, tile) built
     uint32_t mfma_next;    // MFMA product cursor
     uint32_t n_deferred;   // (head, tile) items the scheduler deferred
