@@ -938,7 +938,10 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     // of 64) claims at a time from its own shard's cursor, then from the
     // other shards' in turn once its own is used up.
     const uint32_t n_claim = (n_units + 63u) & ~63u;
-    if ((uint64_t)blockIdx.x * chunk >= n_claim) return;  // more waves than claims
+    // A step with no more units than waves (a small problem, the last steps
+    // of a fixpoint): wave b takes unit b, with no claim at all.
+    const bool direct = n_units <= gridDim.x;
+    if (direct ? blockIdx.x >= n_units : (uint64_t)blockIdx.x * chunk >= n_claim) return;  // more waves than claims
     const uint32_t nblk = n_claim / 64u;
     const uint32_t home = blockIdx.x % SHARDS;
     CounterShard *shards = c->shard;
@@ -958,8 +961,16 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         if (lane == 0) k = atomicAdd(phase == 1 ? &shards[s].prio_next : &shards[s].unit_next, chunk);
         return k;
     };
-    uint32_t pending = claim(sh);
+    uint32_t pending = direct ? 0u : claim(sh);
+    bool first = true;
     for (;;) {
+    uint32_t u0, u1;
+    if (direct) {
+        if (!first) break;
+        first = false;
+        u0 = blockIdx.x;
+        u1 = u0 + 1u;
+    } else {
     const uint32_t kc = uniform(pending);
     const uint32_t blk = (kc / 64u) * SHARDS + sh;
     if (blk >= nblk) {
@@ -978,10 +989,11 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         continue;
     }
     pending = claim(sh);
-    const uint32_t u0 = blk * 64u + kc % 64u;
-    const uint32_t u1 = (u0 + chunk < n_claim) ? u0 + chunk : n_claim;
+    u0 = blk * 64u + kc % 64u;
+    u1 = (u0 + chunk < n_claim) ? u0 + chunk : n_claim;
+    }
     for (uint32_t x = u0; x < u1; ++x) {
-        const uint32_t u = (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
+        const uint32_t u = direct ? x : (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
         if (u >= n_units) continue;
         const uint64_t tu0 = __builtin_amdgcn_s_memtime();
         // (scalar loads: the list entry and the head's fields come through the
