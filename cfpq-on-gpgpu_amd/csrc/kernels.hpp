@@ -86,14 +86,25 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
     return ((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | uniform((uint32_t)v);
 }
 
-// Wave-uniform read of memory that no thread writes during the current
-// kernel (operand metadata, work lists): going through the constant address
-// space lets the compiler use scalar loads through the scalar cache, which
-// is invalidated at every kernel launch.  Never use it for data the running
-// kernel modifies.
+// Wave-uniform read of memory that only the host writes (the plan's
+// descriptor tables, uploaded before the kernels that read them): going
+// through the constant address space lets the compiler use scalar loads
+// through the scalar cache.  Never use it for data that kernels write (work
+// lists, counts, row masks, occupancy): scalar loads of such data were seen
+// to return values older than an earlier launch's stores (the retire
+// launch's reads of the head row masks made the deferral path lose bits
+// intermittently until they went through the vector path).  ld_dev is for
+// those.
 template <class T>
 __device__ __forceinline__ T ld_const(const T *p) {
     return *(const __attribute__((address_space(4))) T *)p;
+}
+// Read of memory that kernels write, always through the vector path (an
+// atomic load at work-group scope is a plain vector load that the compiler
+// never turns into a scalar one).
+template <class T>
+__device__ __forceinline__ T ld_dev(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Work-group-wide dynamic work cursor for the persistent kernels: thread 0
@@ -265,6 +276,11 @@ __device__ __forceinline__ void quarter_transpose(const Row4 &v, const Row4 &cu,
     }
 }
 
+// A wave-uniform 32-byte row of kernel-written data (ld_dev), in scalar registers.
+__device__ __forceinline__ Row4 ld_dev_row_uniform(const uint64_t *p) {
+    return Row4{{uniform64(ld_dev(p)), uniform64(ld_dev(p + 1)), uniform64(ld_dev(p + 2)), uniform64(ld_dev(p + 3))}};
+}
+
 __device__ __forceinline__ Row4 ld_const_row(const uint64_t *p) {
     const __attribute__((address_space(4))) uint64_t *c = (const __attribute__((address_space(4))) uint64_t *)p;
     Row4 r;
@@ -397,13 +413,20 @@ __device__ __forceinline__ void wave_append(bool want, uint64_t val, uint32_t *c
 // paths are off).
 __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, const TermDev *__restrict__ terms,
                                            uint32_t t0, uint32_t t1, int parity, uint32_t T, uint32_t I, uint32_t J,
-                                           uint32_t &qmask, bool &dense, bool &mfma, uint32_t *qcnt, int heavy) {
+                                           uint32_t &qmask, bool &dense, bool &mfma, uint32_t *qcnt, int heavy,
+                                           bool live = true, bool all_lanes = false) {
     // When a tile row spans whole waves (T a multiple of 64), the wave's
     // tiles share the tile row I: the A operand's words of row I (occupancy,
-    // per-quarter, heavy-tile bitsets) are the same for every lane and come in
-    // scalar loads, eight words at a time and together with the B column's
-    // words, instead of a chain of dependent loads per word.
+    // per-quarter, heavy-tile bitsets) are the same for every lane, eight
+    // words at a time and together with the B column's words, instead of a
+    // chain of dependent loads per word.  all_lanes: every lane of the wave
+    // runs this (the tiles to scan are the `live` ones), and the row's 40
+    // words come in one vector load, a word per lane, read lane by lane;
+    // otherwise (callers under divergent conditions) every active lane reads
+    // each word at one address.  (Kernels write these words: never through
+    // the scalar cache.)
     const bool row_uniform = (T % WAVE) == 0;
+    if (row_uniform && all_lanes && !__ballot(live)) return;
     for (uint32_t t = t0; t < t1; ++t) {
         const MatDev L = mats[terms[t].left[parity]];
         const MatDev R = mats[terms[t].right[parity]];
@@ -411,20 +434,36 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
             const uint32_t Iu = uniform(I);
             for (uint32_t w0 = 0; w0 < L.ow; w0 += 8) {
                 uint32_t col[8], qw[QUARTERS][8];
+                if (all_lanes) {
+                    const int lane = threadIdx.x & 63;
+                    const uint32_t lw = w0 + (uint32_t)(lane & 7);
+                    const int lq = (lane >> 3) - 1;  // -1: occupancy word, 0..3: quarter lq
+                    uint32_t rowv = 0;
+                    if (lane < 40 && lw < L.ow)
+                        rowv = lq < 0 ? gld(&L.occR[Iu * L.ow + lw]) : gld(&L.occQ[((uint32_t)lq * T + Iu) * L.ow + lw]);
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const uint32_t w = w0 + (uint32_t)i;
-                    col[i] = w < L.ow ? gld(&R.occC[J * R.ow + w]) & ld_const(&L.occR[Iu * L.ow + w]) : 0u;
+                    for (int i = 0; i < 8; ++i) {
+                        const uint32_t w = w0 + (uint32_t)i;
+                        col[i] = (live && w < L.ow) ? gld(&R.occC[J * R.ow + w]) & lane_read(rowv, (uint32_t)i) : 0u;
 #pragma unroll
-                    for (uint32_t q = 0; q < QUARTERS; ++q)
-                        qw[q][i] = w < L.ow ? ld_const(&L.occQ[(q * T + Iu) * L.ow + w]) : 0u;
+                        for (uint32_t q = 0; q < QUARTERS; ++q) qw[q][i] = lane_read(rowv, 8u + q * 8u + (uint32_t)i);
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const uint32_t w = w0 + (uint32_t)i;
+                        col[i] = w < L.ow ? gld(&R.occC[J * R.ow + w]) & uniform(ld_dev(&L.occR[Iu * L.ow + w])) : 0u;
+#pragma unroll
+                        for (uint32_t q = 0; q < QUARTERS; ++q)
+                            qw[q][i] = w < L.ow ? uniform(ld_dev(&L.occQ[(q * T + Iu) * L.ow + w])) : 0u;
+                    }
                 }
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const uint32_t w = w0 + (uint32_t)i;
                     if (heavy && col[i]) {
-                        if (ld_const(&L.occD[Iu * L.ow + w]) & col[i]) dense = true;
-                        if (ld_const(&L.occX[Iu * L.ow + w]) & col[i]) mfma = true;
+                        if (ld_dev(&L.occD[Iu * L.ow + w]) & col[i]) dense = true;
+                        if (ld_dev(&L.occX[Iu * L.ow + w]) & col[i]) mfma = true;
                     }
 #pragma unroll
                     for (uint32_t q = 0; q < QUARTERS; ++q) {
@@ -436,6 +475,7 @@ __device__ __forceinline__ void scan_terms(const MatDev *__restrict__ mats, cons
             }
             continue;
         }
+        if (!live) continue;
         for (uint32_t w = 0; w < L.ow; ++w) {
             uint32_t col = gld(&R.occC[J * R.ow + w]);
             if (!col) continue;
@@ -623,11 +663,12 @@ __global__ __launch_bounds__(BLOCK) void k_build(const MatDev *__restrict__ mats
     bool dense = false, mfma = false;
     uint32_t qmask = 0;
     uint32_t qcnt[QUARTERS] = {0, 0, 0, 0};
-    if (p < T * T) {
-        const uint32_t I = p / T, J = p - I * T;
-        if (gld(&mats[target].cnt[p]) < TILE_BITS) {
-            scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma, qcnt, heavy);
-        }
+    {
+        // every lane scans (the live tiles: inside the head, not full)
+        const bool in = p < T * T;
+        const uint32_t I = in ? p / T : 0u, J = in ? p - (p / T) * T : 0u;
+        const bool live = in && gld(&mats[target].cnt[p]) < TILE_BITS;
+        scan_terms(mats, terms, t0, t1, parity, T, I, J, qmask, dense, mfma, qcnt, heavy, live, true);
     }
     wave_append(mfma, flag | ((uint64_t)h << 40) | p, &c->n_mfma, mfma_items);
     wave_append(dense && !mfma, flag | ((uint64_t)h << 40) | p, &c->n_dense, dense_items);
@@ -657,7 +698,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
     for (uint32_t u0 = wave_next(&c->unit_next, chunk); u0 < n_units; u0 = wave_next(&c->unit_next, chunk)) {
     const uint32_t u1 = (u0 + chunk < n_units) ? u0 + chunk : n_units;
     for (uint32_t u = u0; u < u1; ++u) {
-        const uint64_t it = ld_const(units + u);
+        const uint64_t it = uniform64(ld_dev(units + u));
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform((uint32_t)((it & ITEM_TILE_MASK) >> 2));
         const uint32_t q = uniform((uint32_t)(it & 3u));
@@ -673,7 +714,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
             const MatDev L = mats[ld_const(&terms[t].left[parity])];
             const MatDev R = mats[ld_const(&terms[t].right[parity])];
             for (uint32_t w = 0; w < L.ow; ++w) {
-                uint32_t kb = uniform(ld_const(&L.occQ[(q * T + I) * L.ow + w]) & ld_const(&R.occC[J * R.ow + w]));
+                uint32_t kb = uniform(ld_dev(&L.occQ[(q * T + I) * L.ow + w]) & ld_dev(&R.occC[J * R.ow + w]));
                 while (kb) {
                     // Up to SPARSE_BATCH candidate k-tiles at once, so that their
                     // dependent loads (row masks, then A rows, then B rows) are
@@ -698,9 +739,9 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse(const MatDev *__restric
                         brm[j] = Row4{{0, 0, 0, 0}};
                         if ((uint32_t)j < nk) {
                             // all three loads are independent and go out together
-                            brm[j] = ld_const_row(R.rmask + (size_t)rts[j] * 4);  // non-empty rows of the B tile
-                            const Row4 cu = ld_const_row(colq_of(L, lts[j], q));  // columns used by this A quarter
-                            const uint64_t arm = ld_const(&L.rmask[(size_t)lts[j] * 4 + q]);
+                            brm[j] = ld_dev_row_uniform(R.rmask + (size_t)rts[j] * 4);  // non-empty rows of the B tile
+                            const Row4 cu = ld_dev_row_uniform(colq_of(L, lts[j], q));  // columns used by this A quarter
+                            const uint64_t arm = uniform64(ld_dev(&L.rmask[(size_t)lts[j] * 4 + q]));
                             const bool hit = ((cu.w[0] & brm[j].w[0]) | (cu.w[1] & brm[j].w[1]) |
                                               (cu.w[2] & brm[j].w[2]) | (cu.w[3] & brm[j].w[3])) != 0;
                             rm[j] = hit ? arm : 0;
@@ -1000,7 +1041,7 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         // scalar cache, which neighbouring units of a chunk share; loading a
         // chunk's entries one per lane through the vector path measured
         // slower on batches of small units)
-        const uint64_t it = phase == 1 ? ld_const(prio_list + u) : ld_const(units + (u < n_heavy ? heavy_base + u : u - n_heavy));
+        const uint64_t it = uniform64(phase == 1 ? ld_dev(prio_list + u) : ld_dev(units + (u < n_heavy ? heavy_base + u : u - n_heavy)));
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform(unit_tile(it));
         // the unit's quarters: one, or every quarter of the tile with candidates
@@ -1311,8 +1352,8 @@ __global__ __launch_bounds__(WAVE) void k_merge_split(const MatDev *__restrict__
     uint32_t out_rows = 0;
     uint64_t new_bits = 0;
     for (uint32_t sq = blockIdx.x; sq < n_split; sq += gridDim.x) {
-        const uint64_t it = ld_const(splits + 2 * (size_t)sq);
-        const uint32_t at = (uint32_t)ld_const(splits + 2 * (size_t)sq + 1);
+        const uint64_t it = uniform64(ld_dev(splits + 2 * (size_t)sq));
+        const uint32_t at = (uint32_t)uniform64(ld_dev(splits + 2 * (size_t)sq + 1));
         const uint32_t h = uniform(item_head(it)), p = uniform(unit_tile(it)), q = uniform((uint32_t)(it & 3u));
         const uint32_t nsub = uniform(unit_nsub(it));
         // the sub-units' partial rows, four sub-units' loads in flight at a time
@@ -1322,7 +1363,7 @@ __global__ __launch_bounds__(WAVE) void k_merge_split(const MatDev *__restrict__
 #pragma unroll
             for (uint32_t j = 0; j < 4; ++j) {
                 const uint32_t sidx = s0 + j;
-                const uint64_t m = sidx < nsub ? ld_const(pmask + at + sidx) : 0ull;
+                const uint64_t m = sidx < nsub ? ld_dev(pmask + at + sidx) : 0ull;
                 part[j] = ((m >> lane) & 1ull) ? load_row(partials + ((size_t)(at + sidx) * WAVE + lane) * TROW_WORDS)
                                                : Row4{{0, 0, 0, 0}};
             }
@@ -1543,7 +1584,7 @@ __global__ __launch_bounds__(BLOCK) void k_product_dense(const MatDev *__restric
         // sweeps: the step that last evaluated this entry (0: none in this
         // call); a tile-product whose two operand tiles have not changed since
         // then was evaluated then, with the same operands, and is skipped
-        const uint32_t last = last_eval ? uniform(ld_const(&last_eval[h])) : 0u;
+        const uint32_t last = last_eval ? uniform(ld_dev(&last_eval[h])) : 0u;
         const MatDev M = mats[uniform(heads[h].target)];
         // the head's row r as it is now (a subset of it later, with the head
         // update fused): the bits this item cannot add
@@ -1876,14 +1917,28 @@ __device__ __forceinline__ void apply_tile(const MatDev *__restrict__ mats, cons
     const MatDev O = mats[uniform(heads[h].out[parity])];
     const uint32_t T = uniform(heads[h].T), I = p / T, J = p - (p / T) * T;
     const bool keepT = uniform(heads[h].colT) != 0;
-    const Row4 rmO = ld_const_row(O.rmask + (size_t)p * 4);
-    const Row4 rmM = ld_const_row(M.rmask + (size_t)p * 4);
-    Row4 ocu[QUARTERS], mcu[QUARTERS];
+    // the tile's row masks and column unions, delta and head: 40 words,
+    // one per lane in a single vector load (per-lane addresses, so never the
+    // scalar cache, which may hold them from an earlier launch), then read
+    // into scalar registers lane by lane
+    const uint64_t *src = lane < 4    ? O.rmask + (size_t)p * 4 + lane
+                          : lane < 8  ? M.rmask + (size_t)p * 4 + (lane - 4)
+                          : lane < 24 ? O.colq + (size_t)p * QUARTERS * TROW_WORDS + (lane - 8)
+                                      : M.colq + (size_t)p * QUARTERS * TROW_WORDS + (lane - 24);
+    const uint64_t meta = lane < 40 ? gld(src) : 0ull;
+    Row4 rmO, rmM, ocu[QUARTERS], mcu[QUARTERS];
 #pragma unroll
-    for (int q = 0; q < QUARTERS; ++q) {
-        ocu[q] = ld_const_row(colq_of(O, p, q));
-        mcu[q] = ld_const_row(colq_of(M, p, q));
+    for (int w = 0; w < 4; ++w) {
+        rmO.w[w] = lane_read64(meta, (uint32_t)w);
+        rmM.w[w] = lane_read64(meta, 4u + (uint32_t)w);
     }
+#pragma unroll
+    for (int q = 0; q < QUARTERS; ++q)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            ocu[q].w[w] = lane_read64(meta, 8u + (uint32_t)(q * 4 + w));
+            mcu[q].w[w] = lane_read64(meta, 24u + (uint32_t)(q * 4 + w));
+        }
     if (merge && lane == 0) {
         const uint32_t old = gld(&M.cnt[p]);
         gst(&M.cnt[p], old + gld(&O.cnt[p]));
@@ -1937,7 +1992,8 @@ __device__ __forceinline__ void clear_tile(const MatDev *__restrict__ mats, cons
     const uint32_t h = uniform(item_head(it));
     const uint32_t p = uniform((uint32_t)(it & ITEM_TILE_MASK));
     const MatDev O = mats[uniform(heads[h].out[parity])];
-    const Row4 rm = ld_const_row(O.rmask + (size_t)p * 4);
+    const uint64_t rml = lane < 4 ? gld(O.rmask + (size_t)p * 4 + lane) : 0ull;  // (per-lane: vector path)
+    const Row4 rm = Row4{{lane_read64(rml, 0), lane_read64(rml, 1), lane_read64(rml, 2), lane_read64(rml, 3)}};
 #pragma unroll
     for (int q = 0; q < QUARTERS; ++q) {
         if ((rm.w[q] >> lane) & 1ull)

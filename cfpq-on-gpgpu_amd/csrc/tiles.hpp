@@ -212,13 +212,17 @@ struct TouchedLists {
     uint32_t seg_cap;
 };
 // Entry e of the concatenated segments (e below the sum of the counts).
+// (the entries are written by kernels with plain stores: read through the
+// vector path, never the scalar cache)
 __device__ __forceinline__ uint64_t touched_entry(const StepCounters *c, TouchedLists tl, uint32_t e) {
     uint32_t n = c->n_touched;
-    if (e < n) return tl.list[e];
+    if (e < n) return __hip_atomic_load(&tl.list[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     e -= n;
     for (int s = 0; s < SHARDS; ++s) {
         n = c->shard[s].n_touched;
-        if (e < n) return tl.list[(size_t)(1 + s) * tl.seg_cap + e];
+        if (e < n)
+            return __hip_atomic_load(&tl.list[(size_t)(1 + s) * tl.seg_cap + e], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
         e -= n;
     }
     return 0;  // not reached for e below the total
