@@ -177,6 +177,32 @@ def test_learned_sg_forest_65536_golden(pkg, gen):
         gr.free()
 
 
+def test_learned_sg_forest_65536_repeatable(pkg, gen):
+    """The deferral path on the 65,536-vertex forest, three times under
+    different launch shapes, each against the golden fingerprints.  (It once
+    lost bits from run to run: the retire launch read head row masks through
+    the scalar cache, which could hold them from an earlier launch.)"""
+    import json
+    import os
+    cf = __import__("importlib").import_module("cfpq-on-gpgpu_amd.cfpq")
+    sched = __import__("importlib").import_module("cfpq-on-gpgpu_amd.sched")
+    tune = __import__("importlib").import_module("cfpq-on-gpgpu_amd.tune")
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixpoints.json")))["config4"]
+    want = {k: tuple(v) for k, v in golden["fingerprints"].items()}
+    q = gen.config(4)
+    weights = sched.load()
+    assert weights is not None, "trained scheduler weights are missing"
+    for waves in (8, 12, 16):
+        opts = dict(tune.tuned_opts(q.n, q.grammar), sched=True, sparse_waves_per_cu=waves)
+        with pkg.Engine(0, **opts) as eng:
+            eng.sched_set_weights(weights)
+            gr = cf.Grammar(q.grammar, text=True)
+            _, _, st = gr.intersection_with_graph(cf.Graph.from_query(q), eng)
+            assert gr.results() == want, waves
+            assert st["sched_deferred"] > 0
+            gr.free()
+
+
 @pytest.mark.parametrize("which", ["dense", "dyck", "sg", "closure"])
 def test_ordered_fixpoint_is_bit_exact(pkg, gen, oracle, which):
     """The scheduler's ranking (its predicted-active items evaluated in a
