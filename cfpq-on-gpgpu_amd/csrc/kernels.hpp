@@ -549,23 +549,32 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
             if (tu) rest &= ~gm;
         }
     }
-    uint32_t nsub[QUARTERS];
+    // hv[q]: the quarter's entries in the heavy list -- its sub-units (more
+    // than one), or 1 for a quarter too light to split but of more than
+    // HEAVY_UNSPLIT_16THS / 16 of the split threshold, which the heavy list
+    // only moves to the front of the claim order, so that it starts with the
+    // step instead of ending it late; 0: the quarter is one unit of the
+    // ordinary list
+    uint32_t hv[QUARTERS];
     uint64_t single[QUARTERS];
     uint32_t total = 0;
+    const uint32_t big = split / 16u * HEAVY_UNSPLIT_16THS;
 #pragma unroll
     for (uint32_t q = 0; q < QUARTERS; ++q) {
         const bool wq = ((rest >> q) & 1u) != 0;
-        nsub[q] = wq ? 1u : 0u;
+        hv[q] = 0;
         uint32_t want = 0;
         if (wq && split && qcnt[q] > split) {
             uint32_t ns = (uint32_t)(((uint64_t)qcnt[q] + piece - 1) / piece);
             ns = ns > SPLIT_MAX ? SPLIT_MAX : (ns < 1 ? 1 : ns);
-            want = ns > 1 ? ns : 0u;
+            want = ns;
+        } else if (wq && split && qcnt[q] > big) {
+            want = 1;
         }
-        // the budget counts sub-units (partial-row slots), reserved for the
-        // wave's heavy quarters with one atomic (lane i's slots end at the
-        // prefix sum of the requests); a refused reservation only leaves
-        // budget unused
+        // the budget counts heavy-list entries (a sub-unit's is also its
+        // partial-row slot), reserved for the wave's heavy quarters with one
+        // atomic (lane i's entries end at the prefix sum of the requests); a
+        // refused reservation only leaves budget unused
         const uint64_t wb = __ballot(want != 0);
         if (wb) {
             uint32_t tot = 0;
@@ -573,9 +582,9 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
             uint32_t old = 0;
             if (lane == 0) old = atomicAdd(&c->split_extra, tot);
             old = uniform(old);
-            if (want) nsub[q] = old + incl > split_budget ? 1u : want;
+            if (want) hv[q] = old + incl > split_budget ? 0u : want;
         }
-        single[q] = __ballot(nsub[q] == 1);
+        single[q] = __ballot(wq && hv[q] == 0);
         total += (uint32_t)__popcll(single[q]);
     }
     // The single units' list positions: one atomic per work-group (block_agg:
@@ -603,31 +612,33 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
     if (total) {
 #pragma unroll
         for (uint32_t q = 0; q < QUARTERS; ++q) {
-            if (nsub[q] == 1) units[at + __popcll(single[q] & below)] = base | (uint64_t)q;
+            if ((single[q] >> lane) & 1u) units[at + __popcll(single[q] & below)] = base | (uint64_t)q;
             at += (uint32_t)__popcll(single[q]);
         }
     }
-    // split quarters: their sub-units and split entries, one atomic per wave each
+    // heavy quarters: their heavy-list entries, and the split entries of the
+    // split ones, one atomic per wave each
 #pragma unroll
     for (uint32_t q = 0; q < QUARTERS; ++q) {
-        const uint32_t hv = nsub[q] > 1 ? nsub[q] : 0u;
-        const uint64_t hb = __ballot(hv != 0);
+        const uint32_t n = hv[q];
+        const uint64_t hb = __ballot(n != 0);
         if (!hb) continue;
+        const uint64_t sb = __ballot(n > 1);  // split quarters
         uint32_t tot = 0;
-        const uint32_t incl = scan_ballot(hv, hb, tot);
+        const uint32_t incl = scan_ballot(n, hb, tot);
         uint32_t hbase = 0, sbase = 0;
         if (lane == 0) {
             hbase = atomicAdd(&c->n_heavy, tot);
-            if (!split_rows) sbase = atomicAdd(&c->n_split, (uint32_t)__popcll(hb));
+            if (!split_rows && sb) sbase = atomicAdd(&c->n_split, (uint32_t)__popcll(sb));
         }
         hbase = uniform(hbase);
         sbase = uniform(sbase);
-        if (hv) {
-            const uint32_t hat = hbase + incl - hv;
-            const uint64_t u = base | (uint64_t)q | ((uint64_t)(hv - 1) << 59) | (split_rows ? ITEM_ROWSPLIT : 0ull);
-            for (uint32_t sidx = 0; sidx < hv; ++sidx) units[heavy_base + hat + sidx] = u | ((uint64_t)sidx << 55);
-            if (!split_rows) {  // candidate split: the quarter is finished by k_merge_split
-                const uint32_t sq = sbase + (uint32_t)__popcll(hb & below);
+        if (n) {
+            const uint32_t hat = hbase + incl - n;
+            const uint64_t u = base | (uint64_t)q | ((uint64_t)(n - 1) << 59) | (split_rows && n > 1 ? ITEM_ROWSPLIT : 0ull);
+            for (uint32_t sidx = 0; sidx < n; ++sidx) units[heavy_base + hat + sidx] = u | ((uint64_t)sidx << 55);
+            if (!split_rows && n > 1) {  // candidate split: the quarter is finished by k_merge_split
+                const uint32_t sq = sbase + (uint32_t)__popcll(sb & below);
                 splits[2 * (size_t)sq] = u;
                 splits[2 * (size_t)sq + 1] = hat;
             }
@@ -967,7 +978,8 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
     const int lane = threadIdx.x & 63;
     const uint64_t clk0 = __builtin_amdgcn_s_memtime();
     // phase 1: the units the scheduler ranked first (prio_list); otherwise
-    // list entries [0, n_heavy) are the sub-units of split units, then the others
+    // list entries [0, n_heavy) are the heavy list (sub-units of split units and
+    // quarters of more than half the split threshold), then the others
     const uint32_t n_heavy = phase == 1 ? 0u : c->n_heavy;
     const uint32_t n_units = phase == 1 ? c->n_prio : c->n_units + n_heavy;
     // Claims x are mapped to list entries by reversing the low six bits of x:
@@ -1037,10 +1049,10 @@ __global__ __launch_bounds__(WAVE) void k_product_sparse_lp(const MatDev *__rest
         const uint32_t u = direct ? x : (x & ~63u) | (__builtin_bitreverse32(x & 63u) >> 26);
         if (u >= n_units) continue;
         const uint64_t tu0 = __builtin_amdgcn_s_memtime();
-        // (scalar loads: the list entry and the head's fields come through the
-        // scalar cache, which neighbouring units of a chunk share; loading a
-        // chunk's entries one per lane through the vector path measured
-        // slower on batches of small units)
+        // (the list entry is written by this step's build, so it is read through
+        // the vector path; the head's fields are host-written and come through
+        // the scalar cache.  Loading a chunk's entries one per lane in one
+        // load measured no faster.)
         const uint64_t it = uniform64(phase == 1 ? ld_dev(prio_list + u) : ld_dev(units + (u < n_heavy ? heavy_base + u : u - n_heavy)));
         const uint32_t h = uniform(item_head(it));
         const uint32_t p = uniform(unit_tile(it));

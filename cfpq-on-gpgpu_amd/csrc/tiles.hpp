@@ -140,9 +140,9 @@ struct StepCounters {
     uint32_t dirty_new;    // tiles that became dirty (deferred) in this step
     uint32_t dirty_cleared; // dirty tiles evaluated in full (or found full) in this step
     uint32_t max_unit_cands; // most candidate k-tiles of one sparse unit
-    uint32_t split_extra;    // extra sub-units created by splitting heavy units (budget counter)
+    uint32_t split_extra;    // heavy-list entries requested (budget counter)
     uint32_t n_split;        // heavy units split
-    uint32_t n_heavy;        // sub-units of split units, listed apart so that they are claimed first
+    uint32_t n_heavy;        // heavy-list entries (sub-units of split units, heavy unsplit quarters), claimed first
     uint32_t n_prio;         // units the scheduler ranked first (evaluated by the step's first phase)
     uint32_t shards_done;    // lane-parallel sparse kernel: bit s = shard s's units all claimed
     uint32_t prio_shards_done;  // the same for the scheduler's first phase
@@ -256,6 +256,13 @@ constexpr uint64_t ITEM_FULL = 1ull << 63;
 constexpr uint64_t ITEM_ROWSPLIT = 1ull << 39;
 constexpr uint32_t MAX_HEADS = 1u << 15;   // heads per plan (15-bit field)
 constexpr uint32_t SPLIT_MAX = 16;         // sub-units per heavy sparse unit
+// Unsplit quarters of more than this many 16ths of the split threshold go to
+// the heavy list (claimed first).  On the 65,536-vertex forest the longest
+// unit of the odd steps is such a quarter (117-119 candidates against a
+// threshold of 128); claimed first it no longer ends its step late.  12/16
+// measured best against 10, 13, 14, 15 and 4-8 (more units moved to the front
+// crowd the first claims; fewer miss the hub).
+constexpr uint32_t HEAVY_UNSPLIT_16THS = 12;
 constexpr uint32_t SPLIT_CANDS = 64;       // default of cfpq_opts.split_cands: a unit with more
                                            // candidates than this is split, one sub-unit per
                                            // split_cands candidates
