@@ -260,8 +260,8 @@ constexpr uint32_t SPLIT_MAX = 16;         // sub-units per heavy sparse unit
 // the heavy list (claimed first).  On the 65,536-vertex forest the longest
 // unit of the odd steps is such a quarter (117-119 candidates against a
 // threshold of 128); claimed first it no longer ends its step late.  12/16
-// measured best against 10, 13, 14, 15 and 4-8 (more units moved to the front
-// crowd the first claims; fewer miss the hub).
+// measured best (13/16 the same within noise; 4, 8, 10, 14 and 15 slower:
+// more units moved to the front crowd the first claims, fewer miss the hub).
 constexpr uint32_t HEAVY_UNSPLIT_16THS = 12;
 constexpr uint32_t SPLIT_CANDS = 64;       // default of cfpq_opts.split_cands: a unit with more
                                            // candidates than this is split, one sub-unit per
