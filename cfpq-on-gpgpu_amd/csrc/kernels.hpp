@@ -566,6 +566,11 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
         uint32_t want = 0;
         if (wq && split && qcnt[q] > split) {
             uint32_t ns = (uint32_t)(((uint64_t)qcnt[q] + piece - 1) / piece);
+            // a quarter of at least SPLIT_MAX / 2 pieces takes all SPLIT_MAX
+            // sub-units: on the forest such quarters' candidates also carry
+            // the most (hit, column) items each, so their sub-units were the
+            // longest of their step
+            if (ns >= SPLIT_MAX / 2) ns = SPLIT_MAX;
             ns = ns > SPLIT_MAX ? SPLIT_MAX : (ns < 1 ? 1 : ns);
             want = ns;
         } else if (wq && split && qcnt[q] > big) {
