@@ -558,7 +558,7 @@ __device__ __forceinline__ void append_units(bool want, uint32_t qmask, const ui
     uint32_t hv[QUARTERS];
     uint64_t single[QUARTERS];
     uint32_t total = 0;
-    const uint32_t big = split / 16u * HEAVY_UNSPLIT_16THS;
+    const uint32_t big = (uint32_t)((uint64_t)split * HEAVY_UNSPLIT_16THS / 16u);
 #pragma unroll
     for (uint32_t q = 0; q < QUARTERS; ++q) {
         const bool wq = ((rest >> q) & 1u) != 0;

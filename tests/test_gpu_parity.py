@@ -580,11 +580,14 @@ def test_fixpoint_split_quarters(pkg, gen, oracle, which, split_rows):
     sub-unit writes its own rows and ORs the quarter's metadata): with
     thresholds of 1 and 2 nearly every quarter is split (and with pieces of
     one candidate, quarters of more than 4 split 16 ways), and the fixpoint
-    must still equal the oracle's word for word, for quarter and tile units."""
+    must still equal the oracle's word for word, for quarter and tile units.
+    Thresholds of 8 and 12 mix the three kinds of entry in one step: unsplit
+    quarters of more than 3/4 of the threshold in the heavy list, split ones
+    (of at least 8 pieces: 16 sub-units) and ordinary ones."""
     q = {"dense": lambda: gen.dense_stress(1100), "dyck": lambda: gen.dyck_er(1200, 4000, seed=9),
          "sg": lambda: gen.sg_forest(6000, seed=10, types_per_vertex=4)}[which]()
     for cands, piece, mode, schedule in ((1, 0, 0, "delta"), (2, 0, 2, "delta"), (1, 0, 0, "rules"),
-                                         (4, 1, 0, "delta")):
+                                         (4, 1, 0, "delta"), (8, 1, 0, "delta"), (12, 4, 0, "delta")):
         st = _fixpoint_vs_oracle(pkg, gen, oracle, q, kernel="auto", split_rows=split_rows, split_cands=cands,
                                  split_piece=piece, sparse_mode=mode, schedule=schedule, dense_bits=0xFFFFFFF0,
                                  mfma_bits=0xFFFFFFF0)
